@@ -1,0 +1,289 @@
+"""dollarstorescf_amd -- MI355X-native direct-SCF Fock-build engine.
+
+Python view of the in-tree C-ABI library ``libdsscf.so`` (include/dsscf.h and
+include/dsscf_scf.h).  Everything numeric runs in the library: HIP kernels
+for ERIs/Boys/screening/J-K on the GPU, the host C++ SCF layer for the rest.
+There is no fallback: if the library or a GPU is missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdsscf.so")
+BASIS_DIR = os.path.join(HERE, "basis")
+
+BOYS_REFERENCE = 0
+BOYS_ACCURATE = 1
+
+ERRORS = {1: "bad argument", 2: "HIP error", 3: "call out of order", 4: "unsupported",
+          5: "device allocation failed", 6: "did not converge"}
+
+
+class DsscfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{ERRORS.get(code, code)}] {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("boys_mode", ctypes.c_int), ("schwarz_threshold", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_pairs", ctypes.c_longlong), ("n_prim_pairs", ctypes.c_longlong),
+                ("n_quartets", ctypes.c_longlong), ("n_prim_quartets", ctypes.c_longlong),
+                ("n_quartets_unscreened", ctypes.c_longlong), ("n_class", ctypes.c_int),
+                ("class_quartets", ctypes.c_longlong * 21), ("class_prim_quartets", ctypes.c_longlong * 21)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libdsscf.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run `python -m dollarstorescf_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        i, d, p, c = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_char_p
+        L.dsscf_last_error.restype = c
+        L.dsscf_job_error.restype = c
+        L.dsscf_job_error.argtypes = [p]
+        L.dsscf_job_context.restype = p
+        L.dsscf_job_context.argtypes = [p]
+        sigs = {
+            "dsscf_ctx_create": [p, i], "dsscf_ctx_destroy": [p], "dsscf_set_options": [p, p],
+            "dsscf_set_basis": [p, i, p, p, p, p, p, p], "dsscf_prepare": [p],
+            "dsscf_jk": [p, p, p, p, p], "dsscf_jk_device": [p, p, p, p, p],
+            "dsscf_eri_batch": [p, p, ctypes.c_longlong, p], "dsscf_schwarz": [p, p],
+            "dsscf_get_stats": [p, p], "dsscf_set_density": [p, p, p], "dsscf_jk_resident": [p],
+            "dsscf_get_jk": [p, p, p], "dsscf_time_jk": [p, i, i, p, p, p], "dsscf_class_l": [i, p],
+            "dsscf_job_create": [p, i], "dsscf_job_destroy": [p],
+            "dsscf_job_set_options": [p, i, d, i, i, d, i],
+            "dsscf_job_set_molecule": [p, i, p, p, i, i], "dsscf_job_load_xyz": [p, c, i, i],
+            "dsscf_job_set_basis_file": [p, c], "dsscf_job_construct": [p], "dsscf_job_construct_host": [p],
+            "dsscf_job_ao": [p, i, p, p, p, p, p, p, p, p, p, i], "dsscf_job_n_ao": [p],
+            "dsscf_job_get": [p, c, p], "dsscf_job_set_density": [p, p], "dsscf_job_fock_build": [p],
+            "dsscf_job_solve": [p, p, p], "dsscf_job_energies": [p, p], "dsscf_job_history": [p, p, i],
+        }
+        for name, args in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = i
+        for name in ("dsscf_max_l", "dsscf_class_count", "dsscf_xform_size"):
+            getattr(L, name).restype = i
+            getattr(L, name).argtypes = []
+        _LIB = L
+    return _LIB
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc, msg_fn=None):
+    if rc != 0:
+        msg = msg_fn() if msg_fn else lib().dsscf_last_error().decode()
+        raise DsscfError(rc, msg)
+
+
+def basis_path(name: str) -> str:
+    return os.path.join(BASIS_DIR, name)
+
+
+def class_list():
+    L = lib()
+    out = []
+    for c in range(L.dsscf_class_count()):
+        l4 = np.zeros(4, np.int32)
+        _check(L.dsscf_class_l(c, _p(l4)))
+        out.append(tuple(int(v) for v in l4))
+    return out
+
+
+class Context:
+    """One dsscf_ctx (device buffers + stream) over explicit shell data."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(lib().dsscf_ctx_create(ctypes.byref(self._h), device))
+        self._own = True
+
+    @classmethod
+    def borrow(cls, handle):
+        obj = cls.__new__(cls)
+        obj._h = ctypes.c_void_p(handle)
+        obj._own = False
+        return obj
+
+    def close(self):
+        if getattr(self, "_own", False) and self._h:
+            lib().dsscf_ctx_destroy(self._h)
+        self._h = None
+
+    __del__ = close
+
+    def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12):
+        o = Options(boys_mode, schwarz_threshold)
+        _check(lib().dsscf_set_options(self._h, ctypes.byref(o)))
+
+    def set_basis(self, l, nprim, centers, exps, coefs, xform):
+        arrs = [np.ascontiguousarray(l, np.int32), np.ascontiguousarray(nprim, np.int32),
+                np.ascontiguousarray(centers, np.float64).reshape(-1), np.ascontiguousarray(exps, np.float64),
+                np.ascontiguousarray(coefs, np.float64), np.ascontiguousarray(xform, np.float64)]
+        _check(lib().dsscf_set_basis(self._h, len(arrs[0]), *[_p(a) for a in arrs]))
+
+    def prepare(self):
+        _check(lib().dsscf_prepare(self._h))
+
+    def jk(self, Ptot_packed, Pa_packed):
+        Pt = np.ascontiguousarray(Ptot_packed, np.float64)
+        Pa = np.ascontiguousarray(Pa_packed, np.float64)
+        J, K = np.empty_like(Pt), np.empty_like(Pt)
+        _check(lib().dsscf_jk(self._h, _p(Pt), _p(Pa), _p(J), _p(K)))
+        return J, K
+
+    def set_density(self, Ptot_packed, Pa_packed):
+        self._Pt = np.ascontiguousarray(Ptot_packed, np.float64)
+        self._Pa = np.ascontiguousarray(Pa_packed, np.float64)
+        _check(lib().dsscf_set_density(self._h, _p(self._Pt), _p(self._Pa)))
+
+    def jk_resident(self):
+        _check(lib().dsscf_jk_resident(self._h))
+
+    def get_jk(self, n_ao):
+        J, K = np.empty(n_ao * (n_ao + 1) // 2), np.empty(n_ao * (n_ao + 1) // 2)
+        _check(lib().dsscf_get_jk(self._h, _p(J), _p(K)))
+        return J, K
+
+    def eri(self, quartets):
+        q = np.ascontiguousarray(quartets, np.int32).reshape(-1, 4)
+        out = np.empty(len(q))
+        _check(lib().dsscf_eri_batch(self._h, _p(q), len(q), _p(out)))
+        return out
+
+    def stats(self) -> Stats:
+        st = Stats()
+        _check(lib().dsscf_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def schwarz(self, n_pairs):
+        Q = np.empty(n_pairs)
+        _check(lib().dsscf_schwarz(self._h, _p(Q)))
+        return Q
+
+    def time_jk(self, warmup=2, iters=10):
+        ms, dom = ctypes.c_double(), ctypes.c_double()
+        cls = ctypes.c_int()
+        _check(lib().dsscf_time_jk(self._h, warmup, iters, ctypes.byref(ms), ctypes.byref(dom), ctypes.byref(cls)))
+        return ms.value, dom.value, cls.value
+
+
+class Job:
+    """One SCF job: molecule + basis -> AOs, 1e terms, device Fock builds, RHF."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(lib().dsscf_job_create(ctypes.byref(self._h), device))
+
+    def _err(self):
+        return lib().dsscf_job_error(self._h).decode()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dsscf_job_destroy(self._h)
+        self._h = None
+
+    __del__ = close
+
+    def set_options(self, diis_size=8, energy_threshold=1e-8, max_iterations=500, boys_mode=BOYS_ACCURATE,
+                    schwarz_threshold=1e-12, verbose=0):
+        _check(lib().dsscf_job_set_options(self._h, diis_size, energy_threshold, max_iterations, boys_mode,
+                                           schwarz_threshold, verbose), self._err)
+        return self
+
+    def set_molecule(self, Z, xyz_bohr, charge=0, multiplicity=1):
+        Z = np.ascontiguousarray(Z, np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, np.float64).reshape(-1)
+        _check(lib().dsscf_job_set_molecule(self._h, len(Z), _p(Z), _p(xyz), charge, multiplicity), self._err)
+        return self
+
+    def load_xyz(self, path, charge=0, multiplicity=1):
+        _check(lib().dsscf_job_load_xyz(self._h, path.encode(), charge, multiplicity), self._err)
+        return self
+
+    def set_basis(self, name_or_path: str):
+        path = name_or_path if os.path.sep in name_or_path else basis_path(name_or_path)
+        _check(lib().dsscf_job_set_basis_file(self._h, path.encode()), self._err)
+        return self
+
+    def construct(self):
+        _check(lib().dsscf_job_construct(self._h), self._err)
+        return self
+
+    def construct_host(self):
+        """AO list and one-electron terms only (no GPU needed)."""
+        _check(lib().dsscf_job_construct_host(self._h), self._err)
+        return self
+
+    def ao(self, i: int) -> dict:
+        cap = 128
+        l, m, npr, nt = (np.zeros(1, np.int32) for _ in range(4))
+        c = np.zeros(3)
+        z, w, tw = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        tx = np.zeros(3 * cap, np.int32)
+        _check(lib().dsscf_job_ao(self._h, i, _p(l), _p(m), _p(c), _p(npr), _p(z), _p(w), _p(nt), _p(tw), _p(tx), cap),
+               self._err)
+        return dict(l=int(l[0]), m=int(m[0]), center=c, zeta=z[:npr[0]].copy(), weight=w[:npr[0]].copy(),
+                    terms=[(float(tw[t]), *map(int, tx[3 * t:3 * t + 3])) for t in range(nt[0])])
+
+    @property
+    def n_ao(self) -> int:
+        return lib().dsscf_job_n_ao(self._h)
+
+    def get(self, name: str) -> np.ndarray:
+        n = self.n_ao
+        out = np.empty(n if name == "eps" else n * (n + 1) // 2)
+        _check(lib().dsscf_job_get(self._h, name.encode(), _p(out)), self._err)
+        return out
+
+    def set_density(self, Pa_packed):
+        Pa = np.ascontiguousarray(Pa_packed, np.float64)
+        _check(lib().dsscf_job_set_density(self._h, _p(Pa)), self._err)
+
+    def fock_build(self):
+        _check(lib().dsscf_job_fock_build(self._h), self._err)
+        return self.get("J"), self.get("K")
+
+    def solve(self):
+        e = ctypes.c_double()
+        it = ctypes.c_int()
+        _check(lib().dsscf_job_solve(self._h, ctypes.byref(e), ctypes.byref(it)), self._err)
+        return e.value, it.value
+
+    def energies(self) -> dict:
+        out = np.empty(5)
+        _check(lib().dsscf_job_energies(self._h, _p(out)), self._err)
+        return dict(zip(["kinetic", "nuclear_attraction", "coulomb", "exchange", "nuclear"], out.tolist()))
+
+    def history(self):
+        n = lib().dsscf_job_history(self._h, None, 0)
+        out = np.empty(max(n, 1))
+        lib().dsscf_job_history(self._h, _p(out), n)
+        return out[:n]
+
+    @property
+    def context(self) -> Context:
+        return Context.borrow(lib().dsscf_job_context(self._h))
+
+
+def single_point(Z, xyz_bohr, basis: str, charge=0, multiplicity=1, device=0, **opts):
+    """Convenience: build and solve one RHF job; returns (energy, iterations, job)."""
+    job = Job(device).set_options(**opts).set_basis(basis).set_molecule(Z, xyz_bohr, charge, multiplicity).construct()
+    e, it = job.solve()
+    return e, it, job

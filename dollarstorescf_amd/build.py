@@ -1,0 +1,78 @@
+"""Build recipe for libdsscf.so (gfx950).
+
+  python -m dollarstorescf_amd.build [--maxl L] [--jobs J]
+
+1. gen_eri.py writes the per-class device code into csrc/_gen/.
+2. hipcc compiles the HIP kernels and the C-ABI for gfx950 (packed FP32
+   instructions disabled), g++ compiles the host SCF layer with OpenMP.
+3. hipcc links everything into dollarstorescf_amd/libdsscf.so (in-tree).
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+GEN = os.path.join(CSRC, "_gen")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libdsscf.so")
+ROOT = os.path.dirname(HERE)
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I" + GEN, "-I" + CSRC, "-I" + os.path.join(ROOT, "include")]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-I" + os.path.join(ROOT, "include")]
+
+HIP_SOURCES = ["eri_kernels.hip", "dsscf_abi.cpp"]
+HOST_SOURCES = ["host/chem.cpp", "host/linalg.cpp", "host/scf.cpp", "host/job_api.cpp"]
+
+DEFAULT_MAXL = 1
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build(maxl: int = DEFAULT_MAXL, jobs: int = 8, verbose: bool = False) -> str:
+    os.makedirs(GEN, exist_ok=True)
+    os.makedirs(OBJ, exist_ok=True)
+    _run([sys.executable, os.path.join(CSRC, "gen_eri.py"), os.path.join(GEN, "eri_classes.inc"), str(maxl)])
+    cmds = []
+    objs = []
+    for s in HIP_SOURCES:
+        o = os.path.join(OBJ, os.path.basename(s) + ".o")
+        lang = ["-x", "hip"]
+        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in HOST_SOURCES:
+        o = os.path.join(OBJ, os.path.basename(s) + ".o")
+        cmds.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for r in ex.map(_run, cmds):
+            if verbose and (r.stdout or r.stderr):
+                print(r.stdout + r.stderr)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--maxl", type=int, default=DEFAULT_MAXL)
+    ap.add_argument("--jobs", type=int, default=8)
+    a = ap.parse_args()
+    print(build(a.maxl, a.jobs, verbose=True))
+
+
+if __name__ == "__main__":
+    main()

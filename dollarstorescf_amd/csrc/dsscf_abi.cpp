@@ -1,0 +1,625 @@
+// dsscf_abi.cpp -- implementation of the C-ABI in include/dsscf.h.
+//
+// Owns the per-context device buffers, builds the shell-pair and
+// primitive-pair records, the Schwarz bounds and the class-sorted quartet
+// lists, and drives the per-class kernels of eri_kernels.hip.
+#include "../../include/dsscf.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "boys_table.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(DSSCF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// 2 pi^(5/2) with the reference's PI (GaussianIntegral.cpp:632, Constants.hpp:5).
+const double kEriScale = 2.0 * std::pow(3.14159265358979, 2.5);
+
+int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+int pair_class_of(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }
+
+template <typename T>
+int dev_alloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * n);
+    if (e != hipSuccess) return fail(DSSCF_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return DSSCF_OK;
+}
+
+template <typename T>
+void dev_free(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct dsscf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dsscf_options opt{DSSCF_BOYS_ACCURATE, 1e-12};
+
+    // basis
+    int n_shell = 0, n_ao = 0;
+    std::vector<int> sh_l, sh_nprim, sh_prim_off, sh_ao, ao_shell;
+    std::vector<double> sh_center, prim_exp, prim_coef, xform;
+    bool have_basis = false;
+
+    // pairs (index a(a+1)/2+b for shells a >= b)
+    std::vector<ShellPair> pairs;
+    std::vector<PrimPair> prims;
+    std::vector<int> pair_first, pair_class;
+    std::vector<double> Q;
+
+    // quartet lists
+    std::vector<long long> q_off, q_cnt, q_prim;
+    long long n_unscreened = 0;
+    bool prepared = false;
+
+    // device
+    ShellPair* d_pairs = nullptr;
+    PrimPair* d_pp = nullptr;
+    double* d_btab = nullptr;
+    uint2* d_q = nullptr;
+    double* d_Q = nullptr;
+    int* d_plist = nullptr;
+    double *d_Dt = nullptr, *d_Da = nullptr, *d_Jh = nullptr, *d_Kh = nullptr;
+    double *d_Pt_pk = nullptr, *d_Pa_pk = nullptr, *d_J_pk = nullptr, *d_K_pk = nullptr;
+
+    void free_geometry()
+    {
+        dev_free(d_pairs);
+        dev_free(d_pp);
+        dev_free(d_q);
+        dev_free(d_Q);
+        dev_free(d_plist);
+        dev_free(d_Dt);
+        dev_free(d_Da);
+        dev_free(d_Jh);
+        dev_free(d_Kh);
+        dev_free(d_Pt_pk);
+        dev_free(d_Pa_pk);
+        dev_free(d_J_pk);
+        dev_free(d_K_pk);
+        prepared = false;
+    }
+};
+
+extern "C" {
+
+const char* dsscf_last_error(void) { return g_err.c_str(); }
+int dsscf_max_l(void) { return dk_max_l(); }
+int dsscf_class_count(void) { return dk_n_class(); }
+int dsscf_xform_size(void) { return dk_xform_size(); }
+
+int dsscf_class_l(int c, int* l4)
+{
+    if (c < 0 || c >= dk_n_class() || !l4) return fail(DSSCF_ERR_ARG, "dsscf_class_l: bad class");
+    dk_class_l(c, l4);
+    return DSSCF_OK;
+}
+
+int dsscf_ctx_create(dsscf_ctx** out, int device)
+{
+    if (!out) return fail(DSSCF_ERR_ARG, "dsscf_ctx_create: null output");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(DSSCF_ERR_ARG, "dsscf_ctx_create: no such device");
+    HIP_TRY(hipSetDevice(device));
+    dsscf_ctx* c = new dsscf_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(DSSCF_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    std::vector<double> tab;
+    dsscf_boys_table(tab);
+    if (int rc = dev_alloc(&c->d_btab, tab.size())) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    e = hipMemcpy(c->d_btab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        dev_free(c->d_btab);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(DSSCF_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return DSSCF_OK;
+}
+
+int dsscf_ctx_destroy(dsscf_ctx* c)
+{
+    if (!c) return DSSCF_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->free_geometry();
+    dev_free(c->d_btab);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DSSCF_OK;
+}
+
+int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
+{
+    if (!c || !o) return fail(DSSCF_ERR_ARG, "dsscf_set_options: null argument");
+    if (o->boys_mode != DSSCF_BOYS_REFERENCE && o->boys_mode != DSSCF_BOYS_ACCURATE)
+        return fail(DSSCF_ERR_ARG, "dsscf_set_options: unknown boys_mode");
+    if (!(o->schwarz_threshold >= 0.0)) return fail(DSSCF_ERR_ARG, "dsscf_set_options: negative threshold");
+    const bool changed = o->boys_mode != c->opt.boys_mode || o->schwarz_threshold != c->opt.schwarz_threshold;
+    c->opt = *o;
+    if (changed) c->prepared = false;
+    return DSSCF_OK;
+}
+
+int dsscf_set_basis(dsscf_ctx* c, int n_shell, const int* shell_l, const int* shell_nprim,
+                    const double* shell_center, const double* prim_exp, const double* prim_coef,
+                    const double* ao_xform)
+{
+    if (!c || n_shell <= 0 || !shell_l || !shell_nprim || !shell_center || !prim_exp || !prim_coef || !ao_xform)
+        return fail(DSSCF_ERR_ARG, "dsscf_set_basis: null argument or no shells");
+    const int maxl = dk_max_l();
+    c->sh_l.assign(shell_l, shell_l + n_shell);
+    c->sh_nprim.assign(shell_nprim, shell_nprim + n_shell);
+    c->sh_center.assign(shell_center, shell_center + 3 * n_shell);
+    c->sh_prim_off.resize(n_shell);
+    c->sh_ao.resize(n_shell);
+    int np = 0, nao = 0;
+    for (int s = 0; s < n_shell; ++s) {
+        if (shell_l[s] < 0 || shell_l[s] > maxl)
+            return fail(DSSCF_ERR_UNSUPPORTED, "dsscf_set_basis: shell angular momentum " + std::to_string(shell_l[s]) +
+                                                   " exceeds the built maximum " + std::to_string(maxl));
+        if (shell_nprim[s] <= 0 || shell_nprim[s] > 100)
+            return fail(DSSCF_ERR_ARG, "dsscf_set_basis: primitive count outside 1-100");
+        c->sh_prim_off[s] = np;
+        c->sh_ao[s] = nao;
+        np += shell_nprim[s];
+        nao += 2 * shell_l[s] + 1;
+    }
+    for (int i = 0; i < np; ++i)
+        if (!(prim_exp[i] > 0.0)) return fail(DSSCF_ERR_ARG, "dsscf_set_basis: non-positive exponent");
+    c->prim_exp.assign(prim_exp, prim_exp + np);
+    c->prim_coef.assign(prim_coef, prim_coef + np);
+    c->xform.assign(ao_xform, ao_xform + dk_xform_size());
+    c->n_shell = n_shell;
+    c->n_ao = nao;
+    c->ao_shell.resize(nao);
+    for (int s = 0; s < n_shell; ++s)
+        for (int m = 0; m < 2 * shell_l[s] + 1; ++m) c->ao_shell[c->sh_ao[s] + m] = s;
+    c->have_basis = true;
+    c->prepared = false;
+    return DSSCF_OK;
+}
+
+namespace {
+
+// Shell pairs and primitive pairs; the first shell of a pair has the higher l
+// (equal l: the higher shell index).
+void build_pairs(dsscf_ctx* c)
+{
+    const int ns = c->n_shell;
+    const size_t npair = (size_t)ns * (ns + 1) / 2;
+    c->pairs.assign(npair, ShellPair{});
+    c->pair_first.assign(npair, 0);
+    c->pair_class.assign(npair, 0);
+    c->prims.clear();
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b <= a; ++b) {
+            const size_t idx = (size_t)a * (a + 1) / 2 + b;
+            const int f = c->sh_l[a] >= c->sh_l[b] ? a : b;
+            const int s = f == a ? b : a;
+            ShellPair& P = c->pairs[idx];
+            const double* A = &c->sh_center[3 * f];
+            const double* B = &c->sh_center[3 * s];
+            double AB2 = 0.0;
+            for (int x = 0; x < 3; ++x) {
+                P.A[x] = A[x];
+                P.AB[x] = A[x] - B[x];
+                AB2 += P.AB[x] * P.AB[x];
+            }
+            P.prim_off = (int)c->prims.size();
+            P.ao_a = c->sh_ao[f];
+            P.ao_b = c->sh_ao[s];
+            for (int i = 0; i < c->sh_nprim[f]; ++i)
+                for (int j = 0; j < c->sh_nprim[s]; ++j) {
+                    const double al = c->prim_exp[c->sh_prim_off[f] + i];
+                    const double be = c->prim_exp[c->sh_prim_off[s] + j];
+                    PrimPair pp{};
+                    pp.p = al + be;
+                    pp.inv_p = 1.0 / pp.p;
+                    for (int x = 0; x < 3; ++x) pp.P[x] = (al * A[x] + be * B[x]) * pp.inv_p;
+                    pp.Kp = c->prim_coef[c->sh_prim_off[f] + i] * c->prim_coef[c->sh_prim_off[s] + j] *
+                            std::exp(-al * be * pp.inv_p * AB2) * pp.inv_p;
+                    c->prims.push_back(pp);
+                }
+            P.nprim = (int)c->prims.size() - P.prim_off;
+            c->pair_first[idx] = f;
+            c->pair_class[idx] = pair_class_of(c->sh_l[f], c->sh_l[s]);
+        }
+}
+
+int pair_index(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
+
+}  // namespace
+
+int dsscf_prepare(dsscf_ctx* c)
+{
+    if (!c) return fail(DSSCF_ERR_ARG, "dsscf_prepare: null context");
+    if (!c->have_basis) return fail(DSSCF_ERR_STATE, "dsscf_prepare: no basis set");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->free_geometry();
+    build_pairs(c);
+    const size_t npair = c->pairs.size();
+    const int N = c->n_ao;
+    if (int rc = dev_alloc(&c->d_pairs, npair)) return rc;
+    if (int rc = dev_alloc(&c->d_pp, c->prims.size())) return rc;
+    if (int rc = dev_alloc(&c->d_Q, npair)) return rc;
+    if (int rc = dev_alloc(&c->d_plist, npair)) return rc;
+    const size_t nn = (size_t)N * N, np = (size_t)N * (N + 1) / 2;
+    for (double** p : {&c->d_Dt, &c->d_Da, &c->d_Jh, &c->d_Kh})
+        if (int rc = dev_alloc(p, nn)) return rc;
+    for (double** p : {&c->d_Pt_pk, &c->d_Pa_pk, &c->d_J_pk, &c->d_K_pk})
+        if (int rc = dev_alloc(p, np)) return rc;
+    HIP_TRY(hipMemcpyAsync(c->d_pairs, c->pairs.data(), sizeof(ShellPair) * npair, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_pp, c->prims.data(), sizeof(PrimPair) * c->prims.size(), hipMemcpyHostToDevice,
+                           c->stream));
+    dsscf_upload_xform(c->xform.data(), (int)c->xform.size(), c->stream);
+    HIP_TRY(hipGetLastError());
+
+    // Schwarz bounds, one launch per pair class.
+    const int maxl = dk_max_l();
+    const int npc = (maxl + 1) * (maxl + 2) / 2;
+    std::vector<std::vector<int>> by_class(npc);
+    for (size_t p = 0; p < npair; ++p) by_class[c->pair_class[p]].push_back((int)p);
+    {
+        std::vector<int> plist;
+        std::vector<int> start(npc);
+        for (int pc = 0; pc < npc; ++pc) {
+            start[pc] = (int)plist.size();
+            plist.insert(plist.end(), by_class[pc].begin(), by_class[pc].end());
+        }
+        HIP_TRY(hipMemcpyAsync(c->d_plist, plist.data(), sizeof(int) * plist.size(), hipMemcpyHostToDevice,
+                               c->stream));
+        for (int l1 = 0; l1 <= maxl; ++l1)
+            for (int l2 = 0; l2 <= l1; ++l2) {
+                const int pc = pair_class_of(l1, l2);
+                HIP_TRY(dsscf_launch_schwarz(l1, l2, c->opt.boys_mode, c->d_plist + start[pc],
+                                             (int)by_class[pc].size(), c->d_pairs, c->d_pp, c->d_btab, c->d_Q,
+                                             kEriScale, c->stream));
+            }
+        c->Q.resize(npair);
+        HIP_TRY(hipMemcpyAsync(c->Q.data(), c->d_Q, sizeof(double) * npair, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+
+    // Sort each pair class by (primitive pairs desc, Q desc) and cut it into
+    // groups of equal primitive-pair count.
+    for (auto& v : by_class)
+        std::sort(v.begin(), v.end(), [&](int x, int y) {
+            if (c->pairs[x].nprim != c->pairs[y].nprim) return c->pairs[x].nprim > c->pairs[y].nprim;
+            if (c->Q[x] != c->Q[y]) return c->Q[x] > c->Q[y];
+            return x < y;
+        });
+    auto groups = [&](const std::vector<int>& v) {
+        std::vector<int> g{0};
+        for (size_t i = 1; i < v.size(); ++i)
+            if (c->pairs[v[i]].nprim != c->pairs[v[i - 1]].nprim) g.push_back((int)i);
+        g.push_back((int)v.size());
+        return g;
+    };
+
+    const int ncls = dk_n_class();
+    const double thr = c->opt.schwarz_threshold;
+    std::vector<uint2> all;
+    c->q_off.assign(ncls, 0);
+    c->q_cnt.assign(ncls, 0);
+    c->q_prim.assign(ncls, 0);
+    c->n_unscreened = 0;
+    for (int X = 0; X < npc; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            const int cls = X * (X + 1) / 2 + Y;
+            c->q_off[cls] = (long long)all.size();
+            const std::vector<int>& SX = by_class[X];
+            const std::vector<int>& SY = by_class[Y];
+            c->n_unscreened += X == Y ? (long long)SX.size() * (SX.size() + 1) / 2 : (long long)SX.size() * SY.size();
+            const std::vector<int> GX = groups(SX), GY = groups(SY);
+            for (size_t gb = 0; gb + 1 < GX.size(); ++gb)
+                for (size_t gk = 0; gk + 1 < GY.size(); ++gk) {
+                    if (X == Y && gk > gb) continue;
+                    for (int i = GX[gb]; i < GX[gb + 1]; ++i) {
+                        const int bi = SX[i];
+                        const int jend = (X == Y && gk == gb) ? i + 1 : GY[gk + 1];
+                        for (int j = GY[gk]; j < jend; ++j) {
+                            const int kj = SY[j];
+                            if (thr > 0.0 && c->Q[bi] * c->Q[kj] < thr) break;  // Q sorted desc in a group
+                            all.push_back(make_uint2((unsigned)bi, (unsigned)kj));
+                            c->q_prim[cls] += (long long)c->pairs[bi].nprim * c->pairs[kj].nprim;
+                        }
+                    }
+                }
+            c->q_cnt[cls] = (long long)all.size() - c->q_off[cls];
+        }
+    if (int rc = dev_alloc(&c->d_q, all.size())) return rc;
+    HIP_TRY(hipMemcpyAsync(c->d_q, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->prepared = true;
+    return DSSCF_OK;
+}
+
+namespace {
+int run_jk(dsscf_ctx* c, const double* d_Pt, const double* d_Pa, double* d_J, double* d_K, std::vector<float>* cls_ms)
+{
+    const int N = c->n_ao;
+    const size_t nn = (size_t)N * N;
+    HIP_TRY(dsscf_launch_unpack(d_Pt, c->d_Dt, N, c->stream));
+    HIP_TRY(dsscf_launch_unpack(d_Pa, c->d_Da, N, c->stream));
+    HIP_TRY(dsscf_launch_fill(c->d_Jh, nn, 0.0, c->stream));
+    HIP_TRY(dsscf_launch_fill(c->d_Kh, nn, 0.0, c->stream));
+    const int ncls = dk_n_class();
+    std::vector<hipEvent_t> ev;
+    if (cls_ms) {
+        ev.resize(ncls + 1);
+        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventRecord(ev[0], c->stream));
+    }
+    for (int cls = 0; cls < ncls; ++cls) {
+        const long long n = c->q_cnt[cls];
+        if (n > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "quartet list longer than 2^31");
+        HIP_TRY(dsscf_launch_jk(cls, c->opt.boys_mode, c->d_q + c->q_off[cls], (int)n, c->d_pairs, c->d_pp,
+                                c->d_btab, c->d_Dt, c->d_Da, c->d_Jh, c->d_Kh, N, kEriScale, c->stream));
+        if (cls_ms) HIP_TRY(hipEventRecord(ev[cls + 1], c->stream));
+    }
+    HIP_TRY(dsscf_launch_fold(c->d_Jh, c->d_Kh, d_J, d_K, N, c->stream));
+    if (cls_ms) {
+        HIP_TRY(hipEventSynchronize(ev[ncls]));
+        cls_ms->assign(ncls, 0.0f);
+        for (int cls = 0; cls < ncls; ++cls) HIP_TRY(hipEventElapsedTime(&(*cls_ms)[cls], ev[cls], ev[cls + 1]));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    return DSSCF_OK;
+}
+}  // namespace
+
+int dsscf_jk_device(dsscf_ctx* c, const double* d_Pt, const double* d_Pa, double* d_J, double* d_K)
+{
+    if (!c || !d_Pt || !d_Pa || !d_J || !d_K) return fail(DSSCF_ERR_ARG, "dsscf_jk_device: null argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_jk_device: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = run_jk(c, d_Pt, d_Pa, d_J, d_K, nullptr)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DSSCF_OK;
+}
+
+int dsscf_jk(dsscf_ctx* c, const double* Pt, const double* Pa, double* J, double* K)
+{
+    if (!c || !Pt || !Pa || !J || !K) return fail(DSSCF_ERR_ARG, "dsscf_jk: null argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_jk: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t np = (size_t)c->n_ao * (c->n_ao + 1) / 2;
+    HIP_TRY(hipMemcpyAsync(c->d_Pt_pk, Pt, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_Pa_pk, Pa, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, nullptr)) return rc;
+    HIP_TRY(hipMemcpyAsync(J, c->d_J_pk, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(K, c->d_K_pk, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DSSCF_OK;
+}
+
+int dsscf_set_density(dsscf_ctx* c, const double* Pt, const double* Pa)
+{
+    if (!c || !Pt || !Pa) return fail(DSSCF_ERR_ARG, "dsscf_set_density: null argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_set_density: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t np = (size_t)c->n_ao * (c->n_ao + 1) / 2;
+    HIP_TRY(hipMemcpyAsync(c->d_Pt_pk, Pt, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_Pa_pk, Pa, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DSSCF_OK;
+}
+
+int dsscf_jk_resident(dsscf_ctx* c)
+{
+    if (!c) return fail(DSSCF_ERR_ARG, "dsscf_jk_resident: null context");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_jk_resident: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, nullptr)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DSSCF_OK;
+}
+
+int dsscf_get_jk(dsscf_ctx* c, double* J, double* K)
+{
+    if (!c || !J || !K) return fail(DSSCF_ERR_ARG, "dsscf_get_jk: null argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_get_jk: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t np = (size_t)c->n_ao * (c->n_ao + 1) / 2;
+    HIP_TRY(hipMemcpyAsync(J, c->d_J_pk, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(K, c->d_K_pk, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DSSCF_OK;
+}
+
+int dsscf_time_jk(dsscf_ctx* c, int warmup, int iters, double* ms_per_build, double* ms_dominant, int* dom_cls)
+{
+    if (!c || iters <= 0 || warmup < 0) return fail(DSSCF_ERR_ARG, "dsscf_time_jk: bad argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_time_jk: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    for (int i = 0; i < warmup; ++i)
+        if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, nullptr)) return rc;
+    const int ncls = dk_n_class();
+    std::vector<double> acc(ncls, 0.0);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; ++i)
+        if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, nullptr)) return rc;
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float total = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&total, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // Per-class durations from a separate pass with events between launches.
+    std::vector<float> ms;
+    for (int i = 0; i < iters; ++i) {
+        if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, &ms)) return rc;
+        for (int k = 0; k < ncls; ++k) acc[k] += ms[k];
+    }
+    int best = 0;
+    for (int k = 1; k < ncls; ++k)
+        if (acc[k] > acc[best]) best = k;
+    if (ms_per_build) *ms_per_build = total / iters;
+    if (ms_dominant) *ms_dominant = acc[best] / iters;
+    if (dom_cls) *dom_cls = best;
+    return DSSCF_OK;
+}
+
+int dsscf_schwarz(dsscf_ctx* c, double* Q)
+{
+    if (!c || !Q) return fail(DSSCF_ERR_ARG, "dsscf_schwarz: null argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_schwarz: call dsscf_prepare first");
+    std::memcpy(Q, c->Q.data(), sizeof(double) * c->Q.size());
+    return DSSCF_OK;
+}
+
+int dsscf_get_stats(dsscf_ctx* c, dsscf_stats* st)
+{
+    if (!c || !st) return fail(DSSCF_ERR_ARG, "dsscf_get_stats: null argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_get_stats: call dsscf_prepare first");
+    std::memset(st, 0, sizeof(*st));
+    st->n_pairs = (long long)c->pairs.size();
+    st->n_prim_pairs = (long long)c->prims.size();
+    st->n_quartets_unscreened = c->n_unscreened;
+    st->n_class = dk_n_class();
+    for (int k = 0; k < st->n_class && k < 21; ++k) {
+        st->class_quartets[k] = c->q_cnt[k];
+        st->class_prim_quartets[k] = c->q_prim[k];
+        st->n_quartets += c->q_cnt[k];
+        st->n_prim_quartets += c->q_prim[k];
+    }
+    return DSSCF_OK;
+}
+
+int dsscf_eri_batch(dsscf_ctx* c, const int* ijkl, long long n, double* out)
+{
+    if (!c || (n > 0 && (!ijkl || !out)) || n < 0) return fail(DSSCF_ERR_ARG, "dsscf_eri_batch: bad argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_eri_batch: call dsscf_prepare first");
+    if (n == 0) return DSSCF_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    const int ncls = dk_n_class();
+    struct Req {
+        long long t;
+        int bra, ket, comp;
+    };
+    // Map every request onto (bra pair, ket pair, AO component in the block).
+    std::vector<std::vector<Req>> per_cls(ncls);
+    for (long long t = 0; t < n; ++t) {
+        int ao[4];
+        for (int k = 0; k < 4; ++k) {
+            ao[k] = ijkl[4 * t + k];
+            if (ao[k] < 0 || ao[k] >= c->n_ao) return fail(DSSCF_ERR_ARG, "dsscf_eri_batch: AO index out of range");
+        }
+        int pidx[2], ca[2], cb[2], nb[2];
+        for (int h = 0; h < 2; ++h) {
+            const int s1 = c->ao_shell[ao[2 * h]], s2 = c->ao_shell[ao[2 * h + 1]];
+            const int p = pair_index(s1, s2);
+            const int f = c->pair_first[p];
+            const int m1 = ao[2 * h] - c->sh_ao[s1], m2 = ao[2 * h + 1] - c->sh_ao[s2];
+            // component order inside the pair block is (first shell, second shell)
+            if (f == s1) {
+                ca[h] = m1;
+                cb[h] = m2;
+            } else {
+                ca[h] = m2;
+                cb[h] = m1;
+            }
+            const int s_second = f == s1 ? s2 : s1;
+            nb[h] = 2 * c->sh_l[s_second] + 1;
+            pidx[h] = p;
+        }
+        int X = c->pair_class[pidx[0]], Y = c->pair_class[pidx[1]];
+        int bra = 0, ket = 1;
+        if (X < Y) {
+            std::swap(X, Y);
+            bra = 1;
+            ket = 0;
+        }
+        const int cls = X * (X + 1) / 2 + Y;
+        const int nket = nb[ket] * (2 * c->sh_l[c->pair_first[pidx[ket]]] + 1);
+        const int comp = (ca[bra] * nb[bra] + cb[bra]) * nket + ca[ket] * nb[ket] + cb[ket];
+        per_cls[cls].push_back(Req{t, pidx[bra], pidx[ket], comp});
+    }
+    for (int cls = 0; cls < ncls; ++cls) {
+        auto& rq = per_cls[cls];
+        if (rq.empty()) continue;
+        int l4[4];
+        dk_class_l(cls, l4);
+        const int nblock = (2 * l4[0] + 1) * (2 * l4[1] + 1) * (2 * l4[2] + 1) * (2 * l4[3] + 1);
+        // unique shell quartets of this class
+        std::vector<long long> key(rq.size());
+        for (size_t i = 0; i < rq.size(); ++i) key[i] = (long long)rq[i].bra * (long long)c->pairs.size() + rq[i].ket;
+        std::vector<long long> uk = key;
+        std::sort(uk.begin(), uk.end());
+        uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
+        std::vector<int4> items(uk.size());
+        for (size_t u = 0; u < uk.size(); ++u)
+            items[u] = make_int4((int)(uk[u] / (long long)c->pairs.size()), (int)(uk[u] % (long long)c->pairs.size()),
+                                 (int)(u * nblock), 0);
+        int4* d_items = nullptr;
+        double* d_out = nullptr;
+        if (int rc = dev_alloc(&d_items, items.size())) return rc;
+        if (int rc = dev_alloc(&d_out, items.size() * nblock)) {
+            dev_free(d_items);
+            return rc;
+        }
+        std::vector<double> blk(items.size() * nblock);
+        hipError_t e = hipMemcpyAsync(d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = dsscf_launch_eri_block(cls, c->opt.boys_mode, d_items, (int)items.size(), c->d_pairs, c->d_pp, c->d_btab,
+                                       d_out, kEriScale, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(blk.data(), d_out, sizeof(double) * blk.size(), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dev_free(d_items);
+        dev_free(d_out);
+        if (e != hipSuccess) return fail(DSSCF_ERR_HIP, std::string("dsscf_eri_batch: ") + hipGetErrorString(e));
+        for (size_t i = 0; i < rq.size(); ++i) {
+            const size_t u = std::lower_bound(uk.begin(), uk.end(), key[i]) - uk.begin();
+            out[rq[i].t] = blk[u * nblock + rq[i].comp];
+        }
+    }
+    return DSSCF_OK;
+}
+
+}  // extern "C"

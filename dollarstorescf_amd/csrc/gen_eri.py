@@ -1,0 +1,387 @@
+"""Generator for the per-class ERI device code (gfx950 HIP).
+
+For every shell-quartet class (la lb|lc ld) (la>=lb, lc>=ld, bra class >= ket
+class) it emits straight-line code for
+
+  * the Head-Gordon-Pople form of the Obara-Saika vertical recursion
+    [e0|f0]^(m) (the four-centre OS scheme of GaussianIntegral.cpp:334-559,
+    reorganised: VRR on the A and C centres, then horizontal transfer),
+  * contraction over primitive pairs (the loops of GaussianIntegral.cpp:573-576),
+  * the horizontal recursions (a,b+1| = (a+1,b| + AB (a,b| and the ket analogue,
+  * the Cartesian -> reference-AO transform (the Cartesian-term loops of
+    GaussianIntegral.cpp:598-601 collapsed into one linear map per l, whose
+    coefficients the host uploads into c_ao_xform),
+  * J/K digestion (ExactCoulomb.cpp:103-123 restated for direct SCF).
+
+Only the recursion *structure* is decided here; every numeric coefficient
+comes from the host at run time.  Output: one .inc file included by
+eri_kernels.hip.
+"""
+from __future__ import annotations
+
+import sys
+from fractions import Fraction
+
+MAXL = 2
+
+
+def cart(l):
+    """Cartesian components of angular momentum l, x-major order."""
+    out = []
+    for a in range(l, -1, -1):
+        for b in range(l - a, -1, -1):
+            out.append((a, b, l - a - b))
+    return out
+
+
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+# ---------------------------------------------------------------------------
+# Sparsity of the reference's real solid harmonics (which Cartesian monomials
+# carry a nonzero weight for AO (l,m)).  Exact rational arithmetic on the
+# polynomial Re/Im (x+iy)^|m| * P_l^|m|(z, r^2), the construction of
+# GTO.cpp:15-88 without its normalization.
+def _poly_mul_add(acc, poly, coef, dx=0, dy=0, dz=0):
+    for (a, b, c), v in poly.items():
+        k = (a + dx, b + dy, c + dz)
+        acc[k] = acc.get(k, Fraction(0)) + v * coef
+
+
+def ao_pattern(l, m):
+    am = abs(m)
+    base = {}
+    i, x = 0, (am if m <= 0 else am - 1)
+    while x >= 0:
+        # binomial(am, x) with alternating sign
+        from math import comb
+        base[(x, am - x, 0)] = Fraction((-1) ** i * comb(am, x))
+        x -= 2
+        i += 1
+    if l == am:
+        poly = base
+    else:
+        prev = dict(base)
+        cur = {}
+        _poly_mul_add(cur, base, Fraction(2 * am + 1), dz=1)
+        for ii in range(am + 1, l):
+            sf = Fraction(ii - am + 1)
+            nxt = {}
+            _poly_mul_add(nxt, cur, Fraction(2 * ii + 1) / sf, dz=1)
+            f = -Fraction(ii + am) / sf
+            _poly_mul_add(nxt, prev, f, dx=2)
+            _poly_mul_add(nxt, prev, f, dy=2)
+            _poly_mul_add(nxt, prev, f, dz=2)
+            prev, cur = cur, nxt
+        poly = cur
+    return [c for c in cart(l) if poly.get(c, 0) != 0]
+
+
+def ao_index_pattern(l, spherical=True):
+    """For each AO of an l-shell, the Cartesian component indices it uses."""
+    cs = cart(l)
+    if not spherical:
+        return [[i] for i in range(len(cs))]
+    return [[cs.index(c) for c in ao_pattern(l, m)] for m in range(-l, l + 1)]
+
+
+def nao(l, spherical=True):
+    return 2 * l + 1 if spherical else ncart(l)
+
+
+# ---------------------------------------------------------------------------
+def vname(e, f, m):
+    return "v%d%d%d_%d%d%d_%d" % (e + f + (m,))
+
+
+def sub1(t, i):
+    return t[:i] + (t[i] - 1,) + t[i + 1:]
+
+
+def pick_dir(t):
+    best = -1
+    for i in range(3):
+        if t[i] > 0 and (best < 0 or t[i] > t[best]):
+            best = i
+    return best
+
+
+XYZ = "xyz"
+
+
+def gen_vrr(la, lb, lc, ld):
+    """Statements computing [e|f]^(0) for e in shells la..la+lb and
+    f in shells lc..lc+ld, from Fm[] (prefactor-scaled Boys values)."""
+    E, Fk = la + lb, lc + ld
+    targets = [(e, f, 0) for le in range(la, E + 1) for e in cart(le)
+               for lf in range(lc, Fk + 1) for f in cart(lf)]
+    order, seen = [], set()
+    stmts = {}
+
+    def need(key):
+        if key in seen:
+            return
+        seen.add(key)
+        e, f, m = key
+        if sum(f) > 0:
+            i = pick_dir(f)
+            f1 = sub1(f, i)
+            terms = [("QC%s" % XYZ[i], (e, f1, m)), ("WQ%s" % XYZ[i], (e, f1, m + 1))]
+            if f1[i] > 0:
+                f2 = sub1(f1, i)
+                terms.append(("%d.0*o2q" % f1[i], (e, f2, m)))
+                terms.append(("-%d.0*o2q*rq" % f1[i], (e, f2, m + 1)))
+            if e[i] > 0:
+                terms.append(("%d.0*o2pq" % e[i], (sub1(e, i), f1, m + 1)))
+        elif sum(e) > 0:
+            i = pick_dir(e)
+            e1 = sub1(e, i)
+            terms = [("PA%s" % XYZ[i], (e1, f, m)), ("WP%s" % XYZ[i], (e1, f, m + 1))]
+            if e1[i] > 0:
+                e2 = sub1(e1, i)
+                terms.append(("%d.0*o2p" % e1[i], (e2, f, m)))
+                terms.append(("-%d.0*o2p*rp" % e1[i], (e2, f, m + 1)))
+        else:
+            stmts[key] = None
+            order.append(key)
+            return
+        for _, k in terms:
+            need(k)
+        stmts[key] = terms
+        order.append(key)
+
+    for t in targets:
+        need(t)
+    lines = []
+    for key in order:
+        e, f, m = key
+        if stmts[key] is None:
+            lines.append("const R %s = Fm[%d];" % (vname(e, f, m), m))
+            continue
+        expr = " + ".join("(%s)*%s" % (c, vname(*k)) for c, k in stmts[key])
+        lines.append("const R %s = %s;" % (vname(e, f, m), expr))
+    return lines, targets
+
+
+def acc_name(e, f):
+    return "a%d%d%d_%d%d%d" % (e + f)
+
+
+def gen_hrr(la, lb, lc, ld):
+    """Statements from contracted accumulators a<e>_<f> to Cartesian c_<a>_<b>_<c>_<d>."""
+    E, Fk = la + lb, lc + ld
+    lines = []
+    # bra HRR for each f in lc..Fk
+    fs = [f for lf in range(lc, Fk + 1) for f in cart(lf)]
+    bra_order, seen, bra_stmt = [], set(), {}
+
+    def need_b(a, b):
+        if (a, b) in seen:
+            return
+        seen.add((a, b))
+        if sum(b) == 0:
+            bra_stmt[(a, b)] = None
+        else:
+            i = pick_dir(b)
+            b1 = sub1(b, i)
+            a1 = a[:i] + (a[i] + 1,) + a[i + 1:]
+            need_b(a1, b1)
+            need_b(a, b1)
+            bra_stmt[(a, b)] = (a1, b1, i)
+        bra_order.append((a, b))
+
+    for a in cart(la):
+        for b in cart(lb):
+            need_b(a, b)
+
+    def bn(a, b, f):
+        return "h%d%d%d%d%d%d_%d%d%d" % (a + b + f)
+
+    for f in fs:
+        for (a, b) in bra_order:
+            st = bra_stmt[(a, b)]
+            if st is None:
+                lines.append("const double %s = %s;" % (bn(a, b, f), acc_name(a, f)))
+            else:
+                a1, b1, i = st
+                lines.append("const double %s = %s + AB%s*%s;" % (bn(a, b, f), bn(a1, b1, f), XYZ[i], bn(a, b1, f)))
+    # ket HRR
+    k_order, kseen, k_stmt = [], set(), {}
+
+    def need_k(c, d):
+        if (c, d) in kseen:
+            return
+        kseen.add((c, d))
+        if sum(d) == 0:
+            k_stmt[(c, d)] = None
+        else:
+            i = pick_dir(d)
+            d1 = sub1(d, i)
+            c1 = c[:i] + (c[i] + 1,) + c[i + 1:]
+            need_k(c1, d1)
+            need_k(c, d1)
+            k_stmt[(c, d)] = (c1, d1, i)
+        k_order.append((c, d))
+
+    for c in cart(lc):
+        for d in cart(ld):
+            need_k(c, d)
+
+    def kn(a, b, c, d):
+        return "c%d%d%d%d%d%d_%d%d%d%d%d%d" % (a + b + c + d)
+
+    for a in cart(la):
+        for b in cart(lb):
+            for (c, d) in k_order:
+                st = k_stmt[(c, d)]
+                if st is None:
+                    lines.append("const double %s = %s;" % (kn(a, b, c, d), bn(a, b, c)))
+                else:
+                    c1, d1, i = st
+                    lines.append("const double %s = %s + CD%s*%s;" % (kn(a, b, c, d), kn(a, b, c1, d1), XYZ[i], kn(a, b, c, d1)))
+    return lines, kn
+
+
+def gen_xform(la, lb, lc, ld, kn, spherical=True):
+    """Cartesian -> AO, out[((ia*NB+ib)*NC+ic)*ND+id]."""
+    pats = [ao_index_pattern(l, spherical) for l in (la, lb, lc, ld)]
+    cs = [cart(l) for l in (la, lb, lc, ld)]
+    nb_, nc_, nd_ = (len(p) for p in pats[1:])
+    lines = []
+    # one index at a time: first d, then c, then b, then a
+    # stage 1: t1[a][b][c][md]
+    def t1(a, b, c, md):
+        return "t1_%d_%d_%d_%d" % (a, b, c, md)
+
+    def t2(a, b, mc, md):
+        return "t2_%d_%d_%d_%d" % (a, b, mc, md)
+
+    def t3(a, mb, mc, md):
+        return "t3_%d_%d_%d_%d" % (a, mb, mc, md)
+
+    def coef(l, m, ci):
+        return "c_ao_xform[%d]" % (xform_offset(l) + m * ncart(l) + ci)
+
+    for a in range(len(cs[0])):
+        for b in range(len(cs[1])):
+            for c in range(len(cs[2])):
+                for md, comps in enumerate(pats[3]):
+                    expr = " + ".join("%s*%s" % (coef(ld, md, ci), kn(cs[0][a], cs[1][b], cs[2][c], cs[3][ci])) for ci in comps)
+                    lines.append("const double %s = %s;" % (t1(a, b, c, md), expr))
+    for a in range(len(cs[0])):
+        for b in range(len(cs[1])):
+            for mc, comps in enumerate(pats[2]):
+                for md in range(nd_):
+                    expr = " + ".join("%s*%s" % (coef(lc, mc, ci), t1(a, b, ci, md)) for ci in comps)
+                    lines.append("const double %s = %s;" % (t2(a, b, mc, md), expr))
+    for a in range(len(cs[0])):
+        for mb, comps in enumerate(pats[1]):
+            for mc in range(nc_):
+                for md in range(nd_):
+                    expr = " + ".join("%s*%s" % (coef(lb, mb, ci), t2(a, ci, mc, md)) for ci in comps)
+                    lines.append("const double %s = %s;" % (t3(a, mb, mc, md), expr))
+    for ma, comps in enumerate(pats[0]):
+        for mb in range(nb_):
+            for mc in range(nc_):
+                for md in range(nd_):
+                    expr = " + ".join("%s*%s" % (coef(la, ma, ci), t3(ci, mb, mc, md)) for ci in comps)
+                    lines.append("out[%d] = %s;" % (((ma * nb_ + mb) * nc_ + mc) * nd_ + md, expr))
+    return lines
+
+
+def xform_offset(l):
+    off = 0
+    for k in range(l):
+        off += (2 * k + 1) * ncart(k)
+    return off
+
+
+def classes(maxl=MAXL):
+    pairs = [(a, b) for a in range(maxl + 1) for b in range(a + 1)]
+    out = []
+    for i, X in enumerate(pairs):
+        for Y in pairs[:i + 1]:
+            out.append(X + Y)
+    return pairs, out
+
+
+def cls_name(c):
+    return "".join("spdf"[l] for l in c)
+
+
+def gen_class(c, spherical=True):
+    la, lb, lc, ld = c
+    L = sum(c)
+    vrr, targets = gen_vrr(la, lb, lc, ld)
+    hrr, kn = gen_hrr(la, lb, lc, ld)
+    xf = gen_xform(la, lb, lc, ld, kn, spherical)
+    na = [nao(l, spherical) for l in c]
+    NAO = na[0] * na[1] * na[2] * na[3]
+    name = cls_name(c)
+    accs = sorted({(e, f) for (e, f, _) in targets})
+    o = []
+    o.append("// ---- class (%s) : L=%d, %d VRR terms, %d AO integrals" % (name, L, len(vrr), NAO))
+    o.append("template <> struct ClassInfo<%d,%d,%d,%d> { static constexpr int L = %d, NAO = %d, NA = %d, NB = %d, NC = %d, ND = %d; };"
+             % (la, lb, lc, ld, L, NAO, *na))
+    o.append("template <int BM, typename R>")
+    o.append("__device__ __forceinline__ void shell_quartet_%s(const ShellPair& bra, const ShellPair& ket,"
+             " const PrimPair* __restrict__ pp, const double* __restrict__ btab, double* __restrict__ out) {" % name)
+    for (e, f) in accs:
+        o.append("  double %s = 0.0;" % acc_name(e, f))
+    o.append("  for (int ib = 0; ib < bra.nprim; ++ib) {")
+    o.append("    const PrimPair P = pp[bra.prim_off + ib];")
+    o.append("    for (int ik = 0; ik < ket.nprim; ++ik) {")
+    o.append("      const PrimPair Q = pp[ket.prim_off + ik];")
+    o.append("      R Fm[%d];" % (L + 1))
+    o.append("      PrimQuartet<R> g; prim_setup<BM, R, %d>(P, Q, bra, ket, btab, g, Fm);" % L)
+    o.append("      const R PAx = g.PA[0], PAy = g.PA[1], PAz = g.PA[2];")
+    o.append("      const R WPx = g.WP[0], WPy = g.WP[1], WPz = g.WP[2];")
+    o.append("      const R QCx = g.QC[0], QCy = g.QC[1], QCz = g.QC[2];")
+    o.append("      const R WQx = g.WQ[0], WQy = g.WQ[1], WQz = g.WQ[2];")
+    o.append("      const R o2p = g.o2p, o2q = g.o2q, o2pq = g.o2pq, rp = g.rp, rq = g.rq;")
+    o.append("      (void)PAx; (void)PAy; (void)PAz; (void)WPx; (void)WPy; (void)WPz; (void)QCx; (void)QCy; (void)QCz;")
+    o.append("      (void)WQx; (void)WQy; (void)WQz; (void)o2p; (void)o2q; (void)o2pq; (void)rp; (void)rq;")
+    for ln in vrr:
+        o.append("      " + ln)
+    for (e, f) in accs:
+        o.append("      %s += (double)%s;" % (acc_name(e, f), vname(e, f, 0)))
+    o.append("    }")
+    o.append("  }")
+    o.append("  const double ABx = bra.AB[0], ABy = bra.AB[1], ABz = bra.AB[2];")
+    o.append("  const double CDx = ket.AB[0], CDy = ket.AB[1], CDz = ket.AB[2];")
+    o.append("  (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;")
+    for ln in hrr:
+        o.append("  " + ln)
+    for ln in xf:
+        o.append("  " + ln)
+    o.append("}")
+    return "\n".join(o)
+
+
+def gen_dispatch(cls_list):
+    o = ["#define DSSCF_FOR_EACH_CLASS(X) \\"]
+    for c in cls_list:
+        o.append("  X(%d, %d, %d, %d, %s) \\" % (*c, cls_name(c)))
+    o.append("")
+    return "\n".join(o)
+
+
+def main(out_path, maxl=MAXL):
+    pairs, cls_list = classes(maxl)
+    text = ["// Generated by gen_eri.py -- do not edit.", "#pragma once", ""]
+    text.append("#define DSSCF_MAXL %d" % maxl)
+    text.append("#define DSSCF_XFORM_SIZE %d" % xform_offset(maxl + 1))
+    text.append("#define DSSCF_NCLASS %d" % len(cls_list))
+    text.append(gen_dispatch(cls_list))
+    text.append("// Cartesian -> reference-AO coefficients, per l: (2l+1) x ncart(l), row-major.")
+    text.append("__constant__ double c_ao_xform[DSSCF_XFORM_SIZE];")
+    text.append("")
+    for c in cls_list:
+        text.append(gen_class(c))
+        text.append("")
+    open(out_path, "w").write("\n".join(text))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1], int(sys.argv[2]) if len(sys.argv) > 2 else MAXL)

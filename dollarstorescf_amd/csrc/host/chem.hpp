@@ -1,0 +1,89 @@
+// chem.hpp -- host-side chemistry data of the engine (namespace flo), the
+// counterparts of DollarStoreSCF's src/chem/{GTO,BasisReader,XYZReader,
+// Molecule,Constants}.  Own code; the names follow the reference so that its
+// callers read the same.
+#pragma once
+#include <array>
+#include <string>
+#include <vector>
+
+namespace flo {
+
+constexpr double PI = 3.14159265358979;      // src/chem/Constants.hpp:5
+constexpr double A_to_a0 = 1.889726126;       // src/chem/Constants.hpp:15
+
+struct vec3 {
+    double x = 0, y = 0, z = 0;
+};
+
+// One Cartesian monomial x^a y^b z^c with its weight (GTO.hpp:8-17).
+struct CartesianPolynomialTerm {
+    double weight = 0.0;
+    unsigned x = 0, y = 0, z = 0;
+};
+
+struct GTOPrimitive {  // GTO.hpp:19-27
+    double zeta = 0.0;
+    double weight = 0.0;               // normalized (GTO.cpp:90-93)
+    double unnormalized_weight = 0.0;  // as read from the basis file
+};
+
+// One real-solid-harmonic AO (l, m) as a Cartesian expansion (GTO.hpp:29-42).
+struct ContractedGaussian {
+    vec3 center;
+    std::vector<GTOPrimitive> primitives;
+    std::vector<CartesianPolynomialTerm> spherical_harmonic;
+    unsigned l = 0;
+    int m = 0;
+
+    ContractedGaussian() = default;
+    ContractedGaussian(const std::vector<GTOPrimitive>& prims, unsigned l, int m, const vec3& center);
+};
+
+struct GTOShell {  // GTO.hpp:44-51
+    unsigned l = 0;
+    std::vector<GTOPrimitive> primitives;
+};
+
+typedef std::vector<GTOShell> AtomicBasis;
+
+struct BasisSet {  // GTO.hpp:55-57
+    AtomicBasis atomic_bases[118];
+};
+
+struct Atom {  // Molecule.hpp:9-22
+    vec3 position;        // bohr
+    unsigned charge = 1;  // = element number
+    int element = 1;
+};
+
+struct Molecule {
+    std::vector<Atom> atoms;
+    size_t size() const { return atoms.size(); }
+    Atom& operator[](size_t i) { return atoms[i]; }
+    const Atom& operator[](size_t i) const { return atoms[i]; }
+};
+
+int getElement(const std::string& symbol);  // Element.cpp:178-185 (Z <= 18 here)
+
+// Real-solid-harmonic expansion of AO (l, m) (GTO.cpp:15-88).
+std::vector<CartesianPolynomialTerm> solidHarmonicTerms(unsigned l, int m);
+
+namespace Gaussian {
+// Gaussian94 basis file (BasisReader.cpp:9-113).  Returns 0 on success, a
+// nonzero code with `error` set otherwise (the reference prints and goes on).
+int readBasis(const std::string& path, BasisSet& out, std::string* error = nullptr);
+}  // namespace Gaussian
+
+// XYZ file in Angstrom -> atoms in bohr (XYZReader.cpp:6-35).
+int readXYZFile(const std::string& path, Molecule& out, std::string* error = nullptr);
+
+// Packed-U one-electron matrices over the AO list (SCFBasis.cpp:56-83):
+// S, T and V = -sum_C Z_C <mu|1/r_C|nu>.  boys_mode as dsscf_options.
+void oneElectronMatrices(const std::vector<ContractedGaussian>& basis, const Molecule& mol, int boys_mode,
+                         std::vector<double>& S, std::vector<double>& T, std::vector<double>& V);
+
+// Host Boys function (reference series or accurate), used by V.
+double boysFunction(double x, unsigned m, int boys_mode);
+
+}  // namespace flo

@@ -1,0 +1,225 @@
+// job_api.cpp -- extern "C" entry points of the SCF layer (include/dsscf_scf.h).
+#include "../../../include/dsscf_scf.h"
+
+#include <cstring>
+#include <string>
+
+#include "scf.hpp"
+
+struct dsscf_job {
+    scf::Job job;
+    flo::BasisSet basis_set;
+    bool have_basis = false;
+};
+
+namespace {
+int set_err(dsscf_job* j, int rc, const std::string& msg)
+{
+    if (j) j->job.error = msg;
+    return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int dsscf_job_create(dsscf_job** out, int device)
+{
+    if (!out) return DSSCF_ERR_ARG;
+    *out = new dsscf_job();
+    (*out)->job.opt.device = device;
+    return DSSCF_OK;
+}
+
+int dsscf_job_destroy(dsscf_job* j)
+{
+    delete j;
+    return DSSCF_OK;
+}
+
+const char* dsscf_job_error(dsscf_job* j) { return j ? j->job.error.c_str() : "null job"; }
+
+int dsscf_job_set_options(dsscf_job* j, int diis_size, double thr, int max_it, int boys_mode, double schwarz, int verbose)
+{
+    if (!j || diis_size < 1 || !(thr > 0.0) || max_it < 1 || !(schwarz >= 0.0) ||
+        (boys_mode != DSSCF_BOYS_REFERENCE && boys_mode != DSSCF_BOYS_ACCURATE))
+        return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_options: bad argument");
+    scf::Options& o = j->job.opt;
+    o.diis_size = diis_size;
+    o.energy_threshold = thr;
+    o.max_iterations = max_it;
+    o.boys_mode = boys_mode;
+    o.schwarz_threshold = schwarz;
+    o.verbose = verbose;
+    return DSSCF_OK;
+}
+
+int dsscf_job_set_molecule(dsscf_job* j, int natom, const int* Z, const double* xyz, int charge, int mult)
+{
+    if (!j || natom <= 0 || !Z || !xyz || mult < 1) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_molecule: bad argument");
+    flo::Molecule m;
+    for (int i = 0; i < natom; ++i) {
+        if (Z[i] < 1 || Z[i] > 118) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_molecule: bad element number");
+        flo::Atom a;
+        a.position = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        a.element = Z[i];
+        a.charge = (unsigned)Z[i];
+        m.atoms.push_back(a);
+    }
+    j->job.error.clear();
+    scf::assignMolecule(j->job, m, charge, (unsigned)mult);
+    if (!j->job.error.empty()) return DSSCF_ERR_ARG;
+    if (j->have_basis) scf::assignBasis(j->job, j->basis_set);
+    return DSSCF_OK;
+}
+
+int dsscf_job_load_xyz(dsscf_job* j, const char* path, int charge, int mult)
+{
+    if (!j || !path) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_load_xyz: null argument");
+    flo::Molecule m;
+    std::string err;
+    if (flo::readXYZFile(path, m, &err)) return set_err(j, DSSCF_ERR_ARG, err);
+    j->job.error.clear();
+    scf::assignMolecule(j->job, m, charge, (unsigned)mult);
+    if (!j->job.error.empty()) return DSSCF_ERR_ARG;
+    if (j->have_basis) scf::assignBasis(j->job, j->basis_set);
+    return DSSCF_OK;
+}
+
+int dsscf_job_set_basis_file(dsscf_job* j, const char* path)
+{
+    if (!j || !path) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_basis_file: null argument");
+    std::string err;
+    if (flo::Gaussian::readBasis(path, j->basis_set, &err)) return set_err(j, DSSCF_ERR_ARG, err);
+    j->have_basis = true;
+    if (j->job.molecule.size()) scf::assignBasis(j->job, j->basis_set);
+    return DSSCF_OK;
+}
+
+int dsscf_job_construct(dsscf_job* j)
+{
+    if (!j) return DSSCF_ERR_ARG;
+    if (!j->have_basis || !j->job.molecule.size()) return set_err(j, DSSCF_ERR_STATE, "molecule and basis must be set first");
+    int rc = scf::constructBasis(j->job);
+    if (rc) return rc;
+    scf::useCoreGuess(j->job);
+    return DSSCF_OK;
+}
+
+int dsscf_job_construct_host(dsscf_job* j)
+{
+    if (!j) return DSSCF_ERR_ARG;
+    if (!j->have_basis || !j->job.molecule.size()) return set_err(j, DSSCF_ERR_STATE, "molecule and basis must be set first");
+    int rc = scf::constructBasisHost(j->job);
+    if (rc) return rc;
+    scf::useCoreGuess(j->job);
+    return DSSCF_OK;
+}
+
+int dsscf_job_ao(dsscf_job* j, int i, int* l, int* m, double* center, int* nprim, double* zeta, double* w,
+                 int* nterm, double* tw, int* txyz, int cap)
+{
+    if (!j || i < 0 || i >= (int)j->job.basis.size() || !l || !m || !center || !nprim || !nterm)
+        return set_err(j, DSSCF_ERR_ARG, "dsscf_job_ao: bad argument");
+    const flo::ContractedGaussian& g = j->job.basis[i];
+    *l = (int)g.l;
+    *m = g.m;
+    center[0] = g.center.x;
+    center[1] = g.center.y;
+    center[2] = g.center.z;
+    *nprim = (int)g.primitives.size();
+    *nterm = (int)g.spherical_harmonic.size();
+    for (int p = 0; p < *nprim && p < cap; ++p) {
+        if (zeta) zeta[p] = g.primitives[p].zeta;
+        if (w) w[p] = g.primitives[p].weight;
+    }
+    for (int t = 0; t < *nterm && t < cap; ++t) {
+        if (tw) tw[t] = g.spherical_harmonic[t].weight;
+        if (txyz) {
+            txyz[3 * t] = (int)g.spherical_harmonic[t].x;
+            txyz[3 * t + 1] = (int)g.spherical_harmonic[t].y;
+            txyz[3 * t + 2] = (int)g.spherical_harmonic[t].z;
+        }
+    }
+    return DSSCF_OK;
+}
+
+int dsscf_job_n_ao(dsscf_job* j) { return j ? j->job.n : 0; }
+
+int dsscf_job_get(dsscf_job* j, const char* name, double* out)
+{
+    if (!j || !name || !out) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_get: null argument");
+    scf::Job& b = j->job;
+    const std::vector<double>* v = nullptr;
+    const std::string n = name;
+    if (n == "S") v = &b.S;
+    else if (n == "T") v = &b.T;
+    else if (n == "V") v = &b.V;
+    else if (n == "H") v = &b.H;
+    else if (n == "Pa") v = &b.Pa;
+    else if (n == "Ptot") v = &b.Ptot;
+    else if (n == "J") v = &b.J;
+    else if (n == "K") v = &b.K;
+    else if (n == "G") v = &b.G;
+    else if (n == "eps") v = &b.eps;
+    if (!v) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_get: unknown matrix '" + n + "'");
+    std::memcpy(out, v->data(), sizeof(double) * v->size());
+    return DSSCF_OK;
+}
+
+int dsscf_job_set_density(dsscf_job* j, const double* Pa)
+{
+    if (!j || !Pa) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_density: null argument");
+    scf::Job& b = j->job;
+    if (b.Pa.empty()) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_set_density: construct first");
+    for (size_t i = 0; i < b.Pa.size(); ++i) {
+        b.Pa[i] = Pa[i];
+        b.Ptot[i] = 2.0 * Pa[i];
+    }
+    b.jk_valid = false;
+    return DSSCF_OK;
+}
+
+int dsscf_job_fock_build(dsscf_job* j)
+{
+    if (!j) return DSSCF_ERR_ARG;
+    if (!j->job.ctx) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_fock_build: construct first");
+    int rc = scf::fockBuild(j->job);
+    if (!rc)
+        for (size_t i = 0; i < j->job.G.size(); ++i) j->job.G[i] = j->job.J[i] - j->job.K[i];
+    return rc;
+}
+
+int dsscf_job_solve(dsscf_job* j, double* energy, int* iterations)
+{
+    if (!j) return DSSCF_ERR_ARG;
+    if (!j->job.ctx) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_solve: construct first");
+    int rc = scf::solveMOs(j->job);
+    if (energy) *energy = j->job.energies.total();
+    if (iterations) *iterations = (int)j->job.iteration;
+    return rc;
+}
+
+int dsscf_job_energies(dsscf_job* j, double* out)
+{
+    if (!j || !out) return DSSCF_ERR_ARG;
+    const scf::Energies& e = j->job.energies;
+    out[0] = e.kinetic;
+    out[1] = e.nuclear_attraction;
+    out[2] = e.electron_repulsion;
+    out[3] = e.exchange;
+    out[4] = e.nuclear;
+    return DSSCF_OK;
+}
+
+int dsscf_job_history(dsscf_job* j, double* out, int cap)
+{
+    if (!j) return 0;
+    const auto& h = j->job.history;
+    const int n = (int)h.size();
+    for (int i = 0; i < n && i < cap && out; ++i) out[i] = h[i];
+    return n;
+}
+
+dsscf_ctx* dsscf_job_context(dsscf_job* j) { return j ? j->job.ctx : nullptr; }
+
+}  // extern "C"

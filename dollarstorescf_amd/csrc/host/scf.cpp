@@ -1,0 +1,410 @@
+// scf.cpp -- RHF driver over the device Fock build (see scf.hpp).
+#include "scf.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace scf {
+
+namespace {
+thread_local Job* g_current = nullptr;
+
+double trace_packed(const std::vector<double>& A, const std::vector<double>& B, int n)
+{
+    // tr(A B) for symmetric A, B in packed-U form
+    double s = 0.0;
+    for (int x = 0; x < n; ++x)
+        for (int y = 0; y <= x; ++y) {
+            const size_t i = (size_t)x * (x + 1) / 2 + y;
+            s += (x == y ? 1.0 : 2.0) * A[i] * B[i];
+        }
+    return s;
+}
+}  // namespace
+
+Job& current()
+{
+    if (!g_current) g_current = new Job();
+    return *g_current;
+}
+
+void setCurrent(Job* job) { g_current = job; }
+
+Job::~Job()
+{
+    if (ctx) dsscf_ctx_destroy(ctx);
+}
+
+// ------------------------------------------------------------------ DIIS --
+void DIIS::resize(int n)
+{
+    size = n;
+    err.clear();
+    res.clear();
+    B.assign((size_t)(n + 1) * (n + 1), 0.0);
+    coef.clear();
+    rms = 0.0;
+}
+
+// DIIS.ipp:18-62: new vectors fill free slots; once full, the vector with
+// the largest B_ii is replaced.  B is bordered by ones with a zero corner.
+void DIIS::addErrorVector(const la::Mat& result, const la::Mat& error)
+{
+    const int W = size + 1;
+    double new_rms = 0.0;
+    for (double v : error) new_rms += v * v;
+    rms = new_rms;
+    int index = 0;
+    int count = (int)err.size();
+    if (count < size) {
+        err.emplace_back();
+        res.emplace_back();
+        index = count;
+        ++count;
+        for (int i = 0; i < count; ++i) B[(size_t)i * W + count] = B[(size_t)count * W + i] = 1.0;
+        B[(size_t)count * W + count] = 0.0;
+    } else {
+        double mx = 0.0;
+        for (int i = 0; i < size; ++i)
+            if (B[(size_t)i * W + i] > mx) {
+                mx = B[(size_t)i * W + i];
+                index = i;
+            }
+    }
+    err[index] = error;
+    res[index] = result;
+    last = index;
+    B[(size_t)index * W + index] = new_rms;
+    for (int i = 0; i < count; ++i) {
+        if (i == index) continue;
+        double dot = 0.0;
+        const la::Mat& e = err[i];
+        for (size_t k = 0; k < e.size(); ++k) dot += e[k] * error[k];
+        B[(size_t)i * W + index] = B[(size_t)index * W + i] = dot;
+    }
+}
+
+// DIIS.ipp:64-77: B c = (0,..,0,1).
+bool DIIS::solve()
+{
+    const int W = size + 1, m = (int)err.size() + 1;
+    std::vector<double> A((size_t)m * m);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) A[(size_t)i * m + j] = B[(size_t)i * W + j];
+    coef.assign(m, 0.0);
+    coef[m - 1] = 1.0;
+    if (!la::solve(A, coef, m)) {
+        // singular system (identical error vectors): keep the newest matrix
+        coef.assign(m, 0.0);
+        coef[last] = 1.0;
+        return false;
+    }
+    return true;
+}
+
+// DIIS.ipp:84-100
+void DIIS::calculateResult(la::Mat& out) const
+{
+    if (res.size() <= 1) {
+        out = res[0];
+        return;
+    }
+    out.assign(res[0].size(), 0.0);
+    for (size_t i = 0; i < res.size(); ++i)
+        for (size_t k = 0; k < out.size(); ++k) out[k] += coef[i] * res[i][k];
+}
+
+// ------------------------------------------------------------ SCFBasis --
+void assignMolecule(Job& job, const flo::Molecule& mol, int charge, unsigned multiplicity)
+{
+    // SCFBasis.cpp:17-37
+    job.molecule = mol;
+    job.atom_bases.assign(mol.size(), flo::AtomicBasis());
+    unsigned total = 0;
+    for (const auto& a : mol.atoms) total += (unsigned)a.element;
+    total -= charge;
+    if (total % 2 == multiplicity % 2) {
+        job.error = "multiplicity does not match the electron count";
+        std::fprintf(stderr, "ERROR: The specified multiplicity does not match the electron count.\n");
+    }
+    job.electron_count[0] = (total + multiplicity) / 2;
+    job.electron_count[1] = (total + 1 - multiplicity) / 2;
+    job.spin_treatment = multiplicity == 1 ? SpinTreatment::restricted : SpinTreatment::unrestricted;
+    job.net_charge = charge;
+    job.multiplicity = multiplicity;
+}
+
+void assignBasis(Job& job, const flo::BasisSet& set)
+{
+    // SCFBasis.cpp:50-54
+    for (size_t i = 0; i < job.molecule.size(); ++i) job.atom_bases[i] = set.atomic_bases[job.molecule[i].element - 1];
+}
+
+int shellData(const Job& job, ShellData& out, std::string* error)
+{
+    out = ShellData();
+    const int maxl = dsscf_max_l();
+    for (const auto& ao : job.basis) {
+        if (ao.m != -(int)ao.l) continue;
+        if ((int)ao.l > maxl) {
+            if (error) *error = "angular momentum " + std::to_string(ao.l) + " not built (max " + std::to_string(maxl) + ")";
+            return DSSCF_ERR_UNSUPPORTED;
+        }
+        out.l.push_back((int)ao.l);
+        out.nprim.push_back((int)ao.primitives.size());
+        out.center.insert(out.center.end(), {ao.center.x, ao.center.y, ao.center.z});
+        for (const auto& p : ao.primitives) {
+            out.exps.push_back(p.zeta);
+            out.coefs.push_back(p.weight);
+        }
+    }
+    out.xform.assign(dsscf_xform_size(), 0.0);
+    size_t off = 0;
+    for (int l = 0; l <= maxl; ++l) {
+        const int nc = (l + 1) * (l + 2) / 2;
+        for (int m = -l; m <= l; ++m)
+            for (const auto& t : flo::solidHarmonicTerms((unsigned)l, m)) {
+                // x-major Cartesian position of x^t.x y^t.y z^t.z
+                int idx = 0;
+                for (int a = l; a > (int)t.x; --a) idx += l - a + 1;
+                idx += l - (int)t.x - (int)t.y;
+                out.xform[off + (size_t)(m + l) * nc + idx] = t.weight;
+            }
+        off += (size_t)(2 * l + 1) * nc;
+    }
+    return DSSCF_OK;
+}
+
+int constructBasisHost(Job& job)
+{
+    // SCFBasis.cpp:107-171
+    for (size_t i = 0; i < job.molecule.size(); ++i)
+        if (job.atom_bases[i].empty()) {
+            job.error = "no basis assigned to atom " + std::to_string(i + 1);
+            return DSSCF_ERR_ARG;
+        }
+    if (job.spin_treatment != SpinTreatment::restricted) {
+        job.error = "only closed-shell (restricted) jobs are supported";
+        return DSSCF_ERR_UNSUPPORTED;
+    }
+    job.basis.clear();
+    job.basis_atoms.clear();
+    job.atom_basis.assign(1, 0);
+    for (size_t i = 0; i < job.molecule.size(); ++i) {
+        for (const auto& sh : job.atom_bases[i])
+            for (int m = -(int)sh.l; m <= (int)sh.l; ++m) {
+                job.basis.emplace_back(sh.primitives, sh.l, m, job.molecule[i].position);
+                job.basis_atoms.push_back((unsigned)i);
+            }
+        job.atom_basis.push_back((unsigned)job.basis.size());
+    }
+    const int n = job.n = (int)job.basis.size();
+    flo::oneElectronMatrices(job.basis, job.molecule, job.opt.boys_mode, job.S, job.T, job.V);
+    job.H.resize(job.S.size());
+    for (size_t i = 0; i < job.H.size(); ++i) job.H[i] = job.T[i] + job.V[i];
+    // Loewdin X = U s^-1/2 U^T (SCFBasis.cpp:85-100)
+    la::unpack(job.S.data(), n, job.Sd);
+    std::vector<double> s;
+    la::Mat U;
+    la::syev(job.Sd, n, s, U);
+    job.X.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < n; ++k) v += U[(size_t)i * n + k] * U[(size_t)j * n + k] / std::sqrt(s[k]);
+            job.X[(size_t)i * n + j] = v;
+        }
+    const size_t np = (size_t)n * (n + 1) / 2;
+    job.Pa.assign(np, 0.0);
+    job.Ptot.assign(np, 0.0);
+    job.J.assign(np, 0.0);
+    job.K.assign(np, 0.0);
+    job.G.assign(np, 0.0);
+    job.jk_valid = false;
+    return DSSCF_OK;
+}
+
+int constructDevice(Job& job)
+{
+    // device side (replaces assignExactRepulsionTensor, ExactCoulomb.cpp:41-72)
+    if (job.basis.empty()) {
+        job.error = "constructDevice: host basis not built";
+        return DSSCF_ERR_STATE;
+    }
+    ShellData sd;
+    if (int rc = shellData(job, sd, &job.error)) return rc;
+    if (!job.ctx) {
+        if (int rc = dsscf_ctx_create(&job.ctx, job.opt.device)) {
+            job.error = dsscf_last_error();
+            return rc;
+        }
+    }
+    dsscf_options o{job.opt.boys_mode, job.opt.schwarz_threshold};
+    int rc = dsscf_set_options(job.ctx, &o);
+    if (!rc)
+        rc = dsscf_set_basis(job.ctx, (int)sd.l.size(), sd.l.data(), sd.nprim.data(), sd.center.data(), sd.exps.data(),
+                             sd.coefs.data(), sd.xform.data());
+    if (!rc) rc = dsscf_prepare(job.ctx);
+    if (rc) {
+        job.error = dsscf_last_error();
+        return rc;
+    }
+    job.jk_valid = false;
+    return DSSCF_OK;
+}
+
+int constructBasis(Job& job)
+{
+    if (int rc = constructBasisHost(job)) return rc;
+    return constructDevice(job);
+}
+
+// ----------------------------------------------------------- SCFSolver --
+void useCoreGuess(Job& job)
+{
+    // SCFSolver.cpp:179-194
+    std::fill(job.Pa.begin(), job.Pa.end(), 0.0);
+    std::fill(job.Ptot.begin(), job.Ptot.end(), 0.0);
+    job.jk_valid = false;
+}
+
+int fockBuild(Job& job)
+{
+    if (!job.ctx) return DSSCF_ERR_STATE;
+    int rc = dsscf_jk(job.ctx, job.Ptot.data(), job.Pa.data(), job.J.data(), job.K.data());
+    if (rc) job.error = dsscf_last_error();
+    job.jk_valid = rc == 0;
+    return rc;
+}
+
+// SCFSolver.cpp:145-172 (restricted): G = J[Ptot] - K[Pa], F = H + G with
+// DIIS from the second iteration, F' = X F X, C = X C', Pa = C_occ C_occ^T,
+// E_elec = 2 sum eps_occ - tr(Pa G).
+int runSCFIteration(Job& job)
+{
+    const int n = job.n;
+    if (int rc = fockBuild(job)) return rc;
+    for (size_t i = 0; i < job.G.size(); ++i) job.G[i] = job.J[i] - job.K[i];
+    std::vector<double> F0p(job.G.size());
+    for (size_t i = 0; i < F0p.size(); ++i) F0p[i] = job.H[i] + job.G[i];
+    la::Mat F0;
+    la::unpack(F0p.data(), n, F0);
+    if (job.iteration == 1) {
+        job.F = F0;
+    } else {
+        // Pulay error S Pa F - (S Pa F)^T (SCFDIIS.cpp:19-25)
+        la::Mat Pd, PF, SPF, e;
+        la::unpack(job.Pa.data(), n, Pd);
+        la::gemm(Pd, F0, PF, n);
+        la::gemm(job.Sd, PF, SPF, n);
+        e.resize((size_t)n * n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) e[(size_t)i * n + j] = SPF[(size_t)i * n + j] - SPF[(size_t)j * n + i];
+        job.diis.addErrorVector(F0, e);
+        job.diis.solve();
+        job.diis.calculateResult(job.F);
+    }
+    // solveRoothaanHall (SCFSolver.cpp:64-75)
+    la::Mat XF, Fp, Cp;
+    la::gemm(job.X, job.F, XF, n);
+    la::gemm(XF, job.X, Fp, n);
+    if (!la::syev(Fp, n, job.eps, Cp)) {
+        job.error = "eigensolver did not converge";
+        return DSSCF_ERR_NOCONV;
+    }
+    la::gemm(job.X, Cp, job.C, n);
+    // computeDensityMatrix (SCFSolver.cpp:77-94)
+    const unsigned nocc = job.electron_count[0];
+    for (int x = 0; x < n; ++x)
+        for (int y = 0; y <= x; ++y) {
+            double v = 0.0;
+            for (unsigned i = 0; i < nocc; ++i) v += job.C[(size_t)x * n + i] * job.C[(size_t)y * n + i];
+            const size_t k = (size_t)x * (x + 1) / 2 + y;
+            job.Pa[k] = v;
+            job.Ptot[k] = 2.0 * v;
+        }
+    job.jk_valid = false;
+    // getTotalElectronicEnergy (SCFSolver.cpp:118-143)
+    double E = 0.0;
+    for (unsigned i = 0; i < nocc; ++i) E += 2.0 * job.eps[i];
+    E -= trace_packed(job.Pa, job.G, n);
+    job.previous_electronic_energy = job.total_electronic_energy;
+    job.total_electronic_energy = E;
+    job.history.push_back(E);
+    return DSSCF_OK;
+}
+
+int solveMOs(Job& job)
+{
+    // SCFSolver.cpp:196-237
+    job.iteration = 0;
+    job.total_electronic_energy = job.previous_electronic_energy = 0.0;
+    job.history.clear();
+    job.diis.resize(job.opt.diis_size);
+    if (job.opt.verbose) std::printf("%9s %22s %22s %14s\n", "iteration", "E_elec", "dE", "[PF]^2");
+    while (!(job.iteration &&
+             std::fabs(job.total_electronic_energy - job.previous_electronic_energy) < job.opt.energy_threshold)) {
+        if ((int)job.iteration >= job.opt.max_iterations) {
+            job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
+            return DSSCF_ERR_NOCONV;
+        }
+        ++job.iteration;
+        if (int rc = runSCFIteration(job)) return rc;
+        if (job.opt.verbose)
+            std::printf("%9u %22.12f %22.12f %14.6e\n", job.iteration, job.total_electronic_energy,
+                        job.total_electronic_energy - job.previous_electronic_energy, job.diis.rms);
+    }
+    return calculateEnergies(job);
+}
+
+int calculateEnergies(Job& job)
+{
+    // Energy.cpp:21-50
+    const int n = job.n;
+    if (!job.jk_valid)
+        if (int rc = fockBuild(job)) return rc;
+    Energies& e = job.energies;
+    e.kinetic = trace_packed(job.Ptot, job.T, n);
+    e.nuclear_attraction = trace_packed(job.Ptot, job.V, n);
+    e.electron_repulsion = trace_packed(job.Pa, job.J, n);
+    e.exchange = -trace_packed(job.Pa, job.K, n);
+    e.nuclear = 0.0;
+    const auto& at = job.molecule.atoms;
+    for (size_t i = 0; i < at.size(); ++i)
+        for (size_t j = i + 1; j < at.size(); ++j) {
+            const double dx = at[i].position.x - at[j].position.x, dy = at[i].position.y - at[j].position.y,
+                         dz = at[i].position.z - at[j].position.z;
+            e.nuclear += (double)(at[i].element * at[j].element) / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    return DSSCF_OK;
+}
+
+// --------------------------------------------------- reference-style API --
+void assignMolecule(const flo::Molecule& m, int charge, unsigned mult) { assignMolecule(current(), m, charge, mult); }
+void assignBasis(const flo::BasisSet& b) { assignBasis(current(), b); }
+int constructBasis() { return constructBasis(current()); }
+void useCoreGuess() { useCoreGuess(current()); }
+int solveMOs() { return solveMOs(current()); }
+
+void computeCoulombTerms(Spin)
+{
+    // SCFSolver.cpp:29-35: G = J.  One device pass yields J and K; K is kept
+    // for the computeExchangeTerms call that follows.
+    Job& job = current();
+    if (fockBuild(job)) return;
+    job.G = job.J;
+}
+
+void computeExchangeTerms(Spin)
+{
+    // SCFSolver.cpp:37-43: G -= K
+    Job& job = current();
+    if (!job.jk_valid && fockBuild(job)) return;
+    for (size_t i = 0; i < job.G.size(); ++i) job.G[i] -= job.K[i];
+}
+
+void calculateEnergies() { calculateEnergies(current()); }
+double getTotalEnergy() { return current().energies.total(); }
+
+}  // namespace scf

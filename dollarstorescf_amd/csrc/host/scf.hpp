@@ -1,0 +1,116 @@
+// scf.hpp -- the SCF layer of the engine (namespace scf), mirroring
+// DollarStoreSCF's src/chem/scf entry points (SCFBasis.hpp:6-14,
+// SCFSolver.hpp:16-23, SCFDIIS.hpp:7-15, ExactCoulomb.hpp:9-15,
+// Energy.hpp:4-20).  The reference keeps its state in namespace globals
+// (SCFCommon.hpp:6-37); here it lives in a Job so several molecules can be
+// in flight, and the free functions act on the current job.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../../include/dsscf.h"
+#include "chem.hpp"
+#include "linalg.hpp"
+
+namespace scf {
+
+enum class SpinTreatment { restricted = 0, unrestricted = 1 };
+enum class Spin { alpha = 0, beta = 1 };
+
+struct Options {
+    int diis_size = 8;                 // SCFSolver.cpp:214
+    double energy_threshold = 1e-8;    // SCFSolver.cpp:26
+    int max_iterations = 500;          // the reference has no cap (SCFSolver.cpp:219)
+    int boys_mode = DSSCF_BOYS_ACCURATE;
+    double schwarz_threshold = 1e-12;
+    int device = 0;
+    int verbose = 0;                   // print the iteration table like SCFSolver.cpp:198-225
+};
+
+// flo::DIISSolver<double, MatrixNd, SymmetricMatrixNd> (DIIS.ipp:5-100).
+struct DIIS {
+    int size = 8;
+    std::vector<la::Mat> err, res;
+    std::vector<double> B;  // (size+1)^2, symmetric, bordered by ones
+    std::vector<double> coef;
+    double rms = 0.0;
+    int last = 0;
+    void resize(int n);
+    void addErrorVector(const la::Mat& result, const la::Mat& error);
+    bool solve();
+    void calculateResult(la::Mat& out) const;
+};
+
+struct Energies {
+    double kinetic = 0, nuclear_attraction = 0, electron_repulsion = 0, exchange = 0, nuclear = 0;
+    double total() const { return kinetic + nuclear_attraction + electron_repulsion + exchange + nuclear; }
+};
+
+struct Job {
+    Options opt;
+    flo::Molecule molecule;
+    std::vector<flo::AtomicBasis> atom_bases;
+    int net_charge = 0;
+    unsigned multiplicity = 1;
+    unsigned electron_count[2] = {0, 0};
+    SpinTreatment spin_treatment = SpinTreatment::restricted;
+
+    std::vector<flo::ContractedGaussian> basis;
+    std::vector<unsigned> basis_atoms, atom_basis;
+    int n = 0;
+
+    // packed-U matrices
+    std::vector<double> S, T, V, H, Pa, Ptot, J, K, G;
+    la::Mat Sd, X, F, C;
+    std::vector<double> eps;
+    DIIS diis;
+    unsigned iteration = 0;
+    double total_electronic_energy = 0, previous_electronic_energy = 0;
+    Energies energies;
+    std::vector<double> history;
+    bool jk_valid = false;
+
+    dsscf_ctx* ctx = nullptr;
+    std::string error;
+
+    ~Job();
+};
+
+Job& current();
+void setCurrent(Job* job);
+
+// SCFBasis.hpp
+void assignMolecule(const flo::Molecule& molecule, int charge, unsigned multiplicity);
+void assignBasis(const flo::BasisSet& basis_set);
+int constructBasis();
+
+// SCFSolver.hpp
+void useCoreGuess();
+int solveMOs();
+void computeCoulombTerms(Spin spin);
+void computeExchangeTerms(Spin spin);
+
+// Energy.hpp
+void calculateEnergies();
+double getTotalEnergy();
+
+// Job-level forms used by the functions above and by the C API.
+void assignMolecule(Job& job, const flo::Molecule& molecule, int charge, unsigned multiplicity);
+void assignBasis(Job& job, const flo::BasisSet& basis_set);
+int constructBasis(Job& job);        // host part + device part
+int constructBasisHost(Job& job);    // AOs, S, T, V, H, Loewdin X
+int constructDevice(Job& job);       // shell upload, Schwarz, quartet lists
+void useCoreGuess(Job& job);
+int fockBuild(Job& job);  // J[Ptot], K[Pa] on the device
+int runSCFIteration(Job& job);
+int solveMOs(Job& job);
+int calculateEnergies(Job& job);
+
+// Shell data for the device in the layout dsscf_set_basis expects.
+struct ShellData {
+    std::vector<int> l, nprim;
+    std::vector<double> center, exps, coefs, xform;
+};
+int shellData(const Job& job, ShellData& out, std::string* error);
+
+}  // namespace scf

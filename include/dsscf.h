@@ -1,0 +1,131 @@
+/*
+ * dsscf.h -- C-ABI of the MI355X direct-SCF Fock-build engine.
+ *
+ * The hot path of DollarStoreSCF (Obara-Saika ERIs, Boys F_m(T), screening
+ * and the J/K contraction into the Fock matrix) behind plain-C entry points:
+ * no C++ or torch types, plain pointers and sizes, int status (0 = ok) with
+ * dsscf_last_error() for the message.  Matrices cross the boundary in the
+ * reference's packed-upper layout: element (y,x), y <= x, at x(x+1)/2 + y
+ * (src/lalib/Matrix.ipp:332-351), so SymmetricMatrixNd::getRawData()
+ * (src/lalib/Matrix.hpp:56) passes through unchanged.
+ *
+ * Threading: one HIP stream per context; contexts are independent, one
+ * context must not be used from two threads at once.
+ */
+#ifndef DSSCF_H
+#define DSSCF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DSSCF_OK = 0,
+    DSSCF_ERR_ARG = 1,         /* bad argument (null pointer, size, index) */
+    DSSCF_ERR_HIP = 2,         /* HIP runtime error */
+    DSSCF_ERR_STATE = 3,       /* call out of order (e.g. jk before prepare) */
+    DSSCF_ERR_UNSUPPORTED = 4, /* angular momentum or mode not built */
+    DSSCF_ERR_NOMEM = 5,       /* device allocation failed */
+    DSSCF_ERR_NOCONV = 6       /* SCF iteration cap reached */
+};
+
+/* Boys-function evaluation on the device. */
+enum {
+    DSSCF_BOYS_REFERENCE = 0, /* the reference's truncated series, GaussianIntegral.cpp:209-223 */
+    DSSCF_BOYS_ACCURATE = 1   /* tabulated Taylor + downward recursion, ~1e-15 relative */
+};
+
+typedef struct dsscf_ctx dsscf_ctx;
+
+typedef struct {
+    int boys_mode;            /* DSSCF_BOYS_* */
+    double schwarz_threshold; /* drop shell quartets with Q_ab*Q_cd below it; 0 keeps all.
+                                 Replaces the (ii|jj)(kk|ll) > 1e-16 screen of
+                                 src/chem/scf/ExactCoulomb.cpp:63 */
+} dsscf_options;
+
+typedef struct {
+    long long n_pairs;             /* unique shell pairs */
+    long long n_prim_pairs;        /* primitive pairs over all shell pairs */
+    long long n_quartets;          /* surviving unique shell quartets */
+    long long n_prim_quartets;     /* primitive quartets of the surviving shell quartets */
+    long long n_quartets_unscreened;
+    int n_class;                   /* shell-quartet classes */
+    long long class_quartets[21];  /* per class (order of dsscf_class_l) */
+    long long class_prim_quartets[21];
+} dsscf_stats;
+
+/* Last error message of the calling thread ("" if none). */
+const char* dsscf_last_error(void);
+
+/* Library facts: highest angular momentum built, number of classes, the
+ * (la,lb,lc,ld) of class c, and the size of the AO-transform table. */
+int dsscf_max_l(void);
+int dsscf_class_count(void);
+int dsscf_class_l(int c, int* l4);
+int dsscf_xform_size(void);
+
+/* Context bound to HIP device `device` with its own stream.  Replaces the
+ * namespace-scope state of src/chem/scf/ExactCoulomb.cpp:13-15
+ * (repulsion_tensor, diagonal_size, tensor_size). */
+int dsscf_ctx_create(dsscf_ctx** ctx, int device);
+int dsscf_ctx_destroy(dsscf_ctx* ctx);
+int dsscf_set_options(dsscf_ctx* ctx, const dsscf_options* opt);
+
+/* Shell data, shells in AO order (atom -> shell in basis-file order, each
+ * shell contributing 2l+1 AOs m=-l..l: src/chem/scf/SCFBasis.cpp:120-140).
+ *   shell_l[n_shell], shell_nprim[n_shell], shell_center[3*n_shell] (bohr)
+ *   prim_exp[], prim_coef[]  -- the reference's normalized primitive weights
+ *                               (GTOPrimitive::weight, src/chem/GTO.cpp:90-93)
+ *   ao_xform[dsscf_xform_size()] -- for each l = 0..max_l, a (2l+1) x ncart(l)
+ *       row-major matrix: weight of Cartesian monomial c (x-major order) in
+ *       AO m (ContractedGaussian::spherical_harmonic, src/chem/GTO.cpp:15-88)
+ * Replaces the std::vector<ContractedGaussian> basis that
+ * assignExactRepulsionTensor reads (ExactCoulomb.cpp:41-72, SCFCommon.hpp:10). */
+int dsscf_set_basis(dsscf_ctx* ctx, int n_shell, const int* shell_l, const int* shell_nprim,
+                    const double* shell_center, const double* prim_exp, const double* prim_coef,
+                    const double* ao_xform);
+
+/* Build pair data, Schwarz bounds and quartet lists on the device.  Replaces
+ * scf::assignExactRepulsionTensor (src/chem/scf/ExactCoulomb.cpp:41-72): no
+ * tensor is stored, integrals are recomputed per Fock build (direct SCF). */
+int dsscf_prepare(dsscf_ctx* ctx);
+
+/* One Fock build: J[Ptot] and K[Palpha] in packed-U form from packed-U
+ * densities (host pointers).  J_mn = sum_ls (mn|ls) Ptot_ls replaces
+ * getExactCoulombMatrix (ExactCoulomb.cpp:103-112) as called by
+ * computeCoulombTerms (SCFSolver.cpp:29-35); K_mn = sum_ls (ml|sn) Pa_ls
+ * replaces getExactExchangeMatrix (ExactCoulomb.cpp:114-123) as called by
+ * computeExchangeTerms (SCFSolver.cpp:37-43). */
+int dsscf_jk(dsscf_ctx* ctx, const double* Ptot, const double* Palpha, double* J, double* K);
+
+/* As dsscf_jk with device pointers (device-resident density / output). */
+int dsscf_jk_device(dsscf_ctx* ctx, const double* d_Ptot, const double* d_Palpha, double* d_J, double* d_K);
+
+/* Device-resident form: upload packed-U densities once, run Fock builds on
+ * them (each call synchronizes the context's stream), read J and K back. */
+int dsscf_set_density(dsscf_ctx* ctx, const double* Ptot, const double* Palpha);
+int dsscf_jk_resident(dsscf_ctx* ctx);
+int dsscf_get_jk(dsscf_ctx* ctx, double* J, double* K);
+
+/* (ij|kl) for n AO quartets ijkl[4*t..4*t+3], written to out[t].  The unit
+ * API flo::electronRepulsionIntegral(mu, sigma, nu, tau) = (mu nu|sigma tau)
+ * (src/chem/GaussianIntegral.cpp:561-642, GaussianIntegral.hpp:12) maps to
+ * ijkl = {mu, nu, sigma, tau}.  No screening is applied. */
+int dsscf_eri_batch(dsscf_ctx* ctx, const int* ijkl, long long n, double* out);
+
+/* Schwarz bounds Q_ab of all unique shell pairs (a >= b, index a(a+1)/2+b). */
+int dsscf_schwarz(dsscf_ctx* ctx, double* Q);
+
+int dsscf_get_stats(dsscf_ctx* ctx, dsscf_stats* st);
+
+/* Run K Fock builds back to back on the current device density and return
+ * the average device time per build in milliseconds (HIP events on the
+ * context's stream).  Timing helper for benchmarks. */
+int dsscf_time_jk(dsscf_ctx* ctx, int warmup, int iters, double* ms_per_build, double* ms_dominant_kernel,
+                  int* dominant_class);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSSCF_H */

@@ -1,0 +1,67 @@
+/*
+ * dsscf_scf.h -- C entry points of the host SCF layer (the engine's
+ * counterparts of DollarStoreSCF's src/chem/scf free functions), for callers
+ * that drive whole jobs rather than single Fock builds: bindings, tests and
+ * the benchmark.  Each call names the reference function it stands for.
+ */
+#ifndef DSSCF_SCF_H
+#define DSSCF_SCF_H
+
+#include "dsscf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dsscf_job dsscf_job;
+
+int dsscf_job_create(dsscf_job** job, int device);
+int dsscf_job_destroy(dsscf_job* job);
+const char* dsscf_job_error(dsscf_job* job);
+
+/* diis_size: SCFSolver.cpp:214 (reference: 8); energy_threshold:
+ * scf::energy_threshold, SCFSolver.cpp:26 (reference: 1e-8); max_iterations:
+ * no reference counterpart (its loop is uncapped, SCFSolver.cpp:219). */
+int dsscf_job_set_options(dsscf_job* job, int diis_size, double energy_threshold, int max_iterations,
+                          int boys_mode, double schwarz_threshold, int verbose);
+
+/* scf::assignMolecule (SCFBasis.cpp:17-37); coordinates in bohr. */
+int dsscf_job_set_molecule(dsscf_job* job, int natom, const int* Z, const double* xyz_bohr, int charge,
+                           int multiplicity);
+/* readXYZFile (XYZReader.cpp:6-35) + scf::assignMolecule. */
+int dsscf_job_load_xyz(dsscf_job* job, const char* path, int charge, int multiplicity);
+/* Gaussian::readBasis (BasisReader.cpp:98-113) + scf::assignBasis (SCFBasis.cpp:50-54). */
+int dsscf_job_set_basis_file(dsscf_job* job, const char* path);
+/* scf::constructBasis (SCFBasis.cpp:107-171) + scf::useCoreGuess (SCFSolver.cpp:179-194). */
+int dsscf_job_construct(dsscf_job* job);
+/* Host half of scf::constructBasis only (AO list, S, T, V, H, Loewdin X);
+ * needs no GPU. */
+int dsscf_job_construct_host(dsscf_job* job);
+int dsscf_job_n_ao(dsscf_job* job);
+/* AO i of the list (ContractedGaussian, src/chem/GTO.hpp:29-42): l, m,
+ * centre, primitives (exponent, normalized weight) and Cartesian terms
+ * (weight, x y z exponents); arrays hold at least `cap` entries. */
+int dsscf_job_ao(dsscf_job* job, int i, int* l, int* m, double* center, int* nprim, double* zeta, double* w,
+                 int* nterm, double* tw, int* txyz, int cap);
+
+/* Packed-U matrices by name: "S","T","V","H","Pa","Ptot","J","K","G"; "eps"
+ * gives n orbital energies. */
+int dsscf_job_get(dsscf_job* job, const char* name, double* out);
+/* Set the alpha density (packed-U); Ptot = 2 Pa. */
+int dsscf_job_set_density(dsscf_job* job, const double* Pa);
+/* One Fock build at the current density (computeCoulombTerms +
+ * computeExchangeTerms, SCFSolver.cpp:29-43). */
+int dsscf_job_fock_build(dsscf_job* job);
+/* scf::solveMOs (SCFSolver.cpp:196-237) + calculateEnergies (Energy.cpp:21-50):
+ * total energy (getTotalEnergy, Energy.cpp:76-78) and iteration count. */
+int dsscf_job_solve(dsscf_job* job, double* energy, int* iterations);
+/* kinetic, nuclear attraction, Coulomb, exchange, nuclear repulsion. */
+int dsscf_job_energies(dsscf_job* job, double* out5);
+/* Electronic energies of every iteration (up to cap); returns the count. */
+int dsscf_job_history(dsscf_job* job, double* out, int cap);
+dsscf_ctx* dsscf_job_context(dsscf_job* job);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSSCF_SCF_H */

@@ -1,0 +1,145 @@
+// ref_driver.cpp -- extern "C" driver over DollarStoreSCF's own integral code.
+//
+// TEST INFRASTRUCTURE ONLY.  Linked by oracle/Makefile against the
+// reference's unmodified translation units (GaussianIntegral.cpp, GTO.cpp,
+// BasisReader.cpp, Element.cpp, TextReader.cpp, TextUtil.cpp) compiled from
+// where they lie under the reference tree, into oracle/_ref/.  It is used to
+// pin the oracle restatement and as the timed "reference" CPU baseline.
+//
+// The AO list is assembled the way scf::constructBasis orders it
+// (src/chem/scf/SCFBasis.cpp:120-140: atom -> shell in file order ->
+// m = -l..l), since SCFBasis.cpp itself does not compile without its lalib
+// and LAPACK dependencies.  The reference's integral routines share global
+// scratch (GaussianIntegral.cpp:10): every call here is single-threaded.
+#include "chem/BasisReader.hpp"
+#include "chem/GTO.hpp"
+#include "chem/GaussianIntegral.hpp"
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+namespace flo {
+double boysFunctionExact(double x, uint m);  // GaussianIntegral.cpp:209 (no header decl)
+}
+
+namespace {
+flo::BasisSet g_basis_set;
+std::vector<flo::ContractedGaussian> g_aos;
+std::vector<int> g_Z;
+std::vector<double> g_xyz;
+}  // namespace
+
+extern "C" {
+
+double ref_boys(double x, unsigned m) { return flo::boysFunctionExact(x, m); }
+
+// Load a Gaussian94 basis file with flo::Gaussian::readBasis (BasisReader.cpp:98).
+int ref_load_basis(const char* path)
+{
+    g_basis_set = flo::Gaussian::readBasis(path);
+    int n = 0;
+    for (int z = 0; z < 118; ++z) n += (int)g_basis_set.atomic_bases[z].size();
+    return n;
+}
+
+// Number of shells and their (l, nprim) for element Z in the loaded basis.
+int ref_element_shells(int Z, int* l, int* nprim, double* zeta, double* unnorm_w, int cap)
+{
+    const flo::AtomicBasis& b = g_basis_set.atomic_bases[Z - 1];
+    int off = 0;
+    for (size_t s = 0; s < b.size(); ++s) {
+        l[s] = (int)b[s].l;
+        nprim[s] = (int)b[s].primitives.size();
+        for (auto& p : b[s].primitives) {
+            if (off < cap) {
+                zeta[off] = p.zeta;
+                unnorm_w[off] = p.unnormalized_weight;
+            }
+            ++off;
+        }
+    }
+    return (int)b.size();
+}
+
+// Build AOs for a molecule (atoms in bohr) in constructBasis order.
+int ref_build(int natom, const int* Z, const double* xyz)
+{
+    g_aos.clear();
+    g_Z.assign(Z, Z + natom);
+    g_xyz.assign(xyz, xyz + 3 * natom);
+    for (int a = 0; a < natom; ++a) {
+        flo::vec3 c(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]);
+        const flo::AtomicBasis& def = g_basis_set.atomic_bases[Z[a] - 1];
+        if (def.empty()) return -1;
+        for (const flo::GTOShell& sh : def) {
+            for (int m = -(int)sh.l; m <= (int)sh.l; ++m)
+                g_aos.push_back(flo::ContractedGaussian(sh.primitives, sh.l, m, c));
+        }
+    }
+    return (int)g_aos.size();
+}
+
+// Describe AO i: l, center, primitive count, normalized weights, term list.
+int ref_ao(int i, int* l, double* center, int* nprim, double* zeta, double* w, int* nterm, double* tw, int* txyz, int cap)
+{
+    const flo::ContractedGaussian& g = g_aos[i];
+    *l = (int)g.l;
+    center[0] = g.center.x;
+    center[1] = g.center.y;
+    center[2] = g.center.z;
+    *nprim = (int)g.primitives.size();
+    for (int p = 0; p < *nprim && p < cap; ++p) {
+        zeta[p] = g.primitives[p].zeta;
+        w[p] = g.primitives[p].weight;
+    }
+    *nterm = (int)g.spherical_harmonic.size();
+    for (int t = 0; t < *nterm && t < cap; ++t) {
+        tw[t] = g.spherical_harmonic[t].weight;
+        txyz[3 * t] = (int)g.spherical_harmonic[t].x;
+        txyz[3 * t + 1] = (int)g.spherical_harmonic[t].y;
+        txyz[3 * t + 2] = (int)g.spherical_harmonic[t].z;
+    }
+    return 0;
+}
+
+// (ij|kl) = electronRepulsionIntegral(i, k, j, l) (the tensor builder's call,
+// ExactCoulomb.cpp:64).
+double ref_eri(int i, int j, int k, int l)
+{
+    return flo::electronRepulsionIntegral(g_aos[i], g_aos[k], g_aos[j], g_aos[l]);
+}
+
+void ref_eri_list(const int* q, long n, double* out)
+{
+    for (long t = 0; t < n; ++t) out[t] = ref_eri(q[4 * t], q[4 * t + 1], q[4 * t + 2], q[4 * t + 3]);
+}
+
+// Wall-clock seconds for evaluating the given AO quartets (CPU baseline).
+double ref_time_eri_list(const int* q, long n, double* out)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    ref_eri_list(q, n, out);
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// Packed-U S, T, V with the reference's argument order (SCFBasis.cpp:56-83).
+void ref_one_electron(double* S, double* T, double* V)
+{
+    const int N = (int)g_aos.size();
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j <= i; ++j) {
+            size_t idx = (size_t)i * (i + 1) / 2 + j;
+            S[idx] = flo::overlapIntegral(g_aos[j], g_aos[i]);
+            T[idx] = flo::kineticEnergyIntegral(g_aos[j], g_aos[i]);
+            double v = 0.0;
+            for (size_t a = 0; a < g_Z.size(); ++a) {
+                flo::vec3 c(g_xyz[3 * a], g_xyz[3 * a + 1], g_xyz[3 * a + 2]);
+                v -= (double)g_Z[a] * flo::nuclearPotentialIntegral(g_aos[j], g_aos[i], c);
+            }
+            V[idx] = v;
+        }
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+"""Fock-build benchmark on one MI355X.
+
+Workload: 8 rigid waters on a seeded 2x2x2 lattice (3.0 A spacing, +-0.3 A
+jitter, random orientations), RHF/6-31G, 104 AOs, s/p shells only, Schwarz
+threshold 1e-12, FP64 throughout.  A step is one Fock build: every surviving
+unique shell quartet evaluated (Boys, OS/HGP VRR, HRR) and digested into J
+and K, with the density resident in HBM.  The density is the SCF density
+after up to 20 iterations of the same job.
+
+Prints one JSON line (metric, value = Fock builds/s, roofline of the dominant
+class kernel, CPU baseline of the reference's own ERI routine).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Fock builds/s, ERI quartets/s; SCF/s + p99 vs FP64 plain at |dE|<=1e-6 Eh"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def cpu_baseline(Z, xyz, n_ao, n_unique_ao_quartets, sample, seed=7):
+    """The reference's electronRepulsionIntegral (oracle/_ref, one core) on a
+    random sample of unique AO quartets of the same system, extrapolated to a
+    full Fock build's unique AO quartets (the stored-tensor J/K lookups are
+    not included)."""
+    from oracle import ref
+
+    if not ref.available():
+        return None
+    import dollarstorescf_amd as ds
+
+    rm = ref.RefMolecule(ds.basis_path("6-31G"), Z, xyz)
+    rng = np.random.default_rng(seed)
+    q = rng.integers(0, n_ao, size=(sample, 4)).astype(np.int32)
+    t, _ = rm.time_eri_list(q)
+    per_q = t / sample
+    return {"value": 1.0 / (per_q * n_unique_ao_quartets), "unit": "Fock builds/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"{sample} random AO quartets of the same 104-AO system through the reference's "
+                      f"electronRepulsionIntegral (GaussianIntegral.cpp:561), {t:.1f} s on 1 core, "
+                      f"extrapolated to {n_unique_ao_quartets} unique AO quartets per Fock build; "
+                      f"J/K tensor lookups excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--boys", choices=["accurate", "reference"], default="accurate")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is a design in "
+              f"DESIGN.md, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import dollarstorescf_amd as ds
+    from dollarstorescf_amd import flops, geometry
+
+    Z, xyz = geometry.water_lattice(1)
+    boys = ds.BOYS_ACCURATE if a.boys == "accurate" else ds.BOYS_REFERENCE
+    job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=20, boys_mode=boys,
+                                schwarz_threshold=1e-12)
+    job.set_basis("6-31G").set_molecule(Z, xyz)
+    t0 = time.perf_counter()
+    job.construct()
+    t_construct = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    try:
+        energy, iters = job.solve()
+        converged = True
+    except ds.DsscfError as e:  # 20-iteration cap
+        if e.code != 6:
+            raise
+        energy, iters, converged = float("nan"), 20, False
+    t_scf = time.perf_counter() - t0
+    n = job.n_ao
+    ctx = job.context
+    st = ctx.stats()
+    Pa = job.get("Pa")
+    ctx.set_density(2.0 * Pa, Pa)
+
+    for _ in range(a.warmup):
+        ctx.jk_resident()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.jk_resident()  # each call ends with a stream synchronize
+    dt = time.perf_counter() - t0
+    ms = dt / a.steps * 1e3
+    builds_per_s = a.steps / dt
+
+    # dominant kernel: per-class durations from HIP events on the context stream
+    ms_event, ms_dom, dom = ctx.time_jk(warmup=1, iters=max(5, min(a.steps, 20)))
+    classes = ds.class_list()
+    cq = list(st.class_quartets)[:len(classes)]
+    cp = list(st.class_prim_quartets)[:len(classes)]
+    dom_flops = flops.class_flops(classes[dom], cq[dom], cp[dom])
+    total_flops = flops.fock_build_flops(classes, cq, cp)
+    achieved = dom_flops / (ms_dom * 1e-3) / 1e12
+    name = "".join("spdf"[l] for l in classes[dom])
+
+    n_unique_ao = (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2
+    cpu = None if a.no_cpu else cpu_baseline(Z, xyz, n, n_unique_ao, a.cpu_sample)
+
+    out = {
+        "metric": METRIC,
+        "value": builds_per_s,
+        "unit": "Fock builds/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded water lattice geometry)",
+        "config": {"workload": "RHF/6-31G Fock build, 8-water 2x2x2 lattice (3.0 A, +-0.3 A jitter, seed 1), "
+                               "104 AOs, Schwarz 1e-12, FP64",
+                   "boys": a.boys, "n_ao": n, "shell_quartets": int(st.n_quartets),
+                   "shell_quartets_unscreened": int(st.n_quartets_unscreened),
+                   "prim_quartets": int(st.n_prim_quartets), "parallelism": "single GPU"},
+        "eri_shell_quartets_per_s": st.n_quartets * builds_per_s,
+        "eri_prim_quartets_per_s": st.n_prim_quartets * builds_per_s,
+        "fock_build_tflops": total_flops / (ms * 1e-3) / 1e12,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": f"k_jk<{name}>", "kernel_ms": ms_dom,
+                     "note": "FP64 vector-ALU bound; peak is MI355X FP64 78.6 TFLOP/s; achieved = algorithmic "
+                             "FLOPs of the class (dollarstorescf_amd/flops.py) / HIP-event kernel time"},
+        "cpu_baseline": cpu,
+        "scf": {"energy": energy, "iterations": iters, "converged": converged, "seconds": t_scf,
+                "construct_seconds": t_construct},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,129 @@
+"""CPU tests of the product's host side: the C-ABI library loads and exports
+every declared symbol, the host chemistry layer (basis parsing, AO
+construction, one-electron terms) agrees with the oracle, error paths report
+codes instead of aborting, the FLOP model reproduces its reference values."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import dollarstorescf_amd as ds
+from dollarstorescf_amd import flops
+from dollarstorescf_amd import geometry as G
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in ("dsscf.h", "dsscf_scf.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(dsscf_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ds.lib()
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert len(declared_symbols()) >= 30
+
+
+def test_library_facts():
+    assert ds.lib().dsscf_max_l() >= 1
+    cls = ds.class_list()
+    assert cls[0] == (0, 0, 0, 0)
+    assert len(cls) == ds.lib().dsscf_class_count()
+
+
+@pytest.mark.parametrize("basis", ["STO-3G", "6-31G", "6-31GP", "6-31GPP", "def2-SVP"])
+def test_host_aos_and_one_electron_match_oracle(basis):
+    Z, xyz = G.water_box(2, seed=5)
+    job = ds.Job().set_options(boys_mode=ds.BOYS_REFERENCE).set_basis(basis).set_molecule(Z, xyz).construct_host()
+    aos = O.build_aos(O.read_basis(ds.basis_path(basis)), Z, xyz)
+    assert job.n_ao == len(aos)
+    for i, a in enumerate(aos):
+        h = job.ao(i)
+        assert (h["l"], h["m"]) == (a.l, a.m)
+        assert [t[1:] for t in h["terms"]] == [tuple(t[1:]) for t in a.terms]
+        assert np.allclose([t[0] for t in h["terms"]], [t[0] for t in a.terms], rtol=1e-15, atol=0)
+        assert np.allclose(h["weight"], [p[1] for p in a.prims], rtol=1e-15, atol=0)
+    ob = O.OracleBasis(aos, Z, xyz)
+    S, T, V = ob.one_electron(0)
+    assert np.max(np.abs(job.get("S") - S)) < 1e-13
+    assert np.max(np.abs(job.get("T") - T)) < 1e-12
+    assert np.max(np.abs(job.get("V") - V)) < 1e-11
+
+
+def test_host_one_electron_accurate_mode_shift_small():
+    """V with the accurate Boys differs from the reference series by < 1e-9."""
+    Z, xyz = G.water()
+    a = ds.Job().set_options(boys_mode=ds.BOYS_ACCURATE).set_basis("6-31G").set_molecule(Z, xyz).construct_host()
+    b = ds.Job().set_options(boys_mode=ds.BOYS_REFERENCE).set_basis("6-31G").set_molecule(Z, xyz).construct_host()
+    d = np.max(np.abs(a.get("V") - b.get("V")))
+    assert 0 < d < 1e-9
+
+
+def test_basis_parser_errors(tmp_path):
+    bad = tmp_path / "bad"
+    bad.write_text("H     0\nS    3   1.00\n  0.34D+01  0.15D+00\n  -1.0  0.5\n  0.1 0.4\n****\n")
+    with pytest.raises(ds.DsscfError) as e:
+        ds.Job().set_basis(str(bad))
+    assert "positive" in str(e.value)
+    with pytest.raises(ds.DsscfError):
+        ds.Job().set_basis(str(tmp_path / "missing"))
+    sp = tmp_path / "sp"
+    sp.write_text("O     0\nSP   2   1.00\n  1.0  0.5  0.6\n  0.3  0.5  0.4\n****\n")
+    job = ds.Job().set_basis(str(sp)).set_molecule([8], np.zeros((1, 3)))
+    # an SP line becomes an S shell followed by a P shell: 4 AOs, l = 0,1,1,1
+    job.construct_host()
+    assert job.n_ao == 4 and [job.ao(i)["l"] for i in range(4)] == [0, 1, 1, 1]
+
+
+def test_multiplicity_mismatch_is_reported():
+    with pytest.raises(ds.DsscfError):
+        ds.Job().set_basis("STO-3G").set_molecule([8, 1], np.array([[0, 0, 0], [0, 0, 1.8]]), multiplicity=1)
+
+
+def test_xyz_reader(tmp_path):
+    f = tmp_path / "w.xyz"
+    f.write_text("3\nwater\nO 0 0 0\nH 0.7576 0.5870 0\nH -0.7576 0.5870 0\n")
+    job = ds.Job().set_basis("STO-3G").load_xyz(str(f)).construct_host()
+    assert job.n_ao == 7
+    Z, xyz = O.read_xyz(str(f))
+    ob = O.OracleBasis(O.build_aos(O.read_basis(ds.basis_path("STO-3G")), Z, xyz), Z, xyz)
+    assert np.max(np.abs(job.get("S") - ob.one_electron(0)[0])) < 1e-14
+
+
+def test_flop_model_reference_values():
+    want = {(0, 0, 0, 0): 69, (1, 0, 0, 0): 106, (1, 1, 0, 0): 211, (1, 0, 1, 0): 301, (1, 1, 1, 0): 790,
+            (1, 1, 1, 1): 2191, (2, 0, 0, 0): 205, (2, 2, 2, 2): 38673}
+    for c, v in want.items():
+        assert flops.f_prim(*c) == v
+    assert flops.n_vrr(1, 1, 1, 1) == 195 and flops.n_vrr(2, 2, 2, 2) == 3666
+
+
+def test_geometry_generators():
+    Z, xyz = G.water_lattice(1)
+    assert len(Z) == 24 and list(Z[:3]) == [8, 1, 1]
+    O_ = xyz[Z == 8] / G.A_TO_A0
+    d = np.linalg.norm(O_[:, None] - O_[None], axis=-1)
+    assert d[d > 0].min() > 2.3
+    for k in range(8):
+        w = xyz[3 * k:3 * k + 3] / G.A_TO_A0
+        assert np.linalg.norm(w[1] - w[0]) == pytest.approx(0.9572)
+        assert np.linalg.norm(w[2] - w[0]) == pytest.approx(0.9572)
+    Z, xyz = G.water_box(6, seed=2)
+    O_ = xyz[Z == 8] / G.A_TO_A0
+    d = np.linalg.norm(O_[:, None] - O_[None], axis=-1)
+    assert d[d > 0].min() >= 2.6 - 1e-9
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr

@@ -51,6 +51,20 @@ def cpu_baseline(Z, xyz, n_ao, n_unique_ao_quartets, sample, seed=7):
                       f"J/K tensor lookups excluded"}
 
 
+def copy_calibration(mb: int, reps: int = 5):
+    """A plain device copy of known size, for scaling the FETCH_SIZE/WRITE_SIZE
+    counters (torch is used here as memory plumbing only)."""
+    import torch
+
+    n = mb * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_mib": mb, "reps": reps, "bytes_read_per_copy": n * 8, "bytes_written_per_copy": n * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,10 +73,15 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--boys", choices=["accurate", "reference"], default="accurate")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     a = ap.parse_args()
+    if a.copy_mb:
+        copy_calibration(a.copy_mb)
+        return
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is a design in "
-              f"DESIGN.md, not a measured path.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is a design, "
+              f"not a measured path.", file=sys.stderr)
         sys.exit(2)
 
     import dollarstorescf_amd as ds

@@ -31,7 +31,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-I" + os.path.join(ROOT,
 HIP_SOURCES = ["eri_kernels.hip", "dsscf_abi.cpp"]
 HOST_SOURCES = ["host/chem.cpp", "host/linalg.cpp", "host/scf.cpp", "host/job_api.cpp"]
 
-DEFAULT_MAXL = 1
+DEFAULT_MAXL = 2
 
 
 def _run(cmd):

@@ -41,6 +41,50 @@ struct __align__(16) PrimPair {
 };
 static_assert(sizeof(PrimPair) == 64, "PrimPair must be one 64 B record");
 
+// One context's share of a fused Fock-build launch (one class): its quartet
+// list and the buffers the quartets read and accumulate into.  blk0 is the
+// first block of the launch that belongs to this segment.
+struct JKSeg {
+    const uint2* q;
+    const ShellPair* pairs;
+    const PrimPair* pp;
+    const double* Dt;
+    const double* Da;
+    double* Jh;
+    double* Kh;
+    int n;
+    int N;
+    int blk0;
+    int pad;
+};
+
+// One context's matrices for the fused prepare/fold kernels; row0 is its
+// first row in the launch's stacked row range.
+struct MatSeg {
+    const double* Ptp;  // packed-U total density
+    const double* Pap;  // packed-U alpha density
+    double* Dt;         // full N x N
+    double* Da;
+    double* Jh;         // half-accumulated J, K (full N x N)
+    double* Kh;
+    double* Jp;         // packed-U outputs
+    double* Kp;
+    int N;
+    int row0;
+};
+
+__device__ __forceinline__ int find_segment_by_block(const int* blk0, int nseg, int b)
+{
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk0[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+
 // Reference Boys series (GaussianIntegral.cpp:209-223): asymptotic form for
 // x > 1.1m+25 (with flo::pow's multiplication order, MathUtil.hpp:57-67),
 // otherwise sum a_i = a_{i-1} 2x/(2m+2i+1) until the first a_i <= 1e-9.
@@ -178,3 +222,5 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
 
 template <int la, int lb, int lc, int ld>
 struct ClassInfo;
+template <int la, int lb, int lc, int ld>
+struct WaveTab;

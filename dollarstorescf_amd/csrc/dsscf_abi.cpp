@@ -56,6 +56,39 @@ void dev_free(T*& p)
     p = nullptr;
 }
 
+
+// ------------------------------------------------------------------ plan --
+// A fused Fock-build launch over one or more contexts on one device: per
+// class, a segment table (one JKSeg per context with quartets of the class);
+// for the prepare/fold kernels, one MatSeg per context over stacked rows.
+struct Plan {
+    int boys_mode = DSSCF_BOYS_ACCURATE;
+    const double* btab = nullptr;
+    int ncls = 0;
+    std::vector<int> cls_nseg, cls_nblocks, cls_off, cls_bs;
+    int nmat = 0, total_rows = 0, maxN = 0;
+    std::vector<JKSeg> h_segs;
+    std::vector<int> h_blk0;
+    std::vector<MatSeg> h_mat;
+    JKSeg* d_segs = nullptr;
+    int* d_blk0 = nullptr;
+    MatSeg* d_mat = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool built = false;
+
+    void release()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        dev_free(d_segs);
+        dev_free(d_blk0);
+        dev_free(d_mat);
+        built = false;
+    }
+};
 }  // namespace
 
 struct dsscf_ctx {
@@ -90,8 +123,12 @@ struct dsscf_ctx {
     double *d_Dt = nullptr, *d_Da = nullptr, *d_Jh = nullptr, *d_Kh = nullptr;
     double *d_Pt_pk = nullptr, *d_Pa_pk = nullptr, *d_J_pk = nullptr, *d_K_pk = nullptr;
 
+    std::vector<int> block_size;  // per class, thread-per-quartet kernels (autotuned)
+    Plan plan;                    // fused launch plan over this context alone
+
     void free_geometry()
     {
+        plan.release();
         dev_free(d_pairs);
         dev_free(d_pp);
         dev_free(d_q);
@@ -371,42 +408,166 @@ int dsscf_prepare(dsscf_ctx* c)
     if (int rc = dev_alloc(&c->d_q, all.size())) return rc;
     HIP_TRY(hipMemcpyAsync(c->d_q, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if ((int)c->block_size.size() != ncls) c->block_size.assign(ncls, 256);
     c->prepared = true;
     return DSSCF_OK;
 }
 
 namespace {
-int run_jk(dsscf_ctx* c, const double* d_Pt, const double* d_Pa, double* d_J, double* d_K, std::vector<float>* cls_ms)
+
+// Build `p` over contexts ctxs[0..n): their packed inputs/outputs at
+// dPt/dPa/dJ/dK (device), block sizes per class from bs (nullptr: each
+// class's default of 256 or the first context's tuned value).
+int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt, const double* const* dPa,
+               double* const* dJ, double* const* dK, const int* bs, hipStream_t stream)
 {
-    const int N = c->n_ao;
-    const size_t nn = (size_t)N * N;
-    HIP_TRY(dsscf_launch_unpack(d_Pt, c->d_Dt, N, c->stream));
-    HIP_TRY(dsscf_launch_unpack(d_Pa, c->d_Da, N, c->stream));
-    HIP_TRY(dsscf_launch_fill(c->d_Jh, nn, 0.0, c->stream));
-    HIP_TRY(dsscf_launch_fill(c->d_Kh, nn, 0.0, c->stream));
+    p.release();
     const int ncls = dk_n_class();
-    std::vector<hipEvent_t> ev;
-    if (cls_ms) {
-        ev.resize(ncls + 1);
-        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-        HIP_TRY(hipEventRecord(ev[0], c->stream));
-    }
+    p.ncls = ncls;
+    p.boys_mode = ctxs[0]->opt.boys_mode;
+    p.btab = ctxs[0]->d_btab;
+    p.cls_nseg.assign(ncls, 0);
+    p.cls_nblocks.assign(ncls, 0);
+    p.cls_off.assign(ncls, 0);
+    p.cls_bs.assign(ncls, 256);
+    p.h_segs.clear();
+    p.h_blk0.clear();
+    p.h_mat.clear();
     for (int cls = 0; cls < ncls; ++cls) {
-        const long long n = c->q_cnt[cls];
-        if (n > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "quartet list longer than 2^31");
-        HIP_TRY(dsscf_launch_jk(cls, c->opt.boys_mode, c->d_q + c->q_off[cls], (int)n, c->d_pairs, c->d_pp,
-                                c->d_btab, c->d_Dt, c->d_Da, c->d_Jh, c->d_Kh, N, kEriScale, c->stream));
-        if (cls_ms) HIP_TRY(hipEventRecord(ev[cls + 1], c->stream));
+        const bool wave = dsscf_class_is_wave(cls) != 0;
+        const int b = wave ? 64 : (bs ? bs[cls] : (ctxs[0]->block_size.empty() ? 256 : ctxs[0]->block_size[cls]));
+        p.cls_bs[cls] = b;
+        p.cls_off[cls] = (int)p.h_segs.size();
+        long long blocks = 0;
+        for (int k = 0; k < n; ++k) {
+            dsscf_ctx* c = ctxs[k];
+            const long long cnt = c->q_cnt[cls];
+            if (cnt <= 0) continue;
+            if (cnt > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "quartet list longer than 2^31");
+            JKSeg sg{};
+            sg.q = c->d_q + c->q_off[cls];
+            sg.pairs = c->d_pairs;
+            sg.pp = c->d_pp;
+            sg.Dt = c->d_Dt;
+            sg.Da = c->d_Da;
+            sg.Jh = c->d_Jh;
+            sg.Kh = c->d_Kh;
+            sg.n = (int)cnt;
+            sg.N = c->n_ao;
+            sg.blk0 = (int)blocks;
+            blocks += wave ? cnt : (cnt + b - 1) / b;
+            if (blocks > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "launch grid too large");
+            p.h_segs.push_back(sg);
+            p.h_blk0.push_back(sg.blk0);
+        }
+        p.cls_nseg[cls] = (int)p.h_segs.size() - p.cls_off[cls];
+        p.cls_nblocks[cls] = (int)blocks;
     }
-    HIP_TRY(dsscf_launch_fold(c->d_Jh, c->d_Kh, d_J, d_K, N, c->stream));
-    if (cls_ms) {
-        HIP_TRY(hipEventSynchronize(ev[ncls]));
-        cls_ms->assign(ncls, 0.0f);
-        for (int cls = 0; cls < ncls; ++cls) HIP_TRY(hipEventElapsedTime(&(*cls_ms)[cls], ev[cls], ev[cls + 1]));
-        for (auto& e : ev) (void)hipEventDestroy(e);
+    int row = 0, maxN = 0;
+    for (int k = 0; k < n; ++k) {
+        dsscf_ctx* c = ctxs[k];
+        MatSeg m{};
+        m.Ptp = dPt[k];
+        m.Pap = dPa[k];
+        m.Dt = c->d_Dt;
+        m.Da = c->d_Da;
+        m.Jh = c->d_Jh;
+        m.Kh = c->d_Kh;
+        m.Jp = dJ[k];
+        m.Kp = dK[k];
+        m.N = c->n_ao;
+        m.row0 = row;
+        row += c->n_ao;
+        maxN = std::max(maxN, c->n_ao);
+        p.h_mat.push_back(m);
     }
+    p.nmat = n;
+    p.total_rows = row;
+    p.maxN = maxN;
+    if (int rc = dev_alloc(&p.d_segs, p.h_segs.size())) return rc;
+    if (int rc = dev_alloc(&p.d_blk0, p.h_blk0.size())) return rc;
+    if (int rc = dev_alloc(&p.d_mat, p.h_mat.size())) return rc;
+    if (!p.h_segs.empty()) {
+        HIP_TRY(hipMemcpyAsync(p.d_segs, p.h_segs.data(), sizeof(JKSeg) * p.h_segs.size(), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(p.d_blk0, p.h_blk0.data(), sizeof(int) * p.h_blk0.size(), hipMemcpyHostToDevice, stream));
+    }
+    HIP_TRY(hipMemcpyAsync(p.d_mat, p.h_mat.data(), sizeof(MatSeg) * p.h_mat.size(), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    p.built = true;
     return DSSCF_OK;
 }
+
+int plan_launch_class(Plan& p, int cls, hipStream_t s)
+{
+    if (p.cls_nseg[cls] == 0) return DSSCF_OK;
+    HIP_TRY(dsscf_launch_jk(cls, p.boys_mode, p.d_segs + p.cls_off[cls], p.d_blk0 + p.cls_off[cls], p.cls_nseg[cls],
+                            p.cls_nblocks[cls], p.cls_bs[cls], p.btab, kEriScale, s));
+    return DSSCF_OK;
+}
+
+// prepare (unpack densities, zero accumulators), all classes, fold.
+int plan_enqueue(Plan& p, hipStream_t s, std::vector<hipEvent_t>* ev)
+{
+    HIP_TRY(dsscf_launch_prep(p.d_mat, p.nmat, p.total_rows, p.maxN, s));
+    if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
+    for (int cls = 0; cls < p.ncls; ++cls) {
+        if (int rc = plan_launch_class(p, cls, s)) return rc;
+        if (ev) HIP_TRY(hipEventRecord((*ev)[cls + 1], s));
+    }
+    HIP_TRY(dsscf_launch_fold(p.d_mat, p.nmat, p.total_rows, p.maxN, s));
+    return DSSCF_OK;
+}
+
+// Run one fused Fock build; with use_graph the launch sequence is captured
+// once into a hipGraph (kernels only) and replayed.
+int plan_run(Plan& p, hipStream_t s, bool use_graph, std::vector<float>* cls_ms)
+{
+    if (cls_ms) {
+        std::vector<hipEvent_t> ev(p.ncls + 1);
+        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+        int rc = plan_enqueue(p, s, &ev);
+        if (!rc) {
+            HIP_TRY(hipEventSynchronize(ev[p.ncls]));
+            cls_ms->assign(p.ncls, 0.0f);
+            for (int k = 0; k < p.ncls; ++k) HIP_TRY(hipEventElapsedTime(&(*cls_ms)[k], ev[k], ev[k + 1]));
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        return rc;
+    }
+    if (!use_graph) return plan_enqueue(p, s, nullptr);
+    if (!p.gexec) {
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = plan_enqueue(p, s, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(DSSCF_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        p.graph = g;
+        HIP_TRY(hipGraphInstantiate(&p.gexec, p.graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(p.gexec, s));
+    return DSSCF_OK;
+}
+
+int ctx_plan(dsscf_ctx* c)
+{
+    if (c->plan.built) return DSSCF_OK;
+    const double* pt = c->d_Pt_pk;
+    const double* pa = c->d_Pa_pk;
+    double* j = c->d_J_pk;
+    double* k = c->d_K_pk;
+    return plan_build(c->plan, &c, 1, &pt, &pa, &j, &k, nullptr, c->stream);
+}
+
+int run_jk(dsscf_ctx* c, std::vector<float>* cls_ms)
+{
+    if (int rc = ctx_plan(c)) return rc;
+    return plan_run(c->plan, c->stream, false, cls_ms);
+}
+
 }  // namespace
 
 int dsscf_jk_device(dsscf_ctx* c, const double* d_Pt, const double* d_Pa, double* d_J, double* d_K)
@@ -414,7 +575,12 @@ int dsscf_jk_device(dsscf_ctx* c, const double* d_Pt, const double* d_Pa, double
     if (!c || !d_Pt || !d_Pa || !d_J || !d_K) return fail(DSSCF_ERR_ARG, "dsscf_jk_device: null argument");
     if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_jk_device: call dsscf_prepare first");
     HIP_TRY(hipSetDevice(c->device));
-    if (int rc = run_jk(c, d_Pt, d_Pa, d_J, d_K, nullptr)) return rc;
+    const size_t np = (size_t)c->n_ao * (c->n_ao + 1) / 2;
+    HIP_TRY(hipMemcpyAsync(c->d_Pt_pk, d_Pt, sizeof(double) * np, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_Pa_pk, d_Pa, sizeof(double) * np, hipMemcpyDeviceToDevice, c->stream));
+    if (int rc = run_jk(c, nullptr)) return rc;
+    HIP_TRY(hipMemcpyAsync(d_J, c->d_J_pk, sizeof(double) * np, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_K, c->d_K_pk, sizeof(double) * np, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return DSSCF_OK;
 }
@@ -427,7 +593,7 @@ int dsscf_jk(dsscf_ctx* c, const double* Pt, const double* Pa, double* J, double
     const size_t np = (size_t)c->n_ao * (c->n_ao + 1) / 2;
     HIP_TRY(hipMemcpyAsync(c->d_Pt_pk, Pt, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_Pa_pk, Pa, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
-    if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, nullptr)) return rc;
+    if (int rc = run_jk(c, nullptr)) return rc;
     HIP_TRY(hipMemcpyAsync(J, c->d_J_pk, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(K, c->d_K_pk, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -451,7 +617,7 @@ int dsscf_jk_resident(dsscf_ctx* c)
     if (!c) return fail(DSSCF_ERR_ARG, "dsscf_jk_resident: null context");
     if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_jk_resident: call dsscf_prepare first");
     HIP_TRY(hipSetDevice(c->device));
-    if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, nullptr)) return rc;
+    if (int rc = run_jk(c, nullptr)) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return DSSCF_OK;
 }
@@ -474,7 +640,7 @@ int dsscf_time_jk(dsscf_ctx* c, int warmup, int iters, double* ms_per_build, dou
     if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_time_jk: call dsscf_prepare first");
     HIP_TRY(hipSetDevice(c->device));
     for (int i = 0; i < warmup; ++i)
-        if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, nullptr)) return rc;
+        if (int rc = run_jk(c, nullptr)) return rc;
     const int ncls = dk_n_class();
     std::vector<double> acc(ncls, 0.0);
     hipEvent_t e0, e1;
@@ -483,7 +649,7 @@ int dsscf_time_jk(dsscf_ctx* c, int warmup, int iters, double* ms_per_build, dou
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipEventRecord(e0, c->stream));
     for (int i = 0; i < iters; ++i)
-        if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, nullptr)) return rc;
+        if (int rc = run_jk(c, nullptr)) return rc;
     HIP_TRY(hipEventRecord(e1, c->stream));
     HIP_TRY(hipEventSynchronize(e1));
     float total = 0.0f;
@@ -493,7 +659,7 @@ int dsscf_time_jk(dsscf_ctx* c, int warmup, int iters, double* ms_per_build, dou
     // Per-class durations from a separate pass with events between launches.
     std::vector<float> ms;
     for (int i = 0; i < iters; ++i) {
-        if (int rc = run_jk(c, c->d_Pt_pk, c->d_Pa_pk, c->d_J_pk, c->d_K_pk, &ms)) return rc;
+        if (int rc = run_jk(c, &ms)) return rc;
         for (int k = 0; k < ncls; ++k) acc[k] += ms[k];
     }
     int best = 0;
@@ -619,6 +785,208 @@ int dsscf_eri_batch(dsscf_ctx* c, const int* ijkl, long long n, double* out)
             out[rq[i].t] = blk[u * nblock + rq[i].comp];
         }
     }
+    return DSSCF_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- batch --
+struct dsscf_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<dsscf_ctx*> ctxs;
+    std::vector<size_t> off;  // packed offset of each context
+    size_t total = 0;         // packed elements over all contexts
+    double* d_buf = nullptr;  // [Pt | Pa | J | K], each `total`
+    double* h_pin = nullptr;  // pinned staging, same layout
+    Plan plan;
+    int use_graph = 1;
+    std::vector<int> block_size;
+
+    void release_buffers()
+    {
+        plan.release();
+        dev_free(d_buf);
+        if (h_pin) (void)hipHostFree(h_pin);
+        h_pin = nullptr;
+    }
+};
+
+extern "C" {
+
+int dsscf_batch_create(dsscf_batch** out, int device)
+{
+    if (!out) return fail(DSSCF_ERR_ARG, "dsscf_batch_create: null output");
+    HIP_TRY(hipSetDevice(device));
+    dsscf_batch* b = new dsscf_batch();
+    b->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(DSSCF_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    b->block_size.assign(dk_n_class(), 256);
+    *out = b;
+    return DSSCF_OK;
+}
+
+int dsscf_batch_destroy(dsscf_batch* b)
+{
+    if (!b) return DSSCF_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    b->release_buffers();
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return DSSCF_OK;
+}
+
+namespace {
+int batch_build_plan(dsscf_batch* b)
+{
+    const int n = (int)b->ctxs.size();
+    std::vector<const double*> pt(n), pa(n);
+    std::vector<double*> j(n), k(n);
+    for (int i = 0; i < n; ++i) {
+        pt[i] = b->d_buf + b->off[i];
+        pa[i] = b->d_buf + b->total + b->off[i];
+        j[i] = b->d_buf + 2 * b->total + b->off[i];
+        k[i] = b->d_buf + 3 * b->total + b->off[i];
+    }
+    return plan_build(b->plan, b->ctxs.data(), n, pt.data(), pa.data(), j.data(), k.data(), b->block_size.data(),
+                      b->stream);
+}
+}  // namespace
+
+int dsscf_batch_set(dsscf_batch* b, dsscf_ctx* const* ctxs, int n)
+{
+    if (!b || n <= 0 || !ctxs) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: bad argument");
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i] || !ctxs[i]->prepared) return fail(DSSCF_ERR_STATE, "dsscf_batch_set: context not prepared");
+        if (ctxs[i]->device != b->device) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: contexts on another device");
+        if (ctxs[i]->opt.boys_mode != ctxs[0]->opt.boys_mode)
+            return fail(DSSCF_ERR_ARG, "dsscf_batch_set: mixed Boys modes in one batch");
+        for (int k = 0; k < i; ++k)
+            if (ctxs[k] == ctxs[i]) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: context listed twice");
+    }
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->release_buffers();
+    b->ctxs.assign(ctxs, ctxs + n);
+    b->off.resize(n);
+    b->total = 0;
+    for (int i = 0; i < n; ++i) {
+        b->off[i] = b->total;
+        b->total += (size_t)ctxs[i]->n_ao * (ctxs[i]->n_ao + 1) / 2;
+    }
+    if (int rc = dev_alloc(&b->d_buf, 4 * b->total)) return rc;
+    HIP_TRY(hipHostMalloc((void**)&b->h_pin, sizeof(double) * 4 * b->total, hipHostMallocDefault));
+    // the contexts' own streams must be idle: the batch writes their buffers
+    for (dsscf_ctx* c : b->ctxs) HIP_TRY(hipStreamSynchronize(c->stream));
+    return batch_build_plan(b);
+}
+
+int dsscf_batch_set_graph(dsscf_batch* b, int use_graph)
+{
+    if (!b) return fail(DSSCF_ERR_ARG, "dsscf_batch_set_graph: null batch");
+    b->use_graph = use_graph;
+    return DSSCF_OK;
+}
+
+int dsscf_batch_jk(dsscf_batch* b, const double* const* Ptot, const double* const* Palpha, double* const* J,
+                   double* const* K)
+{
+    if (!b || !Ptot || !Palpha || !J || !K) return fail(DSSCF_ERR_ARG, "dsscf_batch_jk: null argument");
+    if (!b->plan.built) return fail(DSSCF_ERR_STATE, "dsscf_batch_jk: call dsscf_batch_set first");
+    HIP_TRY(hipSetDevice(b->device));
+    const int n = (int)b->ctxs.size();
+    for (int i = 0; i < n; ++i) {
+        const size_t np = (size_t)b->ctxs[i]->n_ao * (b->ctxs[i]->n_ao + 1) / 2;
+        if (!Ptot[i] || !Palpha[i] || !J[i] || !K[i]) return fail(DSSCF_ERR_ARG, "dsscf_batch_jk: null matrix");
+        std::memcpy(b->h_pin + b->off[i], Ptot[i], sizeof(double) * np);
+        std::memcpy(b->h_pin + b->total + b->off[i], Palpha[i], sizeof(double) * np);
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_buf, b->h_pin, sizeof(double) * 2 * b->total, hipMemcpyHostToDevice, b->stream));
+    if (int rc = plan_run(b->plan, b->stream, b->use_graph != 0, nullptr)) return rc;
+    HIP_TRY(hipMemcpyAsync(b->h_pin + 2 * b->total, b->d_buf + 2 * b->total, sizeof(double) * 2 * b->total,
+                           hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < n; ++i) {
+        const size_t np = (size_t)b->ctxs[i]->n_ao * (b->ctxs[i]->n_ao + 1) / 2;
+        std::memcpy(J[i], b->h_pin + 2 * b->total + b->off[i], sizeof(double) * np);
+        std::memcpy(K[i], b->h_pin + 3 * b->total + b->off[i], sizeof(double) * np);
+    }
+    return DSSCF_OK;
+}
+
+int dsscf_batch_time(dsscf_batch* b, int warmup, int iters, double* ms_per_build)
+{
+    if (!b || iters <= 0 || warmup < 0) return fail(DSSCF_ERR_ARG, "dsscf_batch_time: bad argument");
+    if (!b->plan.built) return fail(DSSCF_ERR_STATE, "dsscf_batch_time: call dsscf_batch_set first");
+    HIP_TRY(hipSetDevice(b->device));
+    for (int i = 0; i < warmup; ++i)
+        if (int rc = plan_run(b->plan, b->stream, b->use_graph != 0, nullptr)) return rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, b->stream));
+    for (int i = 0; i < iters; ++i)
+        if (int rc = plan_run(b->plan, b->stream, b->use_graph != 0, nullptr)) return rc;
+    HIP_TRY(hipEventRecord(e1, b->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_per_build) *ms_per_build = ms / iters;
+    return DSSCF_OK;
+}
+
+// Project autotuner: for every thread-per-quartet class with work, time the
+// class kernel alone at 64, 128 and 256 threads per block (3 launches each,
+// HIP events) and keep the fastest; the result goes into the batch's plan
+// and, for a single-context batch, into the context.
+int dsscf_batch_autotune(dsscf_batch* b, int* chosen)
+{
+    if (!b || !b->plan.built) return fail(DSSCF_ERR_STATE, "dsscf_batch_autotune: call dsscf_batch_set first");
+    HIP_TRY(hipSetDevice(b->device));
+    const int ncls = dk_n_class();
+    const int cand[3] = {64, 128, 256};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    std::vector<int> best = b->block_size;
+    for (int cls = 0; cls < ncls; ++cls) {
+        if (dsscf_class_is_wave(cls) || b->plan.cls_nseg[cls] == 0) continue;
+        float best_ms = 1e30f;
+        for (int ci = 0; ci < 3; ++ci) {
+            b->block_size[cls] = cand[ci];
+            if (int rc = batch_build_plan(b)) return rc;
+            if (int rc = plan_launch_class(b->plan, cls, b->stream)) return rc;  // warm
+            HIP_TRY(hipEventRecord(e0, b->stream));
+            for (int r = 0; r < 3; ++r)
+                if (int rc = plan_launch_class(b->plan, cls, b->stream)) return rc;
+            HIP_TRY(hipEventRecord(e1, b->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best[cls] = cand[ci];
+            }
+        }
+        b->block_size[cls] = best[cls];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->block_size = best;
+    if (int rc = batch_build_plan(b)) return rc;
+    if (b->ctxs.size() == 1) {
+        b->ctxs[0]->block_size = best;
+        b->ctxs[0]->plan.release();
+    }
+    if (chosen)
+        for (int k = 0; k < ncls; ++k) chosen[k] = best[k];
     return DSSCF_OK;
 }
 

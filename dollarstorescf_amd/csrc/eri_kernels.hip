@@ -15,8 +15,8 @@
 
 template <int la, int lb, int lc, int ld, int BM, typename R>
 struct Quartet;
-
-#define DSSCF_QUARTET_SPECIALIZE(a, b, c, d, name)                                                    \
+#define DSSCF_QSPEC_1(a, b, c, d, name)
+#define DSSCF_QSPEC_0(a, b, c, d, name)                                                               \
     template <int BM, typename R>                                                                     \
     struct Quartet<a, b, c, d, BM, R> {                                                               \
         static __device__ __forceinline__ void run(const ShellPair& bra, const ShellPair& ket,        \
@@ -26,7 +26,10 @@ struct Quartet;
             shell_quartet_##name<BM, R>(bra, ket, pp, btab, out);                                     \
         }                                                                                             \
     };
+#define DSSCF_QUARTET_SPECIALIZE(a, b, c, d, name, wave) DSSCF_QSPEC_##wave(a, b, c, d, name)
 DSSCF_FOR_EACH_CLASS(DSSCF_QUARTET_SPECIALIZE)
+
+#include "wave_kernel.hpp"
 
 __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 {
@@ -40,19 +43,23 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // (1+d_ab,cd)) stands for its 8 permutation images; half of them go into
 // Jh/Kh and the other half are the transposes added by the fold kernel.
 template <int la, int lb, int lc, int ld, int BM>
-__global__ void __launch_bounds__(256) k_jk(const uint2* __restrict__ qlist, int nq,
-                                            const ShellPair* __restrict__ pairs,
-                                            const PrimPair* __restrict__ pp,
-                                            const double* __restrict__ btab,
-                                            const double* __restrict__ Dt, const double* __restrict__ Da,
-                                            double* __restrict__ Jh, double* __restrict__ Kh, int N,
-                                            double scale)
+__global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
+                                            int nseg, const double* __restrict__ btab, double scale)
 {
     typedef ClassInfo<la, lb, lc, ld> CI;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nq) return;
-    const uint2 q = qlist[t];
+    const int si = find_segment_by_block(seg_blk0, nseg, (int)blockIdx.x);
+    const JKSeg& S = segs[si];
+    const int t = ((int)blockIdx.x - S.blk0) * (int)blockDim.x + (int)threadIdx.x;
+    if (t >= S.n) return;
+    const ShellPair* __restrict__ pairs = S.pairs;
+    const PrimPair* __restrict__ pp = S.pp;
+    const double* __restrict__ Dt = S.Dt;
+    const double* __restrict__ Da = S.Da;
+    double* __restrict__ Jh = S.Jh;
+    double* __restrict__ Kh = S.Kh;
+    const int N = S.N;
+    const uint2 q = S.q[t];
     const ShellPair bra = pairs[q.x];
     const ShellPair ket = pairs[q.y];
     double v[CI::NAO];
@@ -215,26 +222,46 @@ __global__ void k_fill(double* __restrict__ p, size_t n, double v)
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// packed-U (y<=x at x(x+1)/2+y, Matrix.ipp:332-351) -> full row-major
-__global__ void k_unpack(const double* __restrict__ pk, double* __restrict__ full, int N)
+__device__ __forceinline__ int find_segment_by_row(const MatSeg* __restrict__ segs, int nseg, int r)
 {
-    const int x = blockIdx.y;
-    for (int y = threadIdx.x + blockIdx.x * blockDim.x; y <= x; y += gridDim.x * blockDim.x) {
-        const double v = pk[(size_t)x * (x + 1) / 2 + y];
-        full[(size_t)y * N + x] = v;
-        full[(size_t)x * N + y] = v;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].row0 <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Per context and row x: packed-U densities (element (y,x), y<=x, at
+// x(x+1)/2+y, Matrix.ipp:332-351) -> full row-major Dt, Da; zero Jh, Kh rows.
+__global__ void k_prep(const MatSeg* __restrict__ segs, int nseg)
+{
+    const MatSeg& S = segs[find_segment_by_row(segs, nseg, (int)blockIdx.y)];
+    const int x = (int)blockIdx.y - S.row0, N = S.N;
+    for (int y = threadIdx.x + blockIdx.x * blockDim.x; y < N; y += gridDim.x * blockDim.x) {
+        S.Jh[(size_t)x * N + y] = 0.0;
+        S.Kh[(size_t)x * N + y] = 0.0;
+        if (y <= x) {
+            const size_t o = (size_t)x * (x + 1) / 2 + y;
+            const double t = S.Ptp[o], a = S.Pap[o];
+            S.Dt[(size_t)y * N + x] = t;
+            S.Dt[(size_t)x * N + y] = t;
+            S.Da[(size_t)y * N + x] = a;
+            S.Da[(size_t)x * N + y] = a;
+        }
     }
 }
 
-// J = Jh + Jh^T, K = Kh + Kh^T -> packed-U
-__global__ void k_fold(const double* __restrict__ Jh, const double* __restrict__ Kh, double* __restrict__ J,
-                       double* __restrict__ K, int N)
+// J = Jh + Jh^T, K = Kh + Kh^T -> packed-U, per context.
+__global__ void k_fold(const MatSeg* __restrict__ segs, int nseg)
 {
-    const int x = blockIdx.y;
+    const MatSeg& S = segs[find_segment_by_row(segs, nseg, (int)blockIdx.y)];
+    const int x = (int)blockIdx.y - S.row0, N = S.N;
     for (int y = threadIdx.x + blockIdx.x * blockDim.x; y <= x; y += gridDim.x * blockDim.x) {
         const size_t o = (size_t)x * (x + 1) / 2 + y;
-        J[o] = Jh[(size_t)y * N + x] + Jh[(size_t)x * N + y];
-        K[o] = Kh[(size_t)y * N + x] + Kh[(size_t)x * N + y];
+        S.Jp[o] = S.Jh[(size_t)y * N + x] + S.Jh[(size_t)x * N + y];
+        S.Kp[o] = S.Kh[(size_t)y * N + x] + S.Kh[(size_t)x * N + y];
     }
 }
 
@@ -252,7 +279,7 @@ struct ClassDesc {
     int la, lb, lc, ld;
 };
 const ClassDesc kClasses[] = {
-#define DSSCF_CLASS_DESC(a, b, c, d, name) {a, b, c, d},
+#define DSSCF_CLASS_DESC(a, b, c, d, name, wave) {a, b, c, d},
     DSSCF_FOR_EACH_CLASS(DSSCF_CLASS_DESC)
 #undef DSSCF_CLASS_DESC
 };
@@ -268,26 +295,122 @@ void dk_class_l(int c, int* l)
     l[3] = kClasses[c].ld;
 }
 
-hipError_t dsscf_launch_jk(int cls, int boys_mode, const uint2* q, int nq, const ShellPair* pairs,
-                           const PrimPair* pp, const double* btab, const double* Dt, const double* Da,
-                           double* Jh, double* Kh, int N, double scale, hipStream_t s)
-{
-    if (nq <= 0) return hipSuccess;
-    const dim3 blk(256), grd((nq + 255) / 256);
-    int c = 0;
-#define DSSCF_LAUNCH_JK(a, b, cc, d, name)                                                                  \
-    if (c++ == cls) {                                                                                       \
-        if (boys_mode == BOYS_ACCURATE)                                                                     \
-            hipLaunchKernelGGL((k_jk<a, b, cc, d, BOYS_ACCURATE>), grd, blk, 0, s, q, nq, pairs, pp, btab, \
-                               Dt, Da, Jh, Kh, N, scale);                                                   \
-        else                                                                                                \
-            hipLaunchKernelGGL((k_jk<a, b, cc, d, BOYS_REFERENCE>), grd, blk, 0, s, q, nq, pairs, pp, btab,\
-                               Dt, Da, Jh, Kh, N, scale);                                                   \
-        return hipGetLastError();                                                                           \
+// Thread-per-quartet (wave = 0) or wave-cooperative (wave = 1) launches.
+template <int a, int b, int c, int d, int wave>
+struct Launch {
+    static hipError_t jk(int bm, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int bs,
+                         const double* btab, double scale, hipStream_t s)
+    {
+        const dim3 blk(bs), grd(nblocks);
+        if (bm == BOYS_ACCURATE)
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE>), grd, blk, 0, s, segs, blk0, nseg, btab, scale);
+        else
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE>), grd, blk, 0, s, segs, blk0, nseg, btab, scale);
+        return hipGetLastError();
     }
+    static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
+                            const double* btab, double* out, double scale, hipStream_t s)
+    {
+        const dim3 blk(256), grd((n + 255) / 256);
+        if (bm == BOYS_ACCURATE)
+            hipLaunchKernelGGL((k_eri_block<a, b, c, d, BOYS_ACCURATE>), grd, blk, 0, s, items, n, pairs, pp, btab,
+                               out, scale);
+        else
+            hipLaunchKernelGGL((k_eri_block<a, b, c, d, BOYS_REFERENCE>), grd, blk, 0, s, items, n, pairs, pp, btab,
+                               out, scale);
+        return hipGetLastError();
+    }
+    static hipError_t schwarz(int bm, const int* plist, int n, const ShellPair* pairs, const PrimPair* pp,
+                              const double* btab, double* Q, double scale, hipStream_t s)
+    {
+        const dim3 blk(256), grd((n + 255) / 256);
+        if (bm == BOYS_ACCURATE)
+            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_ACCURATE>), grd, blk, 0, s, plist, n, pairs, pp, btab, Q, scale);
+        else
+            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_REFERENCE>), grd, blk, 0, s, plist, n, pairs, pp, btab, Q, scale);
+        return hipGetLastError();
+    }
+};
+
+template <int a, int b, int c, int d>
+struct Launch<a, b, c, d, 1> {
+    template <int BM, int MODE>
+    static void go(const void* items, int n, const ShellPair* pairs, const PrimPair* pp, const double* btab,
+                   double* out, double scale, hipStream_t s)
+    {
+        const int grd = n < (1 << 20) ? n : (1 << 20);
+        hipLaunchKernelGGL((k_wave<a, b, c, d, BM, MODE>), dim3(grd), dim3(64), 0, s, items, n, pairs, pp, btab,
+                           nullptr, nullptr, 0, out, scale);
+    }
+    static hipError_t jk(int bm, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int,
+                         const double* btab, double scale, hipStream_t s)
+    {
+        if (bm == BOYS_ACCURATE)
+            hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_ACCURATE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
+                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
+        else
+            hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_REFERENCE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
+                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
+        return hipGetLastError();
+    }
+    static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
+                            const double* btab, double* out, double scale, hipStream_t s)
+    {
+        if (bm == BOYS_ACCURATE)
+            go<BOYS_ACCURATE, WAVE_BLOCK>(items, n, pairs, pp, btab, out, scale, s);
+        else
+            go<BOYS_REFERENCE, WAVE_BLOCK>(items, n, pairs, pp, btab, out, scale, s);
+        return hipGetLastError();
+    }
+    static hipError_t schwarz(int bm, const int* plist, int n, const ShellPair* pairs, const PrimPair* pp,
+                              const double* btab, double* Q, double scale, hipStream_t s)
+    {
+        if (bm == BOYS_ACCURATE)
+            go<BOYS_ACCURATE, WAVE_SCHWARZ>(plist, n, pairs, pp, btab, Q, scale, s);
+        else
+            go<BOYS_REFERENCE, WAVE_SCHWARZ>(plist, n, pairs, pp, btab, Q, scale, s);
+        return hipGetLastError();
+    }
+};
+
+// Schwarz bounds exist for the diagonal classes (ab|ab) only.
+template <int a, int b, int c, int d, int wave, bool diag>
+struct SchwarzIf {
+    static hipError_t run(int, const int*, int, const ShellPair*, const PrimPair*, const double*, double*, double,
+                          hipStream_t)
+    {
+        return hipErrorInvalidValue;
+    }
+};
+template <int a, int b, int c, int d, int wave>
+struct SchwarzIf<a, b, c, d, wave, true> {
+    static hipError_t run(int bm, const int* plist, int n, const ShellPair* pairs, const PrimPair* pp,
+                          const double* btab, double* Q, double scale, hipStream_t s)
+    {
+        return Launch<a, b, c, d, wave>::schwarz(bm, plist, n, pairs, pp, btab, Q, scale, s);
+    }
+};
+
+hipError_t dsscf_launch_jk(int cls, int boys_mode, const JKSeg* segs, const int* blk0, int nseg, int nblocks,
+                           int block_size, const double* btab, double scale, hipStream_t s)
+{
+    if (nseg <= 0 || nblocks <= 0) return hipSuccess;
+    int c = 0;
+#define DSSCF_LAUNCH_JK(a, b, cc, d, name, wave) \
+    if (c++ == cls) return Launch<a, b, cc, d, wave>::jk(boys_mode, segs, blk0, nseg, nblocks, block_size, btab, scale, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_JK)
 #undef DSSCF_LAUNCH_JK
     return hipErrorInvalidValue;
+}
+
+int dsscf_class_is_wave(int cls)
+{
+    int c = 0;
+#define DSSCF_WAVE_OF(a, b, cc, d, name, wave) \
+    if (c++ == cls) return wave;
+    DSSCF_FOR_EACH_CLASS(DSSCF_WAVE_OF)
+#undef DSSCF_WAVE_OF
+    return 0;
 }
 
 hipError_t dsscf_launch_eri_block(int cls, int boys_mode, const int4* items, int n, const ShellPair* pairs,
@@ -295,18 +418,9 @@ hipError_t dsscf_launch_eri_block(int cls, int boys_mode, const int4* items, int
                                   hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    const dim3 blk(256), grd((n + 255) / 256);
     int c = 0;
-#define DSSCF_LAUNCH_BLK(a, b, cc, d, name)                                                                       \
-    if (c++ == cls) {                                                                                             \
-        if (boys_mode == BOYS_ACCURATE)                                                                           \
-            hipLaunchKernelGGL((k_eri_block<a, b, cc, d, BOYS_ACCURATE>), grd, blk, 0, s, items, n, pairs, pp,   \
-                               btab, out, scale);                                                                 \
-        else                                                                                                      \
-            hipLaunchKernelGGL((k_eri_block<a, b, cc, d, BOYS_REFERENCE>), grd, blk, 0, s, items, n, pairs, pp,  \
-                               btab, out, scale);                                                                 \
-        return hipGetLastError();                                                                                 \
-    }
+#define DSSCF_LAUNCH_BLK(a, b, cc, d, name, wave) \
+    if (c++ == cls) return Launch<a, b, cc, d, wave>::block(boys_mode, items, n, pairs, pp, btab, out, scale, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_BLK)
 #undef DSSCF_LAUNCH_BLK
     return hipErrorInvalidValue;
@@ -316,25 +430,11 @@ hipError_t dsscf_launch_schwarz(int la, int lb, int boys_mode, const int* plist,
                                 const PrimPair* pp, const double* btab, double* Q, double scale, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    const dim3 blk(256), grd((n + 255) / 256);
-#define DSSCF_LAUNCH_SCH(a, b)                                                                                  \
-    if (la == a && lb == b) {                                                                                   \
-        if (boys_mode == BOYS_ACCURATE)                                                                         \
-            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_ACCURATE>), grd, blk, 0, s, plist, n, pairs, pp, btab, Q, \
-                               scale);                                                                          \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_REFERENCE>), grd, blk, 0, s, plist, n, pairs, pp, btab,   \
-                               Q, scale);                                                                       \
-        return hipGetLastError();                                                                               \
-    }
-    DSSCF_LAUNCH_SCH(0, 0)
-    DSSCF_LAUNCH_SCH(1, 0)
-    DSSCF_LAUNCH_SCH(1, 1)
-#if DSSCF_MAXL >= 2
-    DSSCF_LAUNCH_SCH(2, 0)
-    DSSCF_LAUNCH_SCH(2, 1)
-    DSSCF_LAUNCH_SCH(2, 2)
-#endif
+#define DSSCF_LAUNCH_SCH(a, b, cc, d, name, wave)                                                       \
+    if (a == cc && b == d && la == a && lb == b)                                                          \
+        return SchwarzIf<a, b, cc, d, wave, (a == cc && b == d)>::run(boys_mode, plist, n, pairs, pp, btab, Q, \
+                                                                       scale, s);
+    DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_SCH)
 #undef DSSCF_LAUNCH_SCH
     return hipErrorInvalidValue;
 }
@@ -346,14 +446,14 @@ hipError_t dsscf_launch_fill(double* p, size_t n, double v, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t dsscf_launch_unpack(const double* pk, double* full, int N, hipStream_t s)
+hipError_t dsscf_launch_prep(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_unpack, dim3((N + 255) / 256, N), dim3(256), 0, s, pk, full, N);
+    hipLaunchKernelGGL(k_prep, dim3((maxN + 255) / 256, total_rows), dim3(256), 0, s, segs, nseg);
     return hipGetLastError();
 }
 
-hipError_t dsscf_launch_fold(const double* Jh, const double* Kh, double* J, double* K, int N, hipStream_t s)
+hipError_t dsscf_launch_fold(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fold, dim3((N + 255) / 256, N), dim3(256), 0, s, Jh, Kh, J, K, N);
+    hipLaunchKernelGGL(k_fold, dim3((maxN + 255) / 256, total_rows), dim3(256), 0, s, segs, nseg);
     return hipGetLastError();
 }

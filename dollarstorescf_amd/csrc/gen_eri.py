@@ -359,10 +359,218 @@ def gen_class(c, spherical=True):
     return "\n".join(o)
 
 
+# ---------------------------------------------------------------------------
+# Wave-cooperative (LDS) form for the classes whose accumulators do not fit in
+# one thread's registers: the same VRR/HRR graph, emitted as level-ordered
+# tables that the 64 lanes of a wavefront walk together.
+REG_MAX_ACC = 100
+REG_MAX_HRR = 450
+
+
+def is_wave(c):
+    la, lb, lc, ld = c
+    _, targets = gen_vrr(la, lb, lc, ld)
+    acc = len({(e, f) for (e, f, _) in targets})
+    hrr, _ = gen_hrr(la, lb, lc, ld)
+    return acc > REG_MAX_ACC or len(hrr) > REG_MAX_HRR
+
+
+def vrr_structure(la, lb, lc, ld):
+    """Structured VRR graph with the same recursion choices as gen_vrr."""
+    E, Fk = la + lb, lc + ld
+    targets = [(e, f, 0) for le in range(la, E + 1) for e in cart(le)
+               for lf in range(lc, Fk + 1) for f in cart(lf)]
+    info, level = {}, {}
+
+    def need(key):
+        if key in info:
+            return
+        e, f, m = key
+        if sum(f) > 0:
+            i = pick_dir(f)
+            f1 = sub1(f, i)
+            src = [(e, f1, m), (e, f1, m + 1), None, None, None]
+            n1 = f1[i]
+            if n1 > 0:
+                f2 = sub1(f1, i)
+                src[2], src[3] = (e, f2, m), (e, f2, m + 1)
+            n2 = e[i]
+            if n2 > 0:
+                src[4] = (sub1(e, i), f1, m + 1)
+            info[key] = (2, i, n1, n2, src)
+        elif sum(e) > 0:
+            i = pick_dir(e)
+            e1 = sub1(e, i)
+            src = [(e1, f, m), (e1, f, m + 1), None, None, None]
+            n1 = e1[i]
+            if n1 > 0:
+                e2 = sub1(e1, i)
+                src[2], src[3] = (e2, f, m), (e2, f, m + 1)
+            info[key] = (1, i, n1, 0, src)
+        else:
+            info[key] = (0, 0, m, 0, [None] * 5)
+            level[key] = 0
+            return
+        for k in info[key][4]:
+            if k is not None:
+                need(k)
+        level[key] = 1 + max(level[k] for k in info[key][4] if k is not None)
+
+    for t in targets:
+        need(t)
+    keys = sorted(info, key=lambda k: (level[k], k))
+    return keys, info, level, targets
+
+
+def hrr_structure(la, lb, lc, ld, n_targets, tidx):
+    """Level-ordered HRR entries over Y = [targets | bra HRR | ket HRR]."""
+    Fk = lc + ld
+    fs = [f for lf in range(lc, Fk + 1) for f in cart(lf)]
+    yidx, entries = {}, []
+    for key, t in tidx.items():
+        yidx[("b",) + key] = t  # (e, f) of target -> ("b", a=e, b=0, f)
+    # bra stage: (a, b, f)
+    bra = {}
+
+    def need_b(a, b, f):
+        k = ("b", a, b, f)
+        if k in bra or sum(b) == 0:
+            return
+        i = pick_dir(b)
+        b1 = sub1(b, i)
+        a1 = a[:i] + (a[i] + 1,) + a[i + 1:]
+        need_b(a1, b1, f)
+        need_b(a, b1, f)
+        bra[k] = (("b", a1, b1, f), ("b", a, b1, f), i, sum(b))
+
+    for f in fs:
+        for a in cart(la):
+            for b in cart(lb):
+                need_b(a, b, f)
+    ket = {}
+
+    def need_k(a, b, c, d):
+        k = ("k", a, b, c, d)
+        if k in ket or sum(d) == 0:
+            return
+        i = pick_dir(d)
+        d1 = sub1(d, i)
+        c1 = c[:i] + (c[i] + 1,) + c[i + 1:]
+        need_k(a, b, c1, d1)
+        need_k(a, b, c, d1)
+        ket[k] = (("k", a, b, c1, d1), ("k", a, b, c, d1), i, sum(d))
+
+    for a in cart(la):
+        for b in cart(lb):
+            for c in cart(lc):
+                for d in cart(ld):
+                    need_k(a, b, c, d)
+
+    def tgt_key(a, f):
+        return ("b", a, tuple([0, 0, 0]) if False else (0, 0, 0), f)
+
+    # map bra-stage keys with b = 0 to targets; ket-stage keys with d = 0 to bra stage
+    def resolve(k):
+        if k[0] == "b" and sum(k[2]) == 0:
+            return yidx[("b", k[1], k[3])]
+        if k[0] == "k" and sum(k[4]) == 0:
+            bk = ("b", k[1], k[2], k[3])
+            return resolve(bk) if sum(k[2]) == 0 else yidx[bk]
+        return yidx[k]
+
+    nxt = n_targets
+    levels = []
+    for stage, table in ((0, bra), (1, ket)):
+        maxlev = max([v[3] for v in table.values()], default=0)
+        for lev in range(1, maxlev + 1):
+            start = len(entries)
+            for k in sorted(kk for kk, v in table.items() if v[3] == lev):
+                s1, s2, i, _ = table[k]
+                entries.append((resolve(s1), resolve(s2), i, stage))
+                yidx[k] = nxt
+                nxt += 1
+            levels.append((start, len(entries)))
+    cart_y = []
+    for a in cart(la):
+        for b in cart(lb):
+            for c in cart(lc):
+                for d in cart(ld):
+                    cart_y.append(resolve(("k", a, b, c, d)))
+    return entries, levels, cart_y, nxt
+
+
+def gen_wave_class(c, spherical=True):
+    la, lb, lc, ld = c
+    L = sum(c)
+    name = cls_name(c)
+    keys, info, level, targets = vrr_structure(la, lb, lc, ld)
+    xi = {k: n for n, k in enumerate(keys)}
+    nbase = sum(1 for k in keys if level[k] == 0)
+    zero = len(keys)
+    nlev = max(level.values())
+    vrr_rows, lev_off = [], [0]
+    for lev in range(1, nlev + 1):
+        for k in keys:
+            if level[k] != lev:
+                continue
+            kind, d, n1, n2, src = info[k]
+            s = [xi[t] if t is not None else zero for t in src]
+            assert xi[k] == nbase + len(vrr_rows)
+            vrr_rows.append((s[0] | (s[1] << 16), s[2] | (s[3] << 16), s[4] | (kind << 16) | (d << 18) | (n1 << 20) | (n2 << 24)))
+        lev_off.append(len(vrr_rows))
+    base = [-1] * (L + 1)
+    for k in keys:
+        if level[k] == 0:
+            base[k[2]] = xi[k]
+    tlist = sorted({(e, f) for (e, f, _) in targets})
+    tidx = {(e, f): n for n, (e, f) in enumerate(tlist)}
+    tgt = [xi[(e, f, 0)] for (e, f) in tlist]
+    hent, hlev, cart_y, ny = hrr_structure(la, lb, lc, ld, len(tlist), tidx)
+    na = [nao(l, spherical) for l in c]
+    NAO = na[0] * na[1] * na[2] * na[3]
+    nx = zero + 1
+    lds = max(nx, ny + NAO)
+    o = []
+    o.append("// ---- class (%s) : wave-cooperative, %d VRR entries in %d levels, %d HRR entries, LDS %d doubles"
+             % (name, len(keys), nlev, len(hent), lds))
+    o.append("template <> struct ClassInfo<%d,%d,%d,%d> { static constexpr int L = %d, NAO = %d, NA = %d, NB = %d, NC = %d, ND = %d; };"
+             % (la, lb, lc, ld, L, NAO, *na))
+    def arr(t, nm, vals):
+        return "__device__ const %s %s[%d] = {%s};" % (t, nm, max(len(vals), 1), ",".join(vals) if vals else "0")
+    o.append(arr("int", "wv_%s_vrr" % name, ["%d" % v for r in vrr_rows for v in r]))
+    o.append(arr("int", "wv_%s_vlev" % name, ["%d" % v for v in lev_off]))
+    o.append(arr("short", "wv_%s_base" % name, ["%d" % v for v in base]))
+    o.append(arr("short", "wv_%s_tgt" % name, ["%d" % v for v in tgt]))
+    o.append(arr("int", "wv_%s_hrr" % name, ["%d" % (s1 | (s2 << 16)) for (s1, s2, i, st) in hent] ))
+    o.append(arr("char", "wv_%s_hdir" % name, ["%d" % (i | (st << 2)) for (s1, s2, i, st) in hent]))
+    o.append(arr("int", "wv_%s_hlev" % name, ["%d" % v for lv in hlev for v in lv]))
+    o.append(arr("short", "wv_%s_cart" % name, ["%d" % v for v in cart_y]))
+    o.append("template <> struct WaveTab<%d,%d,%d,%d> {" % c)
+    o.append("  static constexpr int L = %d, NBASE = %d, NVE = %d, ZERO = %d, NLEV = %d, NT = %d, NHE = %d, NHLEV = %d,"
+             " NY = %d, NCART = %d, NAO = %d, LDS = %d;" % (L, nbase, len(vrr_rows), zero, nlev, len(tlist), len(hent),
+                                                           len(hlev), ny, len(cart_y), NAO, lds))
+    for fld, nm in (("vrr", "int"), ("vlev", "int"), ("base", "short"), ("tgt", "short"), ("hrr", "int"),
+                    ("hdir", "char"), ("hlev", "int"), ("cart", "short")):
+        o.append("  static __device__ __forceinline__ const %s* %s() { return wv_%s_%s; }" % (nm, fld, name, fld))
+    o.append("};")
+    return "\n".join(o)
+
+
+def gen_ao_patterns(maxl, spherical=True):
+    cnt, idx = [], []
+    for l in range(maxl + 1):
+        for pat in ao_index_pattern(l, spherical):
+            cnt.append(len(pat))
+            idx.extend(pat + [0] * (3 - len(pat)))
+    return ["// Cartesian components used by AO m of an l-shell (AO order l=0..maxl, m=-l..l).",
+            "__device__ const char ao_pat_n[%d] = {%s};" % (len(cnt), ",".join(map(str, cnt))),
+            "__device__ const char ao_pat_c[%d] = {%s};" % (len(idx), ",".join(map(str, idx)))]
+
+
 def gen_dispatch(cls_list):
     o = ["#define DSSCF_FOR_EACH_CLASS(X) \\"]
     for c in cls_list:
-        o.append("  X(%d, %d, %d, %d, %s) \\" % (*c, cls_name(c)))
+        o.append("  X(%d, %d, %d, %d, %s, %d) \\" % (*c, cls_name(c), 1 if is_wave(c) else 0))
     o.append("")
     return "\n".join(o)
 
@@ -377,8 +585,11 @@ def main(out_path, maxl=MAXL):
     text.append("// Cartesian -> reference-AO coefficients, per l: (2l+1) x ncart(l), row-major.")
     text.append("__constant__ double c_ao_xform[DSSCF_XFORM_SIZE];")
     text.append("")
+    text.extend(gen_ao_patterns(maxl))
+    text.append("__device__ __forceinline__ constexpr int xform_off(int l) { return l == 0 ? 0 : (l == 1 ? 1 : (l == 2 ? 10 : 40)); }")
+    text.append("")
     for c in cls_list:
-        text.append(gen_class(c))
+        text.append(gen_wave_class(c) if is_wave(c) else gen_class(c))
         text.append("")
     open(out_path, "w").write("\n".join(text))
 

@@ -11,14 +11,17 @@ int dk_n_class();
 void dk_class_l(int cls, int* l4);
 void dsscf_upload_xform(const double* x, int n, hipStream_t s);
 
-hipError_t dsscf_launch_jk(int cls, int boys_mode, const uint2* q, int nq, const ShellPair* pairs,
-                           const PrimPair* pp, const double* btab, const double* Dt, const double* Da,
-                           double* Jh, double* Kh, int N, double scale, hipStream_t s);
+// Fused J/K launch of one class over nseg context segments (block_size is
+// ignored by the wave-cooperative classes, which use one 64-lane block per
+// quartet).
+hipError_t dsscf_launch_jk(int cls, int boys_mode, const JKSeg* segs, const int* blk0, int nseg, int nblocks,
+                           int block_size, const double* btab, double scale, hipStream_t s);
+int dsscf_class_is_wave(int cls);
 hipError_t dsscf_launch_eri_block(int cls, int boys_mode, const int4* items, int n, const ShellPair* pairs,
                                   const PrimPair* pp, const double* btab, double* out, double scale,
                                   hipStream_t s);
 hipError_t dsscf_launch_schwarz(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
                                 const PrimPair* pp, const double* btab, double* Q, double scale, hipStream_t s);
 hipError_t dsscf_launch_fill(double* p, size_t n, double v, hipStream_t s);
-hipError_t dsscf_launch_unpack(const double* pk, double* full, int N, hipStream_t s);
-hipError_t dsscf_launch_fold(const double* Jh, const double* Kh, double* J, double* K, int N, hipStream_t s);
+hipError_t dsscf_launch_prep(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);
+hipError_t dsscf_launch_fold(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);

@@ -1,0 +1,234 @@
+// wave_kernel.hpp -- wave-cooperative shell-quartet kernel for the large
+// classes (included by eri_kernels.hip after the generated tables).
+//
+// One 64-lane wavefront per shell quartet.  Per primitive quartet the lanes
+// walk the generated VRR table level by level, each lane evaluating every
+// 64th entry of a level into LDS (entries of one level only read lower
+// levels); the lanes then add the [e0|f0] targets into per-lane register
+// accumulators.  After the primitive loops the accumulators go back to LDS,
+// the bra and ket horizontal recursions run level by level, the Cartesian
+// block is transformed to the reference's AO functions, and the result is
+// digested into J/K (each output element owned by one lane: one atomic per
+// element and quartet), written out, or reduced to a Schwarz bound.
+#pragma once
+
+enum { WAVE_JK = 0, WAVE_BLOCK = 1, WAVE_SCHWARZ = 2 };
+
+__device__ __forceinline__ double pick3(int d, double x, double y, double z) { return d == 0 ? x : (d == 1 ? y : z); }
+
+// MODE WAVE_JK: one quartet per block, the block's context found in segs
+// (items, n, pairs, pp unused).  WAVE_BLOCK / WAVE_SCHWARZ: items over one
+// context, grid-stride.
+template <int la, int lb, int lc, int ld, int BM, int MODE>
+__global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int n,
+                                             const ShellPair* __restrict__ pairs_in,
+                                             const PrimPair* __restrict__ pp_in, const double* __restrict__ btab,
+                                             const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
+                                             int nseg, double* __restrict__ out, double scale)
+{
+    typedef WaveTab<la, lb, lc, ld> W;
+    typedef ClassInfo<la, lb, lc, ld> CI;
+    constexpr int L = W::L;
+    constexpr int SLOTS = (W::NT + 63) / 64;
+    constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
+    constexpr int ncA = (la + 1) * (la + 2) / 2, ncB = (lb + 1) * (lb + 2) / 2;
+    constexpr int ncC = (lc + 1) * (lc + 2) / 2, ncD = (ld + 1) * (ld + 2) / 2;
+    __shared__ double sm[W::LDS];
+    const int lane = threadIdx.x;
+    const int* __restrict__ vrr = W::vrr();
+    const int* __restrict__ vlev = W::vlev();
+    const short* __restrict__ tgt = W::tgt();
+    const int* __restrict__ hrr = W::hrr();
+    const char* __restrict__ hdir = W::hdir();
+    const int* __restrict__ hlev = W::hlev();
+    const short* __restrict__ carty = W::cart();
+
+    const JKSeg* S = nullptr;
+    const ShellPair* __restrict__ pairs = pairs_in;
+    const PrimPair* __restrict__ pp = pp_in;
+    int t0 = blockIdx.x, tstep = gridDim.x;
+    if (MODE == WAVE_JK) {
+        S = &segs[find_segment_by_block(seg_blk0, nseg, (int)blockIdx.x)];
+        pairs = S->pairs;
+        pp = S->pp;
+        t0 = (int)blockIdx.x - S->blk0;
+        n = t0 + 1;  // exactly one quartet per block
+        tstep = 1;
+    }
+    for (int t = t0; t < n; t += tstep) {
+        int bi, ki;
+        if (MODE == WAVE_JK) {
+            const uint2 q = S->q[t];
+            bi = (int)q.x;
+            ki = (int)q.y;
+        } else if (MODE == WAVE_BLOCK) {
+            const int4 it = ((const int4*)items)[t];
+            bi = it.x;
+            ki = it.y;
+        } else {
+            bi = ki = ((const int*)items)[t];
+        }
+        const ShellPair bra = pairs[bi];
+        const ShellPair ket = pairs[ki];
+        double acc[SLOTS];
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) acc[s] = 0.0;
+
+        for (int ib = 0; ib < bra.nprim; ++ib) {
+            const PrimPair P = pp[bra.prim_off + ib];
+            for (int ik = 0; ik < ket.nprim; ++ik) {
+                const PrimPair Q = pp[ket.prim_off + ik];
+                PrimQuartet<double> g;
+                double Fm[L + 1];
+                prim_setup<BM, double, L>(P, Q, bra, ket, btab, g, Fm);
+                if (lane == 0) {
+                    sm[W::ZERO] = 0.0;
+#pragma unroll
+                    for (int m = 0; m <= L; ++m) {
+                        const int b = W::base()[m];
+                        if (b >= 0) sm[b] = Fm[m];
+                    }
+                }
+                __syncthreads();
+                for (int lev = 0; lev < W::NLEV; ++lev) {
+                    const int e1 = vlev[lev + 1];
+                    for (int k = vlev[lev] + lane; k < e1; k += 64) {
+                        const int r0 = vrr[3 * k], r1 = vrr[3 * k + 1], r2 = vrr[3 * k + 2];
+                        const int s0 = r0 & 0xffff, s1 = (r0 >> 16) & 0xffff;
+                        const int s2 = r1 & 0xffff, s3 = (r1 >> 16) & 0xffff, s4 = r2 & 0xffff;
+                        const int kind = (r2 >> 16) & 3, dir = (r2 >> 18) & 3;
+                        const double n1 = (double)((r2 >> 20) & 15), n2 = (double)((r2 >> 24) & 15);
+                        double A, Wc, o2, rr;
+                        if (kind == 1) {
+                            A = pick3(dir, g.PA[0], g.PA[1], g.PA[2]);
+                            Wc = pick3(dir, g.WP[0], g.WP[1], g.WP[2]);
+                            o2 = g.o2p;
+                            rr = g.rp;
+                        } else {
+                            A = pick3(dir, g.QC[0], g.QC[1], g.QC[2]);
+                            Wc = pick3(dir, g.WQ[0], g.WQ[1], g.WQ[2]);
+                            o2 = g.o2q;
+                            rr = g.rq;
+                        }
+                        double v = A * sm[s0] + Wc * sm[s1];
+                        v += n1 * o2 * (sm[s2] - rr * sm[s3]);
+                        v += n2 * g.o2pq * sm[s4];
+                        sm[W::NBASE + k] = v;
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int s = 0; s < SLOTS; ++s) {
+                    const int tt = lane + 64 * s;
+                    if (tt < W::NT) acc[s] += sm[tgt[tt]];
+                }
+                __syncthreads();
+            }
+        }
+        // horizontal recursions over Y = [targets | bra HRR | ket HRR]
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) {
+            const int tt = lane + 64 * s;
+            if (tt < W::NT) sm[tt] = acc[s];
+        }
+        __syncthreads();
+        for (int hl = 0; hl < W::NHLEV; ++hl) {
+            const int e1 = hlev[2 * hl + 1];
+            for (int k = hlev[2 * hl] + lane; k < e1; k += 64) {
+                const int h = hrr[k];
+                const int d = hdir[k];
+                const double c = (d >> 2) ? pick3(d & 3, ket.AB[0], ket.AB[1], ket.AB[2])
+                                          : pick3(d & 3, bra.AB[0], bra.AB[1], bra.AB[2]);
+                sm[W::NT + k] = sm[h & 0xffff] + c * sm[(h >> 16) & 0xffff];
+            }
+            __syncthreads();
+        }
+        // Cartesian -> AO
+        for (int o = lane; o < W::NAO; o += 64) {
+            const int md = o % ND, mc = (o / ND) % NC, mb = (o / (ND * NC)) % NB, ma = o / (ND * NC * NB);
+            const int pa = la * la + ma, pb = lb * lb + mb, pc = lc * lc + mc, pd = ld * ld + md;
+            double sum = 0.0;
+            for (int ta = 0; ta < ao_pat_n[pa]; ++ta) {
+                const int ca = ao_pat_c[3 * pa + ta];
+                const double wa = c_ao_xform[xform_off(la) + ma * ncA + ca];
+                for (int tb = 0; tb < ao_pat_n[pb]; ++tb) {
+                    const int cb = ao_pat_c[3 * pb + tb];
+                    const double wb = wa * c_ao_xform[xform_off(lb) + mb * ncB + cb];
+                    for (int tc = 0; tc < ao_pat_n[pc]; ++tc) {
+                        const int cc = ao_pat_c[3 * pc + tc];
+                        const double wc = wb * c_ao_xform[xform_off(lc) + mc * ncC + cc];
+                        for (int td = 0; td < ao_pat_n[pd]; ++td) {
+                            const int cd = ao_pat_c[3 * pd + td];
+                            const double wd = wc * c_ao_xform[xform_off(ld) + md * ncD + cd];
+                            sum += wd * sm[carty[((ca * ncB + cb) * ncC + cc) * ncD + cd]];
+                        }
+                    }
+                }
+            }
+            sm[W::NY + o] = scale * sum;
+        }
+        __syncthreads();
+        const double* V = sm + W::NY;
+#define VX(i, j, k, l) V[(((i) * NB + (j)) * NC + (k)) * ND + (l)]
+        if (MODE == WAVE_BLOCK) {
+            const int4 it = ((const int4*)items)[t];
+            for (int o = lane; o < W::NAO; o += 64) out[(size_t)it.z + o] = V[o];
+        } else if (MODE == WAVE_SCHWARZ) {
+            if (lane == 0) {
+                double mx = 0.0;
+                for (int i = 0; i < NA; ++i)
+                    for (int j = 0; j < NB; ++j) mx = fmax(mx, fabs(VX(i, j, i, j)));
+                out[bi] = sqrt(mx);
+            }
+        } else {
+            const double* __restrict__ Dt = S->Dt;
+            const double* __restrict__ Da = S->Da;
+            double* __restrict__ Jh = S->Jh;
+            double* __restrict__ Kh = S->Kh;
+            const int N = S->N;
+            double g = 1.0;
+            if (bra.ao_a == bra.ao_b) g *= 0.5;
+            if (ket.ao_a == ket.ao_b) g *= 0.5;
+            if (bi == ki) g *= 0.5;
+            const int a0 = bra.ao_a, b0 = bra.ao_b, c0 = ket.ao_a, d0 = ket.ao_b;
+            constexpr int T0 = NA * NB, T1 = T0 + NC * ND, T2 = T1 + NA * NC, T3 = T2 + NB * NC, T4 = T3 + NA * ND,
+                          T5 = T4 + NB * ND;
+            for (int o = lane; o < T5; o += 64) {
+                double s = 0.0;
+                if (o < T0) {  // Jh[a][b] += 2g sum_cd v Dt_cd
+                    const int i = o / NB, j = o % NB;
+                    for (int k = 0; k < NC; ++k)
+                        for (int l = 0; l < ND; ++l) s += VX(i, j, k, l) * Dt[(size_t)(c0 + k) * N + d0 + l];
+                    unsafeAtomicAdd(&Jh[(size_t)(a0 + i) * N + b0 + j], 2.0 * g * s);
+                } else if (o < T1) {  // Jh[c][d] += 2g sum_ab v Dt_ab
+                    const int k = (o - T0) / ND, l = (o - T0) % ND;
+                    for (int i = 0; i < NA; ++i)
+                        for (int j = 0; j < NB; ++j) s += VX(i, j, k, l) * Dt[(size_t)(a0 + i) * N + b0 + j];
+                    unsafeAtomicAdd(&Jh[(size_t)(c0 + k) * N + d0 + l], 2.0 * g * s);
+                } else if (o < T2) {  // Kh[a][c] += g sum_bd v Da_bd
+                    const int i = (o - T1) / NC, k = (o - T1) % NC;
+                    for (int j = 0; j < NB; ++j)
+                        for (int l = 0; l < ND; ++l) s += VX(i, j, k, l) * Da[(size_t)(b0 + j) * N + d0 + l];
+                    unsafeAtomicAdd(&Kh[(size_t)(a0 + i) * N + c0 + k], g * s);
+                } else if (o < T3) {  // Kh[b][c] += g sum_ad v Da_ad
+                    const int j = (o - T2) / NC, k = (o - T2) % NC;
+                    for (int i = 0; i < NA; ++i)
+                        for (int l = 0; l < ND; ++l) s += VX(i, j, k, l) * Da[(size_t)(a0 + i) * N + d0 + l];
+                    unsafeAtomicAdd(&Kh[(size_t)(b0 + j) * N + c0 + k], g * s);
+                } else if (o < T4) {  // Kh[a][d] += g sum_bc v Da_bc
+                    const int i = (o - T3) / ND, l = (o - T3) % ND;
+                    for (int j = 0; j < NB; ++j)
+                        for (int k = 0; k < NC; ++k) s += VX(i, j, k, l) * Da[(size_t)(b0 + j) * N + c0 + k];
+                    unsafeAtomicAdd(&Kh[(size_t)(a0 + i) * N + d0 + l], g * s);
+                } else {  // Kh[b][d] += g sum_ac v Da_ac
+                    const int j = (o - T4) / ND, l = (o - T4) % ND;
+                    for (int i = 0; i < NA; ++i)
+                        for (int k = 0; k < NC; ++k) s += VX(i, j, k, l) * Da[(size_t)(a0 + i) * N + c0 + k];
+                    unsafeAtomicAdd(&Kh[(size_t)(b0 + j) * N + d0 + l], g * s);
+                }
+            }
+        }
+#undef VX
+        __syncthreads();
+    }
+}

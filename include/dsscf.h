@@ -125,6 +125,28 @@ int dsscf_get_stats(dsscf_ctx* ctx, dsscf_stats* st);
 int dsscf_time_jk(dsscf_ctx* ctx, int warmup, int iters, double* ms_per_build, double* ms_dominant_kernel,
                   int* dominant_class);
 
+/* ---------------------------------------------------------------- batches --
+ * Fock builds of several prepared contexts (different molecules and bases,
+ * one device, one Boys mode) fused into one launch per shell-quartet class:
+ * a prepare kernel, one kernel per class over all contexts' quartet lists,
+ * a fold kernel (captured once into a hipGraph and replayed when graphs are
+ * on).  No reference counterpart: the reference runs one molecule at a time. */
+typedef struct dsscf_batch dsscf_batch;
+int dsscf_batch_create(dsscf_batch** batch, int device);
+int dsscf_batch_destroy(dsscf_batch* batch);
+/* Set the member contexts (replaces any previous set). */
+int dsscf_batch_set(dsscf_batch* batch, dsscf_ctx* const* ctxs, int n);
+int dsscf_batch_set_graph(dsscf_batch* batch, int use_graph);
+/* J[Ptot_i], K[Palpha_i] for every member i (packed-U host arrays). */
+int dsscf_batch_jk(dsscf_batch* batch, const double* const* Ptot, const double* const* Palpha, double* const* J,
+                   double* const* K);
+/* Average device time of `iters` fused builds on the resident densities. */
+int dsscf_batch_time(dsscf_batch* batch, int warmup, int iters, double* ms_per_build);
+/* Choose threads per block for each thread-per-quartet class by timing
+ * 64/128/256 on the batch's own quartet lists; chosen[dsscf_class_count()]
+ * receives the choice (may be null). */
+int dsscf_batch_autotune(dsscf_batch* batch, int* chosen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,3 +72,51 @@ def test_scf_water_sto3g(boys):
     e, it = job.solve()
     ref = O.scf_energy(ds.basis_path("STO-3G"), Z, xyz, mode=boys, diis_size=8, conv=1e-10)
     assert abs(e - ref.energy) < 1e-8
+
+
+@pytest.mark.parametrize("boys", [ds.BOYS_REFERENCE, ds.BOYS_ACCURATE])
+def test_eri_all_unique_water_def2svp(boys):
+    """Every unique (ij|kl) of water/def2-SVP (spherical d): all 21 classes, register and LDS kernels."""
+    Z, xyz = G.water()
+    job = _job("def2-SVP", Z, xyz, boys)
+    n = job.n_ao
+    q = np.array([[i, j, k, l] for i in range(n) for j in range(i + 1) for k in range(n) for l in range(k + 1)
+                  if i * (i + 1) // 2 + j >= k * (k + 1) // 2 + l], np.int32)
+    got = job.context.eri(q)
+    want = _oracle_basis("def2-SVP", Z, xyz).eri_list(q, boys)
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+@pytest.mark.parametrize("basis", ["6-31GP", "def2-SVP"])
+def test_eri_sample_d_two_waters(basis):
+    Z, xyz = G.water_box(2, seed=5)
+    job = _job(basis, Z, xyz, ds.BOYS_REFERENCE)
+    rng = np.random.default_rng(7)
+    q = rng.integers(0, job.n_ao, size=(5000, 4)).astype(np.int32)
+    want = _oracle_basis(basis, Z, xyz).eri_list(q, 0)
+    assert np.max(np.abs(job.context.eri(q) - want)) < 1e-12
+
+
+@pytest.mark.parametrize("basis", ["def2-SVP", "6-31GP"])
+def test_jk_vs_tensor_d(basis):
+    Z, xyz = G.water()
+    job = _job(basis, Z, xyz, ds.BOYS_REFERENCE)
+    n = job.n_ao
+    C = np.random.default_rng(3).standard_normal((n, 5))
+    Pa = O.pack(C @ C.T * 0.1)
+    job.set_density(Pa)
+    J, K = job.fock_build()
+    t = _oracle_basis(basis, Z, xyz).tensor(0)
+    Jr, Kr = O.coulomb(t, n, 2 * Pa), O.exchange(t, n, Pa)
+    scale = max(1.0, np.max(np.abs(Jr)))
+    assert np.max(np.abs(J - Jr)) < 1e-11 * scale
+    assert np.max(np.abs(K - Kr)) < 1e-11 * scale
+
+
+def test_scf_water_def2svp():
+    Z, xyz = G.water()
+    job = ds.Job(0).set_options(energy_threshold=1e-10, boys_mode=ds.BOYS_REFERENCE, schwarz_threshold=0.0)
+    job.set_basis("def2-SVP").set_molecule(Z, xyz).construct()
+    e, _ = job.solve()
+    ref = O.scf_energy(ds.basis_path("def2-SVP"), Z, xyz, mode=0, conv=1e-10)
+    assert abs(e - ref.energy) < 1e-8

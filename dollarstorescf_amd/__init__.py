@@ -75,6 +75,12 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = i
+        for name, args in {"dsscf_batch_create": [p, i], "dsscf_batch_destroy": [p], "dsscf_batch_set": [p, p, i],
+                           "dsscf_batch_set_graph": [p, i], "dsscf_batch_jk": [p, p, p, p, p],
+                           "dsscf_batch_time": [p, i, i, p], "dsscf_batch_autotune": [p, p]}.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = i
         for name in ("dsscf_max_l", "dsscf_class_count", "dsscf_xform_size"):
             getattr(L, name).restype = i
             getattr(L, name).argtypes = []
@@ -182,6 +188,48 @@ class Context:
         cls = ctypes.c_int()
         _check(lib().dsscf_time_jk(self._h, warmup, iters, ctypes.byref(ms), ctypes.byref(dom), ctypes.byref(cls)))
         return ms.value, dom.value, cls.value
+
+
+class Batch:
+    """Fused Fock builds over several prepared contexts (dsscf_batch_*)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(lib().dsscf_batch_create(ctypes.byref(self._h), device))
+        self._members = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dsscf_batch_destroy(self._h)
+        self._h = None
+
+    __del__ = close
+
+    def set(self, contexts, graph=True):
+        self._members = list(contexts)
+        arr = (ctypes.c_void_p * len(contexts))(*[c._h.value for c in contexts])
+        _check(lib().dsscf_batch_set(self._h, arr, len(contexts)))
+        _check(lib().dsscf_batch_set_graph(self._h, 1 if graph else 0))
+
+    def jk(self, Ptots, Pas):
+        n = len(self._members)
+        Pt = [np.ascontiguousarray(x, np.float64) for x in Ptots]
+        Pa = [np.ascontiguousarray(x, np.float64) for x in Pas]
+        J = [np.empty_like(x) for x in Pt]
+        K = [np.empty_like(x) for x in Pt]
+        mk = lambda xs: (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
+        _check(lib().dsscf_batch_jk(self._h, mk(Pt), mk(Pa), mk(J), mk(K)))
+        return J, K
+
+    def time(self, warmup=2, iters=10):
+        ms = ctypes.c_double()
+        _check(lib().dsscf_batch_time(self._h, warmup, iters, ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self):
+        out = np.zeros(lib().dsscf_class_count(), np.int32)
+        _check(lib().dsscf_batch_autotune(self._h, _p(out)))
+        return out
 
 
 class Job:

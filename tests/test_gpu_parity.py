@@ -120,3 +120,34 @@ def test_scf_water_def2svp():
     e, _ = job.solve()
     ref = O.scf_energy(ds.basis_path("def2-SVP"), Z, xyz, mode=0, conv=1e-10)
     assert abs(e - ref.energy) < 1e-8
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_single_context(graph):
+    """Fused batch of mixed molecules/bases == each context's own Fock build;
+    several replays (graph mode replays the captured launch sequence)."""
+    specs = [("STO-3G", G.water()), ("6-31G", G.water_box(2, seed=3)), ("def2-SVP", G.water()),
+             ("6-31GP", G.water_box(3, seed=4))]
+    jobs, dens, singles = [], [], []
+    rng = np.random.default_rng(5)
+    for basis, (Z, xyz) in specs:
+        j = _job(basis, Z, xyz, ds.BOYS_ACCURATE, schwarz=1e-12)
+        n = j.n_ao
+        C = rng.standard_normal((n, 4))
+        Pa = O.pack(C @ C.T * 0.1)
+        j.set_density(Pa)
+        singles.append(j.fock_build())
+        jobs.append(j)
+        dens.append(Pa)
+    b = ds.Batch(0)
+    b.set([j.context for j in jobs], graph=graph)
+    for rep in range(3):
+        J, K = b.jk([2 * p for p in dens], dens)
+        for (Js, Ks), Jb, Kb in zip(singles, J, K):
+            assert np.max(np.abs(Jb - Js)) < 1e-11
+            assert np.max(np.abs(Kb - Ks)) < 1e-11
+    chosen = b.autotune()
+    assert set(int(c) for c in chosen) <= {64, 128, 256}
+    J, K = b.jk([2 * p for p in dens], dens)
+    for (Js, Ks), Jb, Kb in zip(singles, J, K):
+        assert np.max(np.abs(Jb - Js)) < 1e-11

@@ -30,14 +30,17 @@ class DsscfError(RuntimeError):
 
 
 class Options(ctypes.Structure):
-    _fields_ = [("boys_mode", ctypes.c_int), ("schwarz_threshold", ctypes.c_double)]
+    _fields_ = [("boys_mode", ctypes.c_int), ("schwarz_threshold", ctypes.c_double),
+                ("fp32_threshold", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("n_pairs", ctypes.c_longlong), ("n_prim_pairs", ctypes.c_longlong),
                 ("n_quartets", ctypes.c_longlong), ("n_prim_quartets", ctypes.c_longlong),
                 ("n_quartets_unscreened", ctypes.c_longlong), ("n_class", ctypes.c_int),
-                ("class_quartets", ctypes.c_longlong * 21), ("class_prim_quartets", ctypes.c_longlong * 21)]
+                ("class_quartets", ctypes.c_longlong * 21), ("class_prim_quartets", ctypes.c_longlong * 21),
+                ("n_quartets_fp32", ctypes.c_longlong), ("class_quartets_fp32", ctypes.c_longlong * 21),
+                ("class_prim_quartets_fp32", ctypes.c_longlong * 21)]
 
 
 _LIB = None
@@ -64,7 +67,7 @@ def lib():
             "dsscf_get_stats": [p, p], "dsscf_set_density": [p, p, p], "dsscf_jk_resident": [p],
             "dsscf_get_jk": [p, p, p], "dsscf_time_jk": [p, i, i, p, p, p], "dsscf_class_l": [i, p],
             "dsscf_job_create": [p, i], "dsscf_job_destroy": [p],
-            "dsscf_job_set_options": [p, i, d, i, i, d, i],
+            "dsscf_job_set_options": [p, i, d, i, i, d, i], "dsscf_job_set_precision": [p, d],
             "dsscf_job_set_molecule": [p, i, p, p, i, i], "dsscf_job_load_xyz": [p, c, i, i],
             "dsscf_job_set_basis_file": [p, c], "dsscf_job_construct": [p], "dsscf_job_construct_host": [p],
             "dsscf_job_ao": [p, i, p, p, p, p, p, p, p, p, p, i], "dsscf_job_n_ao": [p],
@@ -134,8 +137,8 @@ class Context:
 
     __del__ = close
 
-    def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12):
-        o = Options(boys_mode, schwarz_threshold)
+    def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12, fp32_threshold=0.0):
+        o = Options(boys_mode, schwarz_threshold, fp32_threshold)
         _check(lib().dsscf_set_options(self._h, ctypes.byref(o)))
 
     def set_basis(self, l, nprim, centers, exps, coefs, xform):
@@ -250,9 +253,10 @@ class Job:
     __del__ = close
 
     def set_options(self, diis_size=8, energy_threshold=1e-8, max_iterations=500, boys_mode=BOYS_ACCURATE,
-                    schwarz_threshold=1e-12, verbose=0):
+                    schwarz_threshold=1e-12, verbose=0, fp32_threshold=0.0):
         _check(lib().dsscf_job_set_options(self._h, diis_size, energy_threshold, max_iterations, boys_mode,
                                            schwarz_threshold, verbose), self._err)
+        _check(lib().dsscf_job_set_precision(self._h, fp32_threshold), self._err)
         return self
 
     def set_molecule(self, Z, xyz_bohr, charge=0, multiplicity=1):

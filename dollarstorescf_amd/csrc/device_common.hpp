@@ -45,14 +45,16 @@ static_assert(sizeof(PrimPair) == 64, "PrimPair must be one 64 B record");
 // list and the buffers the quartets read and accumulate into.  blk0 is the
 // first block of the launch that belongs to this segment.
 struct JKSeg {
-    const uint2* q;
+    const uint2* q;            // flat (bra, ket) list: wave-cooperative classes
+    const int4* u;             // work units (bra pair, first ket, ket count): thread-per-quartet classes
+    const unsigned* k;         // ket pair indices referenced by the units
     const ShellPair* pairs;
     const PrimPair* pp;
     const double* Dt;
     const double* Da;
     double* Jh;
     double* Kh;
-    int n;
+    int n;                     // quartets (flat list) or units
     int N;
     int blk0;
     int pad;
@@ -122,6 +124,16 @@ __device__ __forceinline__ double boys_reference_m(double x, unsigned m)
     return exp(-x) / (double)(2 * m + 1) * F;
 }
 
+// 1/sqrt(x) for x > 0: hardware estimate (v_rsq_f64) refined by two Newton
+// steps r <- r (3 - x r^2)/2, each of which doubles the correct bits.
+__device__ __forceinline__ double rsqrt_nr(double x)
+{
+    double r = __builtin_amdgcn_rsq(x);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    return r;
+}
+
 // Accurate F_0..F_L: 7-term Taylor expansion of F_L about the nearest grid
 // point (|dx| <= 0.025, truncation < 2e-15 relative), e^-T from the tabulated
 // e^-x_i times an 8-term series, then the stable downward recursion
@@ -158,8 +170,9 @@ __device__ __forceinline__ void boys_accurate(double T, const double* __restrict
             for (int m = L - 1; m >= 0; --m) F[m] = fma(T2, F[m + 1], eT) * (1.0 / (2 * m + 1));
         }
     } else {
-        const double inv_T = 1.0 / T;
-        F[0] = 0.88622692545275801365 * sqrt(inv_T);
+        const double r = rsqrt_nr(T);
+        const double inv_T = r * r;
+        F[0] = 0.88622692545275801365 * r;
 #pragma unroll
         for (int m = 1; m <= L; ++m) F[m] = F[m - 1] * ((m - 0.5) * inv_T);
     }
@@ -185,17 +198,88 @@ struct PrimQuartet {
 // Per-primitive-quartet quantities (the per-primitive block of
 // GaussianIntegral.cpp:577-595 in HGP form): W-P, W-Q, 1/2p, 1/2q,
 // 1/2(p+q), rho/p, rho/q and prefactor-scaled Boys values.
+// FP32 form for low-bound quartets: setup, small-T Boys (same table, Taylor
+// and downward recursion in float) and the values handed to the VRR in FP32;
+// the large-T Boys branch stays FP64.
+template <int L>
+__device__ __forceinline__ void prim_setup_f32(const PrimPair& P, const PrimPair& Q, const ShellPair& bra,
+                                               const ShellPair& ket, const double* __restrict__ btab,
+                                               PrimQuartet<float>& g, float* Fm)
+{
+    const float p = (float)P.p, q = (float)Q.p;
+    const float inv_pq = __builtin_amdgcn_rcpf(p + q);
+    const float PQx = (float)(P.P[0] - Q.P[0]), PQy = (float)(P.P[1] - Q.P[1]), PQz = (float)(P.P[2] - Q.P[2]);
+    const float rho = p * q * inv_pq;
+    const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+    const float pref = (float)P.Kp * (float)Q.Kp * __builtin_sqrtf(inv_pq);
+    if (T < (float)DSSCF_BOYS_TMAX) {
+        const int i = (int)(T * (float)DSSCF_BOYS_INV_H + 0.5f);
+        const float nd = (float)i * (float)DSSCF_BOYS_H - T;
+        const double* row = btab + i * DSSCF_BOYS_STRIDE;
+        float s = (float)row[1 + L + 4];
+        s = fmaf(s, nd * 0.25f, (float)row[1 + L + 3]);
+        s = fmaf(s, nd * (1.0f / 3.0f), (float)row[1 + L + 2]);
+        s = fmaf(s, nd * 0.5f, (float)row[1 + L + 1]);
+        s = fmaf(s, nd, (float)row[1 + L]);
+        Fm[L] = s;
+        if (L > 0) {
+            float e = 1.0f / 24.0f;
+            e = fmaf(e, nd, 1.0f / 6.0f);
+            e = fmaf(e, nd, 0.5f);
+            e = fmaf(e, nd, 1.0f);
+            e = fmaf(e, nd, 1.0f);
+            const float eT = (float)row[0] * e;
+            const float T2 = 2.0f * T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) Fm[m] = fmaf(T2, Fm[m + 1], eT) * (1.0f / (2 * m + 1));
+        }
+    } else {
+        double F[L + 1];
+        const double inv_T = 1.0 / (double)T;
+        F[0] = 0.88622692545275801365 * sqrt(inv_T);
+#pragma unroll
+        for (int m = 1; m <= L; ++m) F[m] = F[m - 1] * ((m - 0.5) * inv_T);
+#pragma unroll
+        for (int m = 0; m <= L; ++m) Fm[m] = (float)F[m];
+    }
+#pragma unroll
+    for (int m = 0; m <= L; ++m) Fm[m] *= pref;
+    const float qf = q * inv_pq, pf = p * inv_pq;
+    g.PA[0] = (float)(P.P[0] - bra.A[0]);
+    g.PA[1] = (float)(P.P[1] - bra.A[1]);
+    g.PA[2] = (float)(P.P[2] - bra.A[2]);
+    g.QC[0] = (float)(Q.P[0] - ket.A[0]);
+    g.QC[1] = (float)(Q.P[1] - ket.A[1]);
+    g.QC[2] = (float)(Q.P[2] - ket.A[2]);
+    g.WP[0] = -qf * PQx;
+    g.WP[1] = -qf * PQy;
+    g.WP[2] = -qf * PQz;
+    g.WQ[0] = pf * PQx;
+    g.WQ[1] = pf * PQy;
+    g.WQ[2] = pf * PQz;
+    g.o2p = 0.5f * (float)P.inv_p;
+    g.o2q = 0.5f * (float)Q.inv_p;
+    g.o2pq = 0.5f * inv_pq;
+    g.rp = qf;
+    g.rq = pf;
+}
+
 template <int BM, typename R, int L>
 __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q, const ShellPair& bra,
                                            const ShellPair& ket, const double* __restrict__ btab,
                                            PrimQuartet<R>& g, R* Fm)
 {
+    if constexpr (sizeof(R) == 4) {
+        prim_setup_f32<L>(P, Q, bra, ket, btab, g, Fm);
+        return;
+    } else {
     const double p = P.p, q = Q.p;
-    const double inv_pq = 1.0 / (p + q);
+    const double rs = rsqrt_nr(p + q);
+    const double inv_pq = rs * rs;
     const double PQx = P.P[0] - Q.P[0], PQy = P.P[1] - Q.P[1], PQz = P.P[2] - Q.P[2];
     const double rho = p * q * inv_pq;
     const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-    const double pref = P.Kp * Q.Kp * sqrt(inv_pq);
+    const double pref = P.Kp * Q.Kp * rs;
     double F[L + 1];
     boys<BM, L>(T, btab, F);
 #pragma unroll
@@ -218,6 +302,7 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
     g.o2pq = (R)(0.5 * inv_pq);
     g.rp = (R)qf;
     g.rq = (R)pf;
+    }
 }
 
 template <int la, int lb, int lc, int ld>

@@ -65,7 +65,7 @@ struct Plan {
     int boys_mode = DSSCF_BOYS_ACCURATE;
     const double* btab = nullptr;
     int ncls = 0;
-    std::vector<int> cls_nseg, cls_nblocks, cls_off, cls_bs;
+    std::vector<int> cls_nseg, cls_nblocks, cls_off, cls_bs, cls_maxN;
     int nmat = 0, total_rows = 0, maxN = 0;
     std::vector<JKSeg> h_segs;
     std::vector<int> h_blk0;
@@ -94,7 +94,7 @@ struct Plan {
 struct dsscf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    dsscf_options opt{DSSCF_BOYS_ACCURATE, 1e-12};
+    dsscf_options opt{DSSCF_BOYS_ACCURATE, 1e-12, 0.0};
 
     // basis
     int n_shell = 0, n_ao = 0;
@@ -109,7 +109,11 @@ struct dsscf_ctx {
     std::vector<double> Q;
 
     // quartet lists
-    std::vector<long long> q_off, q_cnt, q_prim;
+    // per class: FP64 list [q_off, q_off+q_cnt), FP32 list [q_off32, q_off32+q_cnt32)
+    std::vector<long long> q_off, q_cnt, q_prim, q_off32, q_cnt32, q_prim32;
+    // thread-per-quartet classes: work units (bra pair, first ket, count) into `kets`,
+    // per class [u_off, u_off+u_cnt) FP64 and [u_off32, u_off32+u_cnt32) FP32
+    std::vector<long long> u_off, u_cnt, u_off32, u_cnt32;
     long long n_unscreened = 0;
     bool prepared = false;
 
@@ -118,6 +122,8 @@ struct dsscf_ctx {
     PrimPair* d_pp = nullptr;
     double* d_btab = nullptr;
     uint2* d_q = nullptr;
+    int4* d_units = nullptr;
+    unsigned* d_kets = nullptr;
     double* d_Q = nullptr;
     int* d_plist = nullptr;
     double *d_Dt = nullptr, *d_Da = nullptr, *d_Jh = nullptr, *d_Kh = nullptr;
@@ -132,6 +138,8 @@ struct dsscf_ctx {
         dev_free(d_pairs);
         dev_free(d_pp);
         dev_free(d_q);
+        dev_free(d_units);
+        dev_free(d_kets);
         dev_free(d_Q);
         dev_free(d_plist);
         dev_free(d_Dt);
@@ -210,8 +218,12 @@ int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
     if (!c || !o) return fail(DSSCF_ERR_ARG, "dsscf_set_options: null argument");
     if (o->boys_mode != DSSCF_BOYS_REFERENCE && o->boys_mode != DSSCF_BOYS_ACCURATE)
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: unknown boys_mode");
-    if (!(o->schwarz_threshold >= 0.0)) return fail(DSSCF_ERR_ARG, "dsscf_set_options: negative threshold");
-    const bool changed = o->boys_mode != c->opt.boys_mode || o->schwarz_threshold != c->opt.schwarz_threshold;
+    if (!(o->schwarz_threshold >= 0.0) || !(o->fp32_threshold >= 0.0))
+        return fail(DSSCF_ERR_ARG, "dsscf_set_options: negative threshold");
+    if (o->fp32_threshold > 0.0 && o->boys_mode != DSSCF_BOYS_ACCURATE)
+        return fail(DSSCF_ERR_ARG, "dsscf_set_options: FP32 quartets need the accurate Boys mode");
+    const bool changed = o->boys_mode != c->opt.boys_mode || o->schwarz_threshold != c->opt.schwarz_threshold ||
+                         o->fp32_threshold != c->opt.fp32_threshold;
     c->opt = *o;
     if (changed) c->prepared = false;
     return DSSCF_OK;
@@ -376,37 +388,80 @@ int dsscf_prepare(dsscf_ctx* c)
 
     const int ncls = dk_n_class();
     const double thr = c->opt.schwarz_threshold;
-    std::vector<uint2> all;
-    c->q_off.assign(ncls, 0);
-    c->q_cnt.assign(ncls, 0);
-    c->q_prim.assign(ncls, 0);
+    const double thr32 = c->opt.fp32_threshold;
+    constexpr int kChunk = 512;  // kets per work unit
+    std::vector<uint2> all;      // flat lists (wave-cooperative classes)
+    std::vector<int4> units, units32;
+    std::vector<unsigned> kets, k64, k32;
+    for (auto* v : {&c->q_off, &c->q_cnt, &c->q_prim, &c->q_off32, &c->q_cnt32, &c->q_prim32, &c->u_off, &c->u_cnt,
+                    &c->u_off32, &c->u_cnt32})
+        v->assign(ncls, 0);
     c->n_unscreened = 0;
     for (int X = 0; X < npc; ++X)
         for (int Y = 0; Y <= X; ++Y) {
             const int cls = X * (X + 1) / 2 + Y;
-            c->q_off[cls] = (long long)all.size();
+            const bool wave = dsscf_class_is_wave(cls) != 0;
+            const bool split = thr32 > 0.0 && !wave;
             const std::vector<int>& SX = by_class[X];
             const std::vector<int>& SY = by_class[Y];
             c->n_unscreened += X == Y ? (long long)SX.size() * (SX.size() + 1) / 2 : (long long)SX.size() * SY.size();
-            const std::vector<int> GX = groups(SX), GY = groups(SY);
-            for (size_t gb = 0; gb + 1 < GX.size(); ++gb)
+            const std::vector<int> GY = groups(SY);
+            c->q_off[cls] = (long long)all.size();
+            units32.clear();
+            const long long u0 = (long long)units.size();
+            for (size_t i = 0; i < SX.size(); ++i) {
+                const int bi = SX[i];
+                k64.clear();
+                k32.clear();
                 for (size_t gk = 0; gk + 1 < GY.size(); ++gk) {
-                    if (X == Y && gk > gb) continue;
-                    for (int i = GX[gb]; i < GX[gb + 1]; ++i) {
-                        const int bi = SX[i];
-                        const int jend = (X == Y && gk == gb) ? i + 1 : GY[gk + 1];
-                        for (int j = GY[gk]; j < jend; ++j) {
-                            const int kj = SY[j];
-                            if (thr > 0.0 && c->Q[bi] * c->Q[kj] < thr) break;  // Q sorted desc in a group
-                            all.push_back(make_uint2((unsigned)bi, (unsigned)kj));
-                            c->q_prim[cls] += (long long)c->pairs[bi].nprim * c->pairs[kj].nprim;
+                    const int jend = X == Y ? std::min<int>(GY[gk + 1], (int)i + 1) : GY[gk + 1];
+                    for (int j = GY[gk]; j < jend; ++j) {
+                        const int kj = SY[j];
+                        const double bound = c->Q[bi] * c->Q[kj];
+                        if (thr > 0.0 && bound < thr) break;  // Q sorted desc in a group
+                        const long long np = (long long)c->pairs[bi].nprim * c->pairs[kj].nprim;
+                        if (split && bound < thr32) {
+                            k32.push_back((unsigned)kj);
+                            c->q_prim32[cls] += np;
+                        } else {
+                            k64.push_back((unsigned)kj);
+                            c->q_prim[cls] += np;
                         }
                     }
                 }
-            c->q_cnt[cls] = (long long)all.size() - c->q_off[cls];
+                if (wave) {
+                    for (unsigned kj : k64) all.push_back(make_uint2((unsigned)bi, kj));
+                    continue;
+                }
+                c->q_cnt[cls] += (long long)k64.size();
+                c->q_cnt32[cls] += (long long)k32.size();
+                for (int pass = 0; pass < 2; ++pass) {
+                    const std::vector<unsigned>& kk = pass ? k32 : k64;
+                    std::vector<int4>& uu = pass ? units32 : units;
+                    for (size_t o = 0; o < kk.size(); o += kChunk) {
+                        const int cnt = (int)std::min<size_t>(kChunk, kk.size() - o);
+                        uu.push_back(make_int4(bi, (int)(kets.size() + o), cnt, 0));
+                    }
+                    kets.insert(kets.end(), kk.begin(), kk.end());
+                }
+            }
+            if (wave) c->q_cnt[cls] = (long long)all.size() - c->q_off[cls];
+            c->u_off[cls] = u0;
+            c->u_cnt[cls] = (long long)units.size() - u0;
+            c->u_off32[cls] = (long long)units.size();
+            c->u_cnt32[cls] = (long long)units32.size();
+            units.insert(units.end(), units32.begin(), units32.end());
         }
+    if (kets.size() > 0xffffffffULL) return fail(DSSCF_ERR_UNSUPPORTED, "too many quartets");
+    if (int rc = dev_alloc(&c->d_units, units.size())) return rc;
+    if (int rc = dev_alloc(&c->d_kets, kets.size())) return rc;
+    if (!units.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_units, units.data(), sizeof(int4) * units.size(), hipMemcpyHostToDevice, c->stream));
+    if (!kets.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_kets, kets.data(), sizeof(unsigned) * kets.size(), hipMemcpyHostToDevice, c->stream));
     if (int rc = dev_alloc(&c->d_q, all.size())) return rc;
-    HIP_TRY(hipMemcpyAsync(c->d_q, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice, c->stream));
+    if (!all.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_q, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if ((int)c->block_size.size() != ncls) c->block_size.assign(ncls, 256);
     c->prepared = true;
@@ -426,26 +481,33 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     p.ncls = ncls;
     p.boys_mode = ctxs[0]->opt.boys_mode;
     p.btab = ctxs[0]->d_btab;
-    p.cls_nseg.assign(ncls, 0);
-    p.cls_nblocks.assign(ncls, 0);
-    p.cls_off.assign(ncls, 0);
-    p.cls_bs.assign(ncls, 256);
+    const int nslot = 2 * ncls;  // slot 2*cls: FP64 list, 2*cls+1: FP32 list
+    p.cls_nseg.assign(nslot, 0);
+    p.cls_nblocks.assign(nslot, 0);
+    p.cls_off.assign(nslot, 0);
+    p.cls_bs.assign(nslot, 256);
+    p.cls_maxN.assign(nslot, 0);
     p.h_segs.clear();
     p.h_blk0.clear();
     p.h_mat.clear();
-    for (int cls = 0; cls < ncls; ++cls) {
+    for (int slot = 0; slot < nslot; ++slot) {
+        const int cls = slot / 2, prec32 = slot % 2;
         const bool wave = dsscf_class_is_wave(cls) != 0;
         const int b = wave ? 64 : (bs ? bs[cls] : (ctxs[0]->block_size.empty() ? 256 : ctxs[0]->block_size[cls]));
-        p.cls_bs[cls] = b;
-        p.cls_off[cls] = (int)p.h_segs.size();
+        p.cls_bs[slot] = b;
+        p.cls_off[slot] = (int)p.h_segs.size();
         long long blocks = 0;
+        int slot_maxN = 0;
         for (int k = 0; k < n; ++k) {
             dsscf_ctx* c = ctxs[k];
-            const long long cnt = c->q_cnt[cls];
+            const long long cnt = wave ? (prec32 ? 0 : c->q_cnt[cls]) : (prec32 ? c->u_cnt32[cls] : c->u_cnt[cls]);
             if (cnt <= 0) continue;
-            if (cnt > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "quartet list longer than 2^31");
+            if (cnt > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "work list longer than 2^31");
             JKSeg sg{};
             sg.q = c->d_q + c->q_off[cls];
+            sg.u = c->d_units + (prec32 ? c->u_off32[cls] : c->u_off[cls]);
+            sg.k = c->d_kets;
+            slot_maxN = std::max(slot_maxN, c->n_ao);
             sg.pairs = c->d_pairs;
             sg.pp = c->d_pp;
             sg.Dt = c->d_Dt;
@@ -455,13 +517,20 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
             sg.n = (int)cnt;
             sg.N = c->n_ao;
             sg.blk0 = (int)blocks;
-            blocks += wave ? cnt : (cnt + b - 1) / b;
+            blocks += cnt;  // one block per quartet (wave classes) or per work unit
             if (blocks > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "launch grid too large");
             p.h_segs.push_back(sg);
             p.h_blk0.push_back(sg.blk0);
         }
-        p.cls_nseg[cls] = (int)p.h_segs.size() - p.cls_off[cls];
-        p.cls_nblocks[cls] = (int)blocks;
+        p.cls_nseg[slot] = (int)p.h_segs.size() - p.cls_off[slot];
+        p.cls_nblocks[slot] = (int)blocks;
+        p.cls_maxN[slot] = slot_maxN;
+        if (!wave && p.cls_nseg[slot]) {
+            int l4[4];
+            dk_class_l(cls, l4);
+            const size_t lds = sizeof(double) * (size_t)(2 * l4[0] + 2 * l4[1] + 2) * slot_maxN;
+            if (lds > 64 * 1024) return fail(DSSCF_ERR_UNSUPPORTED, "basis too large for the LDS exchange rows");
+        }
     }
     int row = 0, maxN = 0;
     for (int k = 0; k < n; ++k) {
@@ -497,12 +566,19 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     return DSSCF_OK;
 }
 
+int plan_launch_slot(Plan& p, int slot, hipStream_t s)
+{
+    if (p.cls_nseg[slot] == 0) return DSSCF_OK;
+    HIP_TRY(dsscf_launch_jk(slot / 2, p.boys_mode, slot % 2, p.d_segs + p.cls_off[slot], p.d_blk0 + p.cls_off[slot],
+                            p.cls_nseg[slot], p.cls_nblocks[slot], p.cls_bs[slot], p.cls_maxN[slot], p.btab,
+                            kEriScale, s));
+    return DSSCF_OK;
+}
+
 int plan_launch_class(Plan& p, int cls, hipStream_t s)
 {
-    if (p.cls_nseg[cls] == 0) return DSSCF_OK;
-    HIP_TRY(dsscf_launch_jk(cls, p.boys_mode, p.d_segs + p.cls_off[cls], p.d_blk0 + p.cls_off[cls], p.cls_nseg[cls],
-                            p.cls_nblocks[cls], p.cls_bs[cls], p.btab, kEriScale, s));
-    return DSSCF_OK;
+    if (int rc = plan_launch_slot(p, 2 * cls, s)) return rc;
+    return plan_launch_slot(p, 2 * cls + 1, s);
 }
 
 // prepare (unpack densities, zero accumulators), all classes, fold.
@@ -689,10 +765,13 @@ int dsscf_get_stats(dsscf_ctx* c, dsscf_stats* st)
     st->n_quartets_unscreened = c->n_unscreened;
     st->n_class = dk_n_class();
     for (int k = 0; k < st->n_class && k < 21; ++k) {
-        st->class_quartets[k] = c->q_cnt[k];
-        st->class_prim_quartets[k] = c->q_prim[k];
-        st->n_quartets += c->q_cnt[k];
-        st->n_prim_quartets += c->q_prim[k];
+        st->class_quartets[k] = c->q_cnt[k] + c->q_cnt32[k];
+        st->class_prim_quartets[k] = c->q_prim[k] + c->q_prim32[k];
+        st->class_quartets_fp32[k] = c->q_cnt32[k];
+        st->class_prim_quartets_fp32[k] = c->q_prim32[k];
+        st->n_quartets += c->q_cnt[k] + c->q_cnt32[k];
+        st->n_prim_quartets += c->q_prim[k] + c->q_prim32[k];
+        st->n_quartets_fp32 += c->q_cnt32[k];
     }
     return DSSCF_OK;
 }
@@ -957,7 +1036,8 @@ int dsscf_batch_autotune(dsscf_batch* b, int* chosen)
     HIP_TRY(hipEventCreate(&e1));
     std::vector<int> best = b->block_size;
     for (int cls = 0; cls < ncls; ++cls) {
-        if (dsscf_class_is_wave(cls) || b->plan.cls_nseg[cls] == 0) continue;
+        if (dsscf_class_is_wave(cls) || (b->plan.cls_nseg[2 * cls] == 0 && b->plan.cls_nseg[2 * cls + 1] == 0))
+            continue;
         float best_ms = 1e30f;
         for (int ci = 0; ci < 3; ++ci) {
             b->block_size[cls] = cand[ci];

@@ -41,17 +41,25 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // J_mn = sum_ls (mn|ls) Dt_ls, K_mn = sum_ls (ml|ns) Da_ls (ExactCoulomb.cpp:
 // 103-123).  A unique quartet (ab|cd) with weight g = 1/((1+d_ab)(1+d_cd)
 // (1+d_ab,cd)) stands for its 8 permutation images; half of them go into
-// Jh/Kh and the other half are the transposes added by the fold kernel.
-template <int la, int lb, int lc, int ld, int BM>
+// Jh/Kh and the other half are the transposes added by the fold kernel:
+//   Jh[a][b] += 2g sum_cd v D_cd   Jh[c][d] += 2g sum_ab v D_ab
+//   Kh[a][c] += g v D_bd   Kh[b][c] += g v D_ad   Kh[a][d] += g v D_bc   Kh[b][d] += g v D_ac
+//
+// Work unit = one bra shell pair and a chunk of its ket pairs (bra-major):
+// the bra record and its primitive pairs are uniform across the block; the
+// Jh[a][b] sums are reduced across the block, the Kh rows of shells a and b
+// are accumulated in LDS and flushed once per block; only Jh[c][d] goes
+// straight to global memory per quartet.
+template <int la, int lb, int lc, int ld, int BM, typename R>
 __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
                                             int nseg, const double* __restrict__ btab, double scale)
 {
     typedef ClassInfo<la, lb, lc, ld> CI;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
+    extern __shared__ double krow[];  // [(NA + NB) * N]: Kh rows of shell a, then shell b
     const int si = find_segment_by_block(seg_blk0, nseg, (int)blockIdx.x);
     const JKSeg& S = segs[si];
-    const int t = ((int)blockIdx.x - S.blk0) * (int)blockDim.x + (int)threadIdx.x;
-    if (t >= S.n) return;
+    const int4 u = S.u[(int)blockIdx.x - S.blk0];  // bra pair, first ket, ket count
     const ShellPair* __restrict__ pairs = S.pairs;
     const PrimPair* __restrict__ pp = S.pp;
     const double* __restrict__ Dt = S.Dt;
@@ -59,112 +67,104 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     double* __restrict__ Jh = S.Jh;
     double* __restrict__ Kh = S.Kh;
     const int N = S.N;
-    const uint2 q = S.q[t];
-    const ShellPair bra = pairs[q.x];
-    const ShellPair ket = pairs[q.y];
-    double v[CI::NAO];
-    Quartet<la, lb, lc, ld, BM, double>::run(bra, ket, pp, btab, v);
+    const ShellPair bra = pairs[u.x];
+    const int a0 = bra.ao_a, b0 = bra.ao_b;
+    for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;
+    double dab[NA * NB];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) dab[i * NB + j] = Dt[(size_t)(a0 + i) * N + b0 + j];
+    double jab[NA * NB];
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
+    const double gb = scale * (a0 == b0 ? 0.5 : 1.0);
+    __syncthreads();
 
-    double g = scale;
-    if (bra.ao_a == bra.ao_b) g *= 0.5;
-    if (ket.ao_a == ket.ao_b) g *= 0.5;
-    if (q.x == q.y) g *= 0.5;
-    const int a0 = bra.ao_a, b0 = bra.ao_b, c0 = ket.ao_a, d0 = ket.ao_b;
-
-    // Coulomb: Jh[a][b] += 2g sum_cd v D_cd ; Jh[c][d] += 2g sum_ab v D_ab
-    {
-        double dcd[NC * ND], dab[NA * NB];
+    for (int t = threadIdx.x; t < u.z; t += blockDim.x) {
+        const int ki = (int)S.k[u.y + t];
+        const ShellPair ket = pairs[ki];
+        double v[CI::NAO];
+        Quartet<la, lb, lc, ld, BM, R>::run(bra, ket, pp, btab, v);
+        const int c0 = ket.ao_a, d0 = ket.ao_b;
+        const double g = gb * (c0 == d0 ? 0.5 : 1.0) * (ki == u.x ? 0.5 : 1.0);
+#define VIDX(i, j, k, l) ((((i) * NB + (j)) * NC + (k)) * ND + (l))
+        // Coulomb
 #pragma unroll
         for (int k = 0; k < NC; ++k)
 #pragma unroll
-            for (int l = 0; l < ND; ++l) dcd[k * ND + l] = Dt[(size_t)(c0 + k) * N + d0 + l];
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int j = 0; j < NB; ++j) dab[i * NB + j] = Dt[(size_t)(a0 + i) * N + b0 + j];
-        const double g2 = 2.0 * g;
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
+            for (int l = 0; l < ND; ++l) {
+                const double dcd = Dt[(size_t)(c0 + k) * N + d0 + l];
                 double s = 0.0;
 #pragma unroll
-                for (int kl = 0; kl < NC * ND; ++kl) s += v[(i * NB + j) * NC * ND + kl] * dcd[kl];
-                atomic_add_f64(&Jh[(size_t)(a0 + i) * N + b0 + j], g2 * s);
+                for (int ij = 0; ij < NA * NB; ++ij) {
+                    const double x = v[ij * NC * ND + k * ND + l];
+                    jab[ij] += g * x * dcd;
+                    s += x * dab[ij];
+                }
+                atomic_add_f64(&Jh[(size_t)(c0 + k) * N + d0 + l], 2.0 * g * s);
+            }
+        // Exchange into the LDS rows
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {  // Kh[a][c] += g sum_bd v D_bd
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * Da[(size_t)(b0 + j) * N + d0 + l];
+                atomicAdd(&krow[i * N + c0 + k], g * s);
             }
 #pragma unroll
-        for (int kl = 0; kl < NC * ND; ++kl) {
-            double s = 0.0;
+            for (int j = 0; j < NB; ++j) {  // Kh[b][c] += g sum_ad v D_ad
+                double s = 0.0;
 #pragma unroll
-            for (int ij = 0; ij < NA * NB; ++ij) s += v[ij * NC * ND + kl] * dab[ij];
-            atomic_add_f64(&Jh[(size_t)(c0 + kl / ND) * N + d0 + kl % ND], g2 * s);
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * Da[(size_t)(a0 + i) * N + d0 + l];
+                atomicAdd(&krow[(NA + j) * N + c0 + k], g * s);
+            }
         }
-    }
-    // Exchange: Kh[a][c] += g v D_bd, Kh[b][c] += g v D_ad, Kh[a][d] += g v D_bc, Kh[b][d] += g v D_ac
-    {
-        double dbd[NB * ND], dad[NA * ND], dbc[NB * NC], dac[NA * NC];
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
+        for (int l = 0; l < ND; ++l) {
 #pragma unroll
-            for (int l = 0; l < ND; ++l) dbd[j * ND + l] = Da[(size_t)(b0 + j) * N + d0 + l];
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int l = 0; l < ND; ++l) dad[i * ND + l] = Da[(size_t)(a0 + i) * N + d0 + l];
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int k = 0; k < NC; ++k) dbc[j * NC + k] = Da[(size_t)(b0 + j) * N + c0 + k];
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int k = 0; k < NC; ++k) dac[i * NC + k] = Da[(size_t)(a0 + i) * N + c0 + k];
-#define VIDX(i, j, k, l) ((((i) * NB + (j)) * NC + (k)) * ND + (l))
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
+            for (int i = 0; i < NA; ++i) {  // Kh[a][d] += g sum_bc v D_bc
                 double s = 0.0;
 #pragma unroll
                 for (int j = 0; j < NB; ++j)
 #pragma unroll
-                    for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * dbd[j * ND + l];
-                atomic_add_f64(&Kh[(size_t)(a0 + i) * N + c0 + k], g * s);
+                    for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * Da[(size_t)(b0 + j) * N + c0 + k];
+                atomicAdd(&krow[i * N + d0 + l], g * s);
             }
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
+            for (int j = 0; j < NB; ++j) {  // Kh[b][d] += g sum_ac v D_ac
                 double s = 0.0;
 #pragma unroll
                 for (int i = 0; i < NA; ++i)
 #pragma unroll
-                    for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * dad[i * ND + l];
-                atomic_add_f64(&Kh[(size_t)(b0 + j) * N + c0 + k], g * s);
+                    for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * Da[(size_t)(a0 + i) * N + c0 + k];
+                atomicAdd(&krow[(NA + j) * N + d0 + l], g * s);
             }
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int l = 0; l < ND; ++l) {
-                double s = 0.0;
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-#pragma unroll
-                    for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * dbc[j * NC + k];
-                atomic_add_f64(&Kh[(size_t)(a0 + i) * N + d0 + l], g * s);
-            }
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int l = 0; l < ND; ++l) {
-                double s = 0.0;
-#pragma unroll
-                for (int i = 0; i < NA; ++i)
-#pragma unroll
-                    for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * dac[i * NC + k];
-                atomic_add_f64(&Kh[(size_t)(b0 + j) * N + d0 + l], g * s);
-            }
+        }
 #undef VIDX
+    }
+    // Jh[a][b]: wavefront reduction, one atomic per wavefront and element
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) {
+        double x = jab[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if ((threadIdx.x & 63) == 0 && x != 0.0) atomic_add_f64(&Jh[(size_t)(a0 + k / NB) * N + b0 + k % NB], 2.0 * x);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) {
+        const double x = krow[i];
+        if (x != 0.0) {
+            const int r = i / N, col = i % N;
+            const int row = r < NA ? a0 + r : b0 + (r - NA);
+            atomic_add_f64(&Kh[(size_t)row * N + col], x);
+        }
     }
 }
 
@@ -298,14 +298,20 @@ void dk_class_l(int c, int* l)
 // Thread-per-quartet (wave = 0) or wave-cooperative (wave = 1) launches.
 template <int a, int b, int c, int d, int wave>
 struct Launch {
-    static hipError_t jk(int bm, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int bs,
-                         const double* btab, double scale, hipStream_t s)
+    static hipError_t jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int bs,
+                         int maxN, const double* btab, double scale, hipStream_t s)
     {
         const dim3 blk(bs), grd(nblocks);
-        if (bm == BOYS_ACCURATE)
-            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE>), grd, blk, 0, s, segs, blk0, nseg, btab, scale);
+        const size_t lds = sizeof(double) * (size_t)(ClassInfo<a, b, c, d>::NA + ClassInfo<a, b, c, d>::NB) * maxN;
+        if (fp32)
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, float>), grd, blk, lds, s, segs, blk0, nseg, btab,
+                               scale);
+        else if (bm == BOYS_ACCURATE)
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
+                               scale);
         else
-            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE>), grd, blk, 0, s, segs, blk0, nseg, btab, scale);
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
+                               scale);
         return hipGetLastError();
     }
     static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
@@ -342,7 +348,7 @@ struct Launch<a, b, c, d, 1> {
         hipLaunchKernelGGL((k_wave<a, b, c, d, BM, MODE>), dim3(grd), dim3(64), 0, s, items, n, pairs, pp, btab,
                            nullptr, nullptr, 0, out, scale);
     }
-    static hipError_t jk(int bm, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int,
+    static hipError_t jk(int bm, int, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int, int,
                          const double* btab, double scale, hipStream_t s)
     {
         if (bm == BOYS_ACCURATE)
@@ -391,13 +397,15 @@ struct SchwarzIf<a, b, c, d, wave, true> {
     }
 };
 
-hipError_t dsscf_launch_jk(int cls, int boys_mode, const JKSeg* segs, const int* blk0, int nseg, int nblocks,
-                           int block_size, const double* btab, double scale, hipStream_t s)
+hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
+                           int nblocks, int block_size, int maxN, const double* btab, double scale, hipStream_t s)
 {
     if (nseg <= 0 || nblocks <= 0) return hipSuccess;
     int c = 0;
-#define DSSCF_LAUNCH_JK(a, b, cc, d, name, wave) \
-    if (c++ == cls) return Launch<a, b, cc, d, wave>::jk(boys_mode, segs, blk0, nseg, nblocks, block_size, btab, scale, s);
+#define DSSCF_LAUNCH_JK(a, b, cc, d, name, wave)                                                                 \
+    if (c++ == cls)                                                                                              \
+        return Launch<a, b, cc, d, wave>::jk(boys_mode, fp32, segs, blk0, nseg, nblocks, block_size, maxN, btab, \
+                                             scale, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_JK)
 #undef DSSCF_LAUNCH_JK
     return hipErrorInvalidValue;

@@ -23,6 +23,7 @@ struct Options {
     int max_iterations = 500;          // the reference has no cap (SCFSolver.cpp:219)
     int boys_mode = DSSCF_BOYS_ACCURATE;
     double schwarz_threshold = 1e-12;
+    double fp32_threshold = 0.0;       // quartet bound below which the VRR runs in FP32
     int device = 0;
     int verbose = 0;                   // print the iteration table like SCFSolver.cpp:198-225
 };

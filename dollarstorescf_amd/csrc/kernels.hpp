@@ -14,8 +14,10 @@ void dsscf_upload_xform(const double* x, int n, hipStream_t s);
 // Fused J/K launch of one class over nseg context segments (block_size is
 // ignored by the wave-cooperative classes, which use one 64-lane block per
 // quartet).
-hipError_t dsscf_launch_jk(int cls, int boys_mode, const JKSeg* segs, const int* blk0, int nseg, int nblocks,
-                           int block_size, const double* btab, double scale, hipStream_t s);
+// fp32: the FP32 form of a thread-per-quartet class (always accurate Boys).
+// maxN: largest AO count among the segments (sizes the LDS exchange rows).
+hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
+                           int nblocks, int block_size, int maxN, const double* btab, double scale, hipStream_t s);
 int dsscf_class_is_wave(int cls);
 hipError_t dsscf_launch_eri_block(int cls, int boys_mode, const int4* items, int n, const ShellPair* pairs,
                                   const PrimPair* pp, const double* btab, double* out, double scale,

@@ -42,6 +42,9 @@ typedef struct {
     double schwarz_threshold; /* drop shell quartets with Q_ab*Q_cd below it; 0 keeps all.
                                  Replaces the (ii|jj)(kk|ll) > 1e-16 screen of
                                  src/chem/scf/ExactCoulomb.cpp:63 */
+    double fp32_threshold;    /* quartets with Q_ab*Q_cd below it run their setup, small-T Boys and
+                                 VRR in FP32 (contraction, HRR, J/K accumulation stay FP64); 0 = all
+                                 FP64.  Needs DSSCF_BOYS_ACCURATE. */
 } dsscf_options;
 
 typedef struct {
@@ -53,6 +56,9 @@ typedef struct {
     int n_class;                   /* shell-quartet classes */
     long long class_quartets[21];  /* per class (order of dsscf_class_l) */
     long long class_prim_quartets[21];
+    long long n_quartets_fp32;     /* of n_quartets, evaluated in FP32 */
+    long long class_quartets_fp32[21];
+    long long class_prim_quartets_fp32[21];
 } dsscf_stats;
 
 /* Last error message of the calling thread ("" if none). */

@@ -89,7 +89,7 @@ def main():
 
     Z, xyz = geometry.water_lattice(1)
     boys = ds.BOYS_ACCURATE if a.boys == "accurate" else ds.BOYS_REFERENCE
-    job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=20, boys_mode=boys,
+    job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=30, boys_mode=boys,
                                 schwarz_threshold=1e-12)
     job.set_basis("6-31G").set_molecule(Z, xyz)
     t0 = time.perf_counter()
@@ -102,7 +102,7 @@ def main():
     except ds.DsscfError as e:  # 20-iteration cap
         if e.code != 6:
             raise
-        energy, iters, converged = float("nan"), 20, False
+        energy, iters, converged = float(job.history()[-1]), 30, False
     t_scf = time.perf_counter() - t0
     n = job.n_ao
     ctx = job.context
@@ -129,6 +129,10 @@ def main():
     achieved = dom_flops / (ms_dom * 1e-3) / 1e12
     name = "".join("spdf"[l] for l in classes[dom])
 
+    cls_ms = ctx.time_classes(iters=10)
+    per_class = [{"class": "".join("spdf"[l] for l in c), "quartets": int(q), "prim_quartets": int(pq),
+                  "ms": float(m), "tflops": (flops.class_flops(c, q, pq) / (m * 1e-3) / 1e12) if m > 0 else 0.0}
+                 for c, q, pq, m in zip(classes, cq, cp, cls_ms) if q > 0]
     n_unique_ao = (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2
     cpu = None if a.no_cpu else cpu_baseline(Z, xyz, n, n_unique_ao, a.cpu_sample)
 
@@ -158,6 +162,7 @@ def main():
                      "kernel": f"k_jk<{name}>", "kernel_ms": ms_dom,
                      "note": "FP64 vector-ALU bound; peak is MI355X FP64 78.6 TFLOP/s; achieved = algorithmic "
                              "FLOPs of the class (dollarstorescf_amd/flops.py) / HIP-event kernel time"},
+        "classes": per_class,
         "cpu_baseline": cpu,
         "scf": {"energy": energy, "iterations": iters, "converged": converged, "seconds": t_scf,
                 "construct_seconds": t_construct},

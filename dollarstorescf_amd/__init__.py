@@ -18,6 +18,7 @@ BASIS_DIR = os.path.join(HERE, "basis")
 
 BOYS_REFERENCE = 0
 BOYS_ACCURATE = 1
+DEFAULT_PRIM_THRESHOLD = 1e-17
 
 ERRORS = {1: "bad argument", 2: "HIP error", 3: "call out of order", 4: "unsupported",
           5: "device allocation failed", 6: "did not converge"}
@@ -31,7 +32,7 @@ class DsscfError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("boys_mode", ctypes.c_int), ("schwarz_threshold", ctypes.c_double),
-                ("fp32_threshold", ctypes.c_double)]
+                ("fp32_threshold", ctypes.c_double), ("prim_threshold", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -65,9 +66,9 @@ def lib():
             "dsscf_jk": [p, p, p, p, p], "dsscf_jk_device": [p, p, p, p, p],
             "dsscf_eri_batch": [p, p, ctypes.c_longlong, p], "dsscf_schwarz": [p, p],
             "dsscf_get_stats": [p, p], "dsscf_set_density": [p, p, p], "dsscf_jk_resident": [p],
-            "dsscf_get_jk": [p, p, p], "dsscf_time_jk": [p, i, i, p, p, p], "dsscf_class_l": [i, p],
+            "dsscf_get_jk": [p, p, p], "dsscf_time_jk_classes": [p, i, p], "dsscf_time_jk": [p, i, i, p, p, p], "dsscf_class_l": [i, p],
             "dsscf_job_create": [p, i], "dsscf_job_destroy": [p],
-            "dsscf_job_set_options": [p, i, d, i, i, d, i], "dsscf_job_set_precision": [p, d],
+            "dsscf_job_set_options": [p, i, d, i, i, d, i], "dsscf_job_set_precision": [p, d, d],
             "dsscf_job_set_molecule": [p, i, p, p, i, i], "dsscf_job_load_xyz": [p, c, i, i],
             "dsscf_job_set_basis_file": [p, c], "dsscf_job_construct": [p], "dsscf_job_construct_host": [p],
             "dsscf_job_ao": [p, i, p, p, p, p, p, p, p, p, p, i], "dsscf_job_n_ao": [p],
@@ -137,8 +138,8 @@ class Context:
 
     __del__ = close
 
-    def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12, fp32_threshold=0.0):
-        o = Options(boys_mode, schwarz_threshold, fp32_threshold)
+    def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12, fp32_threshold=0.0, prim_threshold=0.0):
+        o = Options(boys_mode, schwarz_threshold, fp32_threshold, prim_threshold)
         _check(lib().dsscf_set_options(self._h, ctypes.byref(o)))
 
     def set_basis(self, l, nprim, centers, exps, coefs, xform):
@@ -185,6 +186,11 @@ class Context:
         Q = np.empty(n_pairs)
         _check(lib().dsscf_schwarz(self._h, _p(Q)))
         return Q
+
+    def time_classes(self, iters=10):
+        out = np.zeros(lib().dsscf_class_count())
+        _check(lib().dsscf_time_jk_classes(self._h, iters, _p(out)))
+        return out
 
     def time_jk(self, warmup=2, iters=10):
         ms, dom = ctypes.c_double(), ctypes.c_double()
@@ -253,10 +259,14 @@ class Job:
     __del__ = close
 
     def set_options(self, diis_size=8, energy_threshold=1e-8, max_iterations=500, boys_mode=BOYS_ACCURATE,
-                    schwarz_threshold=1e-12, verbose=0, fp32_threshold=0.0):
+                    schwarz_threshold=1e-12, verbose=0, fp32_threshold=0.0, prim_threshold=None):
+        """prim_threshold defaults to 1e-17 in the accurate Boys mode and to 0 (every
+        primitive quartet, as the reference) in the reference mode."""
+        if prim_threshold is None:
+            prim_threshold = DEFAULT_PRIM_THRESHOLD if boys_mode == BOYS_ACCURATE else 0.0
         _check(lib().dsscf_job_set_options(self._h, diis_size, energy_threshold, max_iterations, boys_mode,
                                            schwarz_threshold, verbose), self._err)
-        _check(lib().dsscf_job_set_precision(self._h, fp32_threshold), self._err)
+        _check(lib().dsscf_job_set_precision(self._h, fp32_threshold, prim_threshold), self._err)
         return self
 
     def set_molecule(self, Z, xyz_bohr, charge=0, multiplicity=1):

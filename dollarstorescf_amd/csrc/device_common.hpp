@@ -37,7 +37,8 @@ struct __align__(16) PrimPair {
     double p, inv_p;
     double P[3];
     double Kp;
-    double pad[2];
+    double e;    // screening factor |Kp| / p^(1/4); a shell pair's records are sorted by it, largest first
+    double pad;
 };
 static_assert(sizeof(PrimPair) == 64, "PrimPair must be one 64 B record");
 
@@ -74,6 +75,38 @@ struct MatSeg {
     int N;
     int row0;
 };
+
+// Read-only global-memory view of a pointer that arrived through memory (a
+// segment table), so loads compile to global_load rather than flat_load.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using GPtr = const __attribute__((address_space(1))) T*;
+#else
+template <typename T>
+using GPtr = const T*;
+#endif
+template <typename T>
+__device__ __forceinline__ GPtr<T> gbl(const T* p)
+{
+    return (GPtr<T>)p;
+}
+// Whole record (a multiple of 8 bytes) through global loads.
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p)
+{
+    static_assert(sizeof(T) % 8 == 0, "record size");
+    const GPtr<double> q = gbl((const double*)p);
+    double tmp[sizeof(T) / 8];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 8); ++i) tmp[i] = q[i];
+    T out;
+    __builtin_memcpy(&out, tmp, sizeof(T));
+    return out;
+}
+
+// Bra primitive pairs staged in LDS per block: a shell pair may have at most
+// this many primitive pairs (checked by dsscf_set_basis).
+constexpr int kBraStage = 128;
 
 __device__ __forceinline__ int find_segment_by_block(const int* blk0, int nseg, int b)
 {

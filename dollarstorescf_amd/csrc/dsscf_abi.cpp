@@ -63,6 +63,7 @@ void dev_free(T*& p)
 // for the prepare/fold kernels, one MatSeg per context over stacked rows.
 struct Plan {
     int boys_mode = DSSCF_BOYS_ACCURATE;
+    double pthr = 0.0;  // primitive threshold on e_ab e_cd
     const double* btab = nullptr;
     int ncls = 0;
     std::vector<int> cls_nseg, cls_nblocks, cls_off, cls_bs, cls_maxN;
@@ -94,7 +95,7 @@ struct Plan {
 struct dsscf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    dsscf_options opt{DSSCF_BOYS_ACCURATE, 1e-12, 0.0};
+    dsscf_options opt{DSSCF_BOYS_ACCURATE, 1e-12, 0.0, 0.0};
 
     // basis
     int n_shell = 0, n_ao = 0;
@@ -218,12 +219,12 @@ int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
     if (!c || !o) return fail(DSSCF_ERR_ARG, "dsscf_set_options: null argument");
     if (o->boys_mode != DSSCF_BOYS_REFERENCE && o->boys_mode != DSSCF_BOYS_ACCURATE)
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: unknown boys_mode");
-    if (!(o->schwarz_threshold >= 0.0) || !(o->fp32_threshold >= 0.0))
+    if (!(o->schwarz_threshold >= 0.0) || !(o->fp32_threshold >= 0.0) || !(o->prim_threshold >= 0.0))
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: negative threshold");
     if (o->fp32_threshold > 0.0 && o->boys_mode != DSSCF_BOYS_ACCURATE)
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: FP32 quartets need the accurate Boys mode");
     const bool changed = o->boys_mode != c->opt.boys_mode || o->schwarz_threshold != c->opt.schwarz_threshold ||
-                         o->fp32_threshold != c->opt.fp32_threshold;
+                         o->fp32_threshold != c->opt.fp32_threshold || o->prim_threshold != c->opt.prim_threshold;
     c->opt = *o;
     if (changed) c->prepared = false;
     return DSSCF_OK;
@@ -255,6 +256,11 @@ int dsscf_set_basis(dsscf_ctx* c, int n_shell, const int* shell_l, const int* sh
     }
     for (int i = 0; i < np; ++i)
         if (!(prim_exp[i] > 0.0)) return fail(DSSCF_ERR_ARG, "dsscf_set_basis: non-positive exponent");
+    int maxp = 0;
+    for (int s2 = 0; s2 < n_shell; ++s2) maxp = std::max(maxp, shell_nprim[s2]);
+    if (maxp * maxp > kBraStage)
+        return fail(DSSCF_ERR_UNSUPPORTED, "dsscf_set_basis: shells with more than " + std::to_string((int)std::sqrt((double)kBraStage)) +
+                                               " primitives are not supported (primitive pairs are staged in LDS)");
     c->prim_exp.assign(prim_exp, prim_exp + np);
     c->prim_coef.assign(prim_coef, prim_coef + np);
     c->xform.assign(ao_xform, ao_xform + dk_xform_size());
@@ -307,15 +313,36 @@ void build_pairs(dsscf_ctx* c)
                     for (int x = 0; x < 3; ++x) pp.P[x] = (al * A[x] + be * B[x]) * pp.inv_p;
                     pp.Kp = c->prim_coef[c->sh_prim_off[f] + i] * c->prim_coef[c->sh_prim_off[s] + j] *
                             std::exp(-al * be * pp.inv_p * AB2) * pp.inv_p;
+                    pp.e = std::fabs(pp.Kp) / std::sqrt(std::sqrt(pp.p));
                     c->prims.push_back(pp);
                 }
             P.nprim = (int)c->prims.size() - P.prim_off;
+            std::stable_sort(c->prims.begin() + P.prim_off, c->prims.end(),
+                             [](const PrimPair& x, const PrimPair& y) { return x.e > y.e; });
             c->pair_first[idx] = f;
             c->pair_class[idx] = pair_class_of(c->sh_l[f], c->sh_l[s]);
         }
 }
 
 int pair_index(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
+
+// Primitive quartets of (bra|ket) that survive e_ab e_cd >= pthr (both
+// record lists sorted by e, largest first): two pointers.
+long long prim_count(const dsscf_ctx* c, int bra, int ket, double pthr)
+{
+    const ShellPair& B = c->pairs[bra];
+    const ShellPair& K = c->pairs[ket];
+    if (pthr <= 0.0) return (long long)B.nprim * K.nprim;
+    long long n = 0;
+    int nb = B.nprim;
+    for (int k = 0; k < K.nprim; ++k) {
+        const double ek = c->prims[K.prim_off + k].e;
+        while (nb > 0 && c->prims[B.prim_off + nb - 1].e * ek < pthr) --nb;
+        if (nb == 0) break;
+        n += nb;
+    }
+    return n;
+}
 
 }  // namespace
 
@@ -389,6 +416,7 @@ int dsscf_prepare(dsscf_ctx* c)
     const int ncls = dk_n_class();
     const double thr = c->opt.schwarz_threshold;
     const double thr32 = c->opt.fp32_threshold;
+    const double pthr = c->opt.prim_threshold / kEriScale;
     constexpr int kChunk = 512;  // kets per work unit
     std::vector<uint2> all;      // flat lists (wave-cooperative classes)
     std::vector<int4> units, units32;
@@ -419,7 +447,7 @@ int dsscf_prepare(dsscf_ctx* c)
                         const int kj = SY[j];
                         const double bound = c->Q[bi] * c->Q[kj];
                         if (thr > 0.0 && bound < thr) break;  // Q sorted desc in a group
-                        const long long np = (long long)c->pairs[bi].nprim * c->pairs[kj].nprim;
+                        const long long np = prim_count(c, bi, kj, pthr);
                         if (split && bound < thr32) {
                             k32.push_back((unsigned)kj);
                             c->q_prim32[cls] += np;
@@ -480,6 +508,7 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     const int ncls = dk_n_class();
     p.ncls = ncls;
     p.boys_mode = ctxs[0]->opt.boys_mode;
+    p.pthr = ctxs[0]->opt.prim_threshold / kEriScale;
     p.btab = ctxs[0]->d_btab;
     const int nslot = 2 * ncls;  // slot 2*cls: FP64 list, 2*cls+1: FP32 list
     p.cls_nseg.assign(nslot, 0);
@@ -571,7 +600,7 @@ int plan_launch_slot(Plan& p, int slot, hipStream_t s)
     if (p.cls_nseg[slot] == 0) return DSSCF_OK;
     HIP_TRY(dsscf_launch_jk(slot / 2, p.boys_mode, slot % 2, p.d_segs + p.cls_off[slot], p.d_blk0 + p.cls_off[slot],
                             p.cls_nseg[slot], p.cls_nblocks[slot], p.cls_bs[slot], p.cls_maxN[slot], p.btab,
-                            kEriScale, s));
+                            kEriScale, p.pthr, s));
     return DSSCF_OK;
 }
 
@@ -744,6 +773,22 @@ int dsscf_time_jk(dsscf_ctx* c, int warmup, int iters, double* ms_per_build, dou
     if (ms_per_build) *ms_per_build = total / iters;
     if (ms_dominant) *ms_dominant = acc[best] / iters;
     if (dom_cls) *dom_cls = best;
+    return DSSCF_OK;
+}
+
+int dsscf_time_jk_classes(dsscf_ctx* c, int iters, double* ms_per_class)
+{
+    if (!c || iters <= 0 || !ms_per_class) return fail(DSSCF_ERR_ARG, "dsscf_time_jk_classes: bad argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_time_jk_classes: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    const int ncls = dk_n_class();
+    std::vector<double> acc(ncls, 0.0);
+    std::vector<float> ms;
+    for (int i = 0; i < iters; ++i) {
+        if (int rc = run_jk(c, &ms)) return rc;
+        for (int k = 0; k < ncls; ++k) acc[k] += ms[k];
+    }
+    for (int k = 0; k < ncls; ++k) ms_per_class[k] = acc[k] / iters;
     return DSSCF_OK;
 }
 
@@ -943,8 +988,9 @@ int dsscf_batch_set(dsscf_batch* b, dsscf_ctx* const* ctxs, int n)
     for (int i = 0; i < n; ++i) {
         if (!ctxs[i] || !ctxs[i]->prepared) return fail(DSSCF_ERR_STATE, "dsscf_batch_set: context not prepared");
         if (ctxs[i]->device != b->device) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: contexts on another device");
-        if (ctxs[i]->opt.boys_mode != ctxs[0]->opt.boys_mode)
-            return fail(DSSCF_ERR_ARG, "dsscf_batch_set: mixed Boys modes in one batch");
+        if (ctxs[i]->opt.boys_mode != ctxs[0]->opt.boys_mode ||
+            ctxs[i]->opt.prim_threshold != ctxs[0]->opt.prim_threshold)
+            return fail(DSSCF_ERR_ARG, "dsscf_batch_set: mixed Boys modes or primitive thresholds in one batch");
         for (int k = 0; k < i; ++k)
             if (ctxs[k] == ctxs[i]) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: context listed twice");
     }

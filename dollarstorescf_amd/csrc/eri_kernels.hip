@@ -20,10 +20,12 @@ struct Quartet;
     template <int BM, typename R>                                                                     \
     struct Quartet<a, b, c, d, BM, R> {                                                               \
         static __device__ __forceinline__ void run(const ShellPair& bra, const ShellPair& ket,        \
-                                                   const PrimPair* __restrict__ pp,                   \
-                                                   const double* __restrict__ btab, double* out)      \
+                                                   const PrimPair* __restrict__ bra_pp,               \
+                                                   const PrimPair* __restrict__ ket_pp,               \
+                                                   const double* __restrict__ btab, double pthr,      \
+                                                   double* out)                                       \
         {                                                                                             \
-            shell_quartet_##name<BM, R>(bra, ket, pp, btab, out);                                     \
+            shell_quartet_##name<BM, R>(bra, ket, bra_pp, ket_pp, btab, pthr, out);                   \
         }                                                                                             \
     };
 #define DSSCF_QUARTET_SPECIALIZE(a, b, c, d, name, wave) DSSCF_QSPEC_##wave(a, b, c, d, name)
@@ -52,24 +54,27 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // straight to global memory per quartet.
 template <int la, int lb, int lc, int ld, int BM, typename R>
 __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
-                                            int nseg, const double* __restrict__ btab, double scale)
+                                            int nseg, const double* __restrict__ btab, double scale, double pthr)
 {
     typedef ClassInfo<la, lb, lc, ld> CI;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
     extern __shared__ double krow[];  // [(NA + NB) * N]: Kh rows of shell a, then shell b
+    __shared__ PrimPair braP[kBraStage];
     const int si = find_segment_by_block(seg_blk0, nseg, (int)blockIdx.x);
     const JKSeg& S = segs[si];
-    const int4 u = S.u[(int)blockIdx.x - S.blk0];  // bra pair, first ket, ket count
+    const int4 u = ldg(S.u + ((int)blockIdx.x - S.blk0));  // bra pair, first ket, ket count
     const ShellPair* __restrict__ pairs = S.pairs;
     const PrimPair* __restrict__ pp = S.pp;
-    const double* __restrict__ Dt = S.Dt;
-    const double* __restrict__ Da = S.Da;
+    const GPtr<double> Dt = gbl(S.Dt);
+    const GPtr<double> Da = gbl(S.Da);  // read-only in this kernel
     double* __restrict__ Jh = S.Jh;
     double* __restrict__ Kh = S.Kh;
+    const GPtr<unsigned> kets = gbl(S.k);
     const int N = S.N;
-    const ShellPair bra = pairs[u.x];
+    const ShellPair bra = ldg(pairs + u.x);
     const int a0 = bra.ao_a, b0 = bra.ao_b;
     for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;
+    for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) braP[i] = ldg(pp + bra.prim_off + i);
     double dab[NA * NB];
 #pragma unroll
     for (int i = 0; i < NA; ++i)
@@ -82,10 +87,10 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     __syncthreads();
 
     for (int t = threadIdx.x; t < u.z; t += blockDim.x) {
-        const int ki = (int)S.k[u.y + t];
-        const ShellPair ket = pairs[ki];
+        const int ki = (int)kets[u.y + t];
+        const ShellPair ket = ldg(pairs + ki);
         double v[CI::NAO];
-        Quartet<la, lb, lc, ld, BM, R>::run(bra, ket, pp, btab, v);
+        Quartet<la, lb, lc, ld, BM, R>::run(bra, ket, braP, S.pp + ket.prim_off, btab, pthr, v);
         const int c0 = ket.ao_a, d0 = ket.ao_b;
         const double g = gb * (c0 == d0 ? 0.5 : 1.0) * (ki == u.x ? 0.5 : 1.0);
 #define VIDX(i, j, k, l) ((((i) * NB + (j)) * NC + (k)) * ND + (l))
@@ -185,7 +190,7 @@ __global__ void __launch_bounds__(256) k_eri_block(const int4* __restrict__ item
     const ShellPair bra = pairs[it.x];
     const ShellPair ket = pairs[it.y];
     double v[CI::NAO];
-    Quartet<la, lb, lc, ld, BM, double>::run(bra, ket, pp, btab, v);
+    Quartet<la, lb, lc, ld, BM, double>::run(bra, ket, pp + bra.prim_off, pp + ket.prim_off, btab, 0.0, v);
     double* o = out + (size_t)it.z;
 #pragma unroll
     for (int i = 0; i < CI::NAO; ++i) o[i] = scale * v[i];
@@ -207,7 +212,7 @@ __global__ void __launch_bounds__(256) k_schwarz(const int* __restrict__ plist, 
     const int pidx = plist[t];
     const ShellPair pr = pairs[pidx];
     double v[CI::NAO];
-    Quartet<la, lb, la, lb, BM, double>::run(pr, pr, pp, btab, v);
+    Quartet<la, lb, la, lb, BM, double>::run(pr, pr, pp + pr.prim_off, pp + pr.prim_off, btab, 0.0, v);
     double mx = 0.0;
 #pragma unroll
     for (int i = 0; i < NA; ++i)
@@ -299,19 +304,19 @@ void dk_class_l(int c, int* l)
 template <int a, int b, int c, int d, int wave>
 struct Launch {
     static hipError_t jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int bs,
-                         int maxN, const double* btab, double scale, hipStream_t s)
+                         int maxN, const double* btab, double scale, double pthr, hipStream_t s)
     {
         const dim3 blk(bs), grd(nblocks);
         const size_t lds = sizeof(double) * (size_t)(ClassInfo<a, b, c, d>::NA + ClassInfo<a, b, c, d>::NB) * maxN;
         if (fp32)
             hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, float>), grd, blk, lds, s, segs, blk0, nseg, btab,
-                               scale);
+                               scale, pthr);
         else if (bm == BOYS_ACCURATE)
             hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
-                               scale);
+                               scale, pthr);
         else
             hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
-                               scale);
+                               scale, pthr);
         return hipGetLastError();
     }
     static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
@@ -346,17 +351,17 @@ struct Launch<a, b, c, d, 1> {
     {
         const int grd = n < (1 << 20) ? n : (1 << 20);
         hipLaunchKernelGGL((k_wave<a, b, c, d, BM, MODE>), dim3(grd), dim3(64), 0, s, items, n, pairs, pp, btab,
-                           nullptr, nullptr, 0, out, scale);
+                           nullptr, nullptr, 0, out, scale, 0.0);
     }
     static hipError_t jk(int bm, int, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int, int,
-                         const double* btab, double scale, hipStream_t s)
+                         const double* btab, double scale, double pthr, hipStream_t s)
     {
         if (bm == BOYS_ACCURATE)
             hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_ACCURATE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
-                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
+                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale, pthr);
         else
             hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_REFERENCE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
-                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
+                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale, pthr);
         return hipGetLastError();
     }
     static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
@@ -398,14 +403,15 @@ struct SchwarzIf<a, b, c, d, wave, true> {
 };
 
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                           int nblocks, int block_size, int maxN, const double* btab, double scale, hipStream_t s)
+                           int nblocks, int block_size, int maxN, const double* btab, double scale, double pthr,
+                           hipStream_t s)
 {
     if (nseg <= 0 || nblocks <= 0) return hipSuccess;
     int c = 0;
 #define DSSCF_LAUNCH_JK(a, b, cc, d, name, wave)                                                                 \
     if (c++ == cls)                                                                                              \
         return Launch<a, b, cc, d, wave>::jk(boys_mode, fp32, segs, blk0, nseg, nblocks, block_size, maxN, btab, \
-                                             scale, s);
+                                             scale, pthr, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_JK)
 #undef DSSCF_LAUNCH_JK
     return hipErrorInvalidValue;

@@ -53,12 +53,14 @@ int dsscf_job_set_options(dsscf_job* j, int diis_size, double thr, int max_it, i
     return DSSCF_OK;
 }
 
-int dsscf_job_set_precision(dsscf_job* j, double fp32_threshold)
+int dsscf_job_set_precision(dsscf_job* j, double fp32_threshold, double prim_threshold)
 {
-    if (!j || !(fp32_threshold >= 0.0)) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_precision: bad argument");
+    if (!j || !(fp32_threshold >= 0.0) || !(prim_threshold >= 0.0))
+        return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_precision: bad argument");
     if (fp32_threshold > 0.0 && j->job.opt.boys_mode != DSSCF_BOYS_ACCURATE)
         return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_precision: FP32 quartets need the accurate Boys mode");
     j->job.opt.fp32_threshold = fp32_threshold;
+    j->job.opt.prim_threshold = prim_threshold;
     return DSSCF_OK;
 }
 

@@ -24,6 +24,7 @@ struct Options {
     int boys_mode = DSSCF_BOYS_ACCURATE;
     double schwarz_threshold = 1e-12;
     double fp32_threshold = 0.0;       // quartet bound below which the VRR runs in FP32
+    double prim_threshold = 0.0;       // primitive-quartet screening bound (0 = none)
     int device = 0;
     int verbose = 0;                   // print the iteration table like SCFSolver.cpp:198-225
 };

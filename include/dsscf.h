@@ -45,13 +45,16 @@ typedef struct {
     double fp32_threshold;    /* quartets with Q_ab*Q_cd below it run their setup, small-T Boys and
                                  VRR in FP32 (contraction, HRR, J/K accumulation stay FP64); 0 = all
                                  FP64.  Needs DSSCF_BOYS_ACCURATE. */
+    double prim_threshold;    /* skip primitive quartets whose bound 2 pi^(5/2) e_ab e_cd falls below it,
+                                 e = w_a w_b exp(-ab/p |AB|^2) / p^(5/4) (no reference counterpart: the
+                                 reference evaluates every primitive quartet); 0 = none */
 } dsscf_options;
 
 typedef struct {
     long long n_pairs;             /* unique shell pairs */
     long long n_prim_pairs;        /* primitive pairs over all shell pairs */
     long long n_quartets;          /* surviving unique shell quartets */
-    long long n_prim_quartets;     /* primitive quartets of the surviving shell quartets */
+    long long n_prim_quartets;     /* primitive quartets evaluated (after primitive screening) */
     long long n_quartets_unscreened;
     int n_class;                   /* shell-quartet classes */
     long long class_quartets[21];  /* per class (order of dsscf_class_l) */
@@ -130,6 +133,10 @@ int dsscf_get_stats(dsscf_ctx* ctx, dsscf_stats* st);
  * context's stream).  Timing helper for benchmarks. */
 int dsscf_time_jk(dsscf_ctx* ctx, int warmup, int iters, double* ms_per_build, double* ms_dominant_kernel,
                   int* dominant_class);
+
+/* Average device time of each class's kernel(s) over `iters` Fock builds
+ * (ms_per_class[dsscf_class_count()]). */
+int dsscf_time_jk_classes(dsscf_ctx* ctx, int iters, double* ms_per_class);
 
 /* ---------------------------------------------------------------- batches --
  * Fock builds of several prepared contexts (different molecules and bases,

@@ -25,10 +25,9 @@ const char* dsscf_job_error(dsscf_job* job);
 int dsscf_job_set_options(dsscf_job* job, int diis_size, double energy_threshold, int max_iterations,
                           int boys_mode, double schwarz_threshold, int verbose);
 
-/* Mixed precision: quartets with Schwarz bound product below fp32_threshold
- * run in FP32 (dsscf_options.fp32_threshold); 0 = plain FP64.  No reference
- * counterpart. */
-int dsscf_job_set_precision(dsscf_job* job, double fp32_threshold);
+/* Mixed precision and primitive screening (dsscf_options.fp32_threshold and
+ * .prim_threshold; 0 = off).  No reference counterpart. */
+int dsscf_job_set_precision(dsscf_job* job, double fp32_threshold, double prim_threshold);
 /* scf::assignMolecule (SCFBasis.cpp:17-37); coordinates in bohr. */
 int dsscf_job_set_molecule(dsscf_job* job, int natom, const int* Z, const double* xyz_bohr, int charge,
                            int multiplicity);

@@ -63,7 +63,6 @@ void dev_free(T*& p)
 // for the prepare/fold kernels, one MatSeg per context over stacked rows.
 struct Plan {
     int boys_mode = DSSCF_BOYS_ACCURATE;
-    double pthr = 0.0;  // primitive threshold on e_ab e_cd
     const double* btab = nullptr;
     int ncls = 0;
     std::vector<int> cls_nseg, cls_nblocks, cls_off, cls_bs, cls_maxN;
@@ -131,6 +130,7 @@ struct dsscf_ctx {
     double *d_Pt_pk = nullptr, *d_Pa_pk = nullptr, *d_J_pk = nullptr, *d_K_pk = nullptr;
 
     std::vector<int> block_size;  // per class, thread-per-quartet kernels (autotuned)
+    std::vector<int> chunk;       // per class, kets per work unit
     Plan plan;                    // fused launch plan over this context alone
 
     void free_geometry()
@@ -326,23 +326,28 @@ void build_pairs(dsscf_ctx* c)
 
 int pair_index(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
 
-// Primitive quartets of (bra|ket) that survive e_ab e_cd >= pthr (both
-// record lists sorted by e, largest first): two pointers.
-long long prim_count(const dsscf_ctx* c, int bra, int ket, double pthr)
+// Primitive screening: a primitive pair whose e times the largest e of any
+// primitive pair stays below pthr bounds every primitive quartet it could
+// enter (2 pi^(5/2) e_ab e_cd >= |(ab|cd)| for the s-type prefactor), so it
+// is removed from its shell pair's records once, at prepare time.
+void drop_primitive_pairs(dsscf_ctx* c, double pthr)
 {
-    const ShellPair& B = c->pairs[bra];
-    const ShellPair& K = c->pairs[ket];
-    if (pthr <= 0.0) return (long long)B.nprim * K.nprim;
-    long long n = 0;
-    int nb = B.nprim;
-    for (int k = 0; k < K.nprim; ++k) {
-        const double ek = c->prims[K.prim_off + k].e;
-        while (nb > 0 && c->prims[B.prim_off + nb - 1].e * ek < pthr) --nb;
-        if (nb == 0) break;
-        n += nb;
+    double emax = 0.0;
+    for (const PrimPair& p : c->prims) emax = std::max(emax, p.e);
+    std::vector<PrimPair> kept;
+    kept.reserve(c->prims.size());
+    for (ShellPair& P : c->pairs) {
+        const int off = (int)kept.size();
+        for (int i = 0; i < P.nprim; ++i) {
+            const PrimPair& r = c->prims[P.prim_off + i];
+            if (r.e * emax >= pthr) kept.push_back(r);
+        }
+        P.prim_off = off;
+        P.nprim = (int)kept.size() - off;
     }
-    return n;
+    c->prims.swap(kept);
 }
+
 
 }  // namespace
 
@@ -354,6 +359,7 @@ int dsscf_prepare(dsscf_ctx* c)
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->free_geometry();
     build_pairs(c);
+    if (c->opt.prim_threshold > 0.0) drop_primitive_pairs(c, c->opt.prim_threshold / kEriScale);
     const size_t npair = c->pairs.size();
     const int N = c->n_ao;
     if (int rc = dev_alloc(&c->d_pairs, npair)) return rc;
@@ -416,8 +422,7 @@ int dsscf_prepare(dsscf_ctx* c)
     const int ncls = dk_n_class();
     const double thr = c->opt.schwarz_threshold;
     const double thr32 = c->opt.fp32_threshold;
-    const double pthr = c->opt.prim_threshold / kEriScale;
-    constexpr int kChunk = 512;  // kets per work unit
+    c->chunk.assign(ncls, 512);
     std::vector<uint2> all;      // flat lists (wave-cooperative classes)
     std::vector<int4> units, units32;
     std::vector<unsigned> kets, k64, k32;
@@ -434,6 +439,10 @@ int dsscf_prepare(dsscf_ctx* c)
             const std::vector<int>& SY = by_class[Y];
             c->n_unscreened += X == Y ? (long long)SX.size() * (SX.size() + 1) / 2 : (long long)SX.size() * SY.size();
             const std::vector<int> GY = groups(SY);
+            // kets per work unit: enough units to fill the chip (~2048), 64..512, multiple of 64
+            const long long est = X == Y ? (long long)SX.size() * (SX.size() + 1) / 2 : (long long)SX.size() * SY.size();
+            const int kChunk = (int)std::min<long long>(512, std::max<long long>(64, (est / 2048 + 63) / 64 * 64));
+            c->chunk[cls] = kChunk;
             c->q_off[cls] = (long long)all.size();
             units32.clear();
             const long long u0 = (long long)units.size();
@@ -447,7 +456,7 @@ int dsscf_prepare(dsscf_ctx* c)
                         const int kj = SY[j];
                         const double bound = c->Q[bi] * c->Q[kj];
                         if (thr > 0.0 && bound < thr) break;  // Q sorted desc in a group
-                        const long long np = prim_count(c, bi, kj, pthr);
+                        const long long np = (long long)c->pairs[bi].nprim * c->pairs[kj].nprim;
                         if (split && bound < thr32) {
                             k32.push_back((unsigned)kj);
                             c->q_prim32[cls] += np;
@@ -508,7 +517,6 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     const int ncls = dk_n_class();
     p.ncls = ncls;
     p.boys_mode = ctxs[0]->opt.boys_mode;
-    p.pthr = ctxs[0]->opt.prim_threshold / kEriScale;
     p.btab = ctxs[0]->d_btab;
     const int nslot = 2 * ncls;  // slot 2*cls: FP64 list, 2*cls+1: FP32 list
     p.cls_nseg.assign(nslot, 0);
@@ -522,7 +530,12 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     for (int slot = 0; slot < nslot; ++slot) {
         const int cls = slot / 2, prec32 = slot % 2;
         const bool wave = dsscf_class_is_wave(cls) != 0;
-        const int b = wave ? 64 : (bs ? bs[cls] : (ctxs[0]->block_size.empty() ? 256 : ctxs[0]->block_size[cls]));
+        int b = wave ? 64 : (bs ? bs[cls] : (ctxs[0]->block_size.empty() ? 256 : ctxs[0]->block_size[cls]));
+        if (!wave) {  // no more threads per block than the largest work unit
+            int maxchunk = 64;
+            for (int k = 0; k < n; ++k) maxchunk = std::max(maxchunk, ctxs[k]->chunk.empty() ? 512 : ctxs[k]->chunk[cls]);
+            b = std::min(b, maxchunk);
+        }
         p.cls_bs[slot] = b;
         p.cls_off[slot] = (int)p.h_segs.size();
         long long blocks = 0;
@@ -600,7 +613,7 @@ int plan_launch_slot(Plan& p, int slot, hipStream_t s)
     if (p.cls_nseg[slot] == 0) return DSSCF_OK;
     HIP_TRY(dsscf_launch_jk(slot / 2, p.boys_mode, slot % 2, p.d_segs + p.cls_off[slot], p.d_blk0 + p.cls_off[slot],
                             p.cls_nseg[slot], p.cls_nblocks[slot], p.cls_bs[slot], p.cls_maxN[slot], p.btab,
-                            kEriScale, p.pthr, s));
+                            kEriScale, s));
     return DSSCF_OK;
 }
 

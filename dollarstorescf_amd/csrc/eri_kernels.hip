@@ -22,10 +22,9 @@ struct Quartet;
         static __device__ __forceinline__ void run(const ShellPair& bra, const ShellPair& ket,        \
                                                    const PrimPair* __restrict__ bra_pp,               \
                                                    const PrimPair* __restrict__ ket_pp,               \
-                                                   const double* __restrict__ btab, double pthr,      \
-                                                   double* out)                                       \
+                                                   const double* __restrict__ btab, double* out)      \
         {                                                                                             \
-            shell_quartet_##name<BM, R>(bra, ket, bra_pp, ket_pp, btab, pthr, out);                   \
+            shell_quartet_##name<BM, R>(bra, ket, bra_pp, ket_pp, btab, out);                         \
         }                                                                                             \
     };
 #define DSSCF_QUARTET_SPECIALIZE(a, b, c, d, name, wave) DSSCF_QSPEC_##wave(a, b, c, d, name)
@@ -54,7 +53,7 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // straight to global memory per quartet.
 template <int la, int lb, int lc, int ld, int BM, typename R>
 __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
-                                            int nseg, const double* __restrict__ btab, double scale, double pthr)
+                                            int nseg, const double* __restrict__ btab, double scale)
 {
     typedef ClassInfo<la, lb, lc, ld> CI;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
@@ -90,7 +89,7 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
         const int ki = (int)kets[u.y + t];
         const ShellPair ket = ldg(pairs + ki);
         double v[CI::NAO];
-        Quartet<la, lb, lc, ld, BM, R>::run(bra, ket, braP, S.pp + ket.prim_off, btab, pthr, v);
+        Quartet<la, lb, lc, ld, BM, R>::run(bra, ket, braP, S.pp + ket.prim_off, btab, v);
         const int c0 = ket.ao_a, d0 = ket.ao_b;
         const double g = gb * (c0 == d0 ? 0.5 : 1.0) * (ki == u.x ? 0.5 : 1.0);
 #define VIDX(i, j, k, l) ((((i) * NB + (j)) * NC + (k)) * ND + (l))
@@ -190,7 +189,7 @@ __global__ void __launch_bounds__(256) k_eri_block(const int4* __restrict__ item
     const ShellPair bra = pairs[it.x];
     const ShellPair ket = pairs[it.y];
     double v[CI::NAO];
-    Quartet<la, lb, lc, ld, BM, double>::run(bra, ket, pp + bra.prim_off, pp + ket.prim_off, btab, 0.0, v);
+    Quartet<la, lb, lc, ld, BM, double>::run(bra, ket, pp + bra.prim_off, pp + ket.prim_off, btab, v);
     double* o = out + (size_t)it.z;
 #pragma unroll
     for (int i = 0; i < CI::NAO; ++i) o[i] = scale * v[i];
@@ -212,7 +211,7 @@ __global__ void __launch_bounds__(256) k_schwarz(const int* __restrict__ plist, 
     const int pidx = plist[t];
     const ShellPair pr = pairs[pidx];
     double v[CI::NAO];
-    Quartet<la, lb, la, lb, BM, double>::run(pr, pr, pp + pr.prim_off, pp + pr.prim_off, btab, 0.0, v);
+    Quartet<la, lb, la, lb, BM, double>::run(pr, pr, pp + pr.prim_off, pp + pr.prim_off, btab, v);
     double mx = 0.0;
 #pragma unroll
     for (int i = 0; i < NA; ++i)
@@ -304,19 +303,19 @@ void dk_class_l(int c, int* l)
 template <int a, int b, int c, int d, int wave>
 struct Launch {
     static hipError_t jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int bs,
-                         int maxN, const double* btab, double scale, double pthr, hipStream_t s)
+                         int maxN, const double* btab, double scale, hipStream_t s)
     {
         const dim3 blk(bs), grd(nblocks);
         const size_t lds = sizeof(double) * (size_t)(ClassInfo<a, b, c, d>::NA + ClassInfo<a, b, c, d>::NB) * maxN;
         if (fp32)
             hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, float>), grd, blk, lds, s, segs, blk0, nseg, btab,
-                               scale, pthr);
+                               scale);
         else if (bm == BOYS_ACCURATE)
             hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
-                               scale, pthr);
+                               scale);
         else
             hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
-                               scale, pthr);
+                               scale);
         return hipGetLastError();
     }
     static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
@@ -351,17 +350,17 @@ struct Launch<a, b, c, d, 1> {
     {
         const int grd = n < (1 << 20) ? n : (1 << 20);
         hipLaunchKernelGGL((k_wave<a, b, c, d, BM, MODE>), dim3(grd), dim3(64), 0, s, items, n, pairs, pp, btab,
-                           nullptr, nullptr, 0, out, scale, 0.0);
+                           nullptr, nullptr, 0, out, scale);
     }
     static hipError_t jk(int bm, int, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int, int,
-                         const double* btab, double scale, double pthr, hipStream_t s)
+                         const double* btab, double scale, hipStream_t s)
     {
         if (bm == BOYS_ACCURATE)
             hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_ACCURATE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
-                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale, pthr);
+                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
         else
             hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_REFERENCE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
-                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale, pthr);
+                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
         return hipGetLastError();
     }
     static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
@@ -403,15 +402,14 @@ struct SchwarzIf<a, b, c, d, wave, true> {
 };
 
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                           int nblocks, int block_size, int maxN, const double* btab, double scale, double pthr,
-                           hipStream_t s)
+                           int nblocks, int block_size, int maxN, const double* btab, double scale, hipStream_t s)
 {
     if (nseg <= 0 || nblocks <= 0) return hipSuccess;
     int c = 0;
 #define DSSCF_LAUNCH_JK(a, b, cc, d, name, wave)                                                                 \
     if (c++ == cls)                                                                                              \
         return Launch<a, b, cc, d, wave>::jk(boys_mode, fp32, segs, blk0, nseg, nblocks, block_size, maxN, btab, \
-                                             scale, pthr, s);
+                                             scale, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_JK)
 #undef DSSCF_LAUNCH_JK
     return hipErrorInvalidValue;

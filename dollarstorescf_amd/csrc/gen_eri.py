@@ -327,17 +327,14 @@ def gen_class(c, spherical=True):
     o.append("template <int BM, typename R>")
     o.append("__device__ __forceinline__ void shell_quartet_%s(const ShellPair& bra, const ShellPair& ket,"
              " const PrimPair* __restrict__ bra_pp, const PrimPair* __restrict__ ket_pp,"
-             " const double* __restrict__ btab, double pthr, double* __restrict__ out) {" % name)
+             " const double* __restrict__ btab, double* __restrict__ out) {" % name)
     for (e, f) in accs:
         o.append("  double %s = 0.0;" % acc_name(e, f))
-    o.append("  const double emax_b = bra_pp[0].e;")
     o.append("  for (int ik = 0; ik < ket.nprim; ++ik) {")
     o.append("    const PrimPair Q = ldg(ket_pp + ik);")
-    o.append("    if (Q.e * emax_b < pthr) break;  // records sorted by e, largest first")
     o.append("#pragma unroll 2")
     o.append("    for (int ib = 0; ib < bra.nprim; ++ib) {")
     o.append("      const PrimPair P = bra_pp[ib];")
-    o.append("      if (P.e * Q.e < pthr) break;")
     o.append("      R Fm[%d];" % (L + 1))
     o.append("      PrimQuartet<R> g; prim_setup<BM, R, %d>(P, Q, bra, ket, btab, g, Fm);" % L)
     o.append("      const R PAx = g.PA[0], PAy = g.PA[1], PAz = g.PA[2];")

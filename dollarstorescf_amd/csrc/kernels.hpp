@@ -16,10 +16,8 @@ void dsscf_upload_xform(const double* x, int n, hipStream_t s);
 // quartet).
 // fp32: the FP32 form of a thread-per-quartet class (always accurate Boys).
 // maxN: largest AO count among the segments (sizes the LDS exchange rows).
-// pthr: primitive-quartet threshold on e_ab e_cd (0 = evaluate all).
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                           int nblocks, int block_size, int maxN, const double* btab, double scale, double pthr,
-                           hipStream_t s);
+                           int nblocks, int block_size, int maxN, const double* btab, double scale, hipStream_t s);
 int dsscf_class_is_wave(int cls);
 hipError_t dsscf_launch_eri_block(int cls, int boys_mode, const int4* items, int n, const ShellPair* pairs,
                                   const PrimPair* pp, const double* btab, double* out, double scale,

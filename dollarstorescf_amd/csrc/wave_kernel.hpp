@@ -24,7 +24,7 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
                                              const ShellPair* __restrict__ pairs_in,
                                              const PrimPair* __restrict__ pp_in, const double* __restrict__ btab,
                                              const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
-                                             int nseg, double* __restrict__ out, double scale, double pthr)
+                                             int nseg, double* __restrict__ out, double scale)
 {
     typedef WaveTab<la, lb, lc, ld> W;
     typedef ClassInfo<la, lb, lc, ld> CI;
@@ -76,10 +76,8 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
 
         for (int ib = 0; ib < bra.nprim; ++ib) {
             const PrimPair P = pp[bra.prim_off + ib];
-            if (P.e * pp[ket.prim_off].e < pthr) break;  // records sorted by e, largest first
             for (int ik = 0; ik < ket.nprim; ++ik) {
                 const PrimPair Q = pp[ket.prim_off + ik];
-                if (P.e * Q.e < pthr) break;
                 PrimQuartet<double> g;
                 double Fm[L + 1];
                 prim_setup<BM, double, L>(P, Q, bra, ket, btab, g, Fm);

@@ -79,38 +79,39 @@ def water_box(n: int, seed: int, box: float = 6.0, min_oo: float = 2.6, max_trie
 
 
 def alkane(n: int):
-    """All-trans CnH2n+2 in the xy plane zigzag with tetrahedral angles."""
+    """All-trans CnH2n+2: zigzag carbon backbone in the xy plane (C-C 1.54 A,
+    C-C-C 109.47 deg); each carbon's hydrogens (C-H 1.09 A) sit on the
+    tetrahedral directions left by its two chain neighbours, with virtual
+    neighbours continuing the zigzag past the chain ends (which gives the
+    terminal in-plane hydrogen)."""
     cc, ch = 1.54, 1.09
-    half = math.radians(109.4712206) / 2
-    dx, dy = cc * math.sin(half), cc * math.cos(half)
-    C = [np.array([i * dx, (i % 2) * dy, 0.0]) for i in range(n)]
-    Z, xyz = [], []
-    for p in C:
-        Z.append(6)
-        xyz.append(p)
-    # two out-of-plane H per carbon, bisecting away from the chain
-    for i, p in enumerate(C):
-        up = -1.0 if i % 2 else 1.0
-        side = np.array([0.0, -up * math.cos(half), 0.0])  # in-plane, away from neighbours
-        for sz in (+1, -1):
-            d = side * math.cos(math.radians(109.4712206) / 2) / max(abs(side[1]), 1e-12)
-            v = np.array([0.0, d[1] * math.cos(math.radians(54.75)), sz * math.sin(math.radians(54.75))])
-            v = v / np.linalg.norm(v) * ch
-            Z.append(1)
-            xyz.append(p + v)
-    # terminal H along the chain direction
+    tet = math.radians(109.4712206)
+    dx, dy = cc * math.sin(tet / 2), cc * math.cos(tet / 2)
+
+    def carbon(i):
+        return np.array([i * dx, (i % 2) * dy, 0.0])
+
     if n == 1:
-        Z, xyz = [6], [np.zeros(3)]
-        t = 1.0 / math.sqrt(3.0)
-        for s in [(1, 1, 1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1)]:
+        t = ch / math.sqrt(3.0)
+        xyz = [np.zeros(3)] + [np.array(v, float) * t for v in [(1, 1, 1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1)]]
+        return np.array([6, 1, 1, 1, 1], dtype=np.int32), np.array(xyz) * A_TO_A0
+    Z, xyz = [], []
+    for i in range(n):
+        Z.append(6)
+        xyz.append(carbon(i))
+    for i in range(n):
+        c = carbon(i)
+        b1 = carbon(i - 1) - c
+        b2 = carbon(i + 1) - c
+        u = -(b1 + b2)
+        u /= np.linalg.norm(u)
+        # the two out-of-plane H: H-C-H angle tetrahedral, bisected by u
+        for sz in (1.0, -1.0):
+            d = math.cos(tet / 2) * u + math.sin(tet / 2) * np.array([0.0, 0.0, sz])
             Z.append(1)
-            xyz.append(np.array(s, float) * t * ch)
-        return np.array(Z, dtype=np.int32), np.array(xyz) * A_TO_A0
-    for end, nb in ((0, 1), (n - 1, n - 2)):
-        d = C[end] - C[nb]
-        d[1] = -d[1] if False else d[1]
-        v = np.array([d[0], -d[1], 0.0])
-        v = v / np.linalg.norm(v) * ch
-        Z.append(1)
-        xyz.append(C[end] + v)
+            xyz.append(c + ch * d)
+        if i == 0 or i == n - 1:  # terminal: H where the next chain carbon would be
+            v = b1 if i == 0 else b2
+            Z.append(1)
+            xyz.append(c + ch * v / np.linalg.norm(v))
     return np.array(Z, dtype=np.int32), np.array(xyz) * A_TO_A0

@@ -45,16 +45,17 @@ typedef struct {
     double fp32_threshold;    /* quartets with Q_ab*Q_cd below it run their setup, small-T Boys and
                                  VRR in FP32 (contraction, HRR, J/K accumulation stay FP64); 0 = all
                                  FP64.  Needs DSSCF_BOYS_ACCURATE. */
-    double prim_threshold;    /* skip primitive quartets whose bound 2 pi^(5/2) e_ab e_cd falls below it,
-                                 e = w_a w_b exp(-ab/p |AB|^2) / p^(5/4) (no reference counterpart: the
-                                 reference evaluates every primitive quartet); 0 = none */
+    double prim_threshold;    /* drop primitive pairs with 2 pi^(5/2) e_ab e_max below it, where
+                                 e = w_a w_b exp(-ab/p |AB|^2) / p^(5/4) and e_max is the largest e:
+                                 they cannot lift any primitive quartet above it (no reference
+                                 counterpart: the reference evaluates every primitive); 0 = none */
 } dsscf_options;
 
 typedef struct {
     long long n_pairs;             /* unique shell pairs */
     long long n_prim_pairs;        /* primitive pairs over all shell pairs */
     long long n_quartets;          /* surviving unique shell quartets */
-    long long n_prim_quartets;     /* primitive quartets evaluated (after primitive screening) */
+    long long n_prim_quartets;     /* primitive quartets evaluated (after primitive-pair screening) */
     long long n_quartets_unscreened;
     int n_class;                   /* shell-quartet classes */
     long long class_quartets[21];  /* per class (order of dsscf_class_l) */

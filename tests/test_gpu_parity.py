@@ -151,3 +151,23 @@ def test_batch_matches_single_context(graph):
     J, K = b.jk([2 * p for p in dens], dens)
     for (Js, Ks), Jb, Kb in zip(singles, J, K):
         assert np.max(np.abs(Jb - Js)) < 1e-11
+
+
+def test_jk_primitive_screening_accurate():
+    """Default accurate mode screens primitive quartets (bound 1e-17): J/K stay
+    within 1e-11 (relative to max|J|) of the unscreened accurate-mode oracle."""
+    Z, xyz = G.water_box(3, seed=4)
+    job = ds.Job(0).set_options(boys_mode=ds.BOYS_ACCURATE, schwarz_threshold=0.0)
+    job.set_basis("6-31G").set_molecule(Z, xyz).construct()
+    st = job.context.stats()
+    n = job.n_ao
+    assert st.n_prim_quartets < sum(st.class_prim_quartets) + 1  # evaluated count reported
+    C = np.random.default_rng(8).standard_normal((n, 6))
+    Pa = O.pack(C @ C.T * 0.1)
+    job.set_density(Pa)
+    J, K = job.fock_build()
+    t = _oracle_basis("6-31G", Z, xyz).tensor(1)
+    Jr, Kr = O.coulomb(t, n, 2 * Pa), O.exchange(t, n, Pa)
+    scale = max(1.0, np.max(np.abs(Jr)))
+    assert np.max(np.abs(J - Jr)) < 1e-11 * scale
+    assert np.max(np.abs(K - Kr)) < 1e-11 * scale

@@ -32,7 +32,8 @@ class DsscfError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("boys_mode", ctypes.c_int), ("schwarz_threshold", ctypes.c_double),
-                ("fp32_threshold", ctypes.c_double), ("prim_threshold", ctypes.c_double)]
+                ("fp32_threshold", ctypes.c_double), ("prim_threshold", ctypes.c_double),
+                ("work_item_flops", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -41,7 +42,8 @@ class Stats(ctypes.Structure):
                 ("n_quartets_unscreened", ctypes.c_longlong), ("n_class", ctypes.c_int),
                 ("class_quartets", ctypes.c_longlong * 21), ("class_prim_quartets", ctypes.c_longlong * 21),
                 ("n_quartets_fp32", ctypes.c_longlong), ("class_quartets_fp32", ctypes.c_longlong * 21),
-                ("class_prim_quartets_fp32", ctypes.c_longlong * 21)]
+                ("class_prim_quartets_fp32", ctypes.c_longlong * 21), ("n_work_pairs", ctypes.c_longlong),
+                ("n_work_items", ctypes.c_longlong)]
 
 
 _LIB = None
@@ -138,8 +140,9 @@ class Context:
 
     __del__ = close
 
-    def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12, fp32_threshold=0.0, prim_threshold=0.0):
-        o = Options(boys_mode, schwarz_threshold, fp32_threshold, prim_threshold)
+    def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12, fp32_threshold=0.0, prim_threshold=0.0,
+                    work_item_flops=0.0):
+        o = Options(boys_mode, schwarz_threshold, fp32_threshold, prim_threshold, work_item_flops)
         _check(lib().dsscf_set_options(self._h, ctypes.byref(o)))
 
     def set_basis(self, l, nprim, centers, exps, coefs, xform):

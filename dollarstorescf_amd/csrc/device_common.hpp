@@ -63,12 +63,26 @@ struct JKSeg {
 
 // One context's matrices for the fused prepare/fold kernels; row0 is its
 // first row in the launch's stacked row range.
+// J and K are accumulated into kJKSlices copies of Jh/Kh (slice = the
+// accelerator complex die, XCD, the block runs on), which the fold sums.
+// Every J_cd element of a Fock build takes an atomic add from each bra it
+// meets, and those adds serialize per address; eight copies cut the adds per
+// address eightfold and keep each copy's lines with one XCD's L2.  The
+// atomics stay device-scope, so any block-to-slice map gives the same sums.
+constexpr int kJKSlices = 8;
+
+__device__ __forceinline__ int jk_slice()
+{
+    // s_getreg_b32 hwreg(HW_REG_XCC_ID, 0, 4): the XCD this wavefront runs on
+    return (int)__builtin_amdgcn_s_getreg((3 << 11) | 20) & (kJKSlices - 1);
+}
+
 struct MatSeg {
     const double* Ptp;  // packed-U total density
     const double* Pap;  // packed-U alpha density
     double* Dt;         // full N x N
     double* Da;
-    double* Jh;         // half-accumulated J, K (full N x N)
+    double* Jh;         // half-accumulated J, K (kJKSlices x full N x N)
     double* Kh;
     double* Jp;         // packed-U outputs
     double* Kp;

@@ -36,7 +36,24 @@ int fail(int code, const std::string& msg)
 // 2 pi^(5/2) with the reference's PI (GaussianIntegral.cpp:632, Constants.hpp:5).
 const double kEriScale = 2.0 * std::pow(3.14159265358979, 2.5);
 
+// Smallest primitive-pair run of a sub-pair.
+constexpr int kMinSubPrims = 5;
+
 int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Algorithmic FLOPs of one primitive quartet of class (la lb|lc ld): setup,
+// Boys, Head-Gordon-Pople VRR at 5 FLOPs per entry, contraction into the
+// HRR inputs (the model of dollarstorescf_amd/flops.py).
+double prim_quartet_flops(int la, int lb, int lc, int ld)
+{
+    const int L = la + lb + lc + ld;
+    long long vrr = -(L + 1), contr = 0;
+    for (int e = 0; e <= la + lb; ++e)
+        for (int f = 0; f <= lc + ld; ++f) vrr += (long long)ncart(e) * ncart(f) * (L - e - f + 1);
+    for (int e = la; e <= la + lb; ++e)
+        for (int f = lc; f <= lc + ld; ++f) contr += (long long)ncart(e) * ncart(f);
+    return 40.0 + (27 + 3 * L) + 10.0 * vrr + 2.0 * contr;
+}
 int pair_class_of(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }
 
 template <typename T>
@@ -89,6 +106,20 @@ struct Plan {
         built = false;
     }
 };
+// Fisher-Yates shuffle driven by splitmix64 (deterministic for a seed).
+template <typename T>
+void shuffle_seq(T* v, size_t n, unsigned long long seed)
+{
+    for (size_t i = n; i > 1; --i) {
+        seed += 0x9e3779b97f4a7c15ULL;
+        unsigned long long z = seed;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        z ^= z >> 31;
+        std::swap(v[i - 1], v[z % i]);
+    }
+}
+
 }  // namespace
 
 struct dsscf_ctx {
@@ -115,6 +146,7 @@ struct dsscf_ctx {
     // per class [u_off, u_off+u_cnt) FP64 and [u_off32, u_off32+u_cnt32) FP32
     std::vector<long long> u_off, u_cnt, u_off32, u_cnt32;
     long long n_unscreened = 0;
+    long long n_work_pairs = 0, n_work_items = 0;  // shell pairs + sub-pairs; (sub-)quartets
     bool prepared = false;
 
     // device
@@ -224,7 +256,8 @@ int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
     if (o->fp32_threshold > 0.0 && o->boys_mode != DSSCF_BOYS_ACCURATE)
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: FP32 quartets need the accurate Boys mode");
     const bool changed = o->boys_mode != c->opt.boys_mode || o->schwarz_threshold != c->opt.schwarz_threshold ||
-                         o->fp32_threshold != c->opt.fp32_threshold || o->prim_threshold != c->opt.prim_threshold;
+                         o->fp32_threshold != c->opt.fp32_threshold || o->prim_threshold != c->opt.prim_threshold ||
+                         o->work_item_flops != c->opt.work_item_flops;
     c->opt = *o;
     if (changed) c->prepared = false;
     return DSSCF_OK;
@@ -324,6 +357,9 @@ void build_pairs(dsscf_ctx* c)
         }
 }
 
+void shuffle_run(unsigned* v, size_t n, unsigned long long seed) { shuffle_seq(v, n, seed); }
+void shuffle_units(int4* v, size_t n, unsigned long long seed) { shuffle_seq(v, n, seed); }
+
 int pair_index(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
 
 // Primitive screening: a primitive pair whose e times the largest e of any
@@ -367,8 +403,10 @@ int dsscf_prepare(dsscf_ctx* c)
     if (int rc = dev_alloc(&c->d_Q, npair)) return rc;
     if (int rc = dev_alloc(&c->d_plist, npair)) return rc;
     const size_t nn = (size_t)N * N, np = (size_t)N * (N + 1) / 2;
-    for (double** p : {&c->d_Dt, &c->d_Da, &c->d_Jh, &c->d_Kh})
+    for (double** p : {&c->d_Dt, &c->d_Da})
         if (int rc = dev_alloc(p, nn)) return rc;
+    for (double** p : {&c->d_Jh, &c->d_Kh})  // one N x N copy per XCD slice
+        if (int rc = dev_alloc(p, nn * kJKSlices)) return rc;
     for (double** p : {&c->d_Pt_pk, &c->d_Pa_pk, &c->d_J_pk, &c->d_K_pk})
         if (int rc = dev_alloc(p, np)) return rc;
     HIP_TRY(hipMemcpyAsync(c->d_pairs, c->pairs.data(), sizeof(ShellPair) * npair, hipMemcpyHostToDevice, c->stream));
@@ -422,14 +460,48 @@ int dsscf_prepare(dsscf_ctx* c)
     const int ncls = dk_n_class();
     const double thr = c->opt.schwarz_threshold;
     const double thr32 = c->opt.fp32_threshold;
+    const double item_flops = c->opt.work_item_flops == 0.0 ? DSSCF_WORK_ITEM_FLOPS : c->opt.work_item_flops;
+
+    // Work pairs: the shell pairs (same indices), then sub-pairs.  A sub-pair
+    // is a shell pair restricted to a run of consecutive primitive pairs;
+    // the primitive sum of an ERI splits over such runs, so the sub-quartets
+    // of a shell quartet add up to it and digest into J and K independently.
+    std::vector<ShellPair> work(c->pairs);
+    std::vector<std::vector<std::pair<int, int>>> sub_cache(npair);  // (P, first work index)
+    auto sub_pairs = [&](int p, int P, std::vector<unsigned>& out) {
+        const ShellPair& sp = c->pairs[p];
+        const int m = (sp.nprim + P - 1) / P;
+        if (m <= 1) {
+            out.push_back((unsigned)p);
+            return;
+        }
+        int first = -1;
+        for (const auto& e : sub_cache[p])
+            if (e.first == P) first = e.second;
+        if (first < 0) {
+            first = (int)work.size();
+            sub_cache[p].push_back({P, first});
+            for (int k = 0; k < m; ++k) {  // m near-equal runs
+                const int lo = (int)((long long)sp.nprim * k / m), hi = (int)((long long)sp.nprim * (k + 1) / m);
+                ShellPair s = sp;
+                s.prim_off = sp.prim_off + lo;
+                s.nprim = hi - lo;
+                work.push_back(s);
+            }
+        }
+        for (int k = 0; k < m; ++k) out.push_back((unsigned)(first + k));
+    };
+
     c->chunk.assign(ncls, 512);
     std::vector<uint2> all;      // flat lists (wave-cooperative classes)
     std::vector<int4> units, units32;
-    std::vector<unsigned> kets, k64, k32;
+    std::vector<unsigned> kets, k64, k32, bsub, ksub;
     for (auto* v : {&c->q_off, &c->q_cnt, &c->q_prim, &c->q_off32, &c->q_cnt32, &c->q_prim32, &c->u_off, &c->u_cnt,
                     &c->u_off32, &c->u_cnt32})
         v->assign(ncls, 0);
     c->n_unscreened = 0;
+    c->n_work_items = 0;
+    auto by_cost = [](const int4& x, const int4& y) { return x.w > y.w; };
     for (int X = 0; X < npc; ++X)
         for (int Y = 0; Y <= X; ++Y) {
             const int cls = X * (X + 1) / 2 + Y;
@@ -443,6 +515,13 @@ int dsscf_prepare(dsscf_ctx* c)
             const long long est = X == Y ? (long long)SX.size() * (SX.size() + 1) / 2 : (long long)SX.size() * SY.size();
             const int kChunk = (int)std::min<long long>(512, std::max<long long>(64, (est / 2048 + 63) / 64 * 64));
             c->chunk[cls] = kChunk;
+            // primitive pairs per sub-pair, both sides: P^2 primitive quartets ~ item_flops,
+            // and at least kMinSubPrims (shorter runs cost more in per-item digestion and
+            // J/K atomics than they gain in balance)
+            int l4[4];
+            dk_class_l(cls, l4);
+            const double fq = prim_quartet_flops(l4[0], l4[1], l4[2], l4[3]);
+            const int P = item_flops < 0.0 ? (1 << 30) : std::max(kMinSubPrims, (int)std::sqrt(item_flops / fq));
             c->q_off[cls] = (long long)all.size();
             units32.clear();
             const long long u0 = (long long)units.size();
@@ -472,23 +551,63 @@ int dsscf_prepare(dsscf_ctx* c)
                 }
                 c->q_cnt[cls] += (long long)k64.size();
                 c->q_cnt32[cls] += (long long)k32.size();
+                bsub.clear();
+                sub_pairs(bi, P, bsub);
                 for (int pass = 0; pass < 2; ++pass) {
                     const std::vector<unsigned>& kk = pass ? k32 : k64;
+                    if (kk.empty()) continue;
                     std::vector<int4>& uu = pass ? units32 : units;
-                    for (size_t o = 0; o < kk.size(); o += kChunk) {
-                        const int cnt = (int)std::min<size_t>(kChunk, kk.size() - o);
-                        uu.push_back(make_int4(bi, (int)(kets.size() + o), cnt, 0));
+                    ksub.clear();
+                    for (unsigned kj : kk) sub_pairs((int)kj, P, ksub);
+                    // equal primitive counts side by side: uniform loop trips in a wavefront
+                    std::stable_sort(ksub.begin(), ksub.end(),
+                                     [&](unsigned x, unsigned y) { return work[x].nprim > work[y].nprim; });
+                    // Every bra's kets would otherwise come in nearly the same order, so the
+                    // blocks running at one time would add into the same J_cd elements at
+                    // the same moment; a per-bra shuffle inside each run of equal primitive
+                    // counts spreads those atomics over the matrix.
+                    for (size_t r0 = 0; r0 < ksub.size();) {
+                        size_t r1 = r0 + 1;
+                        while (r1 < ksub.size() && work[ksub[r1]].nprim == work[ksub[r0]].nprim) ++r1;
+                        shuffle_run(ksub.data() + r0, r1 - r0, 0x9e3779b97f4a7c15ULL * (unsigned long long)(bi + 1) + pass);
+                        r0 = r1;
                     }
-                    kets.insert(kets.end(), kk.begin(), kk.end());
+                    const int k0 = (int)kets.size();
+                    kets.insert(kets.end(), ksub.begin(), ksub.end());
+                    for (unsigned sb : bsub) {
+                        c->n_work_items += (long long)ksub.size();
+                        for (size_t o = 0; o < ksub.size(); o += kChunk) {
+                            const int cnt = (int)std::min<size_t>(kChunk, ksub.size() - o);
+                            long long cost = 0;  // primitive quartets of the unit
+                            for (int t = 0; t < cnt; ++t) cost += work[ksub[o + t]].nprim;
+                            cost *= work[sb].nprim;
+                            uu.push_back(make_int4((int)sb, k0 + (int)o, cnt, (int)std::min<long long>(cost, 0x7fffffff)));
+                        }
+                    }
                 }
             }
             if (wave) c->q_cnt[cls] = (long long)all.size() - c->q_off[cls];
+            // heaviest units first, so that the longest blocks start at once; units of
+            // equal cost (often chunks of one bra, which flush the same K rows) in
+            // shuffled order
+            shuffle_units(units.data() + u0, units.size() - (size_t)u0, 0x5851f42d4c957f2dULL + cls);
+            shuffle_units(units32.data(), units32.size(), 0x14057b7ef767814fULL + cls);
+            std::stable_sort(units.begin() + u0, units.end(), by_cost);
+            std::stable_sort(units32.begin(), units32.end(), by_cost);
             c->u_off[cls] = u0;
             c->u_cnt[cls] = (long long)units.size() - u0;
             c->u_off32[cls] = (long long)units.size();
             c->u_cnt32[cls] = (long long)units32.size();
             units.insert(units.end(), units32.begin(), units32.end());
         }
+    if (work.size() > 0x7fffffffULL) return fail(DSSCF_ERR_UNSUPPORTED, "too many sub-pairs");
+    if (work.size() > npair) {  // the JK kernels index the work pairs; Schwarz and ERI batches the first npair
+        dev_free(c->d_pairs);
+        if (int rc = dev_alloc(&c->d_pairs, work.size())) return rc;
+        HIP_TRY(hipMemcpyAsync(c->d_pairs, work.data(), sizeof(ShellPair) * work.size(), hipMemcpyHostToDevice,
+                               c->stream));
+    }
+    c->n_work_pairs = (long long)work.size();
     if (kets.size() > 0xffffffffULL) return fail(DSSCF_ERR_UNSUPPORTED, "too many quartets");
     if (int rc = dev_alloc(&c->d_units, units.size())) return rc;
     if (int rc = dev_alloc(&c->d_kets, kets.size())) return rc;
@@ -831,6 +950,8 @@ int dsscf_get_stats(dsscf_ctx* c, dsscf_stats* st)
         st->n_prim_quartets += c->q_prim[k] + c->q_prim32[k];
         st->n_quartets_fp32 += c->q_cnt32[k];
     }
+    st->n_work_pairs = c->n_work_pairs;
+    st->n_work_items = c->n_work_items;
     return DSSCF_OK;
 }
 
@@ -1001,9 +1122,8 @@ int dsscf_batch_set(dsscf_batch* b, dsscf_ctx* const* ctxs, int n)
     for (int i = 0; i < n; ++i) {
         if (!ctxs[i] || !ctxs[i]->prepared) return fail(DSSCF_ERR_STATE, "dsscf_batch_set: context not prepared");
         if (ctxs[i]->device != b->device) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: contexts on another device");
-        if (ctxs[i]->opt.boys_mode != ctxs[0]->opt.boys_mode ||
-            ctxs[i]->opt.prim_threshold != ctxs[0]->opt.prim_threshold)
-            return fail(DSSCF_ERR_ARG, "dsscf_batch_set: mixed Boys modes or primitive thresholds in one batch");
+        if (ctxs[i]->opt.boys_mode != ctxs[0]->opt.boys_mode)
+            return fail(DSSCF_ERR_ARG, "dsscf_batch_set: mixed Boys modes in one batch");
         for (int k = 0; k < i; ++k)
             if (ctxs[k] == ctxs[i]) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: context listed twice");
     }

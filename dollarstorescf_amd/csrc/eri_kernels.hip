@@ -66,10 +66,11 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     const PrimPair* __restrict__ pp = S.pp;
     const GPtr<double> Dt = gbl(S.Dt);
     const GPtr<double> Da = gbl(S.Da);  // read-only in this kernel
-    double* __restrict__ Jh = S.Jh;
-    double* __restrict__ Kh = S.Kh;
-    const GPtr<unsigned> kets = gbl(S.k);
     const int N = S.N;
+    const size_t slice = (size_t)jk_slice() * N * N;
+    double* __restrict__ Jh = S.Jh + slice;
+    double* __restrict__ Kh = S.Kh + slice;
+    const GPtr<unsigned> kets = gbl(S.k);
     const ShellPair bra = ldg(pairs + u.x);
     const int a0 = bra.ao_a, b0 = bra.ao_b;
     for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;
@@ -91,7 +92,9 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
         double v[CI::NAO];
         Quartet<la, lb, lc, ld, BM, R>::run(bra, ket, braP, S.pp + ket.prim_off, btab, v);
         const int c0 = ket.ao_a, d0 = ket.ao_b;
-        const double g = gb * (c0 == d0 ? 0.5 : 1.0) * (ki == u.x ? 0.5 : 1.0);
+        // (ab|ab): the bra and ket (sub-)pairs belong to one shell pair
+        const bool same = ket.ao_a == a0 && ket.ao_b == b0;
+        const double g = gb * (c0 == d0 ? 0.5 : 1.0) * (same ? 0.5 : 1.0);
 #define VIDX(i, j, k, l) ((((i) * NB + (j)) * NC + (k)) * ND + (l))
         // Coulomb
 #pragma unroll
@@ -243,9 +246,13 @@ __global__ void k_prep(const MatSeg* __restrict__ segs, int nseg)
 {
     const MatSeg& S = segs[find_segment_by_row(segs, nseg, (int)blockIdx.y)];
     const int x = (int)blockIdx.y - S.row0, N = S.N;
+    const size_t NN = (size_t)N * N;
     for (int y = threadIdx.x + blockIdx.x * blockDim.x; y < N; y += gridDim.x * blockDim.x) {
-        S.Jh[(size_t)x * N + y] = 0.0;
-        S.Kh[(size_t)x * N + y] = 0.0;
+#pragma unroll
+        for (int s = 0; s < kJKSlices; ++s) {
+            S.Jh[s * NN + (size_t)x * N + y] = 0.0;
+            S.Kh[s * NN + (size_t)x * N + y] = 0.0;
+        }
         if (y <= x) {
             const size_t o = (size_t)x * (x + 1) / 2 + y;
             const double t = S.Ptp[o], a = S.Pap[o];
@@ -257,15 +264,23 @@ __global__ void k_prep(const MatSeg* __restrict__ segs, int nseg)
     }
 }
 
-// J = Jh + Jh^T, K = Kh + Kh^T -> packed-U, per context.
+// J = sum over slices of Jh + Jh^T, likewise K -> packed-U, per context.
 __global__ void k_fold(const MatSeg* __restrict__ segs, int nseg)
 {
     const MatSeg& S = segs[find_segment_by_row(segs, nseg, (int)blockIdx.y)];
     const int x = (int)blockIdx.y - S.row0, N = S.N;
+    const size_t NN = (size_t)N * N;
     for (int y = threadIdx.x + blockIdx.x * blockDim.x; y <= x; y += gridDim.x * blockDim.x) {
         const size_t o = (size_t)x * (x + 1) / 2 + y;
-        S.Jp[o] = S.Jh[(size_t)y * N + x] + S.Jh[(size_t)x * N + y];
-        S.Kp[o] = S.Kh[(size_t)y * N + x] + S.Kh[(size_t)x * N + y];
+        const size_t yx = (size_t)y * N + x, xy = (size_t)x * N + y;
+        double j = 0.0, k = 0.0;
+#pragma unroll
+        for (int s = 0; s < kJKSlices; ++s) {
+            j += S.Jh[s * NN + yx] + S.Jh[s * NN + xy];
+            k += S.Kh[s * NN + yx] + S.Kh[s * NN + xy];
+        }
+        S.Jp[o] = j;
+        S.Kp[o] = k;
     }
 }
 

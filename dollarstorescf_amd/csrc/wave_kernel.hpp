@@ -183,9 +183,10 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
         } else {
             const double* __restrict__ Dt = S->Dt;
             const double* __restrict__ Da = S->Da;
-            double* __restrict__ Jh = S->Jh;
-            double* __restrict__ Kh = S->Kh;
             const int N = S->N;
+            const size_t slice = (size_t)jk_slice() * N * N;
+            double* __restrict__ Jh = S->Jh + slice;
+            double* __restrict__ Kh = S->Kh + slice;
             double g = 1.0;
             if (bra.ao_a == bra.ao_b) g *= 0.5;
             if (ket.ao_a == ket.ao_b) g *= 0.5;

@@ -49,7 +49,15 @@ typedef struct {
                                  e = w_a w_b exp(-ab/p |AB|^2) / p^(5/4) and e_max is the largest e:
                                  they cannot lift any primitive quartet above it (no reference
                                  counterpart: the reference evaluates every primitive); 0 = none */
+    double work_item_flops;   /* a shell pair with many primitive pairs is evaluated as sub-pairs over
+                                 consecutive ranges of its primitive pairs, so that one thread's
+                                 share of a shell quartet stays near this many FLOPs of primitive
+                                 work, with runs of at least 5 primitive pairs (the sub-quartets
+                                 add up to the shell quartet); 0 = default
+                                 (DSSCF_WORK_ITEM_FLOPS), < 0 = never split */
 } dsscf_options;
+
+#define DSSCF_WORK_ITEM_FLOPS 16384.0
 
 typedef struct {
     long long n_pairs;             /* unique shell pairs */
@@ -63,6 +71,8 @@ typedef struct {
     long long n_quartets_fp32;     /* of n_quartets, evaluated in FP32 */
     long long class_quartets_fp32[21];
     long long class_prim_quartets_fp32[21];
+    long long n_work_pairs;        /* shell pairs plus the sub-pairs of split ones (work_item_flops) */
+    long long n_work_items;        /* (sub-)quartets the thread-per-quartet kernels evaluate */
 } dsscf_stats;
 
 /* Last error message of the calling thread ("" if none). */

@@ -58,23 +58,25 @@ struct JKSeg {
     int n;                     // quartets (flat list) or units
     int N;
     int blk0;
-    int pad;
+    int nslice;                // copies of Jh/Kh (power of two, <= kJKSlicesMax)
 };
 
 // One context's matrices for the fused prepare/fold kernels; row0 is its
 // first row in the launch's stacked row range.
-// J and K are accumulated into kJKSlices copies of Jh/Kh (slice = the
-// accelerator complex die, XCD, the block runs on), which the fold sums.
-// Every J_cd element of a Fock build takes an atomic add from each bra it
-// meets, and those adds serialize per address; eight copies cut the adds per
-// address eightfold and keep each copy's lines with one XCD's L2.  The
-// atomics stay device-scope, so any block-to-slice map gives the same sums.
-constexpr int kJKSlices = 8;
+// J and K are accumulated into up to kJKSlicesMax copies of Jh/Kh (per
+// context, a power of two), which the fold sums.  Every J_cd element takes an
+// atomic add from each bra it meets, and adds to one address serialize; the
+// copies divide the adds per address, and a copy is picked by the XCD the
+// wavefront runs on (then by a block-index bit), so that each copy's lines
+// stay with few L2s.  The atomics stay device-scope: any block-to-copy map
+// gives the same sums.
+constexpr int kJKSlicesMax = 16;
 
-__device__ __forceinline__ int jk_slice()
+__device__ __forceinline__ int jk_slice(int nslice)
 {
     // s_getreg_b32 hwreg(HW_REG_XCC_ID, 0, 4): the XCD this wavefront runs on
-    return (int)__builtin_amdgcn_s_getreg((3 << 11) | 20) & (kJKSlices - 1);
+    const int xcc = (int)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7;
+    return (xcc + 8 * (((int)blockIdx.x >> 3) & 1)) & (nslice - 1);
 }
 
 struct MatSeg {
@@ -82,12 +84,14 @@ struct MatSeg {
     const double* Pap;  // packed-U alpha density
     double* Dt;         // full N x N
     double* Da;
-    double* Jh;         // half-accumulated J, K (kJKSlices x full N x N)
+    double* Jh;         // half-accumulated J, K (nslice x full N x N)
     double* Kh;
     double* Jp;         // packed-U outputs
     double* Kp;
     int N;
     int row0;
+    int nslice;
+    int pad;
 };
 
 // Read-only global-memory view of a pointer that arrived through memory (a

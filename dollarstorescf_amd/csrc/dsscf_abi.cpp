@@ -78,6 +78,8 @@ void dev_free(T*& p)
 // A fused Fock-build launch over one or more contexts on one device: per
 // class, a segment table (one JKSeg per context with quartets of the class);
 // for the prepare/fold kernels, one MatSeg per context over stacked rows.
+constexpr int kStreams = 4;  // streams per fused Fock build (GPU_MAX_HW_QUEUES defaults to 4)
+
 struct Plan {
     int boys_mode = DSSCF_BOYS_ACCURATE;
     const double* btab = nullptr;
@@ -93,6 +95,14 @@ struct Plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool built = false;
+    // Class kernels are independent (they only add into J and K), so they run
+    // on kStreams streams at once: the latency-bound tails of small classes
+    // then overlap the large ones.  cls_stream: stream of each slot (0 = the
+    // caller's), balanced by cls_cost; side streams and fork/join events.
+    std::vector<double> cls_cost;
+    std::vector<int> cls_stream;
+    hipStream_t side[kStreams - 1] = {};
+    hipEvent_t fork = nullptr, join[kStreams - 1] = {};
 
     void release()
     {
@@ -100,6 +110,14 @@ struct Plan {
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
+        for (int i = 0; i < kStreams - 1; ++i) {
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+            if (join[i]) (void)hipEventDestroy(join[i]);
+            side[i] = nullptr;
+            join[i] = nullptr;
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        fork = nullptr;
         dev_free(d_segs);
         dev_free(d_blk0);
         dev_free(d_mat);
@@ -147,6 +165,7 @@ struct dsscf_ctx {
     std::vector<long long> u_off, u_cnt, u_off32, u_cnt32;
     long long n_unscreened = 0;
     long long n_work_pairs = 0, n_work_items = 0;  // shell pairs + sub-pairs; (sub-)quartets
+    int n_slice = 1;                                // copies of Jh/Kh
     bool prepared = false;
 
     // device
@@ -405,8 +424,6 @@ int dsscf_prepare(dsscf_ctx* c)
     const size_t nn = (size_t)N * N, np = (size_t)N * (N + 1) / 2;
     for (double** p : {&c->d_Dt, &c->d_Da})
         if (int rc = dev_alloc(p, nn)) return rc;
-    for (double** p : {&c->d_Jh, &c->d_Kh})  // one N x N copy per XCD slice
-        if (int rc = dev_alloc(p, nn * kJKSlices)) return rc;
     for (double** p : {&c->d_Pt_pk, &c->d_Pa_pk, &c->d_J_pk, &c->d_K_pk})
         if (int rc = dev_alloc(p, np)) return rc;
     HIP_TRY(hipMemcpyAsync(c->d_pairs, c->pairs.data(), sizeof(ShellPair) * npair, hipMemcpyHostToDevice, c->stream));
@@ -608,6 +625,18 @@ int dsscf_prepare(dsscf_ctx* c)
                                c->stream));
     }
     c->n_work_pairs = (long long)work.size();
+    {
+        // J/K copies: about one per 32 atomic adds that each element of J would
+        // otherwise take in a build, as a power of two up to kJKSlicesMax
+        long long items = c->n_work_items;
+        for (int k = 0; k < ncls; ++k)
+            if (dsscf_class_is_wave(k)) items += c->q_cnt[k];
+        const double per_elem = (double)items / std::max<double>(1.0, (double)N * (N + 1) / 2);
+        c->n_slice = 1;
+        while (c->n_slice < kJKSlicesMax && c->n_slice * 32.0 < per_elem) c->n_slice *= 2;
+    }
+    for (double** p : {&c->d_Jh, &c->d_Kh})
+        if (int rc = dev_alloc(p, nn * c->n_slice)) return rc;
     if (kets.size() > 0xffffffffULL) return fail(DSSCF_ERR_UNSUPPORTED, "too many quartets");
     if (int rc = dev_alloc(&c->d_units, units.size())) return rc;
     if (int rc = dev_alloc(&c->d_kets, kets.size())) return rc;
@@ -643,6 +672,8 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     p.cls_off.assign(nslot, 0);
     p.cls_bs.assign(nslot, 256);
     p.cls_maxN.assign(nslot, 0);
+    p.cls_cost.assign(nslot, 0.0);
+    p.cls_stream.assign(nslot, 0);
     p.h_segs.clear();
     p.h_blk0.clear();
     p.h_mat.clear();
@@ -675,9 +706,11 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
             sg.Da = c->d_Da;
             sg.Jh = c->d_Jh;
             sg.Kh = c->d_Kh;
+            sg.nslice = c->n_slice;
             sg.n = (int)cnt;
             sg.N = c->n_ao;
             sg.blk0 = (int)blocks;
+            p.cls_cost[slot] += (double)(prec32 ? c->q_prim32[cls] : c->q_prim[cls]);
             blocks += cnt;  // one block per quartet (wave classes) or per work unit
             if (blocks > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "launch grid too large");
             p.h_segs.push_back(sg);
@@ -686,11 +719,30 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
         p.cls_nseg[slot] = (int)p.h_segs.size() - p.cls_off[slot];
         p.cls_nblocks[slot] = (int)blocks;
         p.cls_maxN[slot] = slot_maxN;
+        {
+            int l4[4];
+            dk_class_l(cls, l4);
+            p.cls_cost[slot] *= prim_quartet_flops(l4[0], l4[1], l4[2], l4[3]) * (prec32 ? 0.5 : 1.0);
+        }
         if (!wave && p.cls_nseg[slot]) {
             int l4[4];
             dk_class_l(cls, l4);
             const size_t lds = sizeof(double) * (size_t)(2 * l4[0] + 2 * l4[1] + 2) * slot_maxN;
             if (lds > 64 * 1024) return fail(DSSCF_ERR_UNSUPPORTED, "basis too large for the LDS exchange rows");
+        }
+    }
+    {  // slots onto streams, heaviest first onto the least loaded
+        std::vector<int> order(nslot);
+        for (int i = 0; i < nslot; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return p.cls_cost[x] > p.cls_cost[y]; });
+        double load[kStreams] = {};
+        for (int slot : order) {
+            if (p.cls_nseg[slot] == 0) continue;
+            int best = 0;
+            for (int k = 1; k < kStreams; ++k)
+                if (load[k] < load[best]) best = k;
+            p.cls_stream[slot] = best;
+            load[best] += p.cls_cost[slot];
         }
     }
     int row = 0, maxN = 0;
@@ -706,6 +758,7 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
         m.Jp = dJ[k];
         m.Kp = dK[k];
         m.N = c->n_ao;
+        m.nslice = c->n_slice;
         m.row0 = row;
         row += c->n_ao;
         maxN = std::max(maxN, c->n_ao);
@@ -742,14 +795,41 @@ int plan_launch_class(Plan& p, int cls, hipStream_t s)
     return plan_launch_slot(p, 2 * cls + 1, s);
 }
 
-// prepare (unpack densities, zero accumulators), all classes, fold.
+// prepare (unpack densities, zero accumulators), all classes, fold.  With
+// ev (per-class timing) every class runs on s in turn; otherwise the classes
+// fork onto the plan's streams and join before the fold.
 int plan_enqueue(Plan& p, hipStream_t s, std::vector<hipEvent_t>* ev)
 {
     HIP_TRY(dsscf_launch_prep(p.d_mat, p.nmat, p.total_rows, p.maxN, s));
-    if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
-    for (int cls = 0; cls < p.ncls; ++cls) {
-        if (int rc = plan_launch_class(p, cls, s)) return rc;
-        if (ev) HIP_TRY(hipEventRecord((*ev)[cls + 1], s));
+    if (ev) {
+        HIP_TRY(hipEventRecord((*ev)[0], s));
+        for (int cls = 0; cls < p.ncls; ++cls) {
+            if (int rc = plan_launch_class(p, cls, s)) return rc;
+            HIP_TRY(hipEventRecord((*ev)[cls + 1], s));
+        }
+    } else {
+        if (!p.fork) {
+            HIP_TRY(hipEventCreateWithFlags(&p.fork, hipEventDisableTiming));
+            for (int i = 0; i < kStreams - 1; ++i) {
+                HIP_TRY(hipStreamCreateWithFlags(&p.side[i], hipStreamNonBlocking));
+                HIP_TRY(hipEventCreateWithFlags(&p.join[i], hipEventDisableTiming));
+            }
+        }
+        bool used[kStreams] = {};
+        for (int slot = 0; slot < 2 * p.ncls; ++slot)
+            if (p.cls_nseg[slot] > 0) used[p.cls_stream[slot]] = true;
+        HIP_TRY(hipEventRecord(p.fork, s));
+        for (int i = 1; i < kStreams; ++i)
+            if (used[i]) HIP_TRY(hipStreamWaitEvent(p.side[i - 1], p.fork, 0));
+        for (int slot = 0; slot < 2 * p.ncls; ++slot) {
+            const int k = p.cls_stream[slot];
+            if (int rc = plan_launch_slot(p, slot, k ? p.side[k - 1] : s)) return rc;
+        }
+        for (int i = 1; i < kStreams; ++i)
+            if (used[i]) {
+                HIP_TRY(hipEventRecord(p.join[i - 1], p.side[i - 1]));
+                HIP_TRY(hipStreamWaitEvent(s, p.join[i - 1], 0));
+            }
     }
     HIP_TRY(dsscf_launch_fold(p.d_mat, p.nmat, p.total_rows, p.maxN, s));
     return DSSCF_OK;

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     const GPtr<double> Dt = gbl(S.Dt);
     const GPtr<double> Da = gbl(S.Da);  // read-only in this kernel
     const int N = S.N;
-    const size_t slice = (size_t)jk_slice() * N * N;
+    const size_t slice = (size_t)jk_slice(S.nslice) * N * N;
     double* __restrict__ Jh = S.Jh + slice;
     double* __restrict__ Kh = S.Kh + slice;
     const GPtr<unsigned> kets = gbl(S.k);
@@ -248,8 +248,7 @@ __global__ void k_prep(const MatSeg* __restrict__ segs, int nseg)
     const int x = (int)blockIdx.y - S.row0, N = S.N;
     const size_t NN = (size_t)N * N;
     for (int y = threadIdx.x + blockIdx.x * blockDim.x; y < N; y += gridDim.x * blockDim.x) {
-#pragma unroll
-        for (int s = 0; s < kJKSlices; ++s) {
+        for (int s = 0; s < S.nslice; ++s) {
             S.Jh[s * NN + (size_t)x * N + y] = 0.0;
             S.Kh[s * NN + (size_t)x * N + y] = 0.0;
         }
@@ -274,8 +273,7 @@ __global__ void k_fold(const MatSeg* __restrict__ segs, int nseg)
         const size_t o = (size_t)x * (x + 1) / 2 + y;
         const size_t yx = (size_t)y * N + x, xy = (size_t)x * N + y;
         double j = 0.0, k = 0.0;
-#pragma unroll
-        for (int s = 0; s < kJKSlices; ++s) {
+        for (int s = 0; s < S.nslice; ++s) {
             j += S.Jh[s * NN + yx] + S.Jh[s * NN + xy];
             k += S.Kh[s * NN + yx] + S.Kh[s * NN + xy];
         }

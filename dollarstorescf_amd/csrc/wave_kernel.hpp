@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             const double* __restrict__ Dt = S->Dt;
             const double* __restrict__ Da = S->Da;
             const int N = S->N;
-            const size_t slice = (size_t)jk_slice() * N * N;
+            const size_t slice = (size_t)jk_slice(S->nslice) * N * N;
             double* __restrict__ Jh = S->Jh + slice;
             double* __restrict__ Kh = S->Kh + slice;
             double g = 1.0;

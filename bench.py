@@ -4,8 +4,8 @@ Workload: 8 rigid waters on a seeded 2x2x2 lattice (3.0 A spacing, +-0.3 A
 jitter, random orientations), RHF/6-31G, 104 AOs, s/p shells only, Schwarz
 threshold 1e-12, FP64 throughout.  A step is one Fock build: every surviving
 unique shell quartet evaluated (Boys, OS/HGP VRR, HRR) and digested into J
-and K, with the density resident in HBM.  The density is the SCF density
-after up to 20 iterations of the same job.
+and K, with the density resident in HBM.  The density is the converged SCF density of the same
+job (at most 30 iterations).
 
 Prints one JSON line (metric, value = Fock builds/s, roofline of the dominant
 class kernel, CPU baseline of the reference's own ERI routine).
@@ -49,6 +49,19 @@ def cpu_baseline(Z, xyz, n_ao, n_unique_ao_quartets, sample, seed=7):
                       f"electronRepulsionIntegral (GaussianIntegral.cpp:561), {t:.1f} s on 1 core, "
                       f"extrapolated to {n_unique_ao_quartets} unique AO quartets per Fock build; "
                       f"J/K tensor lookups excluded"}
+
+
+def traffic_of(kernel: str):
+    """HBM bytes per dispatch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (each counter scaled by what it reads
+    for a device copy of known size), or None when the summary lacks it."""
+    path = os.path.join(ROOT, "profiles", "fock_water8_631g_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if rec is None else float(rec["hbm_bytes"])
 
 
 def copy_calibration(mb: int, reps: int = 5):
@@ -110,6 +123,8 @@ def main():
     Pa = job.get("Pa")
     ctx.set_density(2.0 * Pa, Pa)
 
+    classes = ds.class_list()
+    kernel_ms = np.zeros(len(classes))
     for _ in range(a.warmup):
         ctx.jk_resident()
     t0 = time.perf_counter()
@@ -118,10 +133,21 @@ def main():
     dt = time.perf_counter() - t0
     ms = dt / a.steps * 1e3
     builds_per_s = a.steps / dt
+    # the same steps again with every class kernel (they run concurrently on the
+    # context's streams) bracketed by HIP events on its own stream
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.jk_resident(kernel_ms)
+    ms_instrumented = (time.perf_counter() - t0) / a.steps * 1e3
+    kernel_ms /= a.steps
 
-    # dominant kernel: per-class durations from HIP events on the context stream
-    ms_event, ms_dom, dom = ctx.time_jk(warmup=1, iters=max(5, min(a.steps, 20)))
-    classes = ds.class_list()
+    # dominant kernel and its roofline: the same builds once more with the class
+    # kernels launched one after another on one stream, each bracketed by HIP
+    # events (concurrent kernels share the CUs, so their event spans include
+    # waiting for one another)
+    serial_ms = ctx.time_classes(iters=a.steps)[:len(classes)]
+    dom = int(np.argmax(serial_ms))
+    ms_dom = float(serial_ms[dom])
     cq = list(st.class_quartets)[:len(classes)]
     cp = list(st.class_prim_quartets)[:len(classes)]
     dom_flops = flops.class_flops(classes[dom], cq[dom], cp[dom])
@@ -129,10 +155,13 @@ def main():
     achieved = dom_flops / (ms_dom * 1e-3) / 1e12
     name = "".join("spdf"[l] for l in classes[dom])
 
-    cls_ms = ctx.time_classes(iters=10)
-    per_class = [{"class": "".join("spdf"[l] for l in c), "quartets": int(q), "prim_quartets": int(pq),
-                  "ms": float(m), "tflops": (flops.class_flops(c, q, pq) / (m * 1e-3) / 1e12) if m > 0 else 0.0}
-                 for c, q, pq, m in zip(classes, cq, cp, cls_ms) if q > 0]
+    def class_rows(ms_of):
+        return [{"class": "".join("spdf"[l] for l in c), "quartets": int(q), "prim_quartets": int(pq),
+                 "ms": float(m), "tflops": (flops.class_flops(c, q, pq) / (m * 1e-3) / 1e12) if m > 0 else 0.0}
+                for c, q, pq, m in zip(classes, cq, cp, ms_of) if q > 0]
+
+    per_class = class_rows(kernel_ms)
+    per_class_serial = class_rows(serial_ms)
     n_unique_ao = (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2
     cpu = None if a.no_cpu else cpu_baseline(Z, xyz, n, n_unique_ao, a.cpu_sample)
 
@@ -144,6 +173,7 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": ms,
+        "ms_per_step_event_timed": ms_instrumented,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -158,11 +188,15 @@ def main():
         "eri_prim_quartets_per_s": st.n_prim_quartets * builds_per_s,
         "fock_build_tflops": total_flops / (ms * 1e-3) / 1e12,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_of(f"k_jk<{name}>"),
                      "kernel": f"k_jk<{name}>", "kernel_ms": ms_dom,
                      "note": "FP64 vector-ALU bound; peak is MI355X FP64 78.6 TFLOP/s; achieved = algorithmic "
-                             "FLOPs of the class (dollarstorescf_amd/flops.py) / HIP-event kernel time"},
-        "classes": per_class,
+                             "FLOPs of the class (dollarstorescf_amd/flops.py) / its kernel's average duration "
+                             "(HIP events around it over --steps builds whose class kernels run one after "
+                             "another on one stream, right after the timed steps); traffic = HBM bytes per "
+                             "dispatch from profiles/fock_water8_631g_traffic.json"},
+        "classes": per_class,  # event spans with the classes running concurrently (4 streams)
+        "classes_serial": per_class_serial,  # one stream, classes in turn (the roofline's timing)
         "cpu_baseline": cpu,
         "scf": {"energy": energy, "iterations": iters, "converged": converged, "seconds": t_scf,
                 "construct_seconds": t_construct},

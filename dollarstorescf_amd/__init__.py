@@ -68,6 +68,7 @@ def lib():
             "dsscf_jk": [p, p, p, p, p], "dsscf_jk_device": [p, p, p, p, p],
             "dsscf_eri_batch": [p, p, ctypes.c_longlong, p], "dsscf_schwarz": [p, p],
             "dsscf_get_stats": [p, p], "dsscf_set_density": [p, p, p], "dsscf_jk_resident": [p],
+            "dsscf_jk_resident_timed": [p, p],
             "dsscf_get_jk": [p, p, p], "dsscf_time_jk_classes": [p, i, p], "dsscf_time_jk": [p, i, i, p, p, p], "dsscf_class_l": [i, p],
             "dsscf_job_create": [p, i], "dsscf_job_destroy": [p],
             "dsscf_job_set_options": [p, i, d, i, i, d, i], "dsscf_job_set_precision": [p, d, d],
@@ -166,8 +167,14 @@ class Context:
         self._Pa = np.ascontiguousarray(Pa_packed, np.float64)
         _check(lib().dsscf_set_density(self._h, _p(self._Pt), _p(self._Pa)))
 
-    def jk_resident(self):
-        _check(lib().dsscf_jk_resident(self._h))
+    def jk_resident(self, kernel_ms=None):
+        """One Fock build on the resident density; with kernel_ms (float64 array of
+        class_count() entries) each class kernel's event-timed duration is added to it."""
+        if kernel_ms is None:
+            _check(lib().dsscf_jk_resident(self._h))
+        else:
+            assert kernel_ms.dtype == np.float64 and kernel_ms.size >= lib().dsscf_class_count()
+            _check(lib().dsscf_jk_resident_timed(self._h, _p(kernel_ms)))
 
     def get_jk(self, n_ao):
         J, K = np.empty(n_ao * (n_ao + 1) // 2), np.empty(n_ao * (n_ao + 1) // 2)

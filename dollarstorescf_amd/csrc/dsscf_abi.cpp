@@ -103,6 +103,7 @@ struct Plan {
     std::vector<int> cls_stream;
     hipStream_t side[kStreams - 1] = {};
     hipEvent_t fork = nullptr, join[kStreams - 1] = {};
+    std::vector<hipEvent_t> slot_ev;  // start/end of each slot's kernel (timed builds)
 
     void release()
     {
@@ -118,6 +119,8 @@ struct Plan {
         }
         if (fork) (void)hipEventDestroy(fork);
         fork = nullptr;
+        for (hipEvent_t e : slot_ev) (void)hipEventDestroy(e);
+        slot_ev.clear();
         dev_free(d_segs);
         dev_free(d_blk0);
         dev_free(d_mat);
@@ -795,10 +798,24 @@ int plan_launch_class(Plan& p, int cls, hipStream_t s)
     return plan_launch_slot(p, 2 * cls + 1, s);
 }
 
+// The plan's side streams and fork/join events (created once, outside any
+// stream capture).
+int plan_streams(Plan& p)
+{
+    if (p.fork) return DSSCF_OK;
+    HIP_TRY(hipEventCreateWithFlags(&p.fork, hipEventDisableTiming));
+    for (int i = 0; i < kStreams - 1; ++i) {
+        HIP_TRY(hipStreamCreateWithFlags(&p.side[i], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&p.join[i], hipEventDisableTiming));
+    }
+    return DSSCF_OK;
+}
+
 // prepare (unpack densities, zero accumulators), all classes, fold.  With
 // ev (per-class timing) every class runs on s in turn; otherwise the classes
-// fork onto the plan's streams and join before the fold.
-int plan_enqueue(Plan& p, hipStream_t s, std::vector<hipEvent_t>* ev)
+// fork onto the plan's streams and join before the fold, and with timed each
+// slot's kernel is bracketed by p.slot_ev on the stream it runs on.
+int plan_enqueue(Plan& p, hipStream_t s, std::vector<hipEvent_t>* ev, bool timed = false)
 {
     HIP_TRY(dsscf_launch_prep(p.d_mat, p.nmat, p.total_rows, p.maxN, s));
     if (ev) {
@@ -808,22 +825,23 @@ int plan_enqueue(Plan& p, hipStream_t s, std::vector<hipEvent_t>* ev)
             HIP_TRY(hipEventRecord((*ev)[cls + 1], s));
         }
     } else {
-        if (!p.fork) {
-            HIP_TRY(hipEventCreateWithFlags(&p.fork, hipEventDisableTiming));
-            for (int i = 0; i < kStreams - 1; ++i) {
-                HIP_TRY(hipStreamCreateWithFlags(&p.side[i], hipStreamNonBlocking));
-                HIP_TRY(hipEventCreateWithFlags(&p.join[i], hipEventDisableTiming));
-            }
-        }
+        if (int rc = plan_streams(p)) return rc;
         bool used[kStreams] = {};
         for (int slot = 0; slot < 2 * p.ncls; ++slot)
             if (p.cls_nseg[slot] > 0) used[p.cls_stream[slot]] = true;
         HIP_TRY(hipEventRecord(p.fork, s));
         for (int i = 1; i < kStreams; ++i)
             if (used[i]) HIP_TRY(hipStreamWaitEvent(p.side[i - 1], p.fork, 0));
+        if (timed && p.slot_ev.empty()) {
+            p.slot_ev.assign(4 * p.ncls, nullptr);
+            for (hipEvent_t& e : p.slot_ev) HIP_TRY(hipEventCreate(&e));
+        }
         for (int slot = 0; slot < 2 * p.ncls; ++slot) {
             const int k = p.cls_stream[slot];
-            if (int rc = plan_launch_slot(p, slot, k ? p.side[k - 1] : s)) return rc;
+            hipStream_t st = k ? p.side[k - 1] : s;
+            if (timed && p.cls_nseg[slot]) HIP_TRY(hipEventRecord(p.slot_ev[2 * slot], st));
+            if (int rc = plan_launch_slot(p, slot, st)) return rc;
+            if (timed && p.cls_nseg[slot]) HIP_TRY(hipEventRecord(p.slot_ev[2 * slot + 1], st));
         }
         for (int i = 1; i < kStreams; ++i)
             if (used[i]) {
@@ -853,6 +871,7 @@ int plan_run(Plan& p, hipStream_t s, bool use_graph, std::vector<float>* cls_ms)
     }
     if (!use_graph) return plan_enqueue(p, s, nullptr);
     if (!p.gexec) {
+        if (int rc = plan_streams(p)) return rc;
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         const int rc = plan_enqueue(p, s, nullptr);
         hipGraph_t g = nullptr;
@@ -879,10 +898,12 @@ int ctx_plan(dsscf_ctx* c)
     return plan_build(c->plan, &c, 1, &pt, &pa, &j, &k, nullptr, c->stream);
 }
 
+// One Fock build on the context's stream: a replay of the captured build
+// (per-class timing runs the launches eagerly instead).
 int run_jk(dsscf_ctx* c, std::vector<float>* cls_ms)
 {
     if (int rc = ctx_plan(c)) return rc;
-    return plan_run(c->plan, c->stream, false, cls_ms);
+    return plan_run(c->plan, c->stream, cls_ms == nullptr, cls_ms);
 }
 
 }  // namespace
@@ -936,6 +957,24 @@ int dsscf_jk_resident(dsscf_ctx* c)
     HIP_TRY(hipSetDevice(c->device));
     if (int rc = run_jk(c, nullptr)) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return DSSCF_OK;
+}
+
+int dsscf_jk_resident_timed(dsscf_ctx* c, double* kernel_ms)
+{
+    if (!c || !kernel_ms) return fail(DSSCF_ERR_ARG, "dsscf_jk_resident_timed: null argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_jk_resident_timed: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = ctx_plan(c)) return rc;
+    Plan& p = c->plan;
+    if (int rc = plan_enqueue(p, c->stream, nullptr, true)) return rc;  // eager: events need it
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int slot = 0; slot < 2 * p.ncls; ++slot) {
+        if (!p.cls_nseg[slot]) continue;
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, p.slot_ev[2 * slot], p.slot_ev[2 * slot + 1]));
+        kernel_ms[slot / 2] += ms;
+    }
     return DSSCF_OK;
 }
 

@@ -127,6 +127,11 @@ int dsscf_jk_device(dsscf_ctx* ctx, const double* d_Ptot, const double* d_Palpha
 int dsscf_set_density(dsscf_ctx* ctx, const double* Ptot, const double* Palpha);
 int dsscf_jk_resident(dsscf_ctx* ctx);
 int dsscf_get_jk(dsscf_ctx* ctx, double* J, double* K);
+/* dsscf_jk_resident that also adds each class kernel's duration in ms (HIP
+ * events on the stream the kernel runs on; the build's launches are made
+ * directly rather than replayed from its captured graph; FP64 and FP32 lists
+ * summed) to kernel_ms[class] (dsscf_class_count() entries, not cleared). */
+int dsscf_jk_resident_timed(dsscf_ctx* ctx, double* kernel_ms);
 
 /* (ij|kl) for n AO quartets ijkl[4*t..4*t+3], written to out[t].  The unit
  * API flo::electronRepulsionIntegral(mu, sigma, nu, tau) = (mu nu|sigma tau)

@@ -83,7 +83,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=5_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--boys", choices=["accurate", "reference"], default="accurate")
     ap.add_argument("--copy-mb", type=int, default=0,

@@ -77,6 +77,7 @@ def lib():
             "dsscf_job_ao": [p, i, p, p, p, p, p, p, p, p, p, i], "dsscf_job_n_ao": [p],
             "dsscf_job_get": [p, c, p], "dsscf_job_set_density": [p, p], "dsscf_job_fock_build": [p],
             "dsscf_job_solve": [p, p, p], "dsscf_job_energies": [p, p], "dsscf_job_history": [p, p, i],
+            "dsscf_syev": [i, p, p, p], "dsscf_jobs_construct": [p, i], "dsscf_jobs_solve": [p, i, i, i, p, p, p],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -107,6 +108,37 @@ def _check(rc, msg_fn=None):
 
 def basis_path(name: str) -> str:
     return os.path.join(BASIS_DIR, name)
+
+
+def construct_jobs(jobs):
+    """Job.construct for several jobs, the host parts in parallel."""
+    arr = (ctypes.c_void_p * len(jobs))(*[j._h.value for j in jobs])
+    rc = lib().dsscf_jobs_construct(arr, len(jobs))
+    if rc:
+        bad = [j._err() for j in jobs if j._err()]
+        raise DsscfError(rc, bad[0] if bad else "dsscf_jobs_construct failed")
+    return jobs
+
+
+def solve_jobs(jobs, use_graph=True, autotune=True):
+    """Job.solve for several constructed jobs in lock step, their Fock builds
+    fused into one launch per quartet class.  Returns (energies, iterations,
+    status) arrays; status 0 = converged, else the job's error code."""
+    n = len(jobs)
+    arr = (ctypes.c_void_p * n)(*[j._h.value for j in jobs])
+    e, it, st = np.zeros(n), np.zeros(n, np.int32), np.zeros(n, np.int32)
+    lib().dsscf_jobs_solve(arr, n, 1 if use_graph else 0, 1 if autotune else 0, _p(e), _p(it), _p(st))
+    return e, it, st
+
+
+def syev(A):
+    """Eigenvalues (ascending) and eigenvectors (columns) of a symmetric matrix
+    with the SCF layer's host eigensolver."""
+    A = np.ascontiguousarray(A, np.float64)
+    n = A.shape[0]
+    w, V = np.empty(n), np.empty((n, n))
+    _check(lib().dsscf_syev(n, _p(A), _p(w), _p(V)))
+    return w, V
 
 
 def class_list():

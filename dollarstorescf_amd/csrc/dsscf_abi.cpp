@@ -714,7 +714,8 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
             sg.N = c->n_ao;
             sg.blk0 = (int)blocks;
             p.cls_cost[slot] += (double)(prec32 ? c->q_prim32[cls] : c->q_prim[cls]);
-            blocks += cnt;  // one block per quartet (wave classes) or per work unit
+            // one block per work unit, or per dsscf_wave_group quartets (wave classes)
+            blocks += wave ? (cnt + dsscf_wave_group(cls) - 1) / dsscf_wave_group(cls) : cnt;
             if (blocks > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "launch grid too large");
             p.h_segs.push_back(sg);
             p.h_blk0.push_back(sg.blk0);

@@ -292,6 +292,14 @@ int dk_xform_size() { return DSSCF_XFORM_SIZE; }
 int dk_max_l() { return DSSCF_MAXL; }
 
 namespace {
+template <int a, int b, int c, int d, int wave>
+struct WaveGroupIf {
+    static constexpr int G = 1;
+};
+template <int a, int b, int c, int d>
+struct WaveGroupIf<a, b, c, d, 1> {
+    static constexpr int G = WaveGroup<a, b, c, d>::G;
+};
 struct ClassDesc {
     int la, lb, lc, ld;
 };
@@ -361,7 +369,9 @@ struct Launch<a, b, c, d, 1> {
     static void go(const void* items, int n, const ShellPair* pairs, const PrimPair* pp, const double* btab,
                    double* out, double scale, hipStream_t s)
     {
-        const int grd = n < (1 << 20) ? n : (1 << 20);
+        constexpr int G = WaveGroup<a, b, c, d>::G;
+        const int ng = (n + G - 1) / G;
+        const int grd = ng < (1 << 20) ? ng : (1 << 20);
         hipLaunchKernelGGL((k_wave<a, b, c, d, BM, MODE>), dim3(grd), dim3(64), 0, s, items, n, pairs, pp, btab,
                            nullptr, nullptr, 0, out, scale);
     }
@@ -426,6 +436,16 @@ hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, 
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_JK)
 #undef DSSCF_LAUNCH_JK
     return hipErrorInvalidValue;
+}
+
+int dsscf_wave_group(int cls)
+{
+    static const int g[] = {
+#define DSSCF_WAVE_G(a, b, cc, d, name, wave) WaveGroupIf<a, b, cc, d, wave>::G,
+        DSSCF_FOR_EACH_CLASS(DSSCF_WAVE_G)
+#undef DSSCF_WAVE_G
+    };
+    return (cls >= 0 && cls < (int)(sizeof(g) / sizeof(g[0]))) ? g[cls] : 1;
 }
 
 int dsscf_class_is_wave(int cls)

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 
+#include "multi.hpp"
 #include "scf.hpp"
 
 struct dsscf_job {
@@ -232,5 +233,58 @@ int dsscf_job_history(dsscf_job* j, double* out, int cap)
 }
 
 dsscf_ctx* dsscf_job_context(dsscf_job* j) { return j ? j->job.ctx : nullptr; }
+
+int dsscf_jobs_construct(dsscf_job* const* jobs, int n)
+{
+    if (!jobs || n <= 0) return DSSCF_ERR_ARG;
+    std::vector<scf::Job*> js;
+    for (int i = 0; i < n; ++i) {
+        if (!jobs[i]) return DSSCF_ERR_ARG;
+        if (!jobs[i]->have_basis || !jobs[i]->job.molecule.size())
+            return set_err(jobs[i], DSSCF_ERR_STATE, "molecule and basis must be set first");
+        js.push_back(&jobs[i]->job);
+    }
+    const int rc = scf::constructTogether(js);
+    for (scf::Job* j : js)
+        if (j->ctx) scf::useCoreGuess(*j);
+    return rc;
+}
+
+int dsscf_jobs_solve(dsscf_job* const* jobs, int n, int use_graph, int autotune, double* energies, int* iterations,
+                     int* status)
+{
+    if (!jobs || n <= 0) return DSSCF_ERR_ARG;
+    std::vector<scf::Job*> js;
+    for (int i = 0; i < n; ++i) {
+        if (!jobs[i]) return DSSCF_ERR_ARG;
+        if (!jobs[i]->job.ctx) return set_err(jobs[i], DSSCF_ERR_STATE, "dsscf_jobs_solve: job not constructed");
+        for (int k = 0; k < i; ++k)
+            if (jobs[k] == jobs[i]) return DSSCF_ERR_ARG;
+        js.push_back(&jobs[i]->job);
+    }
+    dsscf_batch* b = nullptr;
+    if (int rc = dsscf_batch_create(&b, js[0]->opt.device)) return rc;
+    dsscf_batch_set_graph(b, use_graph);
+    std::vector<int> st;
+    const int rc = scf::solveTogether(js, b, autotune != 0, &st);
+    dsscf_batch_destroy(b);
+    for (int i = 0; i < n; ++i) {
+        if (energies) energies[i] = st[i] ? 0.0 : js[i]->energies.total();
+        if (iterations) iterations[i] = (int)js[i]->iteration;
+        if (status) status[i] = st[i];
+    }
+    return rc;
+}
+
+int dsscf_syev(int n, const double* A, double* w, double* V)
+{
+    if (n < 0 || (n > 0 && (!A || !w || !V))) return DSSCF_ERR_ARG;
+    la::Mat M(A, A + (size_t)n * n), Vm;
+    std::vector<double> wv;
+    const bool ok = la::syev(M, n, wv, Vm);
+    std::memcpy(w, wv.data(), sizeof(double) * n);
+    std::memcpy(V, Vm.data(), sizeof(double) * n * n);
+    return ok ? DSSCF_OK : DSSCF_ERR_NOCONV;
+}
 
 }  // extern "C"

@@ -46,64 +46,146 @@ void transpose(const Mat& A, Mat& C, int n)
         for (int j = 0; j < n; ++j) C[(size_t)j * n + i] = A[(size_t)i * n + j];
 }
 
-// Cyclic Jacobi: sweep over all (p,q) pairs, zeroing a_pq with the rotation
-// t = sgn(theta)/(|theta| + sqrt(theta^2+1)), theta = (a_qq - a_pp)/(2 a_pq),
-// c = 1/sqrt(1+t^2), s = t c, until the off-diagonal norm falls below
-// 1e-30 of the total.  Rotations accumulate into V.
+// Symmetric eigensolver in two stages (Golub & Van Loan, "Matrix
+// Computations", 8.3):
+//  1. Householder tridiagonalization T = Q^T A Q: for column k a reflector
+//     H = I - 2 v v^T (|v| = 1) maps A[k+1:, k] onto a multiple of e_1;
+//     with p = A v and w = 2 (p - (v^T p) v), H A H = A - v w^T - w v^T on the
+//     trailing block.  Q = H_0 H_1 ... is accumulated afterwards.
+//  2. Implicit symmetric QR steps with the Wilkinson shift on the unreduced
+//     blocks of T, each step a chase of Givens rotations (the first from the
+//     shifted first column, the rest zeroing the bulge); the rotations are
+//     applied to the eigenvector rows Z (row j = eigenvector j), which start
+//     as the columns of Q.  Off-diagonals with |e_k| <= eps (|d_k|+|d_k+1|)
+//     are set to zero (deflation).
 bool syev(const Mat& A0, int n, std::vector<double>& w, Mat& V)
 {
-    Mat A = A0;
+    w.assign(n, 0.0);
     V.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    if (n == 0) return true;
+    Mat A = A0;
     auto a = [&](int i, int j) -> double& { return A[(size_t)i * n + j]; };
-    double total = 0.0;
-    for (double x : A) total += x * x;
-    bool converged = n <= 1;
-    for (int sweep = 0; sweep < 100 && !converged; ++sweep) {
-        double off = 0.0;
-        for (int p = 0; p < n; ++p)
-            for (int q = p + 1; q < n; ++q) off += 2.0 * a(p, q) * a(p, q);
-        if (off <= 1e-30 * total) {
-            converged = true;
+    std::vector<double> refl((size_t)n * n, 0.0);  // row k: reflector of step k (entries k+1..n-1)
+    std::vector<double> p(n), wv(n);
+    for (int k = 0; k + 2 < n; ++k) {
+        double nx = 0.0;
+        for (int i = k + 1; i < n; ++i) nx += a(i, k) * a(i, k);
+        nx = std::sqrt(nx);
+        if (nx == 0.0) continue;
+        const double alpha = a(k + 1, k) > 0.0 ? -nx : nx;
+        double* v = &refl[(size_t)k * n];
+        double nv = 0.0;
+        for (int i = k + 1; i < n; ++i) {
+            v[i] = a(i, k) - (i == k + 1 ? alpha : 0.0);
+            nv += v[i] * v[i];
+        }
+        nv = std::sqrt(nv);
+        if (nv == 0.0) continue;
+        for (int i = k + 1; i < n; ++i) v[i] /= nv;
+        double vp = 0.0;
+        for (int i = k + 1; i < n; ++i) {
+            double s = 0.0;
+            const double* ai = &A[(size_t)i * n];
+            for (int j = k + 1; j < n; ++j) s += ai[j] * v[j];
+            p[i] = s;
+            vp += v[i] * s;
+        }
+        for (int i = k + 1; i < n; ++i) wv[i] = 2.0 * (p[i] - vp * v[i]);
+        for (int i = k + 1; i < n; ++i) {
+            double* ai = &A[(size_t)i * n];
+            const double vi = v[i], wi = wv[i];
+            for (int j = k + 1; j < n; ++j) ai[j] -= vi * wv[j] + wi * v[j];
+        }
+        a(k + 1, k) = a(k, k + 1) = alpha;
+        for (int i = k + 2; i < n; ++i) a(i, k) = a(k, i) = 0.0;
+    }
+    std::vector<double> d(n), e(n, 0.0);
+    for (int i = 0; i < n; ++i) d[i] = a(i, i);
+    for (int i = 0; i + 1 < n; ++i) e[i] = a(i + 1, i);
+    // Q = H_0 H_1 ... H_{n-3}, built as Z <- Z H_k from Z = I (each H_k acts on
+    // columns k+1..n-1), then transposed so that row j of Z is column j of Q.
+    // The QR rotations below act on rows of Z: at the end row j is the
+    // eigenvector of d[j].
+    Mat& Z = V;
+    for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+    for (int k = 0; k + 2 < n; ++k) {
+        const double* v = &refl[(size_t)k * n];
+        bool any = false;
+        for (int i = k + 1; i < n; ++i) any = any || v[i] != 0.0;
+        if (!any) continue;
+        for (int j = 0; j < n; ++j) {
+            double* zj = &Z[(size_t)j * n];
+            double s = 0.0;
+            for (int i = k + 1; i < n; ++i) s += zj[i] * v[i];
+            s *= 2.0;
+            if (s != 0.0)
+                for (int i = k + 1; i < n; ++i) zj[i] -= s * v[i];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) std::swap(Z[(size_t)i * n + j], Z[(size_t)j * n + i]);
+
+    const double eps = 2.220446049250313e-16;
+    auto rotate_rows = [&](int k, double c, double s) {  // z_k, z_k+1 <- G [z_k; z_k+1]
+        double* zk = &Z[(size_t)k * n];
+        double* zl = &Z[(size_t)(k + 1) * n];
+        for (int i = 0; i < n; ++i) {
+            const double x = zk[i], y = zl[i];
+            zk[i] = c * x + s * y;
+            zl[i] = -s * x + c * y;
+        }
+    };
+    bool ok = true;
+    int hi = n - 1;
+    int iters = 0;
+    while (hi > 0) {
+        for (int k = 0; k < hi; ++k)
+            if (std::fabs(e[k]) <= eps * (std::fabs(d[k]) + std::fabs(d[k + 1]))) e[k] = 0.0;
+        if (e[hi - 1] == 0.0) {  // d[hi] converged
+            --hi;
+            iters = 0;
+            continue;
+        }
+        int lo = hi - 1;
+        while (lo > 0 && e[lo - 1] != 0.0) --lo;
+        if (++iters > 60) {
+            ok = false;
             break;
         }
-        for (int p = 0; p < n - 1; ++p)
-            for (int q = p + 1; q < n; ++q) {
-                const double apq = a(p, q);
-                if (std::fabs(apq) < 1e-300) continue;
-                const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
-                const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-                for (int k = 0; k < n; ++k) {  // columns p, q
-                    const double akp = a(k, p), akq = a(k, q);
-                    a(k, p) = c * akp - s * akq;
-                    a(k, q) = s * akp + c * akq;
-                }
-                for (int k = 0; k < n; ++k) {  // rows p, q
-                    const double apk = a(p, k), aqk = a(q, k);
-                    a(p, k) = c * apk - s * aqk;
-                    a(q, k) = s * apk + c * aqk;
-                }
-                a(p, q) = a(q, p) = 0.0;
-                for (int k = 0; k < n; ++k) {
-                    double* vk = &V[(size_t)k * n];
-                    const double vp = vk[p], vq = vk[q];
-                    vk[p] = c * vp - s * vq;
-                    vk[q] = s * vp + c * vq;
-                }
+        // Wilkinson shift from the trailing 2x2 of the block [lo, hi]
+        const double dl = (d[hi - 1] - d[hi]) * 0.5, eh = e[hi - 1];
+        const double mu = d[hi] - eh * eh / (dl + (dl >= 0.0 ? 1.0 : -1.0) * std::hypot(dl, eh));
+        double x = d[lo] - mu, z = e[lo], bulge = 0.0;
+        for (int k = lo; k < hi; ++k) {
+            const double r = std::hypot(x, z);
+            const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : z / r;
+            // T <- G T G^T on rows/cols (k, k+1), G = [[c, s], [-s, c]]
+            if (k > lo) e[k - 1] = r;  // the rotation folded the bulge into e[k-1]
+            const double dk = d[k], dk1 = d[k + 1], ek = e[k];
+            d[k] = c * c * dk + 2.0 * c * s * ek + s * s * dk1;
+            d[k + 1] = s * s * dk - 2.0 * c * s * ek + c * c * dk1;
+            e[k] = c * s * (dk1 - dk) + (c * c - s * s) * ek;
+            if (k + 1 < hi) {
+                const double ek1 = e[k + 1];
+                bulge = s * ek1;
+                e[k + 1] = c * ek1;
             }
+            rotate_rows(k, c, s);
+            x = e[k];
+            z = bulge;
+        }
     }
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
-    w.resize(n);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
     Mat Vs((size_t)n * n);
     for (int j = 0; j < n; ++j) {
-        w[j] = a(order[j], order[j]);
-        for (int i = 0; i < n; ++i) Vs[(size_t)i * n + j] = V[(size_t)i * n + order[j]];
+        w[j] = d[order[j]];
+        const double* zj = &Z[(size_t)order[j] * n];
+        for (int i = 0; i < n; ++i) Vs[(size_t)i * n + j] = zj[i];
     }
     V.swap(Vs);
-    return converged;
+    return ok;
 }
 
 bool solve(std::vector<double> A, std::vector<double>& b, int n)

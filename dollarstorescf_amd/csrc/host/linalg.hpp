@@ -18,10 +18,10 @@ void gemm(const Mat& A, const Mat& B, Mat& C, int n);
 // C = A^T
 void transpose(const Mat& A, Mat& C, int n);
 
-// Eigen-decomposition of a symmetric matrix by cyclic Jacobi rotations.
-// Eigenvalues ascending in w, eigenvectors in the columns of V (dspev_('V','U')
-// semantics, src/lalib/Lapack.cpp:12-39).  Returns false if the sweeps do
-// not converge.
+// Eigen-decomposition of a symmetric matrix (Householder tridiagonalization,
+// implicit QR with Wilkinson shifts).  Eigenvalues ascending in w,
+// eigenvectors in the columns of V (dspev_('V','U') semantics,
+// src/lalib/Lapack.cpp:12-39).  Returns false if the QR steps do not converge.
 bool syev(const Mat& A, int n, std::vector<double>& w, Mat& V);
 
 // Solve A x = b for a small dense system by Gaussian elimination with

@@ -1,0 +1,21 @@
+// multi.hpp -- several SCF jobs on one device (see multi.cpp).
+#pragma once
+#include <vector>
+
+#include "scf.hpp"
+
+namespace scf {
+
+// constructBasisHost for every job in parallel (OpenMP), then constructDevice
+// in order.  Returns the first failure's code (each job keeps its message).
+int constructTogether(const std::vector<Job*>& jobs);
+
+// solveMOs + calculateEnergies for every job, in lock step: each round one
+// fused Fock build (dsscf_batch_jk) over the jobs still iterating or needing
+// their energy build, then every job's host step in parallel.  A job that
+// converges or fails leaves the fused set.  With autotune the batch's block
+// sizes are tuned on the first fused set.  status[i] = 0 or the job's error
+// code.  Returns the first failure's code.
+int solveTogether(const std::vector<Job*>& jobs, dsscf_batch* batch, bool autotune, std::vector<int>* status);
+
+}  // namespace scf

@@ -283,8 +283,14 @@ int fockBuild(Job& job)
 // E_elec = 2 sum eps_occ - tr(Pa G).
 int runSCFIteration(Job& job)
 {
-    const int n = job.n;
     if (int rc = fockBuild(job)) return rc;
+    return scfStep(job);
+}
+
+// The host part of an iteration, with job.J and job.K at the current density.
+int scfStep(Job& job)
+{
+    const int n = job.n;
     for (size_t i = 0; i < job.G.size(); ++i) job.G[i] = job.J[i] - job.K[i];
     std::vector<double> F0p(job.G.size());
     for (size_t i = 0; i < F0p.size(); ++i) F0p[i] = job.H[i] + job.G[i];
@@ -335,6 +341,12 @@ int runSCFIteration(Job& job)
     return DSSCF_OK;
 }
 
+bool converged(const Job& job)
+{
+    return job.iteration &&
+           std::fabs(job.total_electronic_energy - job.previous_electronic_energy) < job.opt.energy_threshold;
+}
+
 int solveMOs(Job& job)
 {
     // SCFSolver.cpp:196-237
@@ -343,8 +355,7 @@ int solveMOs(Job& job)
     job.history.clear();
     job.diis.resize(job.opt.diis_size);
     if (job.opt.verbose) std::printf("%9s %22s %22s %14s\n", "iteration", "E_elec", "dE", "[PF]^2");
-    while (!(job.iteration &&
-             std::fabs(job.total_electronic_energy - job.previous_electronic_energy) < job.opt.energy_threshold)) {
+    while (!converged(job)) {
         if ((int)job.iteration >= job.opt.max_iterations) {
             job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
             return DSSCF_ERR_NOCONV;

@@ -104,7 +104,9 @@ int constructBasisHost(Job& job);    // AOs, S, T, V, H, Loewdin X
 int constructDevice(Job& job);       // shell upload, Schwarz, quartet lists
 void useCoreGuess(Job& job);
 int fockBuild(Job& job);  // J[Ptot], K[Pa] on the device
-int runSCFIteration(Job& job);
+int runSCFIteration(Job& job);  // fockBuild + scfStep
+int scfStep(Job& job);          // G, DIIS, F' = X F X, diagonalization, density, E_elec
+bool converged(const Job& job);  // the stopping test of solveMOs (SCFSolver.cpp:174-177)
 int solveMOs(Job& job);
 int calculateEnergies(Job& job);
 

@@ -19,6 +19,8 @@ void dsscf_upload_xform(const double* x, int n, hipStream_t s);
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
                            int nblocks, int block_size, int maxN, const double* btab, double scale, hipStream_t s);
 int dsscf_class_is_wave(int cls);
+// Shell quartets per wavefront of a wave-cooperative class (1 for the others).
+int dsscf_wave_group(int cls);
 hipError_t dsscf_launch_eri_block(int cls, int boys_mode, const int4* items, int n, const ShellPair* pairs,
                                   const PrimPair* pp, const double* btab, double* out, double scale,
                                   hipStream_t s);

@@ -1,24 +1,49 @@
 // wave_kernel.hpp -- wave-cooperative shell-quartet kernel for the large
 // classes (included by eri_kernels.hip after the generated tables).
 //
-// One 64-lane wavefront per shell quartet.  Per primitive quartet the lanes
-// walk the generated VRR table level by level, each lane evaluating every
-// 64th entry of a level into LDS (entries of one level only read lower
-// levels); the lanes then add the [e0|f0] targets into per-lane register
-// accumulators.  After the primitive loops the accumulators go back to LDS,
-// the bra and ket horizontal recursions run level by level, the Cartesian
-// block is transformed to the reference's AO functions, and the result is
-// digested into J/K (each output element owned by one lane: one atomic per
-// element and quartet), written out, or reduced to a Schwarz bound.
+// One 64-lane wavefront works on G shell quartets at once, 64/G lanes (a
+// group) per quartet, each group with its own LDS table.  Per primitive
+// quartet a group walks the generated VRR table level by level, each lane
+// evaluating every (64/G)-th entry of a level into LDS (entries of one level
+// only read lower levels); the lanes then add the [e0|f0] targets into
+// per-lane register accumulators.  The groups run the same level structure
+// in step (the primitive loop runs to the largest count in the wavefront,
+// groups with fewer primitive quartets idle), so one barrier serves them all.
+// After the primitive loops the accumulators go back to LDS, the bra and ket
+// horizontal recursions run level by level, the Cartesian block is
+// transformed to the reference's AO functions, and the result is digested
+// into J/K (each output element owned by one lane: one atomic per element and
+// quartet), written out, or reduced to a Schwarz bound.
 #pragma once
 
 enum { WAVE_JK = 0, WAVE_BLOCK = 1, WAVE_SCHWARZ = 2 };
 
 __device__ __forceinline__ double pick3(int d, double x, double y, double z) { return d == 0 ? x : (d == 1 ? y : z); }
 
-// MODE WAVE_JK: one quartet per block, the block's context found in segs
-// (items, n, pairs, pp unused).  WAVE_BLOCK / WAVE_SCHWARZ: items over one
-// context, grid-stride.
+// Quartets per wavefront: the largest power of two up to 8 whose G LDS
+// tables fit in DSSCF_WAVE_LDS_BUDGET bytes.
+#ifndef DSSCF_WAVE_LDS_BUDGET
+#define DSSCF_WAVE_LDS_BUDGET 40960
+#endif
+template <int la, int lb, int lc, int ld>
+struct WaveGroup {
+    static constexpr int bytes = (int)sizeof(double) * WaveTab<la, lb, lc, ld>::LDS;
+    static constexpr int G = 8 * bytes <= DSSCF_WAVE_LDS_BUDGET   ? 8
+                             : 4 * bytes <= DSSCF_WAVE_LDS_BUDGET ? 4
+                             : 2 * bytes <= DSSCF_WAVE_LDS_BUDGET ? 2
+                                                                  : 1;
+};
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+// MODE WAVE_JK: block b covers quartets G*(b - blk0) .. +G-1 of the segment
+// its index falls in (items, n, pairs, pp unused).  WAVE_BLOCK / WAVE_SCHWARZ:
+// items over one context, grid-stride over groups of G.
 template <int la, int lb, int lc, int ld, int BM, int MODE>
 __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int n,
                                              const ShellPair* __restrict__ pairs_in,
@@ -28,13 +53,16 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
 {
     typedef WaveTab<la, lb, lc, ld> W;
     typedef ClassInfo<la, lb, lc, ld> CI;
+    constexpr int G = WaveGroup<la, lb, lc, ld>::G;
+    constexpr int LG = 64 / G;
     constexpr int L = W::L;
-    constexpr int SLOTS = (W::NT + 63) / 64;
+    constexpr int SLOTS = (W::NT + LG - 1) / LG;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
     constexpr int ncA = (la + 1) * (la + 2) / 2, ncB = (lb + 1) * (lb + 2) / 2;
     constexpr int ncC = (lc + 1) * (lc + 2) / 2, ncD = (ld + 1) * (ld + 2) / 2;
-    __shared__ double sm[W::LDS];
-    const int lane = threadIdx.x;
+    __shared__ double smem[G * W::LDS];
+    const int grp = (int)threadIdx.x / LG, lg = (int)threadIdx.x % LG;
+    double* __restrict__ sm = smem + grp * W::LDS;
     const int* __restrict__ vrr = W::vrr();
     const int* __restrict__ vlev = W::vlev();
     const short* __restrict__ tgt = W::tgt();
@@ -46,27 +74,32 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
     const JKSeg* S = nullptr;
     const ShellPair* __restrict__ pairs = pairs_in;
     const PrimPair* __restrict__ pp = pp_in;
-    int t0 = blockIdx.x, tstep = gridDim.x;
+    int b0 = blockIdx.x, bstep = gridDim.x, bend = (n + G - 1) / G;
     if (MODE == WAVE_JK) {
         S = &segs[find_segment_by_block(seg_blk0, nseg, (int)blockIdx.x)];
         pairs = S->pairs;
         pp = S->pp;
-        t0 = (int)blockIdx.x - S->blk0;
-        n = t0 + 1;  // exactly one quartet per block
-        tstep = 1;
+        n = S->n;
+        b0 = (int)blockIdx.x - S->blk0;
+        bend = b0 + 1;  // one group of G quartets per block
+        bstep = 1;
     }
-    for (int t = t0; t < n; t += tstep) {
-        int bi, ki;
-        if (MODE == WAVE_JK) {
-            const uint2 q = S->q[t];
-            bi = (int)q.x;
-            ki = (int)q.y;
-        } else if (MODE == WAVE_BLOCK) {
-            const int4 it = ((const int4*)items)[t];
-            bi = it.x;
-            ki = it.y;
-        } else {
-            bi = ki = ((const int*)items)[t];
+    for (int bb = b0; bb < bend; bb += bstep) {  // uniform over the block
+        const int t = bb * G + grp;
+        const bool valid = t < n;
+        int bi = 0, ki = 0;
+        if (valid) {
+            if (MODE == WAVE_JK) {
+                const uint2 q = S->q[t];
+                bi = (int)q.x;
+                ki = (int)q.y;
+            } else if (MODE == WAVE_BLOCK) {
+                const int4 it = ((const int4*)items)[t];
+                bi = it.x;
+                ki = it.y;
+            } else {
+                bi = ki = ((const int*)items)[t];
+            }
         }
         const ShellPair bra = pairs[bi];
         const ShellPair ket = pairs[ki];
@@ -74,14 +107,23 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) acc[s] = 0.0;
 
-        for (int ib = 0; ib < bra.nprim; ++ib) {
-            const PrimPair P = pp[bra.prim_off + ib];
-            for (int ik = 0; ik < ket.nprim; ++ik) {
+        // primitive quartets, flattened; the loop runs to the wavefront's largest count
+        const int npq = valid ? bra.nprim * ket.nprim : 0;
+        const int npq_max = wave_max(npq);
+        int ib = 0, ik = 0;
+        for (int pq = 0; pq < npq_max; ++pq) {
+            const bool active = pq < npq;
+            if (active) {
+                const PrimPair P = pp[bra.prim_off + ib];
                 const PrimPair Q = pp[ket.prim_off + ik];
+                if (++ik == ket.nprim) {
+                    ik = 0;
+                    ++ib;
+                }
                 PrimQuartet<double> g;
                 double Fm[L + 1];
                 prim_setup<BM, double, L>(P, Q, bra, ket, btab, g, Fm);
-                if (lane == 0) {
+                if (lg == 0) {
                     sm[W::ZERO] = 0.0;
 #pragma unroll
                     for (int m = 0; m <= L; ++m) {
@@ -92,7 +134,7 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
                 __syncthreads();
                 for (int lev = 0; lev < W::NLEV; ++lev) {
                     const int e1 = vlev[lev + 1];
-                    for (int k = vlev[lev] + lane; k < e1; k += 64) {
+                    for (int k = vlev[lev] + lg; k < e1; k += LG) {
                         const int r0 = vrr[3 * k], r1 = vrr[3 * k + 1], r2 = vrr[3 * k + 2];
                         const int s0 = r0 & 0xffff, s1 = (r0 >> 16) & 0xffff;
                         const int s2 = r1 & 0xffff, s3 = (r1 >> 16) & 0xffff, s4 = r2 & 0xffff;
@@ -119,22 +161,22 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
                 }
 #pragma unroll
                 for (int s = 0; s < SLOTS; ++s) {
-                    const int tt = lane + 64 * s;
+                    const int tt = lg + LG * s;
                     if (tt < W::NT) acc[s] += sm[tgt[tt]];
                 }
-                __syncthreads();
+                __syncthreads();  // (a block is one wavefront: idle groups need no barrier of their own)
             }
         }
         // horizontal recursions over Y = [targets | bra HRR | ket HRR]
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
-            const int tt = lane + 64 * s;
+            const int tt = lg + LG * s;
             if (tt < W::NT) sm[tt] = acc[s];
         }
         __syncthreads();
         for (int hl = 0; hl < W::NHLEV; ++hl) {
             const int e1 = hlev[2 * hl + 1];
-            for (int k = hlev[2 * hl] + lane; k < e1; k += 64) {
+            for (int k = hlev[2 * hl] + lg; k < e1; k += LG) {
                 const int h = hrr[k];
                 const int d = hdir[k];
                 const double c = (d >> 2) ? pick3(d & 3, ket.AB[0], ket.AB[1], ket.AB[2])
@@ -144,7 +186,7 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             __syncthreads();
         }
         // Cartesian -> AO
-        for (int o = lane; o < W::NAO; o += 64) {
+        for (int o = lg; o < W::NAO; o += LG) {
             const int md = o % ND, mc = (o / ND) % NC, mb = (o / (ND * NC)) % NB, ma = o / (ND * NC * NB);
             const int pa = la * la + ma, pb = lb * lb + mb, pc = lc * lc + mc, pd = ld * ld + md;
             double sum = 0.0;
@@ -170,11 +212,12 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
         __syncthreads();
         const double* V = sm + W::NY;
 #define VX(i, j, k, l) V[(((i) * NB + (j)) * NC + (k)) * ND + (l)]
-        if (MODE == WAVE_BLOCK) {
+        if (!valid) {
+        } else if (MODE == WAVE_BLOCK) {
             const int4 it = ((const int4*)items)[t];
-            for (int o = lane; o < W::NAO; o += 64) out[(size_t)it.z + o] = V[o];
+            for (int o = lg; o < W::NAO; o += LG) out[(size_t)it.z + o] = V[o];
         } else if (MODE == WAVE_SCHWARZ) {
-            if (lane == 0) {
+            if (lg == 0) {
                 double mx = 0.0;
                 for (int i = 0; i < NA; ++i)
                     for (int j = 0; j < NB; ++j) mx = fmax(mx, fabs(VX(i, j, i, j)));
@@ -194,7 +237,7 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             const int a0 = bra.ao_a, b0 = bra.ao_b, c0 = ket.ao_a, d0 = ket.ao_b;
             constexpr int T0 = NA * NB, T1 = T0 + NC * ND, T2 = T1 + NA * NC, T3 = T2 + NB * NC, T4 = T3 + NA * ND,
                           T5 = T4 + NB * ND;
-            for (int o = lane; o < T5; o += 64) {
+            for (int o = lg; o < T5; o += LG) {
                 double s = 0.0;
                 if (o < T0) {  // Jh[a][b] += 2g sum_cd v Dt_cd
                     const int i = o / NB, j = o % NB;

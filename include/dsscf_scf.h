@@ -64,6 +64,28 @@ int dsscf_job_energies(dsscf_job* job, double* out5);
 int dsscf_job_history(dsscf_job* job, double* out, int cap);
 dsscf_ctx* dsscf_job_context(dsscf_job* job);
 
+/* Several jobs at once (no reference counterpart: the reference runs one
+ * molecule per process).  dsscf_jobs_construct: dsscf_job_construct for
+ * each job, the host parts in parallel.  dsscf_jobs_solve: dsscf_job_solve
+ * for each job in lock step; every round the Fock builds of the jobs still
+ * iterating run as one fused launch per quartet class over all of them
+ * (dsscf_batch_*; with use_graph the fused build is a captured hipGraph
+ * replayed until a job leaves the set), the host steps (DIIS,
+ * diagonalization, density) in parallel over jobs.  With autotune the
+ * block-size autotuner runs on the first fused set.  energies[i] and
+ * iterations[i] (either may be null) get each job's results; status[i]
+ * (may be null) 0 or the job's error code (dsscf_job_error has the
+ * message).  Returns 0 if every job converged, else the first failure's code. */
+int dsscf_jobs_construct(dsscf_job* const* jobs, int n);
+int dsscf_jobs_solve(dsscf_job* const* jobs, int n, int use_graph, int autotune, double* energies, int* iterations,
+                     int* status);
+
+/* The SCF layer's symmetric eigensolver (the role of dspev_('V','U'),
+ * src/lalib/Lapack.cpp:12-39): A is n x n row-major symmetric; w gets the
+ * eigenvalues ascending, V (row-major) the eigenvectors as columns.
+ * DSSCF_ERR_NOCONV if the QR iteration did not converge. */
+int dsscf_syev(int n, const double* A, double* w, double* V);
+
 #ifdef __cplusplus
 }
 #endif

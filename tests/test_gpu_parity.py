@@ -204,3 +204,55 @@ def test_graph_replays_match_eager_builds(basis):
         J, K = ctx.get_jk(n)
         assert np.max(np.abs(J - J0)) < 1e-12 * sj, rep
     assert np.all(kms >= 0.0) and kms.sum() > 0.0
+
+
+def test_mixed_precision_alkane_def2svp():
+    """Quartets with Schwarz bound below 1e-5 run their setup, small-T Boys and
+    VRR in FP32 (C4H10/def2-SVP, 106 AOs): J and K at a fixed density stay within
+    1e-7 of the all-FP64 build, and the converged RHF energy within 1e-6 Eh."""
+    Z, xyz = G.alkane(4)
+    res = {}
+    for thr in (0.0, 1e-5):
+        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-9, max_iterations=60, schwarz_threshold=1e-12,
+                                    fp32_threshold=thr)
+        job.set_basis("def2-SVP").set_molecule(Z, xyz).construct()
+        st = job.context.stats()
+        n = job.n_ao
+        C = np.random.default_rng(21).standard_normal((n, 17)) * 0.2
+        Pa = O.pack(C @ C.T)
+        job.set_density(Pa)
+        J, K = job.fock_build()
+        e, it = job.solve()
+        res[thr] = (J, K, e, st.n_quartets_fp32, st.n_quartets)
+    J64, K64, e64, nf64, nq = res[0.0]
+    J32, K32, e32, nf32, nq32 = res[1e-5]
+    assert n == 106 and nf64 == 0 and nq32 == nq
+    assert 0.2 * nq < nf32 < nq  # a large share of the quartets ran in FP32
+    assert np.max(np.abs(J32 - J64)) < 1e-7
+    assert np.max(np.abs(K32 - K64)) < 1e-7
+    assert abs(e32 - e64) < 1e-6
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_jobs_solved_together_match_alone(use_graph):
+    """Uneven jobs (several bases and sizes) solved in lock step with fused
+    Fock builds (and the block-size autotuner) converge to the energies each
+    reaches alone, in the same number of iterations."""
+    specs = [("STO-3G", G.water()), ("6-31G", G.water_box(2, seed=3)), ("6-31GP", G.water_box(3, seed=4)),
+             ("def2-SVP", G.water()), ("6-31G", G.alkane(2)), ("STO-3G", G.water_box(4, seed=9))]
+
+    def make(basis, Z, xyz):
+        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-9, max_iterations=80, schwarz_threshold=1e-12)
+        return job.set_basis(basis).set_molecule(Z, xyz)
+
+    alone = []
+    for basis, (Z, xyz) in specs:
+        j = make(basis, Z, xyz)
+        j.construct()
+        alone.append(j.solve())
+    jobs = ds.construct_jobs([make(b, Z, xyz) for b, (Z, xyz) in specs])
+    e, it, st = ds.solve_jobs(jobs, use_graph=use_graph, autotune=True)
+    assert list(st) == [0] * len(specs)
+    for (ea, ia), eb, ib in zip(alone, e, it):
+        assert abs(ea - eb) < 1e-9
+        assert abs(ia - ib) <= 1

@@ -127,3 +127,34 @@ def test_geometry_generators():
 def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 40, 130])
+def test_host_eigensolver_vs_numpy(n):
+    """The SCF layer's symmetric eigensolver (dspev_ role): eigenvalues equal
+    numpy's eigvalsh, eigenvectors orthonormal and A V = V diag(w)."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    w, V = ds.syev(A)
+    assert np.all(np.diff(w) >= 0.0)
+    assert np.max(np.abs(w - np.linalg.eigvalsh(A))) < 1e-12 * max(1.0, np.abs(w).max())
+    assert np.max(np.abs(V.T @ V - np.eye(n))) < 1e-12
+    assert np.max(np.abs(A @ V - V * w)) < 1e-12 * max(1.0, np.abs(w).max())
+
+
+def test_host_eigensolver_degenerate_and_split():
+    """Degenerate spectra and matrices that split into blocks (zero couplings)."""
+    A = np.eye(30)
+    A[0, 0] = 3.0
+    w, V = ds.syev(A)
+    assert np.allclose(w, np.sort(np.diag(A)), atol=1e-14)
+    assert np.max(np.abs(A @ V - V * w)) < 1e-13
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((12, 12))
+    B = B + B.T
+    B[5, :] = B[:, 5] = 0.0
+    B[5, 5] = -2.0
+    w, V = ds.syev(B)
+    assert np.max(np.abs(w - np.linalg.eigvalsh(B))) < 1e-12
+    assert np.max(np.abs(B @ V - V * w)) < 1e-12

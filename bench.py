@@ -201,6 +201,101 @@ def bench_batch(a):
     print(json.dumps(out))
 
 
+def serve_requests(n_req: int, rate: float, seed: int = 7):
+    """Seeded Poisson stream: arrival times (s) and (basis, Z, xyz) per request.
+    Basis uniform over STO-3G, 6-31G, def2-SVP; molecule a water cluster of
+    1-12 waters (6 A box for up to 6, grown at constant density beyond) or an
+    all-trans alkane CnH2n+2, n = 1-12, with equal odds."""
+    from dollarstorescf_amd import geometry
+
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    out = []
+    for _ in range(n_req):
+        basis = ["STO-3G", "6-31G", "def2-SVP"][int(rng.integers(0, 3))]
+        k = int(rng.integers(1, 13))
+        if rng.random() < 0.5:
+            Z, xyz = geometry.water_box(k, seed=int(rng.integers(0, 2**31 - 1)), box=6.0 * max(1.0, k / 6.0) ** (1 / 3))
+        else:
+            Z, xyz = geometry.alkane(k)
+        out.append((basis, Z, xyz))
+    return arrivals, out
+
+
+def serve_stream(ds, arrivals, reqs, fp32: float, max_active: int, budget: float):
+    """Replay the stream in real time against one server; returns per-request
+    latency (finish - scheduled arrival), energies, status and the wall time
+    from the first arrival to the last finish."""
+    jobs = []
+    for basis, Z, xyz in reqs:
+        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12,
+                                    fp32_threshold=fp32)
+        jobs.append(job.set_basis(basis).set_molecule(Z, xyz))
+    srv = ds.Server(0, budget_bytes=budget, max_active=max_active, workers=4)
+    n = len(reqs)
+    t0 = srv.clock()
+    i, done = 0, {}
+    while len(done) < n:
+        now = srv.clock() - t0
+        while i < n and arrivals[i] <= now:
+            srv.submit(jobs[i], i)
+            i += 1
+        if srv.idle():
+            if i < n:
+                time.sleep(max(0.0, arrivals[i] - (srv.clock() - t0)))
+            continue
+        srv.step()
+        for d in srv.poll():
+            done[d["tag"]] = d
+    srv.close()
+    lat = np.array([done[k]["t_done"] - t0 - arrivals[k] for k in range(n)])
+    e = np.array([done[k]["energy"] for k in range(n)])
+    st = np.array([done[k]["status"] for k in range(n)])
+    wall = max(d["t_done"] for d in done.values()) - t0 - arrivals[0]
+    for j in jobs:
+        j.close()
+    return lat, e, st, wall
+
+
+def bench_serve(a):
+    """Serving: a seeded Poisson stream (mean a.rate requests/s) of RHF
+    requests, replayed in real time against the continuous-batching server
+    (jobs constructed by worker threads, admitted by device-memory budget,
+    one fused Fock build per round over every job on the device).  The
+    stream runs with FP32 below Schwarz 1e-5 (value) and again with every
+    quartet in FP64; reported: p50/p99 latency, solves/s, |dE| per request."""
+    import dollarstorescf_amd as ds
+
+    arrivals, reqs = serve_requests(a.requests, a.rate)
+    runs = {}
+    for mode, thr in (("mixed", 1e-5), ("fp64", 0.0)):
+        lat, e, st, wall = serve_stream(ds, arrivals, reqs, thr, max_active=64, budget=32e9)
+        runs[mode] = {"p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+                      "mean_latency_s": float(lat.mean()), "solves_per_s": len(reqs) / wall, "wall_s": wall,
+                      "failed": int(np.count_nonzero(st)), "_e": e}
+    de = np.abs(runs["mixed"].pop("_e") - runs["fp64"].pop("_e"))
+    n_ao = []
+    for basis, Z, xyz in reqs:
+        j = ds.Job(0).set_basis(basis).set_molecule(Z, xyz)
+        j.construct_host()
+        n_ao.append(j.n_ao)
+        j.close()
+    out = {
+        "metric": METRIC, "value": runs["mixed"]["solves_per_s"], "unit": "SCF solves/s", "n_gpus": 1,
+        "steps": len(reqs), "warmup": 0, "ms_per_step": runs["mixed"]["wall_s"] / len(reqs) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic (seeded Poisson stream of water clusters and alkanes)",
+        "config": {"workload": f"{len(reqs)} RHF requests, Poisson arrivals at {a.rate:g}/s (seed 7), STO-3G / "
+                               "6-31G / def2-SVP, 1-12 waters or C1-C12 alkanes; server: 64 slots, 32 GB budget, "
+                               "4 construction workers",
+                   "n_ao_min": int(min(n_ao)), "n_ao_max": int(max(n_ao)), "parallelism": "single GPU"},
+        "mixed": runs["mixed"], "fp64": runs["fp64"],
+        "p50_latency_s": runs["mixed"]["p50_latency_s"], "p99_latency_s": runs["mixed"]["p99_latency_s"],
+        "max_abs_dE_mixed_vs_fp64": float(de.max()),
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,11 +304,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=5_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--boys", choices=["accurate", "reference"], default="accurate")
-    ap.add_argument("--workload", choices=["water8", "alkane", "batch"], default="water8",
+    ap.add_argument("--workload", choices=["water8", "alkane", "batch", "serve"], default="water8",
                     help="water8: the Fock-build benchmark (default); alkane: C20H42/def2-SVP, FP64 vs "
                          "FP32 below Schwarz 1e-5; batch: 512 RHF/6-31G* jobs of 1-6 waters, fused vs one "
-                         "job at a time")
+                         "job at a time; serve: Poisson stream of SCF requests, mixed precision vs FP64")
     ap.add_argument("--jobs", type=int, default=512, help="jobs of the batch workload")
+    ap.add_argument("--requests", type=int, default=200, help="requests of the serve workload")
+    ap.add_argument("--rate", type=float, default=200.0, help="mean arrival rate of the serve workload (1/s)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     a = ap.parse_args()
@@ -230,6 +327,9 @@ def main():
         return
     if a.workload == "batch":
         bench_batch(a)
+        return
+    if a.workload == "serve":
+        bench_serve(a)
         return
 
     import dollarstorescf_amd as ds

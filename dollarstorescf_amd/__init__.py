@@ -8,6 +8,7 @@ There is no fallback: if the library or a GPU is missing, calls raise.
 from __future__ import annotations
 
 import ctypes
+import time
 import os
 
 import numpy as np
@@ -78,6 +79,9 @@ def lib():
             "dsscf_job_get": [p, c, p], "dsscf_job_set_density": [p, p], "dsscf_job_fock_build": [p],
             "dsscf_job_solve": [p, p, p], "dsscf_job_energies": [p, p], "dsscf_job_history": [p, p, i],
             "dsscf_syev": [i, p, p, p], "dsscf_jobs_construct": [p, i], "dsscf_jobs_solve": [p, i, i, i, p, p, p],
+            "dsscf_server_create": [p, i, d, i, i], "dsscf_server_destroy": [p], "dsscf_server_submit": [p, p, ctypes.c_longlong],
+            "dsscf_server_step": [p, p, p], "dsscf_server_poll": [p, i, p, p, p, p, p, p, p, p],
+            "dsscf_server_idle": [p],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -89,6 +93,10 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = i
+        L.dsscf_server_clock.restype = d
+        L.dsscf_server_clock.argtypes = [p]
+        L.dsscf_ctx_device_bytes.restype = ctypes.c_longlong
+        L.dsscf_ctx_device_bytes.argtypes = [p]
         for name in ("dsscf_max_l", "dsscf_class_count", "dsscf_xform_size"):
             getattr(L, name).restype = i
             getattr(L, name).argtypes = []
@@ -129,6 +137,67 @@ def solve_jobs(jobs, use_graph=True, autotune=True):
     e, it, st = np.zeros(n), np.zeros(n, np.int32), np.zeros(n, np.int32)
     lib().dsscf_jobs_solve(arr, n, 1 if use_graph else 0, 1 if autotune else 0, _p(e), _p(it), _p(st))
     return e, it, st
+
+
+class Server:
+    """Continuous-batching SCF service on one device (dsscf_server_*).  Submit
+    unconstructed jobs; step() runs one round; poll() returns finished ones as
+    dicts (tag, job, status, energy, iterations, t_submit, t_admit, t_done on
+    the server clock)."""
+
+    def __init__(self, device=0, budget_bytes=8e9, max_active=64, workers=4):
+        self._h = ctypes.c_void_p()
+        _check(lib().dsscf_server_create(ctypes.byref(self._h), device, float(budget_bytes), max_active, workers))
+        self._jobs = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dsscf_server_destroy(self._h)
+        self._h = None
+
+    __del__ = close
+
+    def submit(self, job, tag: int):
+        self._jobs[tag] = job
+        rc = lib().dsscf_server_submit(self._h, job._h, tag)
+        if rc:
+            del self._jobs[tag]
+            raise DsscfError(rc, job._err())
+
+    def step(self):
+        a, w = ctypes.c_int(), ctypes.c_int()
+        _check(lib().dsscf_server_step(self._h, ctypes.byref(a), ctypes.byref(w)))
+        return a.value, w.value
+
+    def poll(self, cap=1024):
+        tags = np.zeros(cap, np.int64)
+        st, it = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        e, ts, ta, td = np.zeros(cap), np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        jobs = (ctypes.c_void_p * cap)()
+        n = lib().dsscf_server_poll(self._h, cap, _p(tags), jobs, _p(st), _p(e), _p(it), _p(ts), _p(ta), _p(td))
+        out = []
+        for k in range(n):
+            tag = int(tags[k])
+            out.append({"tag": tag, "job": self._jobs.pop(tag), "status": int(st[k]), "energy": float(e[k]),
+                        "iterations": int(it[k]), "t_submit": float(ts[k]), "t_admit": float(ta[k]),
+                        "t_done": float(td[k])})
+        return out
+
+    def clock(self) -> float:
+        return lib().dsscf_server_clock(self._h)
+
+    def idle(self) -> bool:
+        return lib().dsscf_server_idle(self._h) == 1
+
+    def run_until_idle(self):
+        """Step until every submitted job is back; returns them."""
+        out = []
+        while not self.idle():
+            a, w = self.step()
+            out += self.poll()
+            if a == 0 and w > 0:
+                time.sleep(0.0005)  # jobs still constructing on the workers
+        return out + self.poll()
 
 
 def syev(A):

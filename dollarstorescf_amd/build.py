@@ -29,7 +29,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-I" + os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["eri_kernels.hip", "dsscf_abi.cpp"]
-HOST_SOURCES = ["host/chem.cpp", "host/linalg.cpp", "host/scf.cpp", "host/multi.cpp", "host/job_api.cpp"]
+HOST_SOURCES = ["host/chem.cpp", "host/linalg.cpp", "host/scf.cpp", "host/multi.cpp", "host/serve.cpp",
+                "host/job_api.cpp"]
 
 DEFAULT_MAXL = 2
 

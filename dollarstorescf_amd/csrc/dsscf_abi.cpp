@@ -169,6 +169,7 @@ struct dsscf_ctx {
     long long n_unscreened = 0;
     long long n_work_pairs = 0, n_work_items = 0;  // shell pairs + sub-pairs; (sub-)quartets
     int n_slice = 1;                                // copies of Jh/Kh
+    long long device_bytes = 0;                     // geometry-dependent device memory after prepare
     bool prepared = false;
 
     // device
@@ -652,6 +653,10 @@ int dsscf_prepare(dsscf_ctx* c)
         HIP_TRY(hipMemcpyAsync(c->d_q, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if ((int)c->block_size.size() != ncls) c->block_size.assign(ncls, 256);
+    c->device_bytes = (long long)(sizeof(ShellPair) * work.size() + sizeof(PrimPair) * c->prims.size() +
+                                  (sizeof(double) + sizeof(int)) * npair + sizeof(double) * nn * (2 + 2 * c->n_slice) +
+                                  sizeof(double) * np * 4 + sizeof(int4) * units.size() +
+                                  sizeof(unsigned) * kets.size() + sizeof(uint2) * all.size());
     c->prepared = true;
     return DSSCF_OK;
 }
@@ -1042,6 +1047,12 @@ int dsscf_time_jk_classes(dsscf_ctx* c, int iters, double* ms_per_class)
     }
     for (int k = 0; k < ncls; ++k) ms_per_class[k] = acc[k] / iters;
     return DSSCF_OK;
+}
+
+long long dsscf_ctx_device_bytes(dsscf_ctx* c)
+{
+    if (!c) return 0;
+    return 1024LL * 128 + (c->prepared ? c->device_bytes : 0);  // + Boys table and launch tables
 }
 
 int dsscf_schwarz(dsscf_ctx* c, double* Q)

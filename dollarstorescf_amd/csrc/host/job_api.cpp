@@ -6,6 +6,7 @@
 
 #include "multi.hpp"
 #include "scf.hpp"
+#include "serve.hpp"
 
 struct dsscf_job {
     scf::Job job;
@@ -275,6 +276,62 @@ int dsscf_jobs_solve(dsscf_job* const* jobs, int n, int use_graph, int autotune,
     }
     return rc;
 }
+
+struct dsscf_server {
+    scf::Server* s;
+};
+
+int dsscf_server_create(dsscf_server** out, int device, double budget_bytes, int max_active, int workers)
+{
+    if (!out || !(budget_bytes > 0.0) || max_active < 1 || workers < 1) return DSSCF_ERR_ARG;
+    *out = new dsscf_server{new scf::Server(device, budget_bytes, max_active, workers)};
+    return DSSCF_OK;
+}
+
+int dsscf_server_destroy(dsscf_server* s)
+{
+    if (!s) return DSSCF_OK;
+    delete s->s;
+    delete s;
+    return DSSCF_OK;
+}
+
+int dsscf_server_submit(dsscf_server* s, dsscf_job* j, long long tag)
+{
+    if (!s || !j) return DSSCF_ERR_ARG;
+    if (!j->have_basis || !j->job.molecule.size()) return set_err(j, DSSCF_ERR_STATE, "molecule and basis must be set first");
+    if (j->job.ctx) return set_err(j, DSSCF_ERR_STATE, "dsscf_server_submit: job already constructed");
+    if (j->job.opt.device != s->s->device()) return set_err(j, DSSCF_ERR_ARG, "dsscf_server_submit: job on another device");
+    return s->s->submit(j, &j->job, tag);
+}
+
+int dsscf_server_step(dsscf_server* s, int* n_active, int* n_waiting)
+{
+    if (!s) return DSSCF_ERR_ARG;
+    return s->s->step(n_active, n_waiting);
+}
+
+int dsscf_server_poll(dsscf_server* s, int cap, long long* tags, dsscf_job** jobs, int* status, double* energies,
+                      int* iterations, double* t_submit, double* t_admit, double* t_done)
+{
+    if (!s || cap < 0) return 0;
+    std::vector<scf::Server::Done> d;
+    s->s->poll(d, cap);
+    for (size_t i = 0; i < d.size(); ++i) {
+        if (tags) tags[i] = d[i].tag;
+        if (jobs) jobs[i] = (dsscf_job*)d[i].handle;
+        if (status) status[i] = d[i].status;
+        if (energies) energies[i] = d[i].energy;
+        if (iterations) iterations[i] = d[i].iterations;
+        if (t_submit) t_submit[i] = d[i].t_submit;
+        if (t_admit) t_admit[i] = d[i].t_admit;
+        if (t_done) t_done[i] = d[i].t_done;
+    }
+    return (int)d.size();
+}
+
+double dsscf_server_clock(dsscf_server* s) { return s ? s->s->clock() : 0.0; }
+int dsscf_server_idle(dsscf_server* s) { return s && s->s->idle() ? 1 : 0; }
 
 int dsscf_syev(int n, const double* A, double* w, double* V)
 {

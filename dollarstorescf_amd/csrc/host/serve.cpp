@@ -1,0 +1,268 @@
+// serve.cpp -- continuous-batching SCF service on one device.
+//
+// A submitted job is constructed on the host by a worker thread (AOs,
+// one-electron terms, orthogonalization).  Each round (step) then:
+//  1. admits ready jobs in submission order while a slot is free and their
+//     estimated device footprint fits the budget; a worker prepares each one
+//     on the device (shell pairs, Schwarz bounds, quartet lists) while the
+//     round goes on;
+//  2. runs one fused Fock build (dsscf_batch_jk: one launch per quartet
+//     class over all of them) for every prepared job, from the core guess on;
+//  3. runs every job's host step in parallel (DIIS, diagonalization,
+//     density; or, after convergence, the energies of Energy.cpp:21-50);
+//  4. retires finished jobs: their device memory is freed and they are
+//     handed back through poll().
+// Jobs arriving while others iterate join the next round: the fused set
+// changes from round to round, so its launches are made directly (no graph).
+#include "serve.hpp"
+
+#include <omp.h>
+
+#include <algorithm>
+
+namespace scf {
+
+Server::Server(int device, double budget_bytes, int max_active, int workers)
+    : device_(device), budget_(budget_bytes), max_active_(std::max(1, max_active)), t0_(std::chrono::steady_clock::now())
+{
+    for (int i = 0; i < std::max(1, workers); ++i) workers_.emplace_back([this] { worker_loop(); });
+}
+
+Server::~Server()
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : workers_) t.join();
+    if (batch_) dsscf_batch_destroy(batch_);
+    for (Req* r : active_) {
+        if (r->job->ctx) {
+            dsscf_ctx_destroy(r->job->ctx);
+            r->job->ctx = nullptr;
+        }
+        delete r;
+    }
+    for (Req* r : waiting_) delete r;
+}
+
+double Server::clock() const
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+void Server::worker_loop()
+{
+    for (;;) {
+        std::function<void()> f;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [this] { return stop_ || !tasks_.empty(); });
+            if (stop_ && tasks_.empty()) return;
+            f = std::move(tasks_.front());
+            tasks_.pop_front();
+            ++busy_tasks_;
+        }
+        f();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            --busy_tasks_;
+        }
+    }
+}
+
+// Queue a task for the workers (mu_ must not be held).
+void Server::enqueue(std::function<void()> f)
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        tasks_.push_back(std::move(f));
+    }
+    cv_.notify_one();
+}
+
+// Upper bound of a job's device memory before its quartet lists exist: the
+// unscreened quartets at 6 B each (ket indices, work units, sub-pairs), the
+// matrices with the largest J/K slice count, pair and primitive records.
+double Server::estimate_bytes(const Job& job) const
+{
+    double shells = 0.0, prims = 0.0;
+    for (const auto& ab : job.atom_bases)
+        for (const auto& sh : ab) {
+            shells += 1.0;
+            prims += (double)sh.primitives.size();
+        }
+    const double pairs = shells * (shells + 1.0) / 2.0;
+    const double n = job.n;
+    return 6.0 * pairs * (pairs + 1.0) / 2.0 + 8.0 * n * n * 34.0 + 32.0 * n * (n + 1.0) / 2.0 + 76.0 * pairs +
+           64.0 * prims * prims / 2.0 + 256.0 * 1024.0;
+}
+
+int Server::submit(void* handle, Job* job, long long tag)
+{
+    if (!job) return DSSCF_ERR_ARG;
+    Req* r = new Req{handle, job, tag, HOST, 0, 0.0, 0.0, clock(), 0.0, 0.0};
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        waiting_.push_back(r);
+    }
+    enqueue([this, r] {
+        const int rc = constructBasisHost(*r->job);
+        std::lock_guard<std::mutex> lk(mu_);
+        if (rc) {
+            waiting_.erase(std::find(waiting_.begin(), waiting_.end(), r));
+            retire_locked(r, rc);
+            return;
+        }
+        r->est_bytes = estimate_bytes(*r->job);
+        r->st = READY;
+    });
+    return DSSCF_OK;
+}
+
+// Called with mu_ held: hand r back (frees its device memory).
+void Server::retire_locked(Req* r, int status)
+{
+    if (r->job->ctx) {
+        dsscf_ctx_destroy(r->job->ctx);
+        r->job->ctx = nullptr;
+    }
+    r->st = RETIRED;
+    batch_dirty_ = true;  // the fused plan may point at r's buffers: rebuild before the next launch
+    r->status = status;
+    r->t_done = clock();
+    done_.push_back(Done{r->handle, r->tag, status, status ? 0.0 : r->job->energies.total(), (int)r->job->iteration,
+                         r->t_submit, r->t_admit, r->t_done});
+    delete r;
+}
+
+bool Server::idle()
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    return waiting_.empty() && active_.empty() && tasks_.empty() && busy_tasks_ == 0;
+}
+
+int Server::step(int* n_active, int* n_waiting)
+{
+    std::vector<Req*> members;
+    bool dirty = false;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        // 1. admission, in submission order (a job still constructing holds its place)
+        while (!waiting_.empty() && waiting_.front()->st == READY && (int)active_.size() < max_active_) {
+            Req* r = waiting_.front();
+            if (used_ + r->est_bytes > budget_ && !active_.empty()) break;
+            waiting_.pop_front();
+            if (r->est_bytes > budget_) {  // could never fit
+                r->job->error = "job's device footprint exceeds the server budget";
+                retire_locked(r, DSSCF_ERR_NOMEM);
+                continue;
+            }
+            used_ += r->est_bytes;
+            r->st = PREPARING;
+            r->t_admit = clock();
+            active_.push_back(r);
+            tasks_.push_back([this, r] {  // mu_ is held: queue directly, wake a worker below
+                const int rc = constructDevice(*r->job);
+                std::lock_guard<std::mutex> lk2(mu_);
+                if (!rc) {
+                    useCoreGuess(*r->job);
+                    Job& j = *r->job;
+                    j.iteration = 0;
+                    j.total_electronic_energy = j.previous_electronic_energy = 0.0;
+                    j.history.clear();
+                    j.diis.resize(j.opt.diis_size);
+                    // the budget holds the measured footprint from here on
+                    r->bytes = (double)dsscf_ctx_device_bytes(j.ctx);
+                    used_ += r->bytes - r->est_bytes;
+                    r->est_bytes = r->bytes;
+                    r->st = ITERATING;
+                } else {
+                    r->st = FAILED;
+                    r->status = rc;
+                }
+            });
+        }
+        // jobs whose device preparation failed
+        for (size_t i = 0; i < active_.size();) {
+            Req* r = active_[i];
+            if (r->st == FAILED) {
+                used_ -= r->est_bytes;
+                active_.erase(active_.begin() + i);
+                retire_locked(r, r->status);
+            } else {
+                ++i;
+            }
+        }
+        for (Req* r : active_)
+            if (r->st == ITERATING || r->st == FINAL) members.push_back(r);
+        dirty = batch_dirty_;
+        batch_dirty_ = false;
+    }
+    cv_.notify_all();
+    int rc = DSSCF_OK;
+    if (!members.empty()) {
+        if (!batch_)
+            if ((rc = dsscf_batch_create(&batch_, device_))) return rc;
+        if (dirty || members != members_) {
+            std::vector<dsscf_ctx*> ctxs;
+            for (Req* r : members) ctxs.push_back(r->job->ctx);
+            if ((rc = dsscf_batch_set(batch_, ctxs.data(), (int)ctxs.size()))) return rc;
+            members_ = members;
+        }
+        std::vector<const double*> pt, pa;
+        std::vector<double*> jj, kk;
+        for (Req* r : members) {
+            pt.push_back(r->job->Ptot.data());
+            pa.push_back(r->job->Pa.data());
+            jj.push_back(r->job->J.data());
+            kk.push_back(r->job->K.data());
+        }
+        if ((rc = dsscf_batch_jk(batch_, pt.data(), pa.data(), jj.data(), kk.data()))) return rc;
+        const int m = (int)members.size();
+        std::vector<int> out(m, -1);  // -1 keep iterating, else the final status
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int t = 0; t < m; ++t) {
+            Req* r = members[t];
+            Job& job = *r->job;
+            job.jk_valid = true;
+            if (r->st == FINAL) {
+                out[t] = calculateEnergies(job);
+                continue;
+            }
+            if ((int)job.iteration >= job.opt.max_iterations) {
+                job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
+                out[t] = DSSCF_ERR_NOCONV;
+                continue;
+            }
+            ++job.iteration;
+            const int s = scfStep(job);
+            if (s) out[t] = s;
+            else if (converged(job)) r->st = FINAL;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        for (int t = 0; t < m; ++t) {
+            if (out[t] < 0) continue;
+            Req* r = members[t];
+            used_ -= r->est_bytes;
+            active_.erase(std::find(active_.begin(), active_.end(), r));
+            retire_locked(r, out[t]);
+        }
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    if (n_active) *n_active = (int)active_.size();
+    if (n_waiting) *n_waiting = (int)waiting_.size();
+    return rc;
+}
+
+void Server::poll(std::vector<Done>& out, int cap)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    while (!done_.empty() && (int)out.size() < cap) {
+        out.push_back(done_.front());
+        done_.pop_front();
+    }
+}
+
+}  // namespace scf

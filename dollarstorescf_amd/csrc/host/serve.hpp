@@ -1,0 +1,81 @@
+// serve.hpp -- a persistent SCF service on one device with continuous
+// batching (no reference counterpart: DollarStoreSCF runs one molecule per
+// process).  See serve.cpp.
+#pragma once
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "scf.hpp"
+
+namespace scf {
+
+class Server {
+public:
+    struct Done {
+        void* handle;
+        long long tag;
+        int status;  // 0 or the error code (the job's error string has the message)
+        double energy;
+        int iterations;
+        double t_submit, t_admit, t_done;  // seconds on clock()
+    };
+
+    // budget_bytes: device memory the admitted jobs may hold together;
+    // max_active: jobs on the device at once; workers: host threads for
+    // construction and device preparation.
+    Server(int device, double budget_bytes, int max_active, int workers);
+    ~Server();
+
+    // Queue a job whose molecule, basis and options are set.  The server
+    // constructs it in the background and admits it when its device
+    // footprint fits; handle and tag come back from poll().
+    int submit(void* handle, Job* job, long long tag);
+    // One round: admit ready jobs, one fused Fock build over the jobs on the
+    // device, their host steps in parallel, retire finished jobs.
+    int step(int* n_active, int* n_waiting);
+    // Finished jobs, oldest first, at most cap.
+    void poll(std::vector<Done>& out, int cap);
+    double clock() const;
+    int device() const { return device_; }
+    bool idle();  // nothing queued, constructing, preparing or on the device
+
+private:
+    enum State { HOST, READY, PREPARING, ITERATING, FINAL, FAILED, RETIRED };
+    struct Req {
+        void* handle;
+        Job* job;
+        long long tag;
+        State st;
+        int status;
+        double est_bytes, bytes;
+        double t_submit, t_admit, t_done;
+    };
+    void worker_loop();
+    void enqueue(std::function<void()> f);
+    void retire_locked(Req* r, int status);
+    double estimate_bytes(const Job& job) const;
+
+    int device_;
+    double budget_, used_ = 0.0;
+    int max_active_;
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> tasks_;
+    bool stop_ = false;
+    int busy_tasks_ = 0;
+    std::deque<Req*> waiting_;   // HOST or READY, submission order
+    std::vector<Req*> active_;   // PREPARING, ITERATING or FINAL
+    std::deque<Done> done_;
+    dsscf_batch* batch_ = nullptr;
+    std::vector<Req*> members_;  // the fused set of the last round
+    bool batch_dirty_ = true;    // a job retired since the fused set was built
+    std::chrono::steady_clock::time_point t0_;
+};
+
+}  // namespace scf

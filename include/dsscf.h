@@ -91,6 +91,9 @@ int dsscf_xform_size(void);
 int dsscf_ctx_create(dsscf_ctx** ctx, int device);
 int dsscf_ctx_destroy(dsscf_ctx* ctx);
 int dsscf_set_options(dsscf_ctx* ctx, const dsscf_options* opt);
+/* Device memory the context holds (bytes, approximate: its buffers after
+ * dsscf_prepare; no reference counterpart). */
+long long dsscf_ctx_device_bytes(dsscf_ctx* ctx);
 
 /* Shell data, shells in AO order (atom -> shell in basis-file order, each
  * shell contributing 2l+1 AOs m=-l..l: src/chem/scf/SCFBasis.cpp:120-140).

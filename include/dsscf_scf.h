@@ -80,6 +80,35 @@ int dsscf_jobs_construct(dsscf_job* const* jobs, int n);
 int dsscf_jobs_solve(dsscf_job* const* jobs, int n, int use_graph, int autotune, double* energies, int* iterations,
                      int* status);
 
+/* A persistent SCF service on one device with continuous batching (no
+ * reference counterpart).  Submitted jobs (molecule, basis and options set,
+ * not constructed) are constructed on the host by worker threads.  Each
+ * dsscf_server_step admits ready jobs in submission order while fewer than
+ * max_active are on the device and their device footprint fits
+ * budget_bytes (device preparation runs on the workers while rounds go on),
+ * runs one fused Fock build over every prepared job, then each job's host
+ * step in parallel; a finished job frees its device memory and is handed
+ * back by dsscf_server_poll.  Jobs keep their shell data, density, J and K
+ * with the server between rounds.  The caller keeps ownership of the job
+ * handles (do not touch a job between submit and poll) and polls every job
+ * before dsscf_server_destroy.  One thread drives a server. */
+typedef struct dsscf_server dsscf_server;
+int dsscf_server_create(dsscf_server** server, int device, double budget_bytes, int max_active, int workers);
+int dsscf_server_destroy(dsscf_server* server);
+int dsscf_server_submit(dsscf_server* server, dsscf_job* job, long long tag);
+/* One round; n_active / n_waiting (may be null): jobs on the device and
+ * jobs queued or constructing after it. */
+int dsscf_server_step(dsscf_server* server, int* n_active, int* n_waiting);
+/* Up to cap finished jobs, oldest first: their tag, handle, status (0 or the
+ * job's error code), total energy, iterations, and times on the server's
+ * clock (seconds): submitted, admitted, finished.  Returns the count. */
+int dsscf_server_poll(dsscf_server* server, int cap, long long* tags, dsscf_job** jobs, int* status,
+                      double* energies, int* iterations, double* t_submit, double* t_admit, double* t_done);
+/* Seconds since the server was created; 1 if nothing is queued, constructing
+ * or on the device, else 0 (dsscf_server_idle). */
+double dsscf_server_clock(dsscf_server* server);
+int dsscf_server_idle(dsscf_server* server);
+
 /* The SCF layer's symmetric eigensolver (the role of dspev_('V','U'),
  * src/lalib/Lapack.cpp:12-39): A is n x n row-major symmetric; w gets the
  * eigenvalues ascending, V (row-major) the eigenvectors as columns.

@@ -256,3 +256,50 @@ def test_jobs_solved_together_match_alone(use_graph):
     for (ea, ia), eb, ib in zip(alone, e, it):
         assert abs(ea - eb) < 1e-9
         assert abs(ia - ib) <= 1
+
+
+def _serve_job(basis, Z, xyz, fp32=0.0):
+    job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-9, max_iterations=100, schwarz_threshold=1e-12,
+                                fp32_threshold=fp32)
+    return job.set_basis(basis).set_molecule(Z, xyz)
+
+
+@pytest.mark.parametrize("fp32", [0.0, 1e-5])
+def test_server_result_same_alone_and_in_mix(fp32):
+    """A job served alone and the same job served inside a mix of other
+    molecules and bases (arriving while others iterate, with a footprint
+    budget that forces queueing) converge to the same energy, within 1e-9 Eh;
+    every job in the mix equals its own Job.solve energy within 1e-9 Eh
+    (FP64) and, with FP32 below Schwarz 1e-5, within 1e-6 Eh of it."""
+    probe = ("6-31G", G.water_box(3, seed=4))
+    others = [("STO-3G", G.water()), ("def2-SVP", G.water()), ("6-31G", G.alkane(3)),
+              ("STO-3G", G.water_box(5, seed=2)), ("6-31GP", G.water_box(2, seed=7)), ("def2-SVP", G.alkane(1))]
+    srv = ds.Server(0, budget_bytes=8e9, max_active=8, workers=3)
+    srv.submit(_serve_job(probe[0], *probe[1], fp32=fp32), 0)
+    alone = srv.run_until_idle()
+    assert len(alone) == 1 and alone[0]["status"] == 0
+    srv.close()
+    # the mix: a small budget and four slots force admission in several waves
+    srv = ds.Server(0, budget_bytes=2.5e8, max_active=4, workers=3)
+    specs = others[:3] + [probe] + others[3:]
+    srv.submit(_serve_job(specs[0][0], *specs[0][1], fp32=fp32), 0)
+    srv.submit(_serve_job(specs[1][0], *specs[1][1], fp32=fp32), 1)
+    done = []
+    for _ in range(3):
+        srv.step()
+        done += srv.poll()
+    for k in range(2, len(specs)):
+        srv.submit(_serve_job(specs[k][0], *specs[k][1], fp32=fp32), k)
+        srv.step()
+        done += srv.poll()
+    done += srv.run_until_idle()
+    srv.close()
+    assert sorted(d["tag"] for d in done) == list(range(len(specs)))
+    by_tag = {d["tag"]: d for d in done}
+    assert all(d["status"] == 0 for d in done)
+    assert abs(by_tag[3]["energy"] - alone[0]["energy"]) < 1e-9
+    for k, (basis, (Z, xyz)) in enumerate(specs):
+        j = _serve_job(basis, Z, xyz)  # plain FP64, alone
+        j.construct()
+        e, _ = j.solve()
+        assert abs(by_tag[k]["energy"] - e) < (1e-9 if fp32 == 0.0 else 1e-6), (k, basis)

@@ -158,3 +158,23 @@ def test_host_eigensolver_degenerate_and_split():
     w, V = ds.syev(B)
     assert np.max(np.abs(w - np.linalg.eigvalsh(B))) < 1e-12
     assert np.max(np.abs(B @ V - V * w)) < 1e-12
+
+
+@pytest.mark.timeout(60)
+def test_server_rounds_return_every_job():
+    """The serving loop's host side (worker construction, admission in
+    submission order, retirement, poll) hands back every submitted job.  In
+    a container without a GPU each job's device preparation fails with
+    DSSCF_ERR_HIP; on a GPU host the jobs converge."""
+    srv = ds.Server(0, budget_bytes=1e9, max_active=2, workers=2)
+    for k, (basis, (Z, xyz)) in enumerate([("STO-3G", G.water()), ("6-31G", G.water_box(2, seed=1)),
+                                           ("STO-3G", G.alkane(1))]):
+        job = ds.Job(0).set_options(max_iterations=50).set_basis(basis).set_molecule(Z, xyz)
+        srv.submit(job, k)
+    done = srv.run_until_idle()
+    srv.close()
+    assert sorted(d["tag"] for d in done) == [0, 1, 2]
+    st = {d["status"] for d in done}
+    assert st in ({0}, {2})
+    for d in done:
+        assert d["t_submit"] <= d["t_done"]

@@ -231,7 +231,7 @@ def serve_stream(ds, arrivals, reqs, fp32: float, max_active: int, budget: float
         job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12,
                                     fp32_threshold=fp32)
         jobs.append(job.set_basis(basis).set_molecule(Z, xyz))
-    srv = ds.Server(0, budget_bytes=budget, max_active=max_active, workers=4)
+    srv = ds.Server(0, budget_bytes=budget, max_active=max_active, workers=12)
     n = len(reqs)
     t0 = srv.clock()
     i, done = 0, {}
@@ -247,6 +247,7 @@ def serve_stream(ds, arrivals, reqs, fp32: float, max_active: int, budget: float
         srv.step()
         for d in srv.poll():
             done[d["tag"]] = d
+    stats = srv.stats()
     srv.close()
     lat = np.array([done[k]["t_done"] - t0 - arrivals[k] for k in range(n)])
     e = np.array([done[k]["energy"] for k in range(n)])
@@ -254,7 +255,7 @@ def serve_stream(ds, arrivals, reqs, fp32: float, max_active: int, budget: float
     wall = max(d["t_done"] for d in done.values()) - t0 - arrivals[0]
     for j in jobs:
         j.close()
-    return lat, e, st, wall
+    return lat, e, st, wall, stats
 
 
 def bench_serve(a):
@@ -269,10 +270,10 @@ def bench_serve(a):
     arrivals, reqs = serve_requests(a.requests, a.rate)
     runs = {}
     for mode, thr in (("mixed", 1e-5), ("fp64", 0.0)):
-        lat, e, st, wall = serve_stream(ds, arrivals, reqs, thr, max_active=64, budget=32e9)
+        lat, e, st, wall, stats = serve_stream(ds, arrivals, reqs, thr, max_active=64, budget=32e9)
         runs[mode] = {"p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                       "mean_latency_s": float(lat.mean()), "solves_per_s": len(reqs) / wall, "wall_s": wall,
-                      "failed": int(np.count_nonzero(st)), "_e": e}
+                      "failed": int(np.count_nonzero(st)), "server": stats, "_e": e}
     de = np.abs(runs["mixed"].pop("_e") - runs["fp64"].pop("_e"))
     n_ao = []
     for basis, Z, xyz in reqs:
@@ -287,7 +288,7 @@ def bench_serve(a):
         "data": "synthetic (seeded Poisson stream of water clusters and alkanes)",
         "config": {"workload": f"{len(reqs)} RHF requests, Poisson arrivals at {a.rate:g}/s (seed 7), STO-3G / "
                                "6-31G / def2-SVP, 1-12 waters or C1-C12 alkanes; server: 64 slots, 32 GB budget, "
-                               "4 construction workers",
+                               "12 host workers",
                    "n_ao_min": int(min(n_ao)), "n_ao_max": int(max(n_ao)), "parallelism": "single GPU"},
         "mixed": runs["mixed"], "fp64": runs["fp64"],
         "p50_latency_s": runs["mixed"]["p50_latency_s"], "p99_latency_s": runs["mixed"]["p99_latency_s"],

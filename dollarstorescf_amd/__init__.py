@@ -81,7 +81,7 @@ def lib():
             "dsscf_syev": [i, p, p, p], "dsscf_jobs_construct": [p, i], "dsscf_jobs_solve": [p, i, i, i, p, p, p],
             "dsscf_server_create": [p, i, d, i, i], "dsscf_server_destroy": [p], "dsscf_server_submit": [p, p, ctypes.c_longlong],
             "dsscf_server_step": [p, p, p], "dsscf_server_poll": [p, i, p, p, p, p, p, p, p, p],
-            "dsscf_server_idle": [p],
+            "dsscf_server_idle": [p], "dsscf_server_stats": [p, p],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -189,6 +189,12 @@ class Server:
     def idle(self) -> bool:
         return lib().dsscf_server_idle(self._h) == 1
 
+    def stats(self) -> dict:
+        out = np.zeros(5)
+        _check(lib().dsscf_server_stats(self._h, _p(out)))
+        return dict(zip(["rounds", "set_seconds", "fock_seconds", "host_seconds", "jobs_per_round"],
+                        [out[0], out[1], out[2], out[3], out[4] / max(out[0], 1.0)]))
+
     def run_until_idle(self):
         """Step until every submitted job is back; returns them."""
         out = []
@@ -196,7 +202,7 @@ class Server:
             a, w = self.step()
             out += self.poll()
             if a == 0 and w > 0:
-                time.sleep(0.0005)  # jobs still constructing on the workers
+                time.sleep(0.0005)  # jobs still constructing on the workers (step waits itself otherwise)
         return out + self.poll()
 
 

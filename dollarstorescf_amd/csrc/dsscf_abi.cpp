@@ -105,12 +105,21 @@ struct Plan {
     hipEvent_t fork = nullptr, join[kStreams - 1] = {};
     std::vector<hipEvent_t> slot_ev;  // start/end of each slot's kernel (timed builds)
 
-    void release()
+    size_t cap_segs = 0, cap_blk0 = 0, cap_mat = 0;  // device table capacities (grow only)
+
+    // Forget the launch (graph, built flag); keep tables and streams for the next build.
+    void reset()
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
+        built = false;
+    }
+
+    void release()
+    {
+        reset();
         for (int i = 0; i < kStreams - 1; ++i) {
             if (side[i]) (void)hipStreamDestroy(side[i]);
             if (join[i]) (void)hipEventDestroy(join[i]);
@@ -124,7 +133,7 @@ struct Plan {
         dev_free(d_segs);
         dev_free(d_blk0);
         dev_free(d_mat);
-        built = false;
+        cap_segs = cap_blk0 = cap_mat = 0;
     }
 };
 // Fisher-Yates shuffle driven by splitmix64 (deterministic for a seed).
@@ -669,7 +678,7 @@ namespace {
 int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt, const double* const* dPa,
                double* const* dJ, double* const* dK, const int* bs, hipStream_t stream)
 {
-    p.release();
+    p.reset();
     const int ncls = dk_n_class();
     p.ncls = ncls;
     p.boys_mode = ctxs[0]->opt.boys_mode;
@@ -776,9 +785,18 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     p.nmat = n;
     p.total_rows = row;
     p.maxN = maxN;
-    if (int rc = dev_alloc(&p.d_segs, p.h_segs.size())) return rc;
-    if (int rc = dev_alloc(&p.d_blk0, p.h_blk0.size())) return rc;
-    if (int rc = dev_alloc(&p.d_mat, p.h_mat.size())) return rc;
+    auto grow = [](auto** ptr, size_t& cap, size_t need) -> int {
+        if (need <= cap && *ptr) return DSSCF_OK;
+        dev_free(*ptr);
+        cap = 0;
+        const size_t n = std::max<size_t>(need + need / 2, 16);
+        if (int rc = dev_alloc(ptr, n)) return rc;
+        cap = n;
+        return DSSCF_OK;
+    };
+    if (int rc = grow(&p.d_segs, p.cap_segs, p.h_segs.size())) return rc;
+    if (int rc = grow(&p.d_blk0, p.cap_blk0, p.h_blk0.size())) return rc;
+    if (int rc = grow(&p.d_mat, p.cap_mat, p.h_mat.size())) return rc;
     if (!p.h_segs.empty()) {
         HIP_TRY(hipMemcpyAsync(p.d_segs, p.h_segs.data(), sizeof(JKSeg) * p.h_segs.size(), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(p.d_blk0, p.h_blk0.data(), sizeof(int) * p.h_blk0.size(), hipMemcpyHostToDevice, stream));
@@ -1188,6 +1206,7 @@ struct dsscf_batch {
     size_t total = 0;         // packed elements over all contexts
     double* d_buf = nullptr;  // [Pt | Pa | J | K], each `total`
     double* h_pin = nullptr;  // pinned staging, same layout
+    size_t cap = 0;           // elements of each of the four buffer sections allocated (grow only)
     Plan plan;
     int use_graph = 1;
     std::vector<int> block_size;
@@ -1198,6 +1217,7 @@ struct dsscf_batch {
         dev_free(d_buf);
         if (h_pin) (void)hipHostFree(h_pin);
         h_pin = nullptr;
+        cap = 0;
     }
 };
 
@@ -1260,7 +1280,7 @@ int dsscf_batch_set(dsscf_batch* b, dsscf_ctx* const* ctxs, int n)
     }
     HIP_TRY(hipSetDevice(b->device));
     HIP_TRY(hipStreamSynchronize(b->stream));
-    b->release_buffers();
+    b->plan.reset();  // tables, streams and (if large enough) buffers are reused
     b->ctxs.assign(ctxs, ctxs + n);
     b->off.resize(n);
     b->total = 0;
@@ -1268,8 +1288,16 @@ int dsscf_batch_set(dsscf_batch* b, dsscf_ctx* const* ctxs, int n)
         b->off[i] = b->total;
         b->total += (size_t)ctxs[i]->n_ao * (ctxs[i]->n_ao + 1) / 2;
     }
-    if (int rc = dev_alloc(&b->d_buf, 4 * b->total)) return rc;
-    HIP_TRY(hipHostMalloc((void**)&b->h_pin, sizeof(double) * 4 * b->total, hipHostMallocDefault));
+    if (b->total > b->cap || !b->d_buf) {
+        dev_free(b->d_buf);
+        if (b->h_pin) (void)hipHostFree(b->h_pin);
+        b->h_pin = nullptr;
+        b->cap = 0;
+        const size_t c = std::max<size_t>(b->total + b->total / 2, 1024);
+        if (int rc = dev_alloc(&b->d_buf, 4 * c)) return rc;
+        HIP_TRY(hipHostMalloc((void**)&b->h_pin, sizeof(double) * 4 * c, hipHostMallocDefault));
+        b->cap = c;
+    }
     // the contexts' own streams must be idle: the batch writes their buffers
     for (dsscf_ctx* c : b->ctxs) HIP_TRY(hipStreamSynchronize(c->stream));
     return batch_build_plan(b);

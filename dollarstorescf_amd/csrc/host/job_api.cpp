@@ -333,6 +333,13 @@ int dsscf_server_poll(dsscf_server* s, int cap, long long* tags, dsscf_job** job
 double dsscf_server_clock(dsscf_server* s) { return s ? s->s->clock() : 0.0; }
 int dsscf_server_idle(dsscf_server* s) { return s && s->s->idle() ? 1 : 0; }
 
+int dsscf_server_stats(dsscf_server* s, double* out5)
+{
+    if (!s || !out5) return DSSCF_ERR_ARG;
+    s->s->stats(out5);
+    return DSSCF_OK;
+}
+
 int dsscf_syev(int n, const double* A, double* w, double* V)
 {
     if (n < 0 || (n > 0 && (!A || !w || !V))) return DSSCF_ERR_ARG;

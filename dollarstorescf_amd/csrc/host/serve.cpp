@@ -4,16 +4,17 @@
 // one-electron terms, orthogonalization).  Each round (step) then:
 //  1. admits ready jobs in submission order while a slot is free and their
 //     estimated device footprint fits the budget; a worker prepares each one
-//     on the device (shell pairs, Schwarz bounds, quartet lists) while the
-//     round goes on;
+//     on the device (shell pairs, Schwarz bounds, quartet lists) while rounds
+//     go on;
 //  2. runs one fused Fock build (dsscf_batch_jk: one launch per quartet
-//     class over all of them) for every prepared job, from the core guess on;
-//  3. runs every job's host step in parallel (DIIS, diagonalization,
-//     density; or, after convergence, the energies of Energy.cpp:21-50);
-//  4. retires finished jobs: their device memory is freed and they are
-//     handed back through poll().
-// Jobs arriving while others iterate join the next round: the fused set
-// changes from round to round, so its launches are made directly (no graph).
+//     class) over every job whose next density is ready;
+//  3. hands each of those jobs to a worker for its host step (DIIS,
+//     diagonalization, density; or, after convergence, the energies of
+//     Energy.cpp:21-50), and returns: the next round fuses whichever jobs are
+//     ready then, so a large job's host step holds back no one else.
+// A finished job's worker frees its device memory and queues it for poll().
+// The fused set changes from round to round, so launches are made directly
+// (no graph); the batch keeps its buffers and tables between rounds.
 #include "serve.hpp"
 
 #include <omp.h>
@@ -54,14 +55,16 @@ double Server::clock() const
 
 void Server::worker_loop()
 {
+    omp_set_num_threads(1);  // the workers are the parallelism: one job per thread
     for (;;) {
         std::function<void()> f;
         {
             std::unique_lock<std::mutex> lk(mu_);
-            cv_.wait(lk, [this] { return stop_ || !tasks_.empty(); });
-            if (stop_ && tasks_.empty()) return;
-            f = std::move(tasks_.front());
-            tasks_.pop_front();
+            cv_.wait(lk, [this] { return stop_ || !tasks_.empty() || !steps_.empty(); });
+            if (stop_ && tasks_.empty() && steps_.empty()) return;
+            std::deque<std::function<void()>>& q = steps_.empty() ? tasks_ : steps_;
+            f = std::move(q.front());
+            q.pop_front();
             ++busy_tasks_;
         }
         f();
@@ -112,11 +115,12 @@ int Server::submit(void* handle, Job* job, long long tag)
         std::lock_guard<std::mutex> lk(mu_);
         if (rc) {
             waiting_.erase(std::find(waiting_.begin(), waiting_.end(), r));
-            retire_locked(r, rc);
+            retire_locked(r, rc);  // no context yet
             return;
         }
         r->est_bytes = estimate_bytes(*r->job);
         r->st = READY;
+        ready_cv_.notify_all();
     });
     return DSSCF_OK;
 }
@@ -137,10 +141,56 @@ void Server::retire_locked(Req* r, int status)
     delete r;
 }
 
+// Retire an active job from a worker: the context is destroyed without the
+// lock held (its buffers are in no fused build: the job was not a member of
+// any round since its last build completed).
+void Server::retire(Req* r, int status)
+{
+    if (r->job->ctx) {
+        dsscf_ctx_destroy(r->job->ctx);
+        r->job->ctx = nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    used_ -= r->est_bytes;
+    active_.erase(std::find(active_.begin(), active_.end(), r));
+    retire_locked(r, status);
+}
+
+// One job's host part after a fused Fock build (J, K at its current density).
+void Server::host_step(Req* r, bool final)
+{
+    const double t_start = clock();
+    Job& job = *r->job;
+    job.jk_valid = true;
+    int status = -1;  // -1: keep iterating
+    if (final) {
+        status = calculateEnergies(job);
+    } else if ((int)job.iteration >= job.opt.max_iterations) {
+        job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
+        status = DSSCF_ERR_NOCONV;
+    } else {
+        ++job.iteration;
+        const int rc = scfStep(job);
+        if (rc) status = rc;
+    }
+    const double dt = clock() - t_start;
+    if (status >= 0) {
+        retire(r, status);
+    } else {
+        std::lock_guard<std::mutex> lk(mu_);
+        r->st = converged(job) ? FINAL : ITERATING;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        st_host_ += dt;
+    }
+    ready_cv_.notify_all();
+}
+
 bool Server::idle()
 {
     std::lock_guard<std::mutex> lk(mu_);
-    return waiting_.empty() && active_.empty() && tasks_.empty() && busy_tasks_ == 0;
+    return waiting_.empty() && active_.empty() && tasks_.empty() && steps_.empty() && busy_tasks_ == 0;
 }
 
 int Server::step(int* n_active, int* n_waiting)
@@ -182,6 +232,7 @@ int Server::step(int* n_active, int* n_waiting)
                     r->st = FAILED;
                     r->status = rc;
                 }
+                ready_cv_.notify_all();
             });
         }
         // jobs whose device preparation failed
@@ -203,14 +254,20 @@ int Server::step(int* n_active, int* n_waiting)
     cv_.notify_all();
     int rc = DSSCF_OK;
     if (!members.empty()) {
-        if (!batch_)
+        if (!batch_) {
             if ((rc = dsscf_batch_create(&batch_, device_))) return rc;
+            // direct launches: a capture here would also be broken by the workers'
+            // device allocations and frees, which run while rounds go on
+            dsscf_batch_set_graph(batch_, 0);
+        }
+        const double ta = clock();
         if (dirty || members != members_) {
             std::vector<dsscf_ctx*> ctxs;
             for (Req* r : members) ctxs.push_back(r->job->ctx);
             if ((rc = dsscf_batch_set(batch_, ctxs.data(), (int)ctxs.size()))) return rc;
             members_ = members;
         }
+        const double tb = clock();
         std::vector<const double*> pt, pa;
         std::vector<double*> jj, kk;
         for (Req* r : members) {
@@ -220,40 +277,37 @@ int Server::step(int* n_active, int* n_waiting)
             kk.push_back(r->job->K.data());
         }
         if ((rc = dsscf_batch_jk(batch_, pt.data(), pa.data(), jj.data(), kk.data()))) return rc;
-        const int m = (int)members.size();
-        std::vector<int> out(m, -1);  // -1 keep iterating, else the final status
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int t = 0; t < m; ++t) {
-            Req* r = members[t];
-            Job& job = *r->job;
-            job.jk_valid = true;
-            if (r->st == FINAL) {
-                out[t] = calculateEnergies(job);
-                continue;
+        const double tc = clock();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (Req* r : members) {
+                const bool final = r->st == FINAL;
+                r->st = STEPPING;
+                steps_.push_back([this, r, final] { host_step(r, final); });
             }
-            if ((int)job.iteration >= job.opt.max_iterations) {
-                job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
-                out[t] = DSSCF_ERR_NOCONV;
-                continue;
-            }
-            ++job.iteration;
-            const int s = scfStep(job);
-            if (s) out[t] = s;
-            else if (converged(job)) r->st = FINAL;
+            st_rounds_ += 1.0;
+            st_set_ += tb - ta;
+            st_jk_ += tc - tb;
+            st_members_ += (double)members.size();
         }
-        std::lock_guard<std::mutex> lk(mu_);
-        for (int t = 0; t < m; ++t) {
-            if (out[t] < 0) continue;
-            Req* r = members[t];
-            used_ -= r->est_bytes;
-            active_.erase(std::find(active_.begin(), active_.end(), r));
-            retire_locked(r, out[t]);
-        }
+        cv_.notify_all();
     }
-    std::lock_guard<std::mutex> lk(mu_);
+    std::unique_lock<std::mutex> lk(mu_);
+    if (members.empty() && !active_.empty())  // nothing ready: wait briefly for a worker to finish something
+        ready_cv_.wait_for(lk, std::chrono::milliseconds(2));
     if (n_active) *n_active = (int)active_.size();
     if (n_waiting) *n_waiting = (int)waiting_.size();
     return rc;
+}
+
+void Server::stats(double* out5)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    out5[0] = st_rounds_;
+    out5[1] = st_set_;
+    out5[2] = st_jk_;
+    out5[3] = st_host_;
+    out5[4] = st_members_;
 }
 
 void Server::poll(std::vector<Done>& out, int cap)

@@ -35,17 +35,23 @@ public:
     // constructs it in the background and admits it when its device
     // footprint fits; handle and tag come back from poll().
     int submit(void* handle, Job* job, long long tag);
-    // One round: admit ready jobs, one fused Fock build over the jobs on the
-    // device, their host steps in parallel, retire finished jobs.
+    // One round: admit ready jobs, one fused Fock build over the jobs whose
+    // next density is ready, then hand each of them to a worker for its host
+    // step (the next round does not wait for those).
     int step(int* n_active, int* n_waiting);
     // Finished jobs, oldest first, at most cap.
     void poll(std::vector<Done>& out, int cap);
     double clock() const;
     int device() const { return device_; }
     bool idle();  // nothing queued, constructing, preparing or on the device
+    // rounds with a fused build, seconds in fused-set rebuilds, fused Fock
+    // builds, host steps; jobs per fused build (sum over rounds)
+    void stats(double* out5);
 
 private:
-    enum State { HOST, READY, PREPARING, ITERATING, FINAL, FAILED, RETIRED };
+    // ITERATING / FINAL: waiting for a Fock build (for the next iteration /
+    // for the energies); STEPPING: its host step runs on a worker.
+    enum State { HOST, READY, PREPARING, ITERATING, FINAL, STEPPING, FAILED, RETIRED };
     struct Req {
         void* handle;
         Job* job;
@@ -58,6 +64,8 @@ private:
     void worker_loop();
     void enqueue(std::function<void()> f);
     void retire_locked(Req* r, int status);
+    void retire(Req* r, int status);  // frees the context outside the lock
+    void host_step(Req* r, bool final);
     double estimate_bytes(const Job& job) const;
 
     int device_;
@@ -65,8 +73,10 @@ private:
     int max_active_;
     std::vector<std::thread> workers_;
     std::mutex mu_;
-    std::condition_variable cv_;
-    std::deque<std::function<void()>> tasks_;
+    std::condition_variable cv_;        // work for the workers
+    std::condition_variable ready_cv_;  // a job became ready (for admission or a Fock build)
+    std::deque<std::function<void()>> tasks_;   // construction and device preparation
+    std::deque<std::function<void()>> steps_;   // host steps (taken first)
     bool stop_ = false;
     int busy_tasks_ = 0;
     std::deque<Req*> waiting_;   // HOST or READY, submission order
@@ -75,6 +85,7 @@ private:
     dsscf_batch* batch_ = nullptr;
     std::vector<Req*> members_;  // the fused set of the last round
     bool batch_dirty_ = true;    // a job retired since the fused set was built
+    double st_rounds_ = 0, st_set_ = 0, st_jk_ = 0, st_host_ = 0, st_members_ = 0;
     std::chrono::steady_clock::time_point t0_;
 };
 

@@ -108,6 +108,10 @@ int dsscf_server_poll(dsscf_server* server, int cap, long long* tags, dsscf_job*
  * or on the device, else 0 (dsscf_server_idle). */
 double dsscf_server_clock(dsscf_server* server);
 int dsscf_server_idle(dsscf_server* server);
+/* out5: rounds with a fused build; seconds spent rebuilding the fused set,
+ * in fused Fock builds (upload, kernels, download) and in host steps; the
+ * sum over rounds of the jobs in the fused build. */
+int dsscf_server_stats(dsscf_server* server, double* out5);
 
 /* The SCF layer's symmetric eigensolver (the role of dspev_('V','U'),
  * src/lalib/Lapack.cpp:12-39): A is n x n row-major symmetric; w gets the

@@ -2,9 +2,11 @@
 
   python -m dollarstorescf_amd.build [--maxl L] [--jobs J]
 
-1. gen_eri.py writes the per-class device code into csrc/_gen/.
-2. hipcc compiles the HIP kernels and the C-ABI for gfx950 (packed FP32
-   instructions disabled), g++ compiles the host SCF layer with OpenMP.
+1. gen_eri.py writes the per-class device code into csrc/_gen/, and one
+   small translation unit per shell-quartet class (cls_NN_<class>.hip).
+2. hipcc compiles the class units, the dispatcher and the C-ABI for gfx950
+   (packed FP32 instructions disabled) in parallel; g++ compiles the host SCF
+   layer with OpenMP.
 3. hipcc links everything into dollarstorescf_amd/libdsscf.so (in-tree).
 """
 from __future__ import annotations
@@ -42,16 +44,43 @@ def _run(cmd):
     return r
 
 
+def class_units(maxl: int):
+    """One translation unit per shell-quartet class: it instantiates that
+    class's kernels and launchers (ClassLaunch in eri_class.hpp), so the
+    classes compile in parallel."""
+    sys.path.insert(0, CSRC)
+    try:
+        import gen_eri
+    finally:
+        sys.path.pop(0)
+    _, cls_list = gen_eri.classes(maxl)
+    units = []
+    for k, c in enumerate(cls_list):
+        name = "".join("spdfg"[l] for l in c)
+        path = os.path.join(GEN, "cls_%02d_%s.hip" % (k, name))
+        text = ("// Generated by build.py -- do not edit.\n#include \"eri_class.hpp\"\n"
+                "template struct ClassLaunch<%d, %d, %d, %d, %d>;\n" % (*c, 1 if gen_eri.is_wave(c) else 0))
+        if not os.path.exists(path) or open(path).read() != text:
+            with open(path, "w") as f:
+                f.write(text)
+        units.append(path)
+    for f in os.listdir(GEN):  # units of classes no longer built
+        if f.startswith("cls_") and os.path.join(GEN, f) not in units:
+            os.remove(os.path.join(GEN, f))
+    return units
+
+
 def build(maxl: int = DEFAULT_MAXL, jobs: int = 8, verbose: bool = False) -> str:
     os.makedirs(GEN, exist_ok=True)
     os.makedirs(OBJ, exist_ok=True)
     _run([sys.executable, os.path.join(CSRC, "gen_eri.py"), os.path.join(GEN, "eri_classes.inc"), str(maxl)])
     cmds = []
     objs = []
-    for s in HIP_SOURCES:
-        o = os.path.join(OBJ, os.path.basename(s) + ".o")
-        lang = ["-x", "hip"]
-        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, s), "-o", o])
+    # the largest classes (last in class order) first, so that they start at once
+    hip_units = class_units(maxl)[::-1] + [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    for src in hip_units:
+        o = os.path.join(OBJ, os.path.basename(src) + ".o")
+        cmds.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", src, "-o", o])
         objs.append(o)
     for s in HOST_SOURCES:
         o = os.path.join(OBJ, os.path.basename(s) + ".o")

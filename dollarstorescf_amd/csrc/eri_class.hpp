@@ -1,0 +1,335 @@
+// eri_class.hpp -- gfx950 kernels of the direct-SCF Fock build, per class:
+// included by one generated translation unit per shell-quartet class, which
+// instantiates ClassLaunch for that class (the classes compile in parallel).
+//
+// One thread per contracted shell quartet (thread-per-quartet): the quartet
+// lists are grouped by class (la lb|lc ld) and, inside a class, by the bra
+// and ket primitive-pair counts, so the 64 lanes of a wavefront run the same
+// primitive loop trip counts.  Per primitive quartet: Boys F_m(T), HGP/OS
+// VRR in registers, accumulation of [e0|f0]; per quartet: HRR, transform to
+// the reference's AO functions, J/K digestion with FP64 atomics into
+// half-matrices that a final kernel folds (J = Jh + Jh^T, K = Kh + Kh^T) into
+// packed-U output.
+#pragma once
+#include "device_common.hpp"
+#include "launch.hpp"
+
+#include "eri_classes.inc"
+
+template <int la, int lb, int lc, int ld, int BM, typename R>
+struct Quartet;
+#define DSSCF_QSPEC_1(a, b, c, d, name)
+#define DSSCF_QSPEC_0(a, b, c, d, name)                                                               \
+    template <int BM, typename R>                                                                     \
+    struct Quartet<a, b, c, d, BM, R> {                                                               \
+        static __device__ __forceinline__ void run(const ShellPair& bra, const ShellPair& ket,        \
+                                                   const PrimPair* __restrict__ bra_pp,               \
+                                                   const PrimPair* __restrict__ ket_pp,               \
+                                                   const double* __restrict__ btab, double* out)      \
+        {                                                                                             \
+            shell_quartet_##name<BM, R>(bra, ket, bra_pp, ket_pp, btab, out);                         \
+        }                                                                                             \
+    };
+#define DSSCF_QUARTET_SPECIALIZE(a, b, c, d, name, wave) DSSCF_QSPEC_##wave(a, b, c, d, name)
+DSSCF_FOR_EACH_CLASS(DSSCF_QUARTET_SPECIALIZE)
+
+#include "wave_kernel.hpp"
+
+__device__ __forceinline__ void atomic_add_f64(double* p, double v)
+{
+    // global_atomic_add_f64 (no CAS loop)
+    unsafeAtomicAdd(p, v);
+}
+
+// ----------------------------------------------------------------- J/K --
+// J_mn = sum_ls (mn|ls) Dt_ls, K_mn = sum_ls (ml|ns) Da_ls (ExactCoulomb.cpp:
+// 103-123).  A unique quartet (ab|cd) with weight g = 1/((1+d_ab)(1+d_cd)
+// (1+d_ab,cd)) stands for its 8 permutation images; half of them go into
+// Jh/Kh and the other half are the transposes added by the fold kernel:
+//   Jh[a][b] += 2g sum_cd v D_cd   Jh[c][d] += 2g sum_ab v D_ab
+//   Kh[a][c] += g v D_bd   Kh[b][c] += g v D_ad   Kh[a][d] += g v D_bc   Kh[b][d] += g v D_ac
+//
+// Work unit = one bra shell pair and a chunk of its ket pairs (bra-major):
+// the bra record and its primitive pairs are uniform across the block; the
+// Jh[a][b] sums are reduced across the block, the Kh rows of shells a and b
+// are accumulated in LDS and flushed once per block; only Jh[c][d] goes
+// straight to global memory per quartet.
+template <int la, int lb, int lc, int ld, int BM, typename R>
+__global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
+                                            int nseg, const double* __restrict__ btab, double scale)
+{
+    typedef ClassInfo<la, lb, lc, ld> CI;
+    constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
+    extern __shared__ double krow[];  // [(NA + NB) * N]: Kh rows of shell a, then shell b
+    __shared__ PrimPair braP[kBraStage];
+    const int si = find_segment_by_block(seg_blk0, nseg, (int)blockIdx.x);
+    const JKSeg& S = segs[si];
+    const int4 u = ldg(S.u + ((int)blockIdx.x - S.blk0));  // bra pair, first ket, ket count
+    const ShellPair* __restrict__ pairs = S.pairs;
+    const PrimPair* __restrict__ pp = S.pp;
+    const GPtr<double> Dt = gbl(S.Dt);
+    const GPtr<double> Da = gbl(S.Da);  // read-only in this kernel
+    const int N = S.N;
+    const size_t slice = (size_t)jk_slice(S.nslice) * N * N;
+    double* __restrict__ Jh = S.Jh + slice;
+    double* __restrict__ Kh = S.Kh + slice;
+    const GPtr<unsigned> kets = gbl(S.k);
+    const ShellPair bra = ldg(pairs + u.x);
+    const int a0 = bra.ao_a, b0 = bra.ao_b;
+    for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;
+    for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) braP[i] = ldg(pp + bra.prim_off + i);
+    double dab[NA * NB];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) dab[i * NB + j] = Dt[(size_t)(a0 + i) * N + b0 + j];
+    double jab[NA * NB];
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
+    const double gb = scale * (a0 == b0 ? 0.5 : 1.0);
+    __syncthreads();
+
+    for (int t = threadIdx.x; t < u.z; t += blockDim.x) {
+        const int ki = (int)kets[u.y + t];
+        const ShellPair ket = ldg(pairs + ki);
+        double v[CI::NAO];
+        Quartet<la, lb, lc, ld, BM, R>::run(bra, ket, braP, S.pp + ket.prim_off, btab, v);
+        const int c0 = ket.ao_a, d0 = ket.ao_b;
+        // (ab|ab): the bra and ket (sub-)pairs belong to one shell pair
+        const bool same = ket.ao_a == a0 && ket.ao_b == b0;
+        const double g = gb * (c0 == d0 ? 0.5 : 1.0) * (same ? 0.5 : 1.0);
+#define VIDX(i, j, k, l) ((((i) * NB + (j)) * NC + (k)) * ND + (l))
+        // Coulomb
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+            for (int l = 0; l < ND; ++l) {
+                const double dcd = Dt[(size_t)(c0 + k) * N + d0 + l];
+                double s = 0.0;
+#pragma unroll
+                for (int ij = 0; ij < NA * NB; ++ij) {
+                    const double x = v[ij * NC * ND + k * ND + l];
+                    jab[ij] += g * x * dcd;
+                    s += x * dab[ij];
+                }
+                atomic_add_f64(&Jh[(size_t)(c0 + k) * N + d0 + l], 2.0 * g * s);
+            }
+        // Exchange into the LDS rows
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {  // Kh[a][c] += g sum_bd v D_bd
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * Da[(size_t)(b0 + j) * N + d0 + l];
+                atomicAdd(&krow[i * N + c0 + k], g * s);
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {  // Kh[b][c] += g sum_ad v D_ad
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * Da[(size_t)(a0 + i) * N + d0 + l];
+                atomicAdd(&krow[(NA + j) * N + c0 + k], g * s);
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < ND; ++l) {
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {  // Kh[a][d] += g sum_bc v D_bc
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * Da[(size_t)(b0 + j) * N + c0 + k];
+                atomicAdd(&krow[i * N + d0 + l], g * s);
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {  // Kh[b][d] += g sum_ac v D_ac
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * Da[(size_t)(a0 + i) * N + c0 + k];
+                atomicAdd(&krow[(NA + j) * N + d0 + l], g * s);
+            }
+        }
+#undef VIDX
+    }
+    // Jh[a][b]: wavefront reduction, one atomic per wavefront and element
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) {
+        double x = jab[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if ((threadIdx.x & 63) == 0 && x != 0.0) atomic_add_f64(&Jh[(size_t)(a0 + k / NB) * N + b0 + k % NB], 2.0 * x);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) {
+        const double x = krow[i];
+        if (x != 0.0) {
+            const int r = i / N, col = i % N;
+            const int row = r < NA ? a0 + r : b0 + (r - NA);
+            atomic_add_f64(&Kh[(size_t)row * N + col], x);
+        }
+    }
+}
+
+// --------------------------------------------------------- ERI batch ---
+// Whole AO blocks of listed shell quartets (parity path for the unit API
+// flo::electronRepulsionIntegral, GaussianIntegral.cpp:561).
+template <int la, int lb, int lc, int ld, int BM>
+__global__ void __launch_bounds__(256) k_eri_block(const int4* __restrict__ items, int n,
+                                                   const ShellPair* __restrict__ pairs,
+                                                   const PrimPair* __restrict__ pp,
+                                                   const double* __restrict__ btab, double* __restrict__ out,
+                                                   double scale)
+{
+    typedef ClassInfo<la, lb, lc, ld> CI;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int4 it = items[t];
+    const ShellPair bra = pairs[it.x];
+    const ShellPair ket = pairs[it.y];
+    double v[CI::NAO];
+    Quartet<la, lb, lc, ld, BM, double>::run(bra, ket, pp + bra.prim_off, pp + ket.prim_off, btab, v);
+    double* o = out + (size_t)it.z;
+#pragma unroll
+    for (int i = 0; i < CI::NAO; ++i) o[i] = scale * v[i];
+}
+
+// --------------------------------------------------------- Schwarz -----
+// Q_ab = sqrt(max_{mu in a, nu in b} |(mu nu|mu nu)|) for each listed pair.
+template <int la, int lb, int BM>
+__global__ void __launch_bounds__(256) k_schwarz(const int* __restrict__ plist, int n,
+                                                 const ShellPair* __restrict__ pairs,
+                                                 const PrimPair* __restrict__ pp,
+                                                 const double* __restrict__ btab, double* __restrict__ Q,
+                                                 double scale)
+{
+    typedef ClassInfo<la, lb, la, lb> CI;
+    constexpr int NA = CI::NA, NB = CI::NB;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int pidx = plist[t];
+    const ShellPair pr = pairs[pidx];
+    double v[CI::NAO];
+    Quartet<la, lb, la, lb, BM, double>::run(pr, pr, pp + pr.prim_off, pp + pr.prim_off, btab, v);
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) mx = fmax(mx, fabs(scale * v[(i * NB + j) * NA * NB + i * NB + j]));
+    Q[pidx] = sqrt(mx);
+}
+
+
+// ------------------------------------------------------------ launches --
+template <int a, int b, int c, int d, int wave>
+template <int BM, int MODE>
+void ClassLaunch<a, b, c, d, wave>::wave_go(const void* items, int n, const ShellPair* pairs, const PrimPair* pp,
+                                            const double* btab, double* out, double scale, hipStream_t s)
+{
+    if constexpr (wave) {
+        constexpr int G = WaveGroup<a, b, c, d>::G;
+        const int ng = (n + G - 1) / G;
+        const int grd = ng < (1 << 20) ? ng : (1 << 20);
+        hipLaunchKernelGGL((k_wave<a, b, c, d, BM, MODE>), dim3(grd), dim3(64), 0, s, items, n, pairs, pp, btab,
+                           nullptr, nullptr, 0, out, scale);
+    }
+}
+
+// J/K over nseg context segments: thread-per-quartet classes one block of bs
+// threads per work unit (LDS: the Kh rows of the bra's two shells),
+// wave-cooperative classes one 64-lane block per WaveGroup::G quartets.
+template <int a, int b, int c, int d, int wave>
+hipError_t ClassLaunch<a, b, c, d, wave>::jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg,
+                                             int nblocks, int bs, int maxN, const double* btab, double scale,
+                                             hipStream_t s)
+{
+    if constexpr (wave) {
+        if (bm == BOYS_ACCURATE)
+            hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_ACCURATE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
+                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
+        else
+            hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_REFERENCE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
+                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
+    } else {
+        const dim3 blk(bs), grd(nblocks);
+        const size_t lds = sizeof(double) * (size_t)(ClassInfo<a, b, c, d>::NA + ClassInfo<a, b, c, d>::NB) * maxN;
+        if (fp32)
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, float>), grd, blk, lds, s, segs, blk0, nseg, btab,
+                               scale);
+        else if (bm == BOYS_ACCURATE)
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
+                               scale);
+        else
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
+                               scale);
+    }
+    return hipGetLastError();
+}
+
+template <int a, int b, int c, int d, int wave>
+hipError_t ClassLaunch<a, b, c, d, wave>::block(int bm, const int4* items, int n, const ShellPair* pairs,
+                                                const PrimPair* pp, const double* btab, double* out, double scale,
+                                                hipStream_t s)
+{
+    if constexpr (wave) {
+        if (bm == BOYS_ACCURATE)
+            wave_go<BOYS_ACCURATE, WAVE_BLOCK>(items, n, pairs, pp, btab, out, scale, s);
+        else
+            wave_go<BOYS_REFERENCE, WAVE_BLOCK>(items, n, pairs, pp, btab, out, scale, s);
+    } else {
+        const dim3 blk(256), grd((n + 255) / 256);
+        if (bm == BOYS_ACCURATE)
+            hipLaunchKernelGGL((k_eri_block<a, b, c, d, BOYS_ACCURATE>), grd, blk, 0, s, items, n, pairs, pp, btab,
+                               out, scale);
+        else
+            hipLaunchKernelGGL((k_eri_block<a, b, c, d, BOYS_REFERENCE>), grd, blk, 0, s, items, n, pairs, pp, btab,
+                               out, scale);
+    }
+    return hipGetLastError();
+}
+
+// Schwarz bounds exist for the diagonal classes (ab|ab) only.
+template <int a, int b, int c, int d, int wave>
+hipError_t ClassLaunch<a, b, c, d, wave>::schwarz(int bm, const int* plist, int n, const ShellPair* pairs,
+                                                  const PrimPair* pp, const double* btab, double* Q, double scale,
+                                                  hipStream_t s)
+{
+    if constexpr (a != c || b != d) {
+        return hipErrorInvalidValue;
+    } else if constexpr (wave) {
+        if (bm == BOYS_ACCURATE)
+            wave_go<BOYS_ACCURATE, WAVE_SCHWARZ>(plist, n, pairs, pp, btab, Q, scale, s);
+        else
+            wave_go<BOYS_REFERENCE, WAVE_SCHWARZ>(plist, n, pairs, pp, btab, Q, scale, s);
+        return hipGetLastError();
+    } else {
+        const dim3 blk(256), grd((n + 255) / 256);
+        if (bm == BOYS_ACCURATE)
+            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_ACCURATE>), grd, blk, 0, s, plist, n, pairs, pp, btab, Q, scale);
+        else
+            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_REFERENCE>), grd, blk, 0, s, plist, n, pairs, pp, btab, Q, scale);
+        return hipGetLastError();
+    }
+}
+
+// This translation unit's copy of the AO transform (each unit has its own
+// __constant__ memory).
+template <int a, int b, int c, int d, int wave>
+hipError_t ClassLaunch<a, b, c, d, wave>::upload_xform(const double* x, int n, hipStream_t s)
+{
+    return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_ao_xform), x, sizeof(double) * n, 0, hipMemcpyHostToDevice, s);
+}
+
+template <int a, int b, int c, int d, int wave>
+int ClassLaunch<a, b, c, d, wave>::wave_group()
+{
+    if constexpr (wave) return WaveGroup<a, b, c, d>::G;
+    else return 1;
+}

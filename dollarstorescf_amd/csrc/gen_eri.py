@@ -365,8 +365,8 @@ def gen_class(c, spherical=True):
 # Wave-cooperative (LDS) form for the classes whose accumulators do not fit in
 # one thread's registers: the same VRR/HRR graph, emitted as level-ordered
 # tables that the 64 lanes of a wavefront walk together.
-REG_MAX_ACC = 100
-REG_MAX_HRR = 450
+REG_MAX_ACC = 300
+REG_MAX_HRR = 2000
 
 
 def is_wave(c):
@@ -585,7 +585,7 @@ def main(out_path, maxl=MAXL):
     text.append("#define DSSCF_NCLASS %d" % len(cls_list))
     text.append(gen_dispatch(cls_list))
     text.append("// Cartesian -> reference-AO coefficients, per l: (2l+1) x ncart(l), row-major.")
-    text.append("__constant__ double c_ao_xform[DSSCF_XFORM_SIZE];")
+    text.append("static __constant__ double c_ao_xform[DSSCF_XFORM_SIZE];  // one copy per translation unit")
     text.append("")
     text.extend(gen_ao_patterns(maxl))
     text.append("__device__ __forceinline__ constexpr int xform_off(int l) { return l == 0 ? 0 : (l == 1 ? 1 : (l == 2 ? 10 : 40)); }")

@@ -1,0 +1,23 @@
+// launch.hpp -- host launch interface of one shell-quartet class (defined in
+// eri_class.hpp and instantiated in that class's generated translation unit).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+template <int a, int b, int c, int d, int wave>
+struct ClassLaunch {
+    static hipError_t jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int bs,
+                         int maxN, const double* btab, double scale, hipStream_t s);
+    static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
+                            const double* btab, double* out, double scale, hipStream_t s);
+    static hipError_t schwarz(int bm, const int* plist, int n, const ShellPair* pairs, const PrimPair* pp,
+                              const double* btab, double* Q, double scale, hipStream_t s);
+    static hipError_t upload_xform(const double* x, int n, hipStream_t s);
+    static int wave_group();
+
+private:
+    template <int BM, int MODE>
+    static void wave_go(const void* items, int n, const ShellPair* pairs, const PrimPair* pp, const double* btab,
+                        double* out, double scale, hipStream_t s);
+};

@@ -81,7 +81,7 @@ def lib():
             "dsscf_syev": [i, p, p, p], "dsscf_jobs_construct": [p, i], "dsscf_jobs_solve": [p, i, i, i, p, p, p],
             "dsscf_server_create": [p, i, d, i, i], "dsscf_server_destroy": [p], "dsscf_server_submit": [p, p, ctypes.c_longlong],
             "dsscf_server_step": [p, p, p], "dsscf_server_poll": [p, i, p, p, p, p, p, p, p, p],
-            "dsscf_server_idle": [p], "dsscf_server_stats": [p, p],
+            "dsscf_server_idle": [p], "dsscf_server_stats": [p, p], "dsscf_one_electron": [p, i, p, p, p, p, p],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -293,6 +293,15 @@ class Context:
         out = np.empty(len(q))
         _check(lib().dsscf_eri_batch(self._h, _p(q), len(q), _p(out)))
         return out
+
+    def one_electron(self, Z, xyz_bohr, n_ao: int):
+        """S, T, V (packed-U) of this context's n_ao AOs and the given nuclei, on the device."""
+        Z = np.ascontiguousarray(Z, np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, np.float64).reshape(-1)
+        npk = n_ao * (n_ao + 1) // 2
+        S, T, V = np.empty(npk), np.empty(npk), np.empty(npk)
+        _check(lib().dsscf_one_electron(self._h, len(Z), _p(Z), _p(xyz), _p(S), _p(T), _p(V)))
+        return S, T, V
 
     def stats(self) -> Stats:
         st = Stats()

@@ -38,7 +38,7 @@ struct __align__(16) PrimPair {
     double P[3];
     double Kp;
     double e;    // screening factor |Kp| / p^(1/4); a shell pair's records are sorted by it, largest first
-    double pad;
+    double alpha;  // exponent of the first shell's primitive (p - alpha is the second's)
 };
 static_assert(sizeof(PrimPair) == 64, "PrimPair must be one 64 B record");
 

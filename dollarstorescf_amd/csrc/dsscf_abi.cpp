@@ -192,6 +192,8 @@ struct dsscf_ctx {
     int* d_plist = nullptr;
     double *d_Dt = nullptr, *d_Da = nullptr, *d_Jh = nullptr, *d_Kh = nullptr;
     double *d_Pt_pk = nullptr, *d_Pa_pk = nullptr, *d_J_pk = nullptr, *d_K_pk = nullptr;
+    double* d_xform = nullptr;                   // AO transform (one-electron kernel)
+    std::vector<int> pc_start, pc_count;         // pair classes in d_plist
 
     std::vector<int> block_size;  // per class, thread-per-quartet kernels (autotuned)
     std::vector<int> chunk;       // per class, kets per work unit
@@ -215,6 +217,7 @@ struct dsscf_ctx {
         dev_free(d_Pa_pk);
         dev_free(d_J_pk);
         dev_free(d_K_pk);
+        dev_free(d_xform);
         prepared = false;
     }
 };
@@ -379,6 +382,7 @@ void build_pairs(dsscf_ctx* c)
                     pp.Kp = c->prim_coef[c->sh_prim_off[f] + i] * c->prim_coef[c->sh_prim_off[s] + j] *
                             std::exp(-al * be * pp.inv_p * AB2) * pp.inv_p;
                     pp.e = std::fabs(pp.Kp) / std::sqrt(std::sqrt(pp.p));
+                    pp.alpha = al;
                     c->prims.push_back(pp);
                 }
             P.nprim = (int)c->prims.size() - P.prim_off;
@@ -443,6 +447,9 @@ int dsscf_prepare(dsscf_ctx* c)
     HIP_TRY(hipMemcpyAsync(c->d_pp, c->prims.data(), sizeof(PrimPair) * c->prims.size(), hipMemcpyHostToDevice,
                            c->stream));
     dsscf_upload_xform(c->xform.data(), (int)c->xform.size(), c->stream);
+    if (int rc = dev_alloc(&c->d_xform, c->xform.size())) return rc;
+    HIP_TRY(hipMemcpyAsync(c->d_xform, c->xform.data(), sizeof(double) * c->xform.size(), hipMemcpyHostToDevice,
+                           c->stream));
     HIP_TRY(hipGetLastError());
 
     // Schwarz bounds, one launch per pair class.
@@ -459,6 +466,9 @@ int dsscf_prepare(dsscf_ctx* c)
         }
         HIP_TRY(hipMemcpyAsync(c->d_plist, plist.data(), sizeof(int) * plist.size(), hipMemcpyHostToDevice,
                                c->stream));
+        c->pc_start = start;
+        c->pc_count.assign(npc, 0);
+        for (int pc = 0; pc < npc; ++pc) c->pc_count[pc] = (int)by_class[pc].size();
         for (int l1 = 0; l1 <= maxl; ++l1)
             for (int l2 = 0; l2 <= l1; ++l2) {
                 const int pc = pair_class_of(l1, l2);
@@ -1065,6 +1075,46 @@ int dsscf_time_jk_classes(dsscf_ctx* c, int iters, double* ms_per_class)
     }
     for (int k = 0; k < ncls; ++k) ms_per_class[k] = acc[k] / iters;
     return DSSCF_OK;
+}
+
+int dsscf_one_electron(dsscf_ctx* c, int n_atom, const int* Z, const double* xyz, double* S, double* T, double* V)
+{
+    if (!c || n_atom < 0 || (n_atom > 0 && (!Z || !xyz)) || !S || !T || !V)
+        return fail(DSSCF_ERR_ARG, "dsscf_one_electron: bad argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_one_electron: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t np = (size_t)c->n_ao * (c->n_ao + 1) / 2;
+    std::vector<double> at(4 * (size_t)std::max(n_atom, 1));
+    for (int i = 0; i < n_atom; ++i) {
+        at[4 * i] = xyz[3 * i];
+        at[4 * i + 1] = xyz[3 * i + 1];
+        at[4 * i + 2] = xyz[3 * i + 2];
+        at[4 * i + 3] = (double)Z[i];
+    }
+    double *d_at = nullptr, *d_out = nullptr;
+    if (int rc = dev_alloc(&d_at, at.size())) return rc;
+    if (int rc = dev_alloc(&d_out, 3 * np)) {
+        dev_free(d_at);
+        return rc;
+    }
+    int rc = DSSCF_OK;
+    hipError_t e = hipMemcpyAsync(d_at, at.data(), sizeof(double) * at.size(), hipMemcpyHostToDevice, c->stream);
+    const int maxl = dk_max_l();
+    for (int l1 = 0; l1 <= maxl && e == hipSuccess; ++l1)
+        for (int l2 = 0; l2 <= l1 && e == hipSuccess; ++l2) {
+            const int pc = pair_class_of(l1, l2);
+            e = dsscf_launch_one_electron(l1, l2, c->opt.boys_mode, c->d_plist + c->pc_start[pc], c->pc_count[pc],
+                                          c->d_pairs, c->d_pp, d_at, n_atom, c->d_btab, c->d_xform, d_out,
+                                          d_out + np, d_out + 2 * np, c->stream);
+        }
+    if (e == hipSuccess) e = hipMemcpyAsync(S, d_out, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(T, d_out + np, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(V, d_out + 2 * np, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(DSSCF_ERR_HIP, std::string("dsscf_one_electron: ") + hipGetErrorString(e));
+    dev_free(d_at);
+    dev_free(d_out);
+    return rc;
 }
 
 long long dsscf_ctx_device_bytes(dsscf_ctx* c)

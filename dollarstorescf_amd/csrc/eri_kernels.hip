@@ -66,6 +66,199 @@ __global__ void k_fold(const MatSeg* __restrict__ segs, int nseg)
     }
 }
 
+// ------------------------------------------------- one-electron terms ---
+// S, T and V over the AOs of one shell pair per thread (the pair's first shell
+// has the higher l), the roles of OSOverlap / OSKineticEnergy /
+// OSNuclearPotential (GaussianIntegral.cpp:29-64, 98-157, 225-291) with the
+// reference's PI.  Per primitive pair: 1-D Obara-Saika overlap tables give S
+// and, from the bra exponent, T; V runs the vertical recursion
+//   [e]^(m) = PA_d [e-1_d]^(m) - PC_d [e-1_d]^(m+1)
+//             + (e_d-1)/2p ([e-2_d]^(m) - [e-2_d]^(m+1)),  [0]^(m) = F_m(p PC^2)
+// on the first centre up to la+lb for every nucleus, accumulated over
+// primitives and nuclei; the transfer to the second shell,
+//   (a|b) = sum_k C(b,k) AB^(b-k) (a+k|0)  per direction,
+// and the AO transform follow once per pair.  Each packed-U element belongs
+// to one pair: plain stores.
+namespace one_e {
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int cart_off(int l) { return l * (l + 1) * (l + 2) / 6; }  // components of all lower l
+// x-major position of (x, y, l-x-y) within its shell
+__device__ __forceinline__ int cart_pos(int l, int x, int y)
+{
+    const int r = l - x;  // components with this x come after all with a larger x
+    return r * (r + 1) / 2 + (r - y);
+}
+__device__ __forceinline__ void cart_xyz(int l, int k, int& x, int& y, int& z)
+{
+    int r = 0;
+    while ((r + 1) * (r + 2) / 2 <= k) ++r;  // k in the block of x = l - r
+    x = l - r;
+    y = r - (k - r * (r + 1) / 2);
+    z = l - x - y;
+}
+__device__ __forceinline__ double binom(int n, int k) { return (n == 2 && k == 1) ? 2.0 : 1.0; }  // n <= 2
+}  // namespace one_e
+
+template <int la, int lb, int BM>
+__global__ void __launch_bounds__(64) k_one_electron(const int* __restrict__ plist, int n,
+                                                     const ShellPair* __restrict__ pairs,
+                                                     const PrimPair* __restrict__ pp,
+                                                     const double* __restrict__ atoms, int natom,
+                                                     const double* __restrict__ btab,
+                                                     const double* __restrict__ xform, double* __restrict__ S,
+                                                     double* __restrict__ T, double* __restrict__ V)
+{
+    using namespace one_e;
+    constexpr double kPi = 3.14159265358979;
+    constexpr int L = la + lb, NA = ncart(la), NB = ncart(lb);
+    constexpr int NE = cart_off(L + 1) - cart_off(la);  // components with la <= |e| <= L
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const ShellPair sp = pairs[plist[t]];
+    const double A[3] = {sp.A[0], sp.A[1], sp.A[2]};
+    const double B[3] = {sp.A[0] - sp.AB[0], sp.A[1] - sp.AB[1], sp.A[2] - sp.AB[2]};
+    double s[NA * NB], k[NA * NB], ve[NE];
+    for (int i = 0; i < NA * NB; ++i) s[i] = k[i] = 0.0;
+    for (int i = 0; i < NE; ++i) ve[i] = 0.0;
+    double th[L + 1][cart_off(L + 1)];
+    for (int ip = 0; ip < sp.nprim; ++ip) {
+        const PrimPair P = pp[sp.prim_off + ip];
+        const double p = P.p, al = P.alpha, o2p = 0.5 * P.inv_p;
+        const double K = P.Kp * p;  // w_a w_b exp(-ab/p |AB|^2)
+        // 1-D overlaps S1[x][i][j], i <= la+2, j <= lb
+        double S1[3][la + 3][lb + 1];
+        for (int x = 0; x < 3; ++x) {
+            const double PA = P.P[x] - A[x], PB = P.P[x] - B[x];
+            S1[x][0][0] = sqrt(kPi * P.inv_p);
+            for (int i = 1; i <= la + 2; ++i)
+                S1[x][i][0] = PA * S1[x][i - 1][0] + (i > 1 ? (i - 1) * S1[x][i - 2][0] * o2p : 0.0);
+            for (int j = 1; j <= lb; ++j)
+                for (int i = 0; i <= la + 2; ++i) {
+                    double w = PB * S1[x][i][j - 1];
+                    if (i > 0) w += i * S1[x][i - 1][j - 1] * o2p;
+                    if (j > 1) w += (j - 1) * S1[x][i][j - 2] * o2p;
+                    S1[x][i][j] = w;
+                }
+        }
+        for (int ia = 0; ia < NA; ++ia)
+            for (int ib = 0; ib < NB; ++ib) {
+                int a[3], b[3];
+                cart_xyz(la, ia, a[0], a[1], a[2]);
+                cart_xyz(lb, ib, b[0], b[1], b[2]);
+                double sd[3], dd[3];
+                for (int x = 0; x < 3; ++x) {
+                    const int i = a[x], j = b[x];
+                    sd[x] = S1[x][i][j];
+                    dd[x] = 4.0 * al * al * S1[x][i + 2][j] - 2.0 * al * (2 * i + 1) * S1[x][i][j] +
+                            (i > 1 ? (double)(i * (i - 1)) * S1[x][i - 2][j] : 0.0);
+                }
+                s[ia * NB + ib] += K * sd[0] * sd[1] * sd[2];
+                k[ia * NB + ib] += -0.5 * K * (dd[0] * sd[1] * sd[2] + sd[0] * dd[1] * sd[2] + sd[0] * sd[1] * dd[2]);
+            }
+        // nuclear attraction, every nucleus
+        const double PA[3] = {P.P[0] - A[0], P.P[1] - A[1], P.P[2] - A[2]};
+        for (int c = 0; c < natom; ++c) {
+            const double PC[3] = {P.P[0] - atoms[4 * c], P.P[1] - atoms[4 * c + 1], P.P[2] - atoms[4 * c + 2]};
+            const double R2 = PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2];
+            double F[L + 1];
+            boys<BM, L>(p * R2, btab, F);
+            for (int m = 0; m <= L; ++m) th[m][0] = F[m];
+            for (int l = 1; l <= L; ++l)
+                for (int kk = 0; kk < ncart(l); ++kk) {
+                    int e[3];
+                    cart_xyz(l, kk, e[0], e[1], e[2]);
+                    const int d = e[0] > 0 ? 0 : (e[1] > 0 ? 1 : 2);
+                    int e1[3] = {e[0], e[1], e[2]};
+                    e1[d] -= 1;
+                    const int i1 = cart_off(l - 1) + cart_pos(l - 1, e1[0], e1[1]);
+                    int i2 = -1;
+                    if (e1[d] > 0) {
+                        int e2[3] = {e1[0], e1[1], e1[2]};
+                        e2[d] -= 1;
+                        i2 = cart_off(l - 2) + cart_pos(l - 2, e2[0], e2[1]);
+                    }
+                    const int io = cart_off(l) + kk;
+                    for (int m = 0; m <= L - l; ++m) {
+                        double w = PA[d] * th[m][i1] - PC[d] * th[m + 1][i1];
+                        if (i2 >= 0) w += e1[d] * o2p * (th[m][i2] - th[m + 1][i2]);
+                        th[m][io] = w;
+                    }
+                }
+            const double pref = -atoms[4 * c + 3] * 2.0 * kPi * P.inv_p * K;
+            for (int i = 0; i < NE; ++i) ve[i] += pref * th[0][cart_off(la) + i];
+        }
+    }
+    // (a|b) from (a+k|0) for V
+    const double AB[3] = {sp.AB[0], sp.AB[1], sp.AB[2]};
+    double v[NA * NB];
+    for (int ia = 0; ia < NA; ++ia)
+        for (int ib = 0; ib < NB; ++ib) {
+            int a[3], b[3];
+            cart_xyz(la, ia, a[0], a[1], a[2]);
+            cart_xyz(lb, ib, b[0], b[1], b[2]);
+            double w = 0.0;
+            for (int kx = 0; kx <= b[0]; ++kx)
+                for (int ky = 0; ky <= b[1]; ++ky)
+                    for (int kz = 0; kz <= b[2]; ++kz) {
+                        const int ex = a[0] + kx, ey = a[1] + ky, ez = a[2] + kz, le = ex + ey + ez;
+                        double c = binom(b[0], kx) * binom(b[1], ky) * binom(b[2], kz);
+                        for (int q = 0; q < b[0] - kx; ++q) c *= AB[0];
+                        for (int q = 0; q < b[1] - ky; ++q) c *= AB[1];
+                        for (int q = 0; q < b[2] - kz; ++q) c *= AB[2];
+                        w += c * ve[cart_off(le) - cart_off(la) + cart_pos(le, ex, ey)];
+                    }
+            v[ia * NB + ib] = w;
+        }
+    // Cartesian -> AO and packed-U stores
+    const double* xa = xform + xform_off(la);
+    const double* xb = xform + xform_off(lb);
+    for (int ma = 0; ma < 2 * la + 1; ++ma)
+        for (int mb = 0; mb < 2 * lb + 1; ++mb) {
+            double ss = 0.0, tt = 0.0, vv = 0.0;
+            for (int ia = 0; ia < NA; ++ia) {
+                const double wa = xa[ma * NA + ia];
+                if (wa == 0.0) continue;
+                for (int ib = 0; ib < NB; ++ib) {
+                    const double w = wa * xb[mb * NB + ib];
+                    ss += w * s[ia * NB + ib];
+                    tt += w * k[ia * NB + ib];
+                    vv += w * v[ia * NB + ib];
+                }
+            }
+            const int i = sp.ao_a + ma, j = sp.ao_b + mb;
+            const size_t o = i >= j ? (size_t)i * (i + 1) / 2 + j : (size_t)j * (j + 1) / 2 + i;
+            S[o] = ss;
+            T[o] = tt;
+            V[o] = vv;
+        }
+}
+
+hipError_t dsscf_launch_one_electron(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
+                                     const PrimPair* pp, const double* atoms, int natom, const double* btab,
+                                     const double* xform, double* S, double* T, double* V, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 blk(64), grd((n + 63) / 64);
+#define DSSCF_ONE_E(A_, B_)                                                                                        \
+    if (la == A_ && lb == B_) {                                                                                  \
+        if (boys_mode == BOYS_ACCURATE)                                                                          \
+            hipLaunchKernelGGL((k_one_electron<A_, B_, BOYS_ACCURATE>), grd, blk, 0, s, plist, n, pairs, pp, atoms, \
+                               natom, btab, xform, S, T, V);                                                     \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_one_electron<A_, B_, BOYS_REFERENCE>), grd, blk, 0, s, plist, n, pairs, pp,     \
+                               atoms, natom, btab, xform, S, T, V);                                              \
+        return hipGetLastError();                                                                                \
+    }
+    DSSCF_ONE_E(0, 0)
+    DSSCF_ONE_E(1, 0)
+    DSSCF_ONE_E(1, 1)
+    DSSCF_ONE_E(2, 0)
+    DSSCF_ONE_E(2, 1)
+    DSSCF_ONE_E(2, 2)
+#undef DSSCF_ONE_E
+    return hipErrorInvalidValue;
+}
+
 // -------------------------------------------------------- host launch ---
 void dsscf_upload_xform(const double* x, int n, hipStream_t s)
 {

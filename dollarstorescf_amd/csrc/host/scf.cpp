@@ -176,9 +176,9 @@ int shellData(const Job& job, ShellData& out, std::string* error)
     return DSSCF_OK;
 }
 
-int constructBasisHost(Job& job)
+// AO list in atom -> shell -> m order (SCFBasis.cpp:107-140).
+static int buildAOs(Job& job)
 {
-    // SCFBasis.cpp:107-171
     for (size_t i = 0; i < job.molecule.size(); ++i)
         if (job.atom_bases[i].empty()) {
             job.error = "no basis assigned to atom " + std::to_string(i + 1);
@@ -199,8 +199,14 @@ int constructBasisHost(Job& job)
             }
         job.atom_basis.push_back((unsigned)job.basis.size());
     }
-    const int n = job.n = (int)job.basis.size();
-    flo::oneElectronMatrices(job.basis, job.molecule, job.opt.boys_mode, job.S, job.T, job.V);
+    job.n = (int)job.basis.size();
+    return DSSCF_OK;
+}
+
+// H = T + V, Loewdin X and the SCF buffers, with S, T, V in place.
+static void finishHost(Job& job)
+{
+    const int n = job.n;
     job.H.resize(job.S.size());
     for (size_t i = 0; i < job.H.size(); ++i) job.H[i] = job.T[i] + job.V[i];
     // Loewdin X = U s^-1/2 U^T (SCFBasis.cpp:85-100)
@@ -222,6 +228,36 @@ int constructBasisHost(Job& job)
     job.K.assign(np, 0.0);
     job.G.assign(np, 0.0);
     job.jk_valid = false;
+}
+
+int constructBasisHost(Job& job)
+{
+    // SCFBasis.cpp:107-171, every integral on the host
+    if (int rc = buildAOs(job)) return rc;
+    flo::oneElectronMatrices(job.basis, job.molecule, job.opt.boys_mode, job.S, job.T, job.V);
+    finishHost(job);
+    return DSSCF_OK;
+}
+
+// S, T, V (SCFBasis.cpp:56-83) by the device context of a prepared job.
+static int deviceOneElectron(Job& job)
+{
+    const size_t na = job.molecule.size(), np = (size_t)job.n * (job.n + 1) / 2;
+    std::vector<int> Z(na);
+    std::vector<double> xyz(3 * na);
+    for (size_t i = 0; i < na; ++i) {
+        Z[i] = (int)job.molecule.atoms[i].charge;
+        xyz[3 * i] = job.molecule.atoms[i].position.x;
+        xyz[3 * i + 1] = job.molecule.atoms[i].position.y;
+        xyz[3 * i + 2] = job.molecule.atoms[i].position.z;
+    }
+    job.S.assign(np, 0.0);
+    job.T.assign(np, 0.0);
+    job.V.assign(np, 0.0);
+    if (int rc = dsscf_one_electron(job.ctx, (int)na, Z.data(), xyz.data(), job.S.data(), job.T.data(), job.V.data())) {
+        job.error = dsscf_last_error();
+        return rc;
+    }
     return DSSCF_OK;
 }
 
@@ -256,8 +292,12 @@ int constructDevice(Job& job)
 
 int constructBasis(Job& job)
 {
-    if (int rc = constructBasisHost(job)) return rc;
-    return constructDevice(job);
+    // AOs on the host, the context on the device, then S, T, V there too
+    if (int rc = buildAOs(job)) return rc;
+    if (int rc = constructDevice(job)) return rc;
+    if (int rc = deviceOneElectron(job)) return rc;
+    finishHost(job);
+    return DSSCF_OK;
 }
 
 // ----------------------------------------------------------- SCFSolver --

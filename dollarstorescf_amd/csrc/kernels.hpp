@@ -27,5 +27,10 @@ hipError_t dsscf_launch_eri_block(int cls, int boys_mode, const int4* items, int
 hipError_t dsscf_launch_schwarz(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
                                 const PrimPair* pp, const double* btab, double* Q, double scale, hipStream_t s);
 hipError_t dsscf_launch_fill(double* p, size_t n, double v, hipStream_t s);
+// S, T, V (packed-U) of the pairs plist[0..n) of pair class (la, lb), la >= lb;
+// atoms: x, y, z, Z per nucleus; xform: the AO transform (device copy).
+hipError_t dsscf_launch_one_electron(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
+                                     const PrimPair* pp, const double* atoms, int natom, const double* btab,
+                                     const double* xform, double* S, double* T, double* V, hipStream_t s);
 hipError_t dsscf_launch_prep(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);
 hipError_t dsscf_launch_fold(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);

@@ -136,6 +136,14 @@ int dsscf_get_jk(dsscf_ctx* ctx, double* J, double* K);
  * summed) to kernel_ms[class] (dsscf_class_count() entries, not cleared). */
 int dsscf_jk_resident_timed(dsscf_ctx* ctx, double* kernel_ms);
 
+/* One-electron matrices over the context's AOs (packed-U, N(N+1)/2 each):
+ * overlap S, kinetic T and nuclear attraction V of the n_atom nuclei
+ * (charges Z, positions xyz in bohr).  Replaces the host integrals of
+ * SCFBasis.cpp:56-83 (flo::overlapIntegral, kineticEnergyIntegral,
+ * nuclearAttractionIntegral, GaussianIntegral.cpp:29-332).  Needs
+ * dsscf_prepare; synchronous. */
+int dsscf_one_electron(dsscf_ctx* ctx, int n_atom, const int* Z, const double* xyz, double* S, double* T, double* V);
+
 /* (ij|kl) for n AO quartets ijkl[4*t..4*t+3], written to out[t].  The unit
  * API flo::electronRepulsionIntegral(mu, sigma, nu, tau) = (mu nu|sigma tau)
  * (src/chem/GaussianIntegral.cpp:561-642, GaussianIntegral.hpp:12) maps to

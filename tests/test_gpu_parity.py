@@ -1,4 +1,7 @@
 """GPU parity of the Fock-build hot path against the oracle (through the C-ABI)."""
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -303,3 +306,26 @@ def test_server_result_same_alone_and_in_mix(fp32):
         j.construct()
         e, _ = j.solve()
         assert abs(by_tag[k]["energy"] - e) < (1e-9 if fp32 == 0.0 else 1e-6), (k, basis)
+
+
+@pytest.mark.parametrize("key", ["water_sto3g", "water_def2svp"])
+def test_one_electron_device_matches_reference_fixture(key):
+    """S, T, V computed on the device (reference Boys series) against the
+    matrices the reference's own integral routines produced (1e-12)."""
+    c = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_integrals.json")))[key]
+    job = ds.Job(0).set_options(boys_mode=ds.BOYS_REFERENCE, schwarz_threshold=0.0)
+    job.set_basis(c["basis"]).set_molecule(np.array(c["Z"], np.int32), np.array(c["xyz_bohr"])).construct()
+    for name in ("S", "T", "V"):
+        assert np.max(np.abs(job.get(name) - np.array(c[name]))) < 1e-12, name
+
+
+@pytest.mark.parametrize("basis,mol", [("6-31GP", "water3"), ("def2-SVP", "alkane3"), ("STO-3G", "alkane1")])
+def test_one_electron_device_matches_host(basis, mol):
+    """Device S, T, V (Job.construct) equal the host restatement
+    (Job.construct_host) in the accurate Boys mode, d shells included."""
+    Z, xyz = G.water_box(3, seed=5) if mol == "water3" else G.alkane(int(mol[-1]))
+    dev = ds.Job(0).set_basis(basis).set_molecule(Z, xyz).construct()
+    host = ds.Job(0).set_basis(basis).set_molecule(Z, xyz).construct_host()
+    for name in ("S", "T", "V", "H"):
+        a, b = dev.get(name), host.get(name)
+        assert np.max(np.abs(a - b)) < 1e-12 * max(1.0, np.abs(b).max()), name

@@ -12,6 +12,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <atomic>
+#include <thread>
 
 #include "boys_table.hpp"
 #include "kernels.hpp"
@@ -533,113 +535,214 @@ int dsscf_prepare(dsscf_ctx* c)
     };
 
     c->chunk.assign(ncls, 512);
-    std::vector<uint2> all;      // flat lists (wave-cooperative classes)
-    std::vector<int4> units, units32;
-    std::vector<unsigned> kets, k64, k32, bsub, ksub;
     for (auto* v : {&c->q_off, &c->q_cnt, &c->q_prim, &c->q_off32, &c->q_cnt32, &c->q_prim32, &c->u_off, &c->u_cnt,
                     &c->u_off32, &c->u_cnt32})
         v->assign(ncls, 0);
     c->n_unscreened = 0;
     c->n_work_items = 0;
-    auto by_cost = [](const int4& x, const int4& y) { return x.w > y.w; };
+
+    // Per class: its pair classes, work-unit size and sub-pair run length.
+    struct ClassJob {
+        int X, Y, cls, P, kChunk;
+        bool wave, split;
+    };
+    std::vector<ClassJob> cj;
     for (int X = 0; X < npc; ++X)
         for (int Y = 0; Y <= X; ++Y) {
-            const int cls = X * (X + 1) / 2 + Y;
-            const bool wave = dsscf_class_is_wave(cls) != 0;
-            const bool split = thr32 > 0.0 && !wave;
-            const std::vector<int>& SX = by_class[X];
-            const std::vector<int>& SY = by_class[Y];
-            c->n_unscreened += X == Y ? (long long)SX.size() * (SX.size() + 1) / 2 : (long long)SX.size() * SY.size();
-            const std::vector<int> GY = groups(SY);
+            ClassJob j{X, Y, X * (X + 1) / 2 + Y, 0, 0, false, false};
+            j.wave = dsscf_class_is_wave(j.cls) != 0;
+            j.split = thr32 > 0.0 && !j.wave;
+            const size_t nx = by_class[X].size(), ny = by_class[Y].size();
+            const long long est = X == Y ? (long long)nx * (nx + 1) / 2 : (long long)nx * ny;
+            c->n_unscreened += est;
             // kets per work unit: enough units to fill the chip (~2048), 64..512, multiple of 64
-            const long long est = X == Y ? (long long)SX.size() * (SX.size() + 1) / 2 : (long long)SX.size() * SY.size();
-            const int kChunk = (int)std::min<long long>(512, std::max<long long>(64, (est / 2048 + 63) / 64 * 64));
-            c->chunk[cls] = kChunk;
+            j.kChunk = (int)std::min<long long>(512, std::max<long long>(64, (est / 2048 + 63) / 64 * 64));
+            c->chunk[j.cls] = j.kChunk;
             // primitive pairs per sub-pair, both sides: P^2 primitive quartets ~ item_flops,
             // and at least kMinSubPrims (shorter runs cost more in per-item digestion and
             // J/K atomics than they gain in balance)
             int l4[4];
-            dk_class_l(cls, l4);
+            dk_class_l(j.cls, l4);
             const double fq = prim_quartet_flops(l4[0], l4[1], l4[2], l4[3]);
-            const int P = item_flops < 0.0 ? (1 << 30) : std::max(kMinSubPrims, (int)std::sqrt(item_flops / fq));
-            c->q_off[cls] = (long long)all.size();
-            units32.clear();
-            const long long u0 = (long long)units.size();
-            for (size_t i = 0; i < SX.size(); ++i) {
-                const int bi = SX[i];
-                k64.clear();
-                k32.clear();
-                for (size_t gk = 0; gk + 1 < GY.size(); ++gk) {
-                    const int jend = X == Y ? std::min<int>(GY[gk + 1], (int)i + 1) : GY[gk + 1];
-                    for (int j = GY[gk]; j < jend; ++j) {
-                        const int kj = SY[j];
-                        const double bound = c->Q[bi] * c->Q[kj];
-                        if (thr > 0.0 && bound < thr) break;  // Q sorted desc in a group
-                        const long long np = (long long)c->pairs[bi].nprim * c->pairs[kj].nprim;
-                        if (split && bound < thr32) {
-                            k32.push_back((unsigned)kj);
-                            c->q_prim32[cls] += np;
-                        } else {
-                            k64.push_back((unsigned)kj);
-                            c->q_prim[cls] += np;
-                        }
-                    }
-                }
-                if (wave) {
-                    for (unsigned kj : k64) all.push_back(make_uint2((unsigned)bi, kj));
-                    continue;
-                }
-                c->q_cnt[cls] += (long long)k64.size();
-                c->q_cnt32[cls] += (long long)k32.size();
-                bsub.clear();
-                sub_pairs(bi, P, bsub);
-                for (int pass = 0; pass < 2; ++pass) {
-                    const std::vector<unsigned>& kk = pass ? k32 : k64;
-                    if (kk.empty()) continue;
-                    std::vector<int4>& uu = pass ? units32 : units;
-                    ksub.clear();
-                    for (unsigned kj : kk) sub_pairs((int)kj, P, ksub);
-                    // equal primitive counts side by side: uniform loop trips in a wavefront
-                    std::stable_sort(ksub.begin(), ksub.end(),
-                                     [&](unsigned x, unsigned y) { return work[x].nprim > work[y].nprim; });
-                    // Every bra's kets would otherwise come in nearly the same order, so the
-                    // blocks running at one time would add into the same J_cd elements at
-                    // the same moment; a per-bra shuffle inside each run of equal primitive
-                    // counts spreads those atomics over the matrix.
-                    for (size_t r0 = 0; r0 < ksub.size();) {
-                        size_t r1 = r0 + 1;
-                        while (r1 < ksub.size() && work[ksub[r1]].nprim == work[ksub[r0]].nprim) ++r1;
-                        shuffle_run(ksub.data() + r0, r1 - r0, 0x9e3779b97f4a7c15ULL * (unsigned long long)(bi + 1) + pass);
-                        r0 = r1;
-                    }
-                    const int k0 = (int)kets.size();
-                    kets.insert(kets.end(), ksub.begin(), ksub.end());
-                    for (unsigned sb : bsub) {
-                        c->n_work_items += (long long)ksub.size();
-                        for (size_t o = 0; o < ksub.size(); o += kChunk) {
-                            const int cnt = (int)std::min<size_t>(kChunk, ksub.size() - o);
-                            long long cost = 0;  // primitive quartets of the unit
-                            for (int t = 0; t < cnt; ++t) cost += work[ksub[o + t]].nprim;
-                            cost *= work[sb].nprim;
-                            uu.push_back(make_int4((int)sb, k0 + (int)o, cnt, (int)std::min<long long>(cost, 0x7fffffff)));
-                        }
+            j.P = item_flops < 0.0 ? (1 << 30) : std::max(kMinSubPrims, (int)std::sqrt(item_flops / fq));
+            cj.push_back(j);
+        }
+    // every sub-pair any class will use, created before the parallel part
+    for (const ClassJob& j : cj) {
+        if (j.wave) continue;
+        std::vector<unsigned> tmp;
+        for (int pc : {j.X, j.Y})
+            for (int pidx : by_class[pc]) {
+                tmp.clear();
+                sub_pairs(pidx, j.P, tmp);
+            }
+    }
+    auto sub_lookup = [&](int p, int P, std::vector<unsigned>& out) {  // read-only from here on
+        const int m = (c->pairs[p].nprim + P - 1) / P;
+        if (m <= 1) {
+            out.push_back((unsigned)p);
+            return;
+        }
+        int first = -1;
+        for (const auto& e : sub_cache[p])
+            if (e.first == P) first = e.second;
+        for (int k = 0; k < m; ++k) out.push_back((unsigned)(first + k));
+    };
+
+    // Tasks: a class's bras cut into ranges of about equal quartet counts; each
+    // task builds its own lists, merged below in class and range order.
+    struct Task {
+        int job, i0, i1;
+        std::vector<uint2> all;
+        std::vector<int4> units, units32;  // ket offsets local to `kets`
+        std::vector<unsigned> kets;
+        long long q_cnt = 0, q_cnt32 = 0, q_prim = 0, q_prim32 = 0, items = 0;
+    };
+    std::vector<Task> tasks;
+    for (int jn = 0; jn < (int)cj.size(); ++jn) {
+        const ClassJob& j = cj[jn];
+        const int nx = (int)by_class[j.X].size();
+        const long long ny = (long long)by_class[j.Y].size();
+        const long long total = j.X == j.Y ? (long long)nx * (nx + 1) / 2 : (long long)nx * ny;
+        const long long per = std::max<long long>(1 << 18, total / 64);
+        long long acc = 0;
+        int i0 = 0;
+        for (int i = 0; i < nx; ++i) {
+            acc += j.X == j.Y ? (long long)i + 1 : ny;
+            if (acc >= per || i == nx - 1) {
+                Task t;
+                t.job = jn;
+                t.i0 = i0;
+                t.i1 = i + 1;
+                tasks.push_back(std::move(t));
+                i0 = i + 1;
+                acc = 0;
+            }
+        }
+    }
+    auto run_task = [&](Task& t) {
+        const ClassJob& j = cj[t.job];
+        const std::vector<int>& SX = by_class[j.X];
+        const std::vector<int>& SY = by_class[j.Y];
+        const std::vector<int> GY = groups(SY);
+        std::vector<unsigned> k64, k32, bsub, ksub;
+        for (int i = t.i0; i < t.i1; ++i) {
+            const int bi = SX[i];
+            k64.clear();
+            k32.clear();
+            for (size_t gk = 0; gk + 1 < GY.size(); ++gk) {
+                const int jend = j.X == j.Y ? std::min<int>(GY[gk + 1], i + 1) : GY[gk + 1];
+                for (int jj = GY[gk]; jj < jend; ++jj) {
+                    const int kj = SY[jj];
+                    const double bound = c->Q[bi] * c->Q[kj];
+                    if (thr > 0.0 && bound < thr) break;  // Q sorted desc in a group
+                    const long long np = (long long)c->pairs[bi].nprim * c->pairs[kj].nprim;
+                    if (j.split && bound < thr32) {
+                        k32.push_back((unsigned)kj);
+                        t.q_prim32 += np;
+                    } else {
+                        k64.push_back((unsigned)kj);
+                        t.q_prim += np;
                     }
                 }
             }
-            if (wave) c->q_cnt[cls] = (long long)all.size() - c->q_off[cls];
-            // heaviest units first, so that the longest blocks start at once; units of
-            // equal cost (often chunks of one bra, which flush the same K rows) in
-            // shuffled order
-            shuffle_units(units.data() + u0, units.size() - (size_t)u0, 0x5851f42d4c957f2dULL + cls);
-            shuffle_units(units32.data(), units32.size(), 0x14057b7ef767814fULL + cls);
-            std::stable_sort(units.begin() + u0, units.end(), by_cost);
-            std::stable_sort(units32.begin(), units32.end(), by_cost);
-            c->u_off[cls] = u0;
-            c->u_cnt[cls] = (long long)units.size() - u0;
-            c->u_off32[cls] = (long long)units.size();
-            c->u_cnt32[cls] = (long long)units32.size();
-            units.insert(units.end(), units32.begin(), units32.end());
+            if (j.wave) {
+                for (unsigned kj : k64) t.all.push_back(make_uint2((unsigned)bi, kj));
+                continue;
+            }
+            t.q_cnt += (long long)k64.size();
+            t.q_cnt32 += (long long)k32.size();
+            bsub.clear();
+            sub_lookup(bi, j.P, bsub);
+            for (int pass = 0; pass < 2; ++pass) {
+                const std::vector<unsigned>& kk = pass ? k32 : k64;
+                if (kk.empty()) continue;
+                std::vector<int4>& uu = pass ? t.units32 : t.units;
+                ksub.clear();
+                for (unsigned kj : kk) sub_lookup((int)kj, j.P, ksub);
+                // equal primitive counts side by side: uniform loop trips in a wavefront
+                std::stable_sort(ksub.begin(), ksub.end(),
+                                 [&](unsigned x, unsigned y) { return work[x].nprim > work[y].nprim; });
+                // Every bra's kets would otherwise come in nearly the same order, so the
+                // blocks running at one time would add into the same J_cd elements at
+                // the same moment; a per-bra shuffle inside each run of equal primitive
+                // counts spreads those atomics over the matrix.
+                for (size_t r0 = 0; r0 < ksub.size();) {
+                    size_t r1 = r0 + 1;
+                    while (r1 < ksub.size() && work[ksub[r1]].nprim == work[ksub[r0]].nprim) ++r1;
+                    shuffle_run(ksub.data() + r0, r1 - r0, 0x9e3779b97f4a7c15ULL * (unsigned long long)(bi + 1) + pass);
+                    r0 = r1;
+                }
+                const int k0 = (int)t.kets.size();
+                t.kets.insert(t.kets.end(), ksub.begin(), ksub.end());
+                for (unsigned sb : bsub) {
+                    t.items += (long long)ksub.size();
+                    for (size_t o = 0; o < ksub.size(); o += j.kChunk) {
+                        const int cnt = (int)std::min<size_t>(j.kChunk, ksub.size() - o);
+                        long long cost = 0;  // primitive quartets of the unit
+                        for (int q = 0; q < cnt; ++q) cost += work[ksub[o + q]].nprim;
+                        cost *= work[sb].nprim;
+                        uu.push_back(make_int4((int)sb, k0 + (int)o, cnt, (int)std::min<long long>(cost, 0x7fffffff)));
+                    }
+                }
+            }
         }
+    };
+    {
+        std::atomic<int> next{0};
+        auto worker = [&] {
+            for (int k; (k = next.fetch_add(1)) < (int)tasks.size();) run_task(tasks[k]);
+        };
+        const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
+        std::vector<std::thread> th;
+        for (int k = 1; k < std::min<int>(nt, (int)tasks.size()); ++k) th.emplace_back(worker);
+        worker();
+        for (auto& x : th) x.join();
+    }
+
+    // merge in class order: per class its FP64 units (heaviest first), then its FP32 units
+    std::vector<uint2> all;  // flat lists (wave-cooperative classes)
+    std::vector<int4> units, units32;
+    std::vector<unsigned> kets;
+    auto by_cost = [](const int4& x, const int4& y) { return x.w > y.w; };
+    size_t tk = 0;
+    for (int jn = 0; jn < (int)cj.size(); ++jn) {
+        const int cls = cj[jn].cls;
+        c->q_off[cls] = (long long)all.size();
+        const long long u0 = (long long)units.size();
+        units32.clear();
+        for (; tk < tasks.size() && tasks[tk].job == jn; ++tk) {
+            Task& t = tasks[tk];
+            const int kbase = (int)kets.size();
+            if (kets.size() + t.kets.size() > 0x7fffffffULL) return fail(DSSCF_ERR_UNSUPPORTED, "too many quartets");
+            kets.insert(kets.end(), t.kets.begin(), t.kets.end());
+            for (int4 u : t.units) units.push_back(make_int4(u.x, u.y + kbase, u.z, u.w));
+            for (int4 u : t.units32) units32.push_back(make_int4(u.x, u.y + kbase, u.z, u.w));
+            all.insert(all.end(), t.all.begin(), t.all.end());
+            c->q_cnt[cls] += t.q_cnt;
+            c->q_cnt32[cls] += t.q_cnt32;
+            c->q_prim[cls] += t.q_prim;
+            c->q_prim32[cls] += t.q_prim32;
+            c->n_work_items += t.items;
+            std::vector<unsigned>().swap(t.kets);
+            std::vector<int4>().swap(t.units);
+            std::vector<int4>().swap(t.units32);
+            std::vector<uint2>().swap(t.all);
+        }
+        if (cj[jn].wave) c->q_cnt[cls] = (long long)all.size() - c->q_off[cls];
+        // heaviest units first, so that the longest blocks start at once; units of
+        // equal cost (often chunks of one bra, which flush the same K rows) in
+        // shuffled order
+        shuffle_units(units.data() + u0, units.size() - (size_t)u0, 0x5851f42d4c957f2dULL + cls);
+        shuffle_units(units32.data(), units32.size(), 0x14057b7ef767814fULL + cls);
+        std::stable_sort(units.begin() + u0, units.end(), by_cost);
+        std::stable_sort(units32.begin(), units32.end(), by_cost);
+        c->u_off[cls] = u0;
+        c->u_cnt[cls] = (long long)units.size() - u0;
+        c->u_off32[cls] = (long long)units.size();
+        c->u_cnt32[cls] = (long long)units32.size();
+        units.insert(units.end(), units32.begin(), units32.end());
+    }
     if (work.size() > 0x7fffffffULL) return fail(DSSCF_ERR_UNSUPPORTED, "too many sub-pairs");
     if (work.size() > npair) {  // the JK kernels index the work pairs; Schwarz and ERI batches the first npair
         dev_free(c->d_pairs);

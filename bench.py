@@ -133,6 +133,45 @@ def bench_alkane(a):
     print(json.dumps(out))
 
 
+def bench_scf(a):
+    """Whole SCF solves from the core guess with the iteration's host part
+    (DIIS, X F X, diagonalization, density) on the CPU and on the device, on
+    the 8-water/6-31G lattice (104 AOs) and all-trans C20H42/def2-SVP (490 AOs).
+    Construction is not timed; the Fock builds are the same device path in
+    both modes."""
+    import dollarstorescf_amd as ds
+    from dollarstorescf_amd import geometry
+
+    Zw, xw = geometry.water()
+    for step in (False, True):  # load rocBLAS/rocSOLVER kernels and warm the device
+        ds.Job(0).set_options(device_step=step).set_basis("STO-3G").set_molecule(Zw, xw).construct().solve()
+    systems = [("water8_631g", "6-31G", geometry.water_lattice(1)), ("c20h42_def2svp", "def2-SVP", geometry.alkane(20))]
+    rows = {}
+    for name, basis, (Z, xyz) in systems:
+        row = {}
+        for mode, step in (("host_step", False), ("device_step", True)):
+            job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=80,
+                                        schwarz_threshold=1e-12, device_step=step)
+            job.set_basis(basis).set_molecule(Z, xyz).construct()
+            t0 = time.perf_counter()
+            e, it = job.solve()
+            dt = time.perf_counter() - t0
+            row[mode] = {"seconds": dt, "iterations": it, "energy": e, "ms_per_iteration": dt / (it + 1) * 1e3,
+                         "n_ao": job.n_ao}
+        row["speedup"] = row["host_step"]["seconds"] / row["device_step"]["seconds"]
+        row["dE"] = row["device_step"]["energy"] - row["host_step"]["energy"]
+        rows[name] = row
+    big = rows["c20h42_def2svp"]["device_step"]
+    out = {"metric": METRIC, "value": 1.0 / big["seconds"], "unit": "SCF solves/s", "n_gpus": 1, "steps": 1,
+           "warmup": 1, "ms_per_step": big["seconds"] * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded water lattice, all-trans alkane)",
+           "config": {"workload": "RHF SCF from the core guess to |dE| < 1e-8, device vs host iteration step: "
+                                  "8-water/6-31G (104 AOs) and C20H42/def2-SVP (490 AOs), Schwarz 1e-12, FP64",
+                      "parallelism": "single GPU"},
+           "systems": rows}
+    print(json.dumps(out))
+
+
 def batch_specs(n_jobs: int, seed: int = 2024):
     """n_jobs molecules of 1-6 rigid waters (uniform), each placed by rejection
     sampling in a 6 A box with minimum O-O distance 2.6 A and random
@@ -305,10 +344,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=5_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--boys", choices=["accurate", "reference"], default="accurate")
-    ap.add_argument("--workload", choices=["water8", "alkane", "batch", "serve"], default="water8",
+    ap.add_argument("--workload", choices=["water8", "alkane", "batch", "serve", "scf"], default="water8",
                     help="water8: the Fock-build benchmark (default); alkane: C20H42/def2-SVP, FP64 vs "
                          "FP32 below Schwarz 1e-5; batch: 512 RHF/6-31G* jobs of 1-6 waters, fused vs one "
-                         "job at a time; serve: Poisson stream of SCF requests, mixed precision vs FP64")
+                         "job at a time; serve: Poisson stream of SCF requests, mixed precision vs FP64; scf: whole "
+                         "SCF solves with the iteration step on the device vs the host")
     ap.add_argument("--jobs", type=int, default=512, help="jobs of the batch workload")
     ap.add_argument("--requests", type=int, default=200, help="requests of the serve workload")
     ap.add_argument("--rate", type=float, default=200.0, help="mean arrival rate of the serve workload (1/s)")
@@ -331,6 +371,9 @@ def main():
         return
     if a.workload == "serve":
         bench_serve(a)
+        return
+    if a.workload == "scf":
+        bench_scf(a)
         return
 
     import dollarstorescf_amd as ds

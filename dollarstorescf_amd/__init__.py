@@ -73,6 +73,7 @@ def lib():
             "dsscf_get_jk": [p, p, p], "dsscf_time_jk_classes": [p, i, p], "dsscf_time_jk": [p, i, i, p, p, p], "dsscf_class_l": [i, p],
             "dsscf_job_create": [p, i], "dsscf_job_destroy": [p],
             "dsscf_job_set_options": [p, i, d, i, i, d, i], "dsscf_job_set_precision": [p, d, d],
+            "dsscf_job_set_step": [p, i],
             "dsscf_job_set_molecule": [p, i, p, p, i, i], "dsscf_job_load_xyz": [p, c, i, i],
             "dsscf_job_set_basis_file": [p, c], "dsscf_job_construct": [p], "dsscf_job_construct_host": [p],
             "dsscf_job_ao": [p, i, p, p, p, p, p, p, p, p, p, i], "dsscf_job_n_ao": [p],
@@ -385,14 +386,16 @@ class Job:
     __del__ = close
 
     def set_options(self, diis_size=8, energy_threshold=1e-8, max_iterations=500, boys_mode=BOYS_ACCURATE,
-                    schwarz_threshold=1e-12, verbose=0, fp32_threshold=0.0, prim_threshold=None):
+                    schwarz_threshold=1e-12, verbose=0, fp32_threshold=0.0, prim_threshold=None, device_step=False):
         """prim_threshold defaults to 1e-17 in the accurate Boys mode and to 0 (every
-        primitive quartet, as the reference) in the reference mode."""
+        primitive quartet, as the reference) in the reference mode.  device_step runs
+        each iteration's DIIS, X F X, diagonalization and density on the device."""
         if prim_threshold is None:
             prim_threshold = DEFAULT_PRIM_THRESHOLD if boys_mode == BOYS_ACCURATE else 0.0
         _check(lib().dsscf_job_set_options(self._h, diis_size, energy_threshold, max_iterations, boys_mode,
                                            schwarz_threshold, verbose), self._err)
         _check(lib().dsscf_job_set_precision(self._h, fp32_threshold, prim_threshold), self._err)
+        _check(lib().dsscf_job_set_step(self._h, 1 if device_step else 0), self._err)
         return self
 
     def set_molecule(self, Z, xyz_bohr, charge=0, multiplicity=1):
@@ -436,9 +439,10 @@ class Job:
 
     def get(self, name: str) -> np.ndarray:
         n = self.n_ao
-        out = np.empty(n if name == "eps" else n * (n + 1) // 2)
+        size = {"eps": n, "C": n * n}.get(name, n * (n + 1) // 2)
+        out = np.empty(size)
         _check(lib().dsscf_job_get(self._h, name.encode(), _p(out)), self._err)
-        return out
+        return out.reshape(n, n) if name == "C" else out
 
     def set_density(self, Pa_packed):
         Pa = np.ascontiguousarray(Pa_packed, np.float64)

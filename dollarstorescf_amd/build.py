@@ -30,7 +30,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-I" + GEN, "-I" + CSRC, "-I" + os.path.join(ROOT, "include")]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-I" + os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["eri_kernels.hip", "dsscf_abi.cpp"]
+HIP_SOURCES = ["eri_kernels.hip", "dsscf_abi.cpp", "scf_device.hip"]
 HOST_SOURCES = ["host/chem.cpp", "host/linalg.cpp", "host/scf.cpp", "host/multi.cpp", "host/serve.cpp",
                 "host/job_api.cpp"]
 
@@ -91,7 +91,8 @@ def build(maxl: int = DEFAULT_MAXL, jobs: int = 8, verbose: bool = False) -> str
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout + r.stderr)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp",
+          "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -66,6 +66,15 @@ int dsscf_job_set_precision(dsscf_job* j, double fp32_threshold, double prim_thr
     return DSSCF_OK;
 }
 
+int dsscf_job_set_step(dsscf_job* j, int on_device)
+{
+    if (!j || (on_device != 0 && on_device != 1)) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_step: bad argument");
+    if (on_device && (j->job.opt.diis_size < 1 || j->job.opt.diis_size > 16))
+        return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_step: the device step keeps at most 16 DIIS vectors");
+    j->job.opt.device_step = on_device;
+    return DSSCF_OK;
+}
+
 int dsscf_job_set_molecule(dsscf_job* j, int natom, const int* Z, const double* xyz, int charge, int mult)
 {
     if (!j || natom <= 0 || !Z || !xyz || mult < 1) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_molecule: bad argument");
@@ -174,7 +183,9 @@ int dsscf_job_get(dsscf_job* j, const char* name, double* out)
     else if (n == "K") v = &b.K;
     else if (n == "G") v = &b.G;
     else if (n == "eps") v = &b.eps;
+    else if (n == "C") v = &b.C;
     if (!v) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_get: unknown matrix '" + n + "'");
+    if (v->empty()) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_get: '" + n + "' not computed yet");
     std::memcpy(out, v->data(), sizeof(double) * v->size());
     return DSSCF_OK;
 }

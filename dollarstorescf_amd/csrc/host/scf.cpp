@@ -33,6 +33,7 @@ void setCurrent(Job* job) { g_current = job; }
 
 Job::~Job()
 {
+    scfdev::destroy(dstep);
     if (ctx) dsscf_ctx_destroy(ctx);
 }
 
@@ -49,40 +50,51 @@ void DIIS::resize(int n)
 
 // DIIS.ipp:18-62: new vectors fill free slots; once full, the vector with
 // the largest B_ii is replaced.  B is bordered by ones with a zero corner.
-void DIIS::addErrorVector(const la::Mat& result, const la::Mat& error)
+int DIIS::reserveSlot()
 {
     const int W = size + 1;
-    double new_rms = 0.0;
-    for (double v : error) new_rms += v * v;
-    rms = new_rms;
-    int index = 0;
     int count = (int)err.size();
     if (count < size) {
         err.emplace_back();
         res.emplace_back();
-        index = count;
+        const int index = count;
         ++count;
         for (int i = 0; i < count; ++i) B[(size_t)i * W + count] = B[(size_t)count * W + i] = 1.0;
         B[(size_t)count * W + count] = 0.0;
-    } else {
-        double mx = 0.0;
-        for (int i = 0; i < size; ++i)
-            if (B[(size_t)i * W + i] > mx) {
-                mx = B[(size_t)i * W + i];
-                index = i;
-            }
+        last = index;
+        return index;
     }
+    int index = 0;
+    double mx = 0.0;
+    for (int i = 0; i < size; ++i)
+        if (B[(size_t)i * W + i] > mx) {
+            mx = B[(size_t)i * W + i];
+            index = i;
+        }
+    last = index;
+    return index;
+}
+
+void DIIS::setRow(int index, const double* dots)
+{
+    const int W = size + 1, count = (int)err.size();
+    rms = dots[index];
+    for (int i = 0; i < count; ++i) B[(size_t)i * W + index] = B[(size_t)index * W + i] = dots[i];
+}
+
+void DIIS::addErrorVector(const la::Mat& result, const la::Mat& error)
+{
+    const int index = reserveSlot();
     err[index] = error;
     res[index] = result;
-    last = index;
-    B[(size_t)index * W + index] = new_rms;
-    for (int i = 0; i < count; ++i) {
-        if (i == index) continue;
+    std::vector<double> dots(err.size());
+    for (size_t i = 0; i < err.size(); ++i) {
         double dot = 0.0;
         const la::Mat& e = err[i];
         for (size_t k = 0; k < e.size(); ++k) dot += e[k] * error[k];
-        B[(size_t)i * W + index] = B[(size_t)index * W + i] = dot;
+        dots[i] = dot;
     }
+    setRow(index, dots.data());
 }
 
 // DIIS.ipp:64-77: B c = (0,..,0,1).
@@ -387,8 +399,80 @@ bool converged(const Job& job)
            std::fabs(job.total_electronic_energy - job.previous_electronic_energy) < job.opt.energy_threshold;
 }
 
+// solveMOs with the iteration's host part on the device: the densities,
+// J/K, the DIIS history and the eigenproblem stay in HBM; per iteration the
+// host sees <e_new, e_i> (its B matrix and the bordered solve), the orbital
+// energies and tr(Pa G).  Same stopping test, DIIS rules and energy as the
+// host loop below.
+static int solveMOsDevice(Job& job)
+{
+    const int n = job.n;
+    const unsigned nocc = job.electron_count[0];
+    if (job.dstep) {
+        scfdev::destroy(job.dstep);
+        job.dstep = nullptr;
+    }
+    if (int rc = scfdev::create(&job.dstep, job.opt.device, n, (int)nocc, job.opt.diis_size, job.H.data(),
+                                job.Sd.data(), job.X.data(), &job.error))
+        return rc;
+    if (int rc = scfdev::set_density(job.dstep, job.Pa.data(), &job.error)) return rc;
+    std::vector<double> dots(job.opt.diis_size);
+    while (!converged(job)) {
+        if ((int)job.iteration >= job.opt.max_iterations) {
+            job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
+            return DSSCF_ERR_NOCONV;
+        }
+        ++job.iteration;
+        if (int rc = scfdev::fock_build(job.dstep, job.ctx, &job.error)) return rc;
+        double trPG = 0.0;
+        if (job.iteration == 1) {
+            if (int rc = scfdev::assemble(job.dstep, -1, 0, nullptr, nullptr, &job.error)) return rc;
+            if (int rc = scfdev::diagonalize_direct(job.dstep, job.eps, &trPG, &job.error)) return rc;
+        } else {
+            const int slot = job.diis.reserveSlot();
+            const int count = job.diis.count();
+            double rms = 0.0;
+            if (int rc = scfdev::assemble(job.dstep, slot, count, dots.data(), &rms, &job.error)) return rc;
+            job.diis.setRow(slot, dots.data());
+            job.diis.solve();
+            if (int rc = scfdev::extrapolate(job.dstep, job.diis.coef.data(), count, job.eps, &trPG, &job.error))
+                return rc;
+        }
+        double E = 0.0;
+        for (unsigned i = 0; i < nocc; ++i) E += 2.0 * job.eps[i];
+        E -= trPG;
+        job.previous_electronic_energy = job.total_electronic_energy;
+        job.total_electronic_energy = E;
+        job.history.push_back(E);
+        if (job.opt.verbose)
+            std::printf("%9u %22.12f %22.12f %14.6e\n", job.iteration, job.total_electronic_energy,
+                        job.total_electronic_energy - job.previous_electronic_energy, job.diis.rms);
+    }
+    // J and K at the final density for the energies, then everything the
+    // host API reads back
+    if (int rc = scfdev::fock_build(job.dstep, job.ctx, &job.error)) return rc;
+    job.C.resize((size_t)n * n);
+    if (int rc = scfdev::download(job.dstep, job.Pa.data(), job.Ptot.data(), job.J.data(), job.K.data(), nullptr,
+                                  job.C.data(), &job.error))
+        return rc;
+    for (size_t i = 0; i < job.G.size(); ++i) job.G[i] = job.J[i] - job.K[i];
+    job.jk_valid = true;
+    return calculateEnergies(job);
+}
+
 int solveMOs(Job& job)
 {
+    if (job.opt.device_step) {
+        job.iteration = 0;
+        job.total_electronic_energy = job.previous_electronic_energy = 0.0;
+        job.history.clear();
+        job.diis.resize(job.opt.diis_size);
+        if (!job.ctx) {
+            job.error = "solveMOs: construct the job on the device first";
+            return DSSCF_ERR_STATE;
+        }
+        return solveMOsDevice(job);
+    }
     // SCFSolver.cpp:196-237
     job.iteration = 0;
     job.total_electronic_energy = job.previous_electronic_energy = 0.0;

@@ -11,6 +11,7 @@
 #include "../../../include/dsscf.h"
 #include "chem.hpp"
 #include "linalg.hpp"
+#include "scf_device.hpp"
 
 namespace scf {
 
@@ -26,6 +27,7 @@ struct Options {
     double fp32_threshold = 0.0;       // quartet bound below which the VRR runs in FP32
     double prim_threshold = 0.0;       // primitive-quartet screening bound (0 = none)
     int device = 0;
+    int device_step = 0;               // 1: DIIS, X F X, diagonalization and density on the device
     int verbose = 0;                   // print the iteration table like SCFSolver.cpp:198-225
 };
 
@@ -39,6 +41,12 @@ struct DIIS {
     int last = 0;
     void resize(int n);
     void addErrorVector(const la::Mat& result, const la::Mat& error);
+    // addErrorVector in two halves, for error vectors kept elsewhere (the
+    // device step): the slot the new vector takes (a free one, else the one
+    // with the largest B_ii), then its row of B from <e_new, e_i>, i < count().
+    int reserveSlot();
+    void setRow(int index, const double* dots);
+    int count() const { return (int)err.size(); }
     bool solve();
     void calculateResult(la::Mat& out) const;
 };
@@ -73,6 +81,7 @@ struct Job {
     bool jk_valid = false;
 
     dsscf_ctx* ctx = nullptr;
+    scfdev::Step* dstep = nullptr;  // device SCF step state (opt.device_step)
     std::string error;
 
     ~Job();

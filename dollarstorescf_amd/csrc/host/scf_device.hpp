@@ -1,0 +1,51 @@
+// scf_device.hpp -- the host part of an SCF iteration run on the device
+// (DIIS extrapolation, F' = X F X, diagonalization, density, energy terms):
+// the roles of SCFSolver.cpp:45-94 and 118-143, DIIS.ipp:18-100 and
+// Lapack.cpp:12-39 (dspev_) with the matrices resident in HBM.  Products go
+// through rocBLAS dgemm, the eigenproblem through rocSOLVER dsyevd, the
+// element-wise parts through small kernels; only the DIIS bookkeeping (which
+// slot to replace, the bordered B system of at most 17 x 17) and a few
+// scalars per iteration cross to the host.
+//
+// No HIP types here: scf.cpp (host compiler) includes it.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../../include/dsscf.h"
+
+namespace scfdev {
+
+struct Step;  // device buffers, rocBLAS handle and stream of one job
+
+// n AOs, nocc doubly occupied orbitals, DIIS history of kmax vectors; Hp
+// packed-U core Hamiltonian, S and X full n x n symmetric (row- or
+// column-major alike).
+int create(Step** s, int device, int n, int nocc, int kmax, const double* Hp, const double* S, const double* X,
+           std::string* err);
+void destroy(Step* s);
+
+// Packed-U alpha density from the host (Ptot = 2 Pa on the device).
+int set_density(Step* s, const double* Pa, std::string* err);
+
+// J[Ptot], K[Pa] by the job's context at the resident density.
+int fock_build(Step* s, dsscf_ctx* ctx, std::string* err);
+
+// One iteration after the Fock build.  diis_slot < 0: F = H + G directly
+// (first iteration); otherwise F0 = H + G and its Pulay error go into
+// history slot diis_slot, `dots` receives <e_new, e_i> for i < count
+// (the caller fills its B matrix) and the caller then calls extrapolate.
+int assemble(Step* s, int diis_slot, int count, double* dots, double* rms, std::string* err);
+// F = sum_i c_i F_i over the first `count` slots, then diagonalize and form
+// the new density; eps (n values) and tr(Pa_new G_old) come back.
+int extrapolate(Step* s, const double* coef, int count, std::vector<double>& eps, double* tr_PG, std::string* err);
+int diagonalize_direct(Step* s, std::vector<double>& eps, double* tr_PG, std::string* err);  // F = F0
+
+// Copies to the host: packed-U Pa, Ptot, J, K, G (any may be null) and the
+// row-major coefficient matrix C (may be null).
+int download(Step* s, double* Pa, double* Ptot, double* J, double* K, double* G, double* C, std::string* err);
+
+// Device bytes held by the step buffers.
+long long device_bytes(const Step* s);
+
+}  // namespace scfdev

@@ -28,6 +28,15 @@ int dsscf_job_set_options(dsscf_job* job, int diis_size, double energy_threshold
 /* Mixed precision and primitive screening (dsscf_options.fp32_threshold and
  * .prim_threshold; 0 = off).  No reference counterpart. */
 int dsscf_job_set_precision(dsscf_job* job, double fp32_threshold, double prim_threshold);
+/* Where the host part of each iteration runs (no reference counterpart):
+ * 0 = host (default): G, DIIS, F' = X F X, the eigensolver and the density
+ * on the CPU, J/K copied across per iteration; 1 = device: the same steps
+ * (SCFSolver.cpp:45-94 and 118-143, DIIS.ipp:18-100, dspev_ in
+ * Lapack.cpp:12-39) with every matrix resident in HBM (rocBLAS dgemm,
+ * rocSOLVER dsyevd, small kernels); only the DIIS B matrix row, the orbital
+ * energies and tr(Pa G) come to the host.  The device step keeps at most 16
+ * DIIS vectors.  Used by dsscf_job_solve. */
+int dsscf_job_set_step(dsscf_job* job, int on_device);
 /* scf::assignMolecule (SCFBasis.cpp:17-37); coordinates in bohr. */
 int dsscf_job_set_molecule(dsscf_job* job, int natom, const int* Z, const double* xyz_bohr, int charge,
                            int multiplicity);
@@ -48,7 +57,9 @@ int dsscf_job_ao(dsscf_job* job, int i, int* l, int* m, double* center, int* npr
                  int* nterm, double* tw, int* txyz, int cap);
 
 /* Packed-U matrices by name: "S","T","V","H","Pa","Ptot","J","K","G"; "eps"
- * gives n orbital energies. */
+ * gives n orbital energies (ascending), "C" the n x n row-major MO
+ * coefficients (column i = orbital i; scf::coefficient_matrix,
+ * SCFCommon.hpp) once the job is solved. */
 int dsscf_job_get(dsscf_job* job, const char* name, double* out);
 /* Set the alpha density (packed-U); Ptot = 2 Pa. */
 int dsscf_job_set_density(dsscf_job* job, const double* Pa);

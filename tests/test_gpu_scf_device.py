@@ -1,0 +1,79 @@
+"""The SCF iteration's host part on the device (DIIS, X F X, rocSOLVER dsyevd,
+density, energy terms) against the host step and the oracle, through the C-ABI
+(dsscf_job_set_step)."""
+import numpy as np
+import pytest
+
+import dollarstorescf_amd as ds
+from dollarstorescf_amd import geometry as G
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(basis, Z, xyz, device_step, diis=8, conv=1e-10, boys=ds.BOYS_ACCURATE, schwarz=1e-12):
+    job = ds.Job(0).set_options(diis_size=diis, energy_threshold=conv, max_iterations=80, boys_mode=boys,
+                                schwarz_threshold=schwarz, device_step=device_step)
+    job.set_basis(basis).set_molecule(Z, xyz).construct()
+    e, it = job.solve()
+    return job, e, it
+
+
+def test_device_step_water_sto3g_vs_oracle():
+    """Config 0 (water/STO-3G, DIIS 6, converged to 1e-10) with the device step:
+    the total energy within 1e-8 Eh of the oracle's RHF in the reference Boys mode."""
+    Z, xyz = G.water()
+    _, e, _ = _solve("STO-3G", Z, xyz, True, diis=6, boys=ds.BOYS_REFERENCE, schwarz=0.0)
+    ref = O.scf_energy(ds.basis_path("STO-3G"), Z, xyz, mode=ds.BOYS_REFERENCE, diis_size=8, conv=1e-10)
+    assert abs(e - ref.energy) < 1e-8
+
+
+@pytest.mark.parametrize("case", ["water_sto3g", "water8_631g", "butane_def2svp", "water2_631gp"])
+def test_device_step_matches_host_step(case):
+    """Same job with the iteration's host part on the CPU and on the device:
+    the same number of iterations (within one), total energies within 1e-9 Eh,
+    densities within 1e-5 (both stop at |dE| < 1e-10), and the same
+    per-iteration energies while they are still far from convergence (the two
+    steps differ only in rounding)."""
+    Z, xyz, basis = {
+        "water_sto3g": (*G.water(), "STO-3G"),
+        "water8_631g": (*G.water_lattice(1), "6-31G"),
+        "butane_def2svp": (*G.alkane(4), "def2-SVP"),
+        "water2_631gp": (*G.water_box(2, seed=3), "6-31GP"),
+    }[case]
+    jh, eh, ith = _solve(basis, Z, xyz, False)
+    jd, ed, itd = _solve(basis, Z, xyz, True)
+    assert abs(ith - itd) <= 1, (ith, itd)
+    assert abs(eh - ed) < 1e-9, (eh, ed)
+    assert np.max(np.abs(jh.get("Pa") - jd.get("Pa"))) < 1e-5
+    hh, hd = jh.history(), jd.history()
+    assert np.max(np.abs(hh[:3] - hd[:3])) < 1e-8
+    # the energy terms come from the same final J/K path
+    for k, v in jh.energies().items():
+        assert abs(v - jd.energies()[k]) < 1e-5, k
+
+
+def test_device_step_orbitals():
+    """Orbitals of the device step solve the Roothaan-Hall equations
+    F C = S C eps with C^T S C = 1 (F = H + G at the converged density), and
+    Pa = C_occ C_occ^T."""
+    Z, xyz = G.water_box(2, seed=3)
+    job, _, _ = _solve("6-31G", Z, xyz, True)
+    n = job.n_ao
+    S = O.unpack(job.get("S"), n)
+    F = O.unpack(job.get("H") + job.get("J") - job.get("K"), n)
+    C = job.get("C")
+    eps = job.get("eps")
+    assert np.all(np.diff(eps) >= -1e-12)
+    assert np.max(np.abs(C.T @ S @ C - np.eye(n))) < 1e-9
+    # F is built from the final density while C diagonalized the previous
+    # (DIIS-extrapolated) Fock matrix: equal to the convergence level only
+    assert np.max(np.abs(F @ C - S @ C * eps)) < 1e-4
+    nocc = 10
+    assert np.max(np.abs(O.unpack(job.get("Pa"), n) - C[:, :nocc] @ C[:, :nocc].T)) < 1e-12
+
+
+def test_device_step_rejects_long_history():
+    job = ds.Job(0)
+    with pytest.raises(ds.DsscfError):
+        job.set_options(diis_size=20, device_step=True)

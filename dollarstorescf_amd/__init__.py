@@ -34,7 +34,7 @@ class DsscfError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("boys_mode", ctypes.c_int), ("schwarz_threshold", ctypes.c_double),
                 ("fp32_threshold", ctypes.c_double), ("prim_threshold", ctypes.c_double),
-                ("work_item_flops", ctypes.c_double)]
+                ("work_item_flops", ctypes.c_double), ("group_prims", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -83,6 +83,7 @@ def lib():
             "dsscf_server_create": [p, i, d, i, i], "dsscf_server_destroy": [p], "dsscf_server_submit": [p, p, ctypes.c_longlong],
             "dsscf_server_step": [p, p, p], "dsscf_server_poll": [p, i, p, p, p, p, p, p, p, p],
             "dsscf_server_idle": [p], "dsscf_server_stats": [p, p], "dsscf_one_electron": [p, i, p, p, p, p, p],
+            "dsscf_boys_eval": [p, i, i, i, p, p],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -250,8 +251,8 @@ class Context:
     __del__ = close
 
     def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12, fp32_threshold=0.0, prim_threshold=0.0,
-                    work_item_flops=0.0):
-        o = Options(boys_mode, schwarz_threshold, fp32_threshold, prim_threshold, work_item_flops)
+                    work_item_flops=0.0, group_prims=0.0):
+        o = Options(boys_mode, schwarz_threshold, fp32_threshold, prim_threshold, work_item_flops, group_prims)
         _check(lib().dsscf_set_options(self._h, ctypes.byref(o)))
 
     def set_basis(self, l, nprim, centers, exps, coefs, xform):
@@ -303,6 +304,14 @@ class Context:
         S, T, V = np.empty(npk), np.empty(npk), np.empty(npk)
         _check(lib().dsscf_one_electron(self._h, len(Z), _p(Z), _p(xyz), _p(S), _p(T), _p(V)))
         return S, T, V
+
+    def boys(self, T, L: int, mode: int = 2):
+        """F_0..F_L at the arguments T on the device (mode 0 reference series, 1 accurate
+        Taylor table, 2 accurate interval table in LDS); shape (len(T), L+1)."""
+        T = np.ascontiguousarray(T, np.float64)
+        out = np.empty((len(T), L + 1))
+        _check(lib().dsscf_boys_eval(self._h, mode, L, len(T), _p(T), _p(out)))
+        return out
 
     def stats(self) -> Stats:
         st = Stats()

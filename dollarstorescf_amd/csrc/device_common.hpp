@@ -20,8 +20,25 @@
 #define DSSCF_BOYS_MMAX 14
 
 // Boys-function modes: the reference's truncated series, or the accurate
-// table/asymptotic evaluation.
-enum { BOYS_REFERENCE = 0, BOYS_ACCURATE = 1 };
+// table/asymptotic evaluation.  BOYS_ACCURATE_LDS is the accurate mode
+// evaluated from a block's LDS copy of the interval table below (device
+// internal; the results agree with BOYS_ACCURATE to ~1e-16 relative).
+enum { BOYS_REFERENCE = 0, BOYS_ACCURATE = 1, BOYS_ACCURATE_LDS = 2 };
+
+// Interval table (after the 801-row table in the same buffer): for each
+// m = 0..DSSCF_BOYS_MMAX, DSSCF_BPOLY_ROWS intervals [c-1/4, c+1/4) with
+// c = (i + 1/2)/2 covering [0, 40).  Coefficient q of an interval: q = 0 is
+// e^-c, q = 1..12 the Taylor coefficients (-1)^k F_{m+k}(c)/k!, k = q-1 =
+// 0..11 (remainder below 5e-17 relative for |T-c| <= 1/4), q = 13 padding.
+// Stored as pairs (q = 2j, 2j+1) in 16 B cells, pair-major:
+// cell (j, i) at doubles 2 (j DSSCF_BPOLY_ROWS + i), so one 16 B load
+// fetches two coefficients and lanes reading pair j of different intervals
+// hit consecutive LDS addresses.
+#define DSSCF_BPOLY_ROWS 80
+#define DSSCF_BPOLY_COEF 14
+#define DSSCF_BPOLY_TERMS 12
+#define DSSCF_BPOLY_OFFSET (DSSCF_BOYS_ROWS * DSSCF_BOYS_STRIDE)
+#define DSSCF_BPOLY_M_STRIDE (DSSCF_BPOLY_ROWS * DSSCF_BPOLY_COEF)
 
 struct __align__(16) ShellPair {
     double A[3];     // centre of the first shell (higher l)
@@ -229,11 +246,70 @@ __device__ __forceinline__ void boys_accurate(double T, const double* __restrict
     }
 }
 
+// Accurate F_0..F_L from an LDS copy of the interval table of m = L
+// (DSSCF_BPOLY_COEF x DSSCF_BPOLY_ROWS, coefficient-major): a degree-11
+// Taylor polynomial of F_L about the interval centre, e^-T = e^-c e^-u from
+// a degree-12 series in u = T - c (|u| <= 1/4), then the downward recursion;
+// T >= 40 as boys_accurate.
+template <int L>
+__device__ __forceinline__ void boys_interval(double T, const double* __restrict__ tab, double* F)
+{
+    static_assert(DSSCF_BPOLY_TERMS == 12 && DSSCF_BPOLY_COEF == 14, "pair layout below");
+    if (T < DSSCF_BOYS_TMAX) {
+        const int i = (int)(T * 2.0);
+        const double u = T - (0.5 * i + 0.25);
+        const double2* c = reinterpret_cast<const double2*>(tab) + i;
+        const double2 p0 = c[0], p1 = c[DSSCF_BPOLY_ROWS], p2 = c[2 * DSSCF_BPOLY_ROWS],
+                      p3 = c[3 * DSSCF_BPOLY_ROWS], p4 = c[4 * DSSCF_BPOLY_ROWS], p5 = c[5 * DSSCF_BPOLY_ROWS],
+                      p6 = c[6 * DSSCF_BPOLY_ROWS];
+        double s = p6.x;  // a_11
+        s = fma(s, u, p5.y);
+        s = fma(s, u, p5.x);
+        s = fma(s, u, p4.y);
+        s = fma(s, u, p4.x);
+        s = fma(s, u, p3.y);
+        s = fma(s, u, p3.x);
+        s = fma(s, u, p2.y);
+        s = fma(s, u, p2.x);
+        s = fma(s, u, p1.y);
+        s = fma(s, u, p1.x);
+        s = fma(s, u, p0.y);  // a_0
+        F[L] = s;
+        if (L > 0) {
+            double e = 1.0 / 479001600.0;  // sum_k (-u)^k / k!, k <= 12
+            e = fma(e, -u, 1.0 / 39916800.0);
+            e = fma(e, -u, 1.0 / 3628800.0);
+            e = fma(e, -u, 1.0 / 362880.0);
+            e = fma(e, -u, 1.0 / 40320.0);
+            e = fma(e, -u, 1.0 / 5040.0);
+            e = fma(e, -u, 1.0 / 720.0);
+            e = fma(e, -u, 1.0 / 120.0);
+            e = fma(e, -u, 1.0 / 24.0);
+            e = fma(e, -u, 1.0 / 6.0);
+            e = fma(e, -u, 0.5);
+            e = fma(e, -u, 1.0);
+            e = fma(e, -u, 1.0);
+            const double eT = p0.x * e;
+            const double T2 = 2.0 * T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = fma(T2, F[m + 1], eT) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double r = rsqrt_nr(T);
+        const double inv_T = r * r;
+        F[0] = 0.88622692545275801365 * r;
+#pragma unroll
+        for (int m = 1; m <= L; ++m) F[m] = F[m - 1] * ((m - 0.5) * inv_T);
+    }
+}
+
 template <int BM, int L>
 __device__ __forceinline__ void boys(double T, const double* __restrict__ tab, double* F)
 {
     if (BM == BOYS_ACCURATE) {
         boys_accurate<L>(T, tab, F);
+    } else if (BM == BOYS_ACCURATE_LDS) {
+        boys_interval<L>(T, tab, F);
     } else {
 #pragma unroll
         for (int m = 0; m <= L; ++m) F[m] = boys_reference_m(T, (unsigned)m);

@@ -40,6 +40,30 @@ const double kEriScale = 2.0 * std::pow(3.14159265358979, 2.5);
 
 // Smallest primitive-pair run of a sub-pair.
 constexpr int kMinSubPrims = 5;
+constexpr int kMaxGroup = 16;  // lanes per quartet, at most
+
+// Lanes per quartet for a (sub-)quartet of nb x nk primitive quartets: the
+// largest power of two G <= kMaxGroup with nb*nk/G >= target, split into
+// gb lanes over bra primitives and gk = G/gb over ket primitives (each at
+// most its primitive count) so that the busiest lane's share
+// ceil(nb/gb)*ceil(nk/gk) is smallest.  Returns gb | gk << 8.
+int quartet_group(int nb, int nk, double target)
+{
+    if (target <= 0.0) return 1 | (1 << 8);
+    int G = 1;
+    while (G < kMaxGroup && (double)nb * nk / (2.0 * G) >= target) G *= 2;
+    int best = 1 | (1 << 8), best_load = nb * nk + 1;
+    for (int gb = 1; gb <= G; gb *= 2) {
+        const int gk = G / gb;
+        if (gb > nb || gk > nk) continue;
+        const int load = ((nb + gb - 1) / gb) * ((nk + gk - 1) / gk);
+        if (load < best_load) {
+            best_load = load;
+            best = gb | (gk << 8);
+        }
+    }
+    return best;
+}
 
 int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
@@ -294,7 +318,7 @@ int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: FP32 quartets need the accurate Boys mode");
     const bool changed = o->boys_mode != c->opt.boys_mode || o->schwarz_threshold != c->opt.schwarz_threshold ||
                          o->fp32_threshold != c->opt.fp32_threshold || o->prim_threshold != c->opt.prim_threshold ||
-                         o->work_item_flops != c->opt.work_item_flops;
+                         o->work_item_flops != c->opt.work_item_flops || o->group_prims != c->opt.group_prims;
     c->opt = *o;
     if (changed) c->prepared = false;
     return DSSCF_OK;
@@ -503,6 +527,7 @@ int dsscf_prepare(dsscf_ctx* c)
     const double thr = c->opt.schwarz_threshold;
     const double thr32 = c->opt.fp32_threshold;
     const double item_flops = c->opt.work_item_flops == 0.0 ? DSSCF_WORK_ITEM_FLOPS : c->opt.work_item_flops;
+    const double group_prims = c->opt.group_prims == 0.0 ? DSSCF_GROUP_PRIMS : c->opt.group_prims;
 
     // Work pairs: the shell pairs (same indices), then sub-pairs.  A sub-pair
     // is a shell pair restricted to a run of consecutive primitive pairs;
@@ -677,12 +702,22 @@ int dsscf_prepare(dsscf_ctx* c)
                 t.kets.insert(t.kets.end(), ksub.begin(), ksub.end());
                 for (unsigned sb : bsub) {
                     t.items += (long long)ksub.size();
-                    for (size_t o = 0; o < ksub.size(); o += j.kChunk) {
-                        const int cnt = (int)std::min<size_t>(j.kChunk, ksub.size() - o);
+                    // kets come by primitive count, so lanes per quartet only fall along
+                    // the list: a unit keeps one group size and about kChunk lanes
+                    for (size_t o = 0; o < ksub.size();) {
+                        const int grp = quartet_group(work[sb].nprim, work[ksub[o]].nprim, group_prims);
+                        const int G = (grp & 0xff) * (grp >> 8);
+                        const size_t lim = std::max<size_t>(1, (size_t)j.kChunk / G);
+                        size_t e = o + 1;
+                        while (e < ksub.size() && e - o < lim &&
+                               quartet_group(work[sb].nprim, work[ksub[e]].nprim, group_prims) == grp)
+                            ++e;
+                        const int cnt = (int)(e - o);
                         long long cost = 0;  // primitive quartets of the unit
                         for (int q = 0; q < cnt; ++q) cost += work[ksub[o + q]].nprim;
                         cost *= work[sb].nprim;
                         uu.push_back(make_int4((int)sb, k0 + (int)o, cnt, (int)std::min<long long>(cost, 0x7fffffff)));
+                        o = e;
                     }
                 }
             }
@@ -737,6 +772,12 @@ int dsscf_prepare(dsscf_ctx* c)
         shuffle_units(units32.data(), units32.size(), 0x14057b7ef767814fULL + cls);
         std::stable_sort(units.begin() + u0, units.end(), by_cost);
         std::stable_sort(units32.begin(), units32.end(), by_cost);
+        // the device reads the lanes per quartet where the sort key was
+        auto set_group = [&](int4& u) {
+            u.w = quartet_group(work[u.x].nprim, work[kets[u.y]].nprim, group_prims);
+        };
+        for (size_t k = (size_t)u0; k < units.size(); ++k) set_group(units[k]);
+        for (int4& u : units32) set_group(u);
         c->u_off[cls] = u0;
         c->u_cnt[cls] = (long long)units.size() - u0;
         c->u_off32[cls] = (long long)units.size();
@@ -1057,6 +1098,29 @@ int dsscf_jk_device(dsscf_ctx* c, const double* d_Pt, const double* d_Pa, double
     HIP_TRY(hipMemcpyAsync(d_J, c->d_J_pk, sizeof(double) * np, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(d_K, c->d_K_pk, sizeof(double) * np, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return DSSCF_OK;
+}
+
+int dsscf_boys_eval(dsscf_ctx* c, int mode, int L, int n, const double* T, double* F)
+{
+    if (!c || n < 0 || (n > 0 && (!T || !F))) return fail(DSSCF_ERR_ARG, "dsscf_boys_eval: bad argument");
+    if (mode < 0 || mode > 2 || L < 0 || L > 8) return fail(DSSCF_ERR_ARG, "dsscf_boys_eval: mode 0..2, L 0..8");
+    if (n == 0) return DSSCF_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    double *dT = nullptr, *dF = nullptr;
+    if (int rc = dev_alloc(&dT, (size_t)n)) return rc;
+    if (int rc = dev_alloc(&dF, (size_t)n * (L + 1))) {
+        dev_free(dT);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(dT, T, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = dsscf_launch_boys_eval(mode, L, dT, n, c->d_btab, dF, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(F, dF, sizeof(double) * n * (L + 1), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free(dT);
+    dev_free(dF);
+    if (e != hipSuccess) return fail(DSSCF_ERR_HIP, std::string("dsscf_boys_eval: ") + hipGetErrorString(e));
     return DSSCF_OK;
 }
 

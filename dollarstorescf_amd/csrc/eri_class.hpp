@@ -25,9 +25,10 @@ struct Quartet;
         static __device__ __forceinline__ void run(const ShellPair& bra, const ShellPair& ket,        \
                                                    const PrimPair* __restrict__ bra_pp,               \
                                                    const PrimPair* __restrict__ ket_pp,               \
-                                                   const double* __restrict__ btab, double* out)      \
+                                                   const double* __restrict__ btab, double* out,      \
+                                                   int gb = 1, int rb = 0, int gk = 1, int rk = 0)    \
         {                                                                                             \
-            shell_quartet_##name<BM, R>(bra, ket, bra_pp, ket_pp, btab, out);                         \
+            shell_quartet_##name<BM, R>(bra, ket, bra_pp, ket_pp, btab, out, gb, rb, gk, rk);         \
         }                                                                                             \
     };
 #define DSSCF_QUARTET_SPECIALIZE(a, b, c, d, name, wave) DSSCF_QSPEC_##wave(a, b, c, d, name)
@@ -53,7 +54,10 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // the bra record and its primitive pairs are uniform across the block; the
 // Jh[a][b] sums are reduced across the block, the Kh rows of shells a and b
 // are accumulated in LDS and flushed once per block; only Jh[c][d] goes
-// straight to global memory per quartet.
+// straight to global memory per quartet.  A unit's fourth field gives the
+// lanes per quartet, gb (bra primitives) | gk (ket primitives) << 8: a group
+// of gb*gk lanes splits a primitive-heavy quartet's primitive loops, and the
+// group's first lane digests the summed integrals.
 template <int la, int lb, int lc, int ld, int BM, typename R>
 __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
                                             int nseg, const double* __restrict__ btab, double scale)
@@ -76,6 +80,17 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     const GPtr<unsigned> kets = gbl(S.k);
     const ShellPair bra = ldg(pairs + u.x);
     const int a0 = bra.ao_a, b0 = bra.ao_b;
+    // FP64 accurate mode: the Boys function from the block's LDS copy of the
+    // interval table of m = L (a per-lane gather from the 801-row table
+    // touches up to 64 cache lines per load)
+    constexpr bool kLds = BM == BOYS_ACCURATE && sizeof(R) == 8;
+    constexpr int BMq = kLds ? BOYS_ACCURATE_LDS : BM;
+    __shared__ __align__(16) double bpoly[kLds ? DSSCF_BPOLY_M_STRIDE : 1];
+    if (kLds) {
+        const GPtr<double> src = gbl(btab + DSSCF_BPOLY_OFFSET + CI::L * DSSCF_BPOLY_M_STRIDE);
+        for (int i = threadIdx.x; i < DSSCF_BPOLY_M_STRIDE; i += blockDim.x) bpoly[i] = src[i];
+    }
+    const double* __restrict__ tq = kLds ? bpoly : btab;
     for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;
     for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) braP[i] = ldg(pp + bra.prim_off + i);
     double dab[NA * NB];
@@ -89,11 +104,15 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     const double gb = scale * (a0 == b0 ? 0.5 : 1.0);
     __syncthreads();
 
-    for (int t = threadIdx.x; t < u.z; t += blockDim.x) {
+    const int lgb = u.w & 0xff, lgk = (u.w >> 8) & 0xff, gl = lgb * lgk;  // lanes per quartet (power of two)
+    const int lane = (int)threadIdx.x & (gl - 1);
+    const int rb = lane & (lgb - 1), rk = lane / lgb;
+    for (int t = (int)threadIdx.x / gl; t < u.z; t += (int)blockDim.x / gl) {
         const int ki = (int)kets[u.y + t];
         const ShellPair ket = ldg(pairs + ki);
         double v[CI::NAO];
-        Quartet<la, lb, lc, ld, BM, R>::run(bra, ket, braP, S.pp + ket.prim_off, btab, v);
+        Quartet<la, lb, lc, ld, BMq, R>::run(bra, ket, braP, S.pp + ket.prim_off, tq, v, lgb, rb, lgk, rk);
+        if (lane != 0) continue;
         const int c0 = ket.ao_a, d0 = ket.ao_b;
         // (ab|ab): the bra and ket (sub-)pairs belong to one shell pair
         const bool same = ket.ao_a == a0 && ket.ao_b == b0;

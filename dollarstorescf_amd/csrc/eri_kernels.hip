@@ -368,3 +368,59 @@ hipError_t dsscf_launch_fold(const MatSeg* segs, int nseg, int total_rows, int m
     hipLaunchKernelGGL(k_fold, dim3((maxN + 255) / 256, total_rows), dim3(256), 0, s, segs, nseg);
     return hipGetLastError();
 }
+
+// ------------------------------------------------ Boys-function probe ---
+// F_0..F_L at n arguments in one of the device modes (the ERI kernels'
+// own evaluation, for checking against tabulated values): out[k*(L+1)+m].
+template <int L, int BM>
+__global__ void __launch_bounds__(256) k_boys_eval(const double* __restrict__ T, int n,
+                                                   const double* __restrict__ btab, double* __restrict__ out)
+{
+    __shared__ __align__(16) double tab[BM == BOYS_ACCURATE_LDS ? DSSCF_BPOLY_M_STRIDE : 1];
+    const double* t = btab;
+    if (BM == BOYS_ACCURATE_LDS) {
+        for (int i = threadIdx.x; i < DSSCF_BPOLY_M_STRIDE; i += blockDim.x)
+            tab[i] = btab[DSSCF_BPOLY_OFFSET + L * DSSCF_BPOLY_M_STRIDE + i];
+        __syncthreads();
+        t = tab;
+    }
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double f[L + 1];
+    boys<BM, L>(T[k], t, f);
+#pragma unroll
+    for (int m = 0; m <= L; ++m) out[(size_t)k * (L + 1) + m] = f[m];
+}
+
+template <int BM>
+static hipError_t launch_boys_eval_mode(int L, const double* T, int n, const double* btab, double* out, hipStream_t s)
+{
+    const dim3 blk(256), grd((n + 255) / 256);
+    switch (L) {
+#define DSSCF_BOYS_CASE(L_)                                                                   \
+    case L_:                                                                                \
+        hipLaunchKernelGGL((k_boys_eval<L_, BM>), grd, blk, 0, s, T, n, btab, out);          \
+        return hipGetLastError();
+        DSSCF_BOYS_CASE(0)
+        DSSCF_BOYS_CASE(1)
+        DSSCF_BOYS_CASE(2)
+        DSSCF_BOYS_CASE(3)
+        DSSCF_BOYS_CASE(4)
+        DSSCF_BOYS_CASE(5)
+        DSSCF_BOYS_CASE(6)
+        DSSCF_BOYS_CASE(7)
+        DSSCF_BOYS_CASE(8)
+#undef DSSCF_BOYS_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t dsscf_launch_boys_eval(int mode, int L, const double* T, int n, const double* btab, double* out,
+                                  hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (mode == BOYS_REFERENCE) return launch_boys_eval_mode<BOYS_REFERENCE>(L, T, n, btab, out, s);
+    if (mode == BOYS_ACCURATE) return launch_boys_eval_mode<BOYS_ACCURATE>(L, T, n, btab, out, s);
+    if (mode == BOYS_ACCURATE_LDS) return launch_boys_eval_mode<BOYS_ACCURATE_LDS>(L, T, n, btab, out, s);
+    return hipErrorInvalidValue;
+}

@@ -324,16 +324,21 @@ def gen_class(c, spherical=True):
     o.append("// ---- class (%s) : L=%d, %d VRR terms, %d AO integrals" % (name, L, len(vrr), NAO))
     o.append("template <> struct ClassInfo<%d,%d,%d,%d> { static constexpr int L = %d, NAO = %d, NA = %d, NB = %d, NC = %d, ND = %d; };"
              % (la, lb, lc, ld, L, NAO, *na))
+    o.append("// A group of gb*gk lanes (aligned, power of two) may share one quartet:")
+    o.append("// lane (rb, rk) takes bra primitives rb, rb+gb, .. and ket primitives rk,")
+    o.append("// rk+gk, ..; the [e0|f0] sums are then added across the group, so every")
+    o.append("// lane of it ends with the whole quartet.  gb = gk = 1: one lane.")
     o.append("template <int BM, typename R>")
     o.append("__device__ __forceinline__ void shell_quartet_%s(const ShellPair& bra, const ShellPair& ket,"
              " const PrimPair* __restrict__ bra_pp, const PrimPair* __restrict__ ket_pp,"
-             " const double* __restrict__ btab, double* __restrict__ out) {" % name)
+             " const double* __restrict__ btab, double* __restrict__ out,"
+             " int gb = 1, int rb = 0, int gk = 1, int rk = 0) {" % name)
     for (e, f) in accs:
         o.append("  double %s = 0.0;" % acc_name(e, f))
-    o.append("  for (int ik = 0; ik < ket.nprim; ++ik) {")
+    o.append("  for (int ik = rk; ik < ket.nprim; ik += gk) {")
     o.append("    const PrimPair Q = ldg(ket_pp + ik);")
     o.append("#pragma unroll 2")
-    o.append("    for (int ib = 0; ib < bra.nprim; ++ib) {")
+    o.append("    for (int ib = rb; ib < bra.nprim; ib += gb) {")
     o.append("      const PrimPair P = bra_pp[ib];")
     o.append("      R Fm[%d];" % (L + 1))
     o.append("      PrimQuartet<R> g; prim_setup<BM, R, %d>(P, Q, bra, ket, btab, g, Fm);" % L)
@@ -349,6 +354,10 @@ def gen_class(c, spherical=True):
     for (e, f) in accs:
         o.append("      %s += (double)%s;" % (acc_name(e, f), vname(e, f, 0)))
     o.append("    }")
+    o.append("  }")
+    o.append("  for (int off = 1; off < gb * gk; off <<= 1) {")
+    for (e, f) in accs:
+        o.append("    %s += __shfl_xor(%s, off);" % (acc_name(e, f), acc_name(e, f)))
     o.append("  }")
     o.append("  const double ABx = bra.AB[0], ABy = bra.AB[1], ABz = bra.AB[2];")
     o.append("  const double CDx = ket.AB[0], CDy = ket.AB[1], CDz = ket.AB[2];")

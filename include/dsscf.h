@@ -55,9 +55,15 @@ typedef struct {
                                  work, with runs of at least 5 primitive pairs (the sub-quartets
                                  add up to the shell quartet); 0 = default
                                  (DSSCF_WORK_ITEM_FLOPS), < 0 = never split */
+    double group_prims;       /* a (sub-)quartet with many primitive quartets is shared by a group of
+                                 up to 16 lanes that split its primitive loops, so that each lane
+                                 keeps about this many primitive quartets; its first lane digests
+                                 the summed integrals into J and K; 0 = default
+                                 (DSSCF_GROUP_PRIMS), < 0 = one lane per quartet */
 } dsscf_options;
 
 #define DSSCF_WORK_ITEM_FLOPS 16384.0
+#define DSSCF_GROUP_PRIMS 8.0
 
 typedef struct {
     long long n_pairs;             /* unique shell pairs */
@@ -149,6 +155,14 @@ int dsscf_one_electron(dsscf_ctx* ctx, int n_atom, const int* Z, const double* x
  * (src/chem/GaussianIntegral.cpp:561-642, GaussianIntegral.hpp:12) maps to
  * ijkl = {mu, nu, sigma, tau}.  No screening is applied. */
 int dsscf_eri_batch(dsscf_ctx* ctx, const int* ijkl, long long n, double* out);
+
+/* F_0..F_L (L <= 8) of the Boys function at n arguments T[], evaluated on the
+ * device exactly as the ERI kernels do: mode 0 = the reference series
+ * (boysFunctionExact, GaussianIntegral.cpp:209-223), 1 = accurate from the
+ * 801-row Taylor table, 2 = accurate from the interval table staged in LDS
+ * (the FP64 J/K kernels' form).  F[k*(L+1)+m] receives F_m(T[k]).  A
+ * checking aid (no reference counterpart beyond the series); synchronous. */
+int dsscf_boys_eval(dsscf_ctx* ctx, int mode, int L, int n, const double* T, double* F);
 
 /* Schwarz bounds Q_ab of all unique shell pairs (a >= b, index a(a+1)/2+b). */
 int dsscf_schwarz(dsscf_ctx* ctx, double* Q);

@@ -329,3 +329,30 @@ def test_one_electron_device_matches_host(basis, mol):
     for name in ("S", "T", "V", "H"):
         a, b = dev.get(name), host.get(name)
         assert np.max(np.abs(a - b)) < 1e-12 * max(1.0, np.abs(b).max()), name
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_device_boys_vs_mpmath(mode):
+    """The device's accurate Boys forms (1: 801-row Taylor table, 2: interval
+    table in LDS, the FP64 J/K kernels' form) for F_0..F_L, L <= 8, against
+    50-digit mpmath values: 2e-15 relative, plus 1e-18 absolute for the
+    asymptotic form at T >= 40, which drops terms of order e^-T / 2T."""
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "boys.json")))
+    x = np.array(g["x"])
+    mp = np.array(g["mpmath"])
+    ctx = ds.Context(0)
+    for L in range(9):
+        F = ctx.boys(x, L, mode)
+        ref = mp[:, :L + 1]
+        assert np.all(np.abs(F - ref) <= 2e-15 * ref + 1e-18), L
+
+
+def test_device_boys_interval_matches_taylor_table():
+    """Interval table vs 801-row table on a dense grid across every interval
+    edge and the T = 40 switch: the two accurate forms agree to 4e-15 relative."""
+    T = np.concatenate([np.linspace(0.0, 45.0, 90001), np.arange(1, 81) * 0.5 - 1e-13, np.arange(0, 81) * 0.5])
+    ctx = ds.Context(0)
+    for L in (0, 1, 4, 8):
+        a = ctx.boys(T, L, 1)
+        b = ctx.boys(T, L, 2)
+        assert np.all(np.abs(a - b) <= 4e-15 * np.abs(a)), L

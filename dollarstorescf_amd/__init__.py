@@ -73,7 +73,7 @@ def lib():
             "dsscf_get_jk": [p, p, p], "dsscf_time_jk_classes": [p, i, p], "dsscf_time_jk": [p, i, i, p, p, p], "dsscf_class_l": [i, p],
             "dsscf_job_create": [p, i], "dsscf_job_destroy": [p],
             "dsscf_job_set_options": [p, i, d, i, i, d, i], "dsscf_job_set_precision": [p, d, d],
-            "dsscf_job_set_step": [p, i],
+            "dsscf_job_set_step": [p, i], "dsscf_job_set_orbitals": [p, p, p], "dsscf_job_write_molden": [p, c, i],
             "dsscf_job_set_molecule": [p, i, p, p, i, i], "dsscf_job_load_xyz": [p, c, i, i],
             "dsscf_job_set_basis_file": [p, c], "dsscf_job_construct": [p], "dsscf_job_construct_host": [p],
             "dsscf_job_ao": [p, i, p, p, p, p, p, p, p, p, p, i], "dsscf_job_n_ao": [p],
@@ -452,6 +452,19 @@ class Job:
         out = np.empty(size)
         _check(lib().dsscf_job_get(self._h, name.encode(), _p(out)), self._err)
         return out.reshape(n, n) if name == "C" else out
+
+    def set_orbitals(self, C, eps):
+        """Replace the orbitals (C: n x n, column i = orbital i; eps: n energies)."""
+        C = np.ascontiguousarray(C, np.float64)
+        eps = np.ascontiguousarray(eps, np.float64)
+        n = self.n_ao
+        if C.shape != (n, n) or eps.shape != (n,):
+            raise ValueError("set_orbitals: C must be n x n and eps n long")
+        _check(lib().dsscf_job_set_orbitals(self._h, _p(C), _p(eps)), self._err)
+
+    def write_molden(self, path: str, write_virtual: bool = True):
+        """Molden file of the current orbitals (scf::writeOrbitalsMolden)."""
+        _check(lib().dsscf_job_write_molden(self._h, path.encode(), 1 if write_virtual else 0), self._err)
 
     def set_density(self, Pa_packed):
         Pa = np.ascontiguousarray(Pa_packed, np.float64)

@@ -32,7 +32,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-I" + os.path.join(ROOT,
 
 HIP_SOURCES = ["eri_kernels.hip", "dsscf_abi.cpp", "scf_device.hip"]
 HOST_SOURCES = ["host/chem.cpp", "host/linalg.cpp", "host/scf.cpp", "host/multi.cpp", "host/serve.cpp",
-                "host/job_api.cpp"]
+                "host/job_api.cpp", "host/molden.cpp"]
 
 DEFAULT_MAXL = 2
 

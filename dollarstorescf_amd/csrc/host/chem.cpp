@@ -12,6 +12,8 @@ namespace flo {
 static const char* kSymbols[18] = {"H", "He", "Li", "Be", "B", "C", "N", "O", "F",
                                    "Ne", "Na", "Mg", "Al", "Si", "P", "S", "Cl", "Ar"};
 
+const char* elementSymbol(int Z) { return Z >= 1 && Z <= 18 ? kSymbols[Z - 1] : "X"; }
+
 int getElement(const std::string& s)
 {
     if (s.empty() || s.size() > 2) return 0;

@@ -65,6 +65,7 @@ struct Molecule {
 };
 
 int getElement(const std::string& symbol);  // Element.cpp:178-185 (Z <= 18 here)
+const char* elementSymbol(int Z);            // element_symbols[Z-1], Element.cpp:4
 
 // Real-solid-harmonic expansion of AO (l, m) (GTO.cpp:15-88).
 std::vector<CartesianPolynomialTerm> solidHarmonicTerms(unsigned l, int m);

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 
+#include "molden.hpp"
 #include "multi.hpp"
 #include "scf.hpp"
 #include "serve.hpp"
@@ -188,6 +189,22 @@ int dsscf_job_get(dsscf_job* j, const char* name, double* out)
     if (v->empty()) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_get: '" + n + "' not computed yet");
     std::memcpy(out, v->data(), sizeof(double) * v->size());
     return DSSCF_OK;
+}
+
+int dsscf_job_set_orbitals(dsscf_job* j, const double* C, const double* eps)
+{
+    if (!j || !C || !eps) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_orbitals: null argument");
+    scf::Job& b = j->job;
+    if (b.n <= 0) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_set_orbitals: construct first");
+    b.C.assign(C, C + (size_t)b.n * b.n);
+    b.eps.assign(eps, eps + b.n);
+    return DSSCF_OK;
+}
+
+int dsscf_job_write_molden(dsscf_job* j, const char* path, int write_virtual)
+{
+    if (!j || !path) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_write_molden: null argument");
+    return scf::writeOrbitalsMolden(j->job, path, write_virtual != 0, &j->job.error);
 }
 
 int dsscf_job_set_density(dsscf_job* j, const double* Pa)

@@ -61,6 +61,15 @@ int dsscf_job_ao(dsscf_job* job, int i, int* l, int* m, double* center, int* npr
  * coefficients (column i = orbital i; scf::coefficient_matrix,
  * SCFCommon.hpp) once the job is solved. */
 int dsscf_job_get(dsscf_job* job, const char* name, double* out);
+/* Replace the job's orbitals: C n x n row-major (column i = orbital i), eps
+ * n orbital energies.  No reference counterpart (its orbitals come only from
+ * solveMOs); for writing orbitals computed elsewhere. */
+int dsscf_job_set_orbitals(dsscf_job* job, const double* C, const double* eps);
+/* scf::writeOrbitalsMolden(path, write_virtual) (WriteOrbitals.cpp:17-137,
+ * declared WriteOrbitals.hpp:5): [Atoms] in bohr, [GTO], [MO] of the job's
+ * orbitals (all n, or the occupied ones), byte-identical to the reference's
+ * file for the same orbitals.  Restricted jobs, l <= 2. */
+int dsscf_job_write_molden(dsscf_job* job, const char* path, int write_virtual);
 /* Set the alpha density (packed-U); Ptot = 2 Pa. */
 int dsscf_job_set_density(dsscf_job* job, const double* Pa);
 /* One Fock build at the current density (computeCoulombTerms +

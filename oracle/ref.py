@@ -36,6 +36,8 @@ def lib():
         L.ref_time_eri_list.restype = d
         L.ref_time_eri_list.argtypes = [p, ctypes.c_long, p]
         L.ref_one_electron.argtypes = [p, p, p]
+        L.ref_molden.restype = i
+        L.ref_molden.argtypes = [ctypes.c_char_p, p, p, i, i]
         _LIB = L
     return _LIB
 
@@ -91,6 +93,18 @@ class RefMolecule:
         S, T, V = np.empty(D), np.empty(D), np.empty(D)
         lib().ref_one_electron(_p(S), _p(T), _p(V))
         return S, T, V
+
+
+    def molden(self, path: str, C, eps, nocc: int, write_virtual: bool):
+        """The reference's writeOrbitalsMolden text for orbitals C (n x n,
+        column i = orbital i) and energies eps, through its own FormattedStream."""
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        self.rebuild()
+        if lib().ref_molden(path.encode(), _p(C), _p(eps), nocc, 1 if write_virtual else 0) != 0:
+            raise OSError("ref_molden: cannot write " + path)
+        with open(path) as f:
+            return f.read()
 
 
 def boys(x: float, m: int) -> float:

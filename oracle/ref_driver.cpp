@@ -2,8 +2,9 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Linked by oracle/Makefile against the
 // reference's unmodified translation units (GaussianIntegral.cpp, GTO.cpp,
-// BasisReader.cpp, Element.cpp, TextReader.cpp, TextUtil.cpp) compiled from
-// where they lie under the reference tree, into oracle/_ref/.  It is used to
+// BasisReader.cpp, Element.cpp, TextReader.cpp, TextUtil.cpp,
+// FormattedStream.cpp) compiled from where they lie under the reference
+// tree, into oracle/_ref/.  It is used to
 // pin the oracle restatement and as the timed "reference" CPU baseline.
 //
 // The AO list is assembled the way scf::constructBasis orders it
@@ -12,8 +13,12 @@
 // and LAPACK dependencies.  The reference's integral routines share global
 // scratch (GaussianIntegral.cpp:10): every call here is single-threaded.
 #include "chem/BasisReader.hpp"
+#include "chem/Element.hpp"
 #include "chem/GTO.hpp"
 #include "chem/GaussianIntegral.hpp"
+#include "util/FormattedStream.hpp"
+
+#include <fstream>
 
 #include <chrono>
 #include <string>
@@ -140,6 +145,91 @@ void ref_one_electron(double* S, double* T, double* V)
             }
             V[idx] = v;
         }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// scf::writeOrbitalsMolden (src/chem/scf/WriteOrbitals.cpp:17-137), restricted
+// case, over the AOs of the last ref_build and orbitals given as arrays
+// (C row-major n x n, column i = orbital i; eps ascending; nocc doubly
+// occupied).  The text goes through the reference's own FormattedStream and
+// NumberFormat, element_symbols and the AO/primitive data its GTO and
+// BasisReader code built; only the loops over the scf globals are restated
+// here (WriteOrbitals.cpp includes SCFCommon.hpp, which needs lalib).
+int ref_molden(const char* path, const double* C, const double* eps, int nocc, int write_virtual)
+{
+    using namespace flo;
+    const int n = (int)g_aos.size();
+    const int natom = (int)g_Z.size();
+    std::vector<int> atom_basis(1, 0);
+    for (int a = 0, k = 0; a < natom; ++a) {
+        for (const GTOShell& sh : g_basis_set.atomic_bases[g_Z[a] - 1]) k += 2 * (int)sh.l + 1;
+        atom_basis.push_back(k);
+    }
+    const unsigned perm[5][9] = {{0, 0, 0, 0, 0, 0, 0, 0, 0},
+                                 {0, 2, 1, 0, 0, 0, 0, 0, 0},
+                                 {2, 1, 3, 0, 4, 0, 0, 0, 0},
+                                 {3, 2, 4, 1, 5, 0, 6, 0, 0},
+                                 {4, 3, 5, 2, 6, 1, 7, 0, 8}};
+    const char labels[5] = {'s', 'p', 'd', 'f', 'g'};
+    std::ofstream file(path, std::ios::binary);
+    if (!file) return -1;
+    file << "[Molden Format]\n";
+    file << "[5D]\n";
+    file << "[9G]\n";
+    file << "[Atoms] AU\n";
+    FormattedStream fs;
+    fs.write_to = &file;
+    fs.addRow(NumberFormat(), TextAlignment::left, 2);
+    fs.addRow(NumberFormat::crudeFormatPositive(4, 4), TextAlignment::right, 4);
+    fs.addRow(NumberFormat::crudeFormatPositive(4, 4), TextAlignment::right, 4);
+    fs.addRow(NumberFormat::crudeFormat(8, 8), TextAlignment::right, 8);
+    fs.addRow(NumberFormat::crudeFormat(8, 8), TextAlignment::right, 8);
+    fs.addRow(NumberFormat::crudeFormat(8, 8), TextAlignment::right, 8);
+    for (int a = 0; a < natom; ++a)
+        fs << element_symbols[g_Z[a] - 1] << ',' << (i64)(a + 1) << ',' << (i64)g_Z[a] << ',' << g_xyz[3 * a] << ','
+           << g_xyz[3 * a + 1] << ',' << g_xyz[3 * a + 2] << '\n';
+    fs.resetRows();
+    fs.addRow(NumberFormat::scientificFormat(14, 8), TextAlignment::right, 14);
+    fs.addRow(NumberFormat::scientificFormat(14, 8), TextAlignment::right, 14);
+    file << "[GTO]\n";
+    for (int a = 0; a < natom; ++a) {
+        file << (a + 1) << " 0\n";
+        for (int j = atom_basis[a]; j < atom_basis[a + 1];) {
+            const ContractedGaussian& g = g_aos[j];
+            if (g.l < 5) {
+                file << labels[g.l] << ' ' << g.primitives.size() << " 1.00\n";
+                for (size_t k = 0; k < g.primitives.size(); ++k)
+                    fs << g.primitives[k].zeta << ',' << g.primitives[k].unnormalized_weight << '\n';
+            }
+            j += 2 * (int)g.l + 1;
+        }
+        file << '\n';
+    }
+    file << "[MO]\n";
+    fs.resetRows();
+    fs.addRow(NumberFormat::crudeFormatPositive(5, 5), TextAlignment::right, 5);
+    fs.addRow(NumberFormat::scientificFormat(14, 8), TextAlignment::right, 14);
+    for (int i = 0; i < (write_virtual ? n : nocc); ++i) {
+        file << "Sym=   a" << i << '\n';
+        file << "Ene=   " << eps[i] << '\n';
+        file << "Spin=  Alpha\n";
+        file << "Occup= " << ((i < nocc) * 2) << '\n';
+        int ao_index = 0;
+        for (int a = 0; a < natom; ++a)
+            for (int k = atom_basis[a]; k < atom_basis[a + 1];) {
+                const ContractedGaussian& g = g_aos[k];
+                if (g.l < 5)
+                    for (int m = 0; m <= 2 * (int)g.l; ++m) {
+                        ++ao_index;
+                        fs << (i64)ao_index << ',' << C[(size_t)(k + perm[g.l][m]) * n + i] << '\n';
+                    }
+                k += 2 * (int)g.l + 1;
+            }
+    }
+    return 0;
 }
 
 }  // extern "C"

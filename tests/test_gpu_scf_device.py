@@ -77,3 +77,43 @@ def test_device_step_rejects_long_history():
     job = ds.Job(0)
     with pytest.raises(ds.DsscfError):
         job.set_options(diis_size=20, device_step=True)
+
+
+@pytest.mark.parametrize("device_step", [False, True])
+def test_molden_of_solved_job(device_step, tmp_path):
+    """A solved job's Molden file (dsscf_job_write_molden) carries its orbital
+    energies and coefficients: parsed back, in the reference's AO order
+    (Molden p order x, y, z from m = -1, 0, 1 via WriteOrbitals.cpp:25-31),
+    they match the job's C and eps to the printed 8 significant digits."""
+    Z, xyz = G.water_box(2, seed=3)
+    job, _, _ = _solve("def2-SVP", Z, xyz, device_step)
+    n = job.n_ao
+    path = str(tmp_path / "w2.molden")
+    job.write_molden(path)
+    text = open(path).read().split("[MO]\n")[1].splitlines()
+    C = job.get("C")
+    eps = job.get("eps")
+    # AO position in the file -> the reference's AO index (perm per shell)
+    order, k = [], 0
+    ao_l = [job.ao(i)["l"] for i in range(n)]
+    perm = {0: [0], 1: [0, 2, 1], 2: [2, 1, 3, 0, 4]}
+    while k < n:
+        l = ao_l[k]
+        order += [k + p for p in perm[l]]
+        k += 2 * l + 1
+    mo, row = -1, 0
+    for line in text:
+        if line.startswith("Sym="):
+            mo += 1
+            row = 0
+        elif line.startswith("Ene="):
+            assert abs(float(line.split()[1]) - eps[mo]) <= 1e-5 * max(1.0, abs(eps[mo]))
+        elif line.startswith("Occup="):
+            assert int(line.split()[1]) == (2 if mo < 20 else 0)
+        elif line.strip() and not line.startswith("Spin"):
+            idx, val = line.split()
+            assert int(idx) == row + 1
+            want = C[order[row], mo]
+            assert abs(float(val) - want) <= 1e-7 * max(abs(want), 1e-3), (mo, row)
+            row += 1
+    assert mo == n - 1

@@ -178,3 +178,40 @@ def test_server_rounds_return_every_job():
     assert st in ({0}, {2})
     for d in done:
         assert d["t_submit"] <= d["t_done"]
+
+
+def _molden_cases():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "molden.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", ["water_sto3g", "water2_def2svp", "ethane_631gp"])
+def test_molden_matches_reference_writer(case, tmp_path):
+    """dsscf_job_write_molden reproduces, byte for byte, the file that the
+    reference's writeOrbitalsMolden logic and its own FormattedStream produce
+    for the same orbitals (fixture from oracle/_ref; orbitals synthetic, with
+    values at the number formatter's rounding and notation boundaries)."""
+    g = _molden_cases()[case]
+    Z = np.array(g["Z"], np.int32)
+    xyz = np.array(g["xyz_bohr"]).reshape(-1, 3)
+    job = ds.Job(0).set_basis(g["basis"]).set_molecule(Z, xyz).construct_host()
+    n = job.n_ao
+    Ck = np.array(g["C_columns"])
+    C = np.zeros((n, n))
+    C[:, :Ck.shape[1]] = Ck
+    job.set_orbitals(C, np.array(g["eps"]))
+    path = str(tmp_path / "out.molden")
+    job.write_molden(path, g["write_virtual"])
+    with open(path) as f:
+        got = f.read()
+    assert got == g["molden"]
+
+
+def test_molden_needs_orbitals(tmp_path):
+    Z, xyz = G.water()
+    job = ds.Job(0).set_basis("STO-3G").set_molecule(Z, xyz).construct_host()
+    with pytest.raises(ds.DsscfError):
+        job.write_molden(str(tmp_path / "x.molden"))
+    with pytest.raises(ValueError):
+        job.set_orbitals(np.zeros((3, 3)), np.zeros(3))

@@ -74,6 +74,8 @@ def lib():
             "dsscf_job_create": [p, i], "dsscf_job_destroy": [p],
             "dsscf_job_set_options": [p, i, d, i, i, d, i], "dsscf_job_set_precision": [p, d, d],
             "dsscf_job_set_step": [p, i], "dsscf_job_set_orbitals": [p, p, p], "dsscf_job_write_molden": [p, c, i],
+            "dsscf_job_gradient": [p, p, p], "dsscf_eri_gradient": [p, p, i, p, p], "dsscf_job_n_atom": [p],
+            "dsscf_job_gradient_terms": [p, p, p, p, p],
             "dsscf_job_set_molecule": [p, i, p, p, i, i], "dsscf_job_load_xyz": [p, c, i, i],
             "dsscf_job_set_basis_file": [p, c], "dsscf_job_construct": [p], "dsscf_job_construct_host": [p],
             "dsscf_job_ao": [p, i, p, p, p, p, p, p, p, p, p, i], "dsscf_job_n_ao": [p],
@@ -305,6 +307,15 @@ class Context:
         _check(lib().dsscf_one_electron(self._h, len(Z), _p(Z), _p(xyz), _p(S), _p(T), _p(V)))
         return S, T, V
 
+    def eri_gradient(self, Pa_packed, ao_atom, natom: int):
+        """Two-electron part of the RHF gradient at alpha density Pa (packed-U):
+        (natom, 3), Eh/bohr; ao_atom gives each AO's atom."""
+        Pa = np.ascontiguousarray(Pa_packed, np.float64)
+        at = np.ascontiguousarray(ao_atom, np.int32)
+        g = np.empty(3 * natom)
+        _check(lib().dsscf_eri_gradient(self._h, _p(Pa), natom, _p(at), _p(g)))
+        return g.reshape(natom, 3)
+
     def boys(self, T, L: int, mode: int = 2):
         """F_0..F_L at the arguments T on the device (mode 0 reference series, 1 accurate
         Taylor table, 2 accurate interval table in LDS); shape (len(T), L+1)."""
@@ -461,6 +472,28 @@ class Job:
         if C.shape != (n, n) or eps.shape != (n,):
             raise ValueError("set_orbitals: C must be n x n and eps n long")
         _check(lib().dsscf_job_set_orbitals(self._h, _p(C), _p(eps)), self._err)
+
+    def gradient(self, parts: bool = False):
+        """dE/dR of the solved job, (natom, 3) in Eh/bohr; with parts=True also the
+        one-electron, two-electron and nuclear-repulsion parts."""
+        na = lib().dsscf_job_n_atom(self._h)
+        g = np.empty(3 * na)
+        pt = np.empty(9 * na)
+        _check(lib().dsscf_job_gradient(self._h, _p(g), _p(pt)), self._err)
+        g = g.reshape(na, 3)
+        if parts:
+            return g, pt.reshape(3, na, 3)
+        return g
+
+    def gradient_terms(self, Pt_packed, W_packed):
+        """Host terms of the gradient for given densities: (one-electron, nuclear
+        repulsion), each (natom, 3)."""
+        na = lib().dsscf_job_n_atom(self._h)
+        Pt = np.ascontiguousarray(Pt_packed, np.float64)
+        W = np.ascontiguousarray(W_packed, np.float64)
+        g1, gn = np.empty(3 * na), np.empty(3 * na)
+        _check(lib().dsscf_job_gradient_terms(self._h, _p(Pt), _p(W), _p(g1), _p(gn)), self._err)
+        return g1.reshape(na, 3), gn.reshape(na, 3)
 
     def write_molden(self, path: str, write_virtual: bool = True):
         """Molden file of the current orbitals (scf::writeOrbitalsMolden)."""

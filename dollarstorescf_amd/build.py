@@ -3,7 +3,8 @@
   python -m dollarstorescf_amd.build [--maxl L] [--jobs J]
 
 1. gen_eri.py writes the per-class device code into csrc/_gen/, and one
-   small translation unit per shell-quartet class (cls_NN_<class>.hip).
+   small translation unit per shell-quartet class (cls_NN_<class>.hip);
+   gen_grad.py writes the s/p gradient classes (grad_classes.inc).
 2. hipcc compiles the class units, the dispatcher and the C-ABI for gfx950
    (packed FP32 instructions disabled) in parallel; g++ compiles the host SCF
    layer with OpenMP.
@@ -30,9 +31,9 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-I" + GEN, "-I" + CSRC, "-I" + os.path.join(ROOT, "include")]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-I" + os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["eri_kernels.hip", "dsscf_abi.cpp", "scf_device.hip"]
+HIP_SOURCES = ["eri_kernels.hip", "dsscf_abi.cpp", "scf_device.hip", "grad_kernels.hip"]
 HOST_SOURCES = ["host/chem.cpp", "host/linalg.cpp", "host/scf.cpp", "host/multi.cpp", "host/serve.cpp",
-                "host/job_api.cpp", "host/molden.cpp"]
+                "host/job_api.cpp", "host/molden.cpp", "host/gradient.cpp"]
 
 DEFAULT_MAXL = 2
 
@@ -74,6 +75,7 @@ def build(maxl: int = DEFAULT_MAXL, jobs: int = 8, verbose: bool = False) -> str
     os.makedirs(GEN, exist_ok=True)
     os.makedirs(OBJ, exist_ok=True)
     _run([sys.executable, os.path.join(CSRC, "gen_eri.py"), os.path.join(GEN, "eri_classes.inc"), str(maxl)])
+    _run([sys.executable, os.path.join(CSRC, "gen_grad.py"), os.path.join(GEN, "grad_classes.inc")])
     cmds = []
     objs = []
     # the largest classes (last in class order) first, so that they start at once

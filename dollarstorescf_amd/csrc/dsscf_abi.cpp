@@ -1124,6 +1124,101 @@ int dsscf_boys_eval(dsscf_ctx* c, int mode, int L, int n, const double* T, doubl
     return DSSCF_OK;
 }
 
+int dsscf_eri_gradient(dsscf_ctx* c, const double* Pa, int natom, const int* ao_atom, double* grad)
+{
+    if (!c || !Pa || natom <= 0 || !ao_atom || !grad) return fail(DSSCF_ERR_ARG, "dsscf_eri_gradient: bad argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_eri_gradient: call dsscf_prepare first");
+    const int N = c->n_ao;
+    for (int i = 0; i < N; ++i)
+        if (ao_atom[i] < 0 || ao_atom[i] >= natom) return fail(DSSCF_ERR_ARG, "dsscf_eri_gradient: AO atom index");
+    if ((size_t)3 * natom * sizeof(double) > 48 * 1024)
+        return fail(DSSCF_ERR_UNSUPPORTED, "dsscf_eri_gradient: more than 2048 atoms");
+    const int ncls = dk_n_class();
+    for (int cls = 0; cls < ncls; ++cls) {
+        int l4[4];
+        dk_class_l(cls, l4);
+        const bool sp = l4[0] <= 1 && l4[1] <= 1 && l4[2] <= 1 && l4[3] <= 1;
+        if (!sp && (c->q_cnt[cls] > 0 || c->q_cnt32[cls] > 0 || c->u_cnt[cls] > 0 || c->u_cnt32[cls] > 0))
+            return fail(DSSCF_ERR_UNSUPPORTED, "dsscf_eri_gradient: gradients are built for s and p shells only");
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t np = (size_t)N * (N + 1) / 2;
+    std::vector<double> Pt(np);
+    for (size_t i = 0; i < np; ++i) Pt[i] = 2.0 * Pa[i];
+    HIP_TRY(hipMemcpyAsync(c->d_Pt_pk, Pt.data(), sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_Pa_pk, Pa, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    // full densities from the packed ones (the prepare kernel of a Fock build)
+    MatSeg m{};
+    m.Ptp = c->d_Pt_pk;
+    m.Pap = c->d_Pa_pk;
+    m.Dt = c->d_Dt;
+    m.Da = c->d_Da;
+    m.Jh = c->d_Jh;
+    m.Kh = c->d_Kh;
+    m.N = N;
+    m.nslice = c->n_slice;
+    // per class: up to two segments (the FP64 and the FP32 work units; both run in FP64 here)
+    std::vector<JKSeg> segs;
+    std::vector<int> blk0, cls_of;
+    for (int cls = 0; cls < ncls; ++cls)
+        for (int pass = 0; pass < 2; ++pass) {
+            const long long cnt = pass ? c->u_cnt32[cls] : c->u_cnt[cls];
+            if (cnt <= 0) continue;
+            JKSeg sg{};
+            sg.u = c->d_units + (pass ? c->u_off32[cls] : c->u_off[cls]);
+            sg.k = c->d_kets;
+            sg.pairs = c->d_pairs;
+            sg.pp = c->d_pp;
+            sg.Dt = c->d_Dt;
+            sg.Da = c->d_Da;
+            sg.n = (int)cnt;
+            sg.N = N;
+            sg.blk0 = 0;
+            segs.push_back(sg);
+            blk0.push_back(0);
+            cls_of.push_back(cls);
+        }
+    MatSeg* d_m = nullptr;
+    JKSeg* d_segs = nullptr;
+    int *d_blk0 = nullptr, *d_atom = nullptr;
+    double* d_grad = nullptr;
+    auto cleanup = [&] {
+        dev_free(d_m);
+        dev_free(d_segs);
+        dev_free(d_blk0);
+        dev_free(d_atom);
+        dev_free(d_grad);
+    };
+    int rc = dev_alloc(&d_m, 1);
+    if (!rc && !segs.empty()) rc = dev_alloc(&d_segs, segs.size());
+    if (!rc && !segs.empty()) rc = dev_alloc(&d_blk0, blk0.size());
+    if (!rc) rc = dev_alloc(&d_atom, (size_t)N);
+    if (!rc) rc = dev_alloc(&d_grad, (size_t)3 * natom);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_m, &m, sizeof(MatSeg), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && !segs.empty())
+        e = hipMemcpyAsync(d_segs, segs.data(), sizeof(JKSeg) * segs.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && !segs.empty())
+        e = hipMemcpyAsync(d_blk0, blk0.data(), sizeof(int) * blk0.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_atom, ao_atom, sizeof(int) * N, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_grad, 0, sizeof(double) * 3 * natom, c->stream);
+    if (e == hipSuccess) e = dsscf_launch_prep(d_m, 1, N, N, c->stream);
+    for (size_t k = 0; e == hipSuccess && k < segs.size(); ++k) {
+        int l4[4];
+        dk_class_l(cls_of[k], l4);
+        e = dsscf_launch_grad(l4, c->opt.boys_mode, d_segs + k, d_blk0 + k, 1, segs[k].n, c->d_btab, c->d_xform, d_atom,
+                              d_grad, natom, kEriScale, c->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(grad, d_grad, sizeof(double) * 3 * natom, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(DSSCF_ERR_HIP, std::string("dsscf_eri_gradient: ") + hipGetErrorString(e));
+    return DSSCF_OK;
+}
+
 int dsscf_jk(dsscf_ctx* c, const double* Pt, const double* Pa, double* J, double* K)
 {
     if (!c || !Pt || !Pa || !J || !K) return fail(DSSCF_ERR_ARG, "dsscf_jk: null argument");

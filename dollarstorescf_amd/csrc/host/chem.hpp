@@ -84,6 +84,15 @@ int readXYZFile(const std::string& path, Molecule& out, std::string* error = nul
 void oneElectronMatrices(const std::vector<ContractedGaussian>& basis, const Molecule& mol, int boys_mode,
                          std::vector<double>& S, std::vector<double>& T, std::vector<double>& V);
 
+// One-electron part of the RHF gradient (Gradient.cpp:61-116): kinetic and
+// nuclear attraction weighted by the total density Pt, overlap by the
+// energy-weighted density W (packed-U), into grad[3*natom].  s and p shells.
+void oneElectronGradient(const std::vector<ContractedGaussian>& basis, const std::vector<unsigned>& basis_atoms,
+                         const Molecule& mol, int boys_mode, const std::vector<double>& Pt,
+                         const std::vector<double>& W, std::vector<double>& grad);
+// Gradient of the nuclear repulsion (Gradient.cpp:118-132).
+void nuclearRepulsionGradient(const Molecule& mol, std::vector<double>& grad);
+
 // Host Boys function (reference series or accurate), used by V.
 double boysFunction(double x, unsigned m, int boys_mode);
 

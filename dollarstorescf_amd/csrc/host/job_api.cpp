@@ -201,6 +201,33 @@ int dsscf_job_set_orbitals(dsscf_job* j, const double* C, const double* eps)
     return DSSCF_OK;
 }
 
+int dsscf_job_n_atom(dsscf_job* j) { return j ? (int)j->job.molecule.size() : 0; }
+
+int dsscf_job_gradient_terms(dsscf_job* j, const double* Pt, const double* W, double* one_e, double* nuclear)
+{
+    if (!j || !Pt || !W || !one_e || !nuclear) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_gradient_terms: null argument");
+    scf::Job& b = j->job;
+    if (b.basis.empty()) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_gradient_terms: construct first");
+    for (const auto& g : b.basis)
+        if (g.l > 1) return set_err(j, DSSCF_ERR_UNSUPPORTED, "dsscf_job_gradient_terms: s and p shells only");
+    const size_t np = (size_t)b.n * (b.n + 1) / 2;
+    std::vector<double> pt(Pt, Pt + np), w(W, W + np), g1, gn;
+    flo::oneElectronGradient(b.basis, b.basis_atoms, b.molecule, b.opt.boys_mode, pt, w, g1);
+    flo::nuclearRepulsionGradient(b.molecule, gn);
+    std::memcpy(one_e, g1.data(), sizeof(double) * g1.size());
+    std::memcpy(nuclear, gn.data(), sizeof(double) * gn.size());
+    return DSSCF_OK;
+}
+
+int dsscf_job_gradient(dsscf_job* j, double* grad, double* parts)
+{
+    if (!j || !grad) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_gradient: null argument");
+    std::vector<double> g;
+    if (int rc = scf::gradient(j->job, g, parts)) return rc;
+    std::memcpy(grad, g.data(), sizeof(double) * g.size());
+    return DSSCF_OK;
+}
+
 int dsscf_job_write_molden(dsscf_job* j, const char* path, int write_virtual)
 {
     if (!j || !path) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_write_molden: null argument");

@@ -118,6 +118,10 @@ int scfStep(Job& job);          // G, DIIS, F' = X F X, diagonalization, density
 bool converged(const Job& job);  // the stopping test of solveMOs (SCFSolver.cpp:174-177)
 int solveMOs(Job& job);
 int calculateEnergies(Job& job);
+// scf::getEnergyGradient (Gradient.cpp:14-133), restricted: dE/dR of a solved
+// job into grad[3*natom]; parts (may be null) gets the one-electron,
+// two-electron and nuclear-repulsion parts, 3*natom each.
+int gradient(Job& job, std::vector<double>& grad, double* parts);
 
 // Shell data for the device in the layout dsscf_set_basis expects.
 struct ShellData {

@@ -38,3 +38,9 @@ hipError_t dsscf_launch_fold(const MatSeg* segs, int nseg, int total_rows, int m
 // out[k*(L+1)+m].
 hipError_t dsscf_launch_boys_eval(int mode, int L, const double* T, int n, const double* btab, double* out,
                                   hipStream_t s);
+// ERI part of the nuclear gradient for one s/p class (l4) over nseg context
+// segments of J/K work units: grad[3*natom] += its contribution (see
+// grad_kernels.hip); xform the AO transform, ao_atom[N] the atom of each AO.
+hipError_t dsscf_launch_grad(const int* l4, int boys_mode, const JKSeg* segs, const int* blk0, int nseg, int nblocks,
+                             const double* btab, const double* xform, const int* ao_atom, double* grad, int natom,
+                             double scale, hipStream_t s);

@@ -164,6 +164,16 @@ int dsscf_eri_batch(dsscf_ctx* ctx, const int* ijkl, long long n, double* out);
  * checking aid (no reference counterpart beyond the series); synchronous. */
 int dsscf_boys_eval(dsscf_ctx* ctx, int mode, int L, int n, const double* T, double* F);
 
+/* Two-electron part of the RHF nuclear gradient at alpha density Pa
+ * (packed-U; Ptot = 2 Pa): grad[3*natom] (x, y, z per atom, Eh/bohr)
+ * receives sum over AO quartets of the pair density times d(ij|kl)/dR, the
+ * quantity scf::addExactTwoElectronGradient adds (ExactCoulomb.cpp:125-195,
+ * over flo::electronRepulsionGradient, GaussianIntegral.cpp:1052-1459);
+ * ao_atom[N] gives each AO's atom.  Uses the prepared quartet lists (Schwarz
+ * screening as the Fock build; 0 keeps every quartet).  s and p shells only:
+ * DSSCF_ERR_UNSUPPORTED if the basis has d shells.  Synchronous. */
+int dsscf_eri_gradient(dsscf_ctx* ctx, const double* Pa, int natom, const int* ao_atom, double* grad);
+
 /* Schwarz bounds Q_ab of all unique shell pairs (a >= b, index a(a+1)/2+b). */
 int dsscf_schwarz(dsscf_ctx* ctx, double* Q);
 

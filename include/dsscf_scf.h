@@ -78,6 +78,21 @@ int dsscf_job_fock_build(dsscf_job* job);
 /* scf::solveMOs (SCFSolver.cpp:196-237) + calculateEnergies (Energy.cpp:21-50):
  * total energy (getTotalEnergy, Energy.cpp:76-78) and iteration count. */
 int dsscf_job_solve(dsscf_job* job, double* energy, int* iterations);
+/* scf::getEnergyGradient (Gradient.cpp:14-133), restricted, of a solved job:
+ * grad[3*natom] = dE/dR (Eh/bohr, x y z per atom).  The two-electron part
+ * runs on the device (dsscf_eri_gradient), the one-electron and
+ * nuclear-repulsion parts on the host.  parts (may be null) receives the
+ * one-electron, two-electron and nuclear-repulsion parts, 3*natom each.
+ * s and p shells only (DSSCF_ERR_UNSUPPORTED otherwise). */
+int dsscf_job_gradient(dsscf_job* job, double* grad, double* parts);
+/* The host terms of the gradient for given packed-U total density Pt and
+ * energy-weighted density W: one_e[3*natom] the kinetic, overlap and
+ * nuclear-attraction terms (Gradient.cpp:61-116), nuclear[3*natom] the
+ * nuclear repulsion (Gradient.cpp:118-132).  Needs no GPU (a constructed
+ * or host-constructed job).  s and p shells only. */
+int dsscf_job_gradient_terms(dsscf_job* job, const double* Pt, const double* W, double* one_e, double* nuclear);
+/* Atoms of the job's molecule (scf::molecule.size()). */
+int dsscf_job_n_atom(dsscf_job* job);
 /* kinetic, nuclear attraction, Coulomb, exchange, nuclear repulsion. */
 int dsscf_job_energies(dsscf_job* job, double* out5);
 /* Electronic energies of every iteration (up to cap); returns the count. */

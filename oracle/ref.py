@@ -36,6 +36,8 @@ def lib():
         L.ref_time_eri_list.restype = d
         L.ref_time_eri_list.argtypes = [p, ctypes.c_long, p]
         L.ref_one_electron.argtypes = [p, p, p]
+        L.ref_gradient.argtypes = [p, p, p, i, p]
+        L.ref_eri_gradient.argtypes = [i, i, i, i, p]
         L.ref_molden.restype = i
         L.ref_molden.argtypes = [ctypes.c_char_p, p, p, i, i]
         _LIB = L
@@ -94,6 +96,20 @@ class RefMolecule:
         lib().ref_one_electron(_p(S), _p(T), _p(V))
         return S, T, V
 
+
+    def gradient(self, Pt, Pa, W, part: int = 7):
+        """getEnergyGradient restated over the reference's derivative integrals:
+        part 1 one-electron, 2 two-electron, 4 nuclear repulsion (sums allowed)."""
+        self.rebuild()
+        g = np.empty(3 * len(self.Z))
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (Pt, Pa, W)]
+        lib().ref_gradient(_p(arrs[0]), _p(arrs[1]), _p(arrs[2]), part, _p(g))
+        return g.reshape(-1, 3)
+
+    def eri_gradient(self, i, j, k, l):
+        out = np.empty(12)
+        lib().ref_eri_gradient(i, j, k, l, _p(out))
+        return out.reshape(4, 3)
 
     def molden(self, path: str, C, eps, nocc: int, write_virtual: bool):
         """The reference's writeOrbitalsMolden text for orbitals C (n x n,

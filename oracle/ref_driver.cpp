@@ -233,3 +233,121 @@ int ref_molden(const char* path, const double* C, const double* eps, int nocc, i
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// electronRepulsionGradient for AOs (i j|k l): 12 values, d/dA of centres of
+// i, k, j, l in that order (the reference's return order, GaussianIntegral.cpp
+// :1459, for its call electronRepulsionGradient(i, k, j, l), ExactCoulomb.cpp:141).
+void ref_eri_gradient(int i, int j, int k, int l, double* out)
+{
+    const auto g = flo::electronRepulsionGradient(g_aos[i], g_aos[k], g_aos[j], g_aos[l]);
+    for (int t = 0; t < 4; ++t) {
+        out[3 * t] = g[t].x;
+        out[3 * t + 1] = g[t].y;
+        out[3 * t + 2] = g[t].z;
+    }
+}
+
+// scf::getEnergyGradient (Gradient.cpp:14-133) for a restricted job over the
+// AOs of the last ref_build, with the densities given as packed-U arrays:
+// Pt total density, Pa alpha density, W the energy-weighted density
+// (2 sum_i<nocc C_mi C_ni eps_i, Gradient.cpp:45-58).  part selects the
+// terms: 1 = one-electron (kinetic, overlap, nuclear attraction), 2 = the
+// two-electron loop (addExactTwoElectronGradient, ExactCoulomb.cpp:125-195),
+// 4 = nuclear repulsion; grad[3*natom] receives their sum.  The loops follow
+// the reference's; the integrals and their derivatives are its own functions.
+void ref_gradient(const double* Pt, const double* Pa, const double* W, int part, double* grad)
+{
+    using namespace flo;
+    const int N = (int)g_aos.size(), natom = (int)g_Z.size();
+    std::vector<int> atom_of(N);
+    for (int a = 0, k = 0; a < natom; ++a)
+        for (const GTOShell& sh : g_basis_set.atomic_bases[g_Z[a] - 1])
+            for (int m = 0; m < 2 * (int)sh.l + 1; ++m) atom_of[k++] = a;
+    auto pk = [](int i, int j) { return i >= j ? (size_t)i * (i + 1) / 2 + j : (size_t)j * (j + 1) / 2 + i; };
+    auto pos = [&](int a) { return vec3(g_xyz[3 * a], g_xyz[3 * a + 1], g_xyz[3 * a + 2]); };
+    for (int t = 0; t < 3 * natom; ++t) grad[t] = 0.0;
+    auto add = [&](int a, const vec3& v, double s) {
+        grad[3 * a] += s * v.x;
+        grad[3 * a + 1] += s * v.y;
+        grad[3 * a + 2] += s * v.z;
+    };
+    if (part & 1) {
+        for (int i = 0; i < N; ++i) {
+            const int ai = atom_of[i];
+            for (int j = i; j < N; ++j) {
+                const int aj = atom_of[j];
+                double density = Pt[pk(i, j)];
+                if (i != j) density *= 2.0;
+                if (ai != aj) {
+                    const vec3 kg = density * kineticEnergyGradient(g_aos[i], g_aos[j]);
+                    add(ai, kg, 1.0);
+                    add(aj, kg, -1.0);
+                    double wd = W[pk(i, j)];
+                    if (i != j) wd *= 2.0;
+                    const vec3 og = wd * overlapGradient(g_aos[i], g_aos[j]);
+                    add(ai, og, -1.0);
+                    add(aj, og, 1.0);
+                }
+                for (int c = 0; c < natom; ++c) {
+                    if (ai == aj && c == ai) continue;
+                    const auto ng = nuclearPotentialGradient(g_aos[i], g_aos[j], pos(c));
+                    const vec3 gm = -density * (double)g_Z[c] * ng[0];
+                    const vec3 gn = -density * (double)g_Z[c] * ng[1];
+                    add(c, gm + gn, -1.0);
+                    add(ai, gm, 1.0);
+                    add(aj, gn, 1.0);
+                }
+            }
+        }
+    }
+    if (part & 2) {
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j <= i; ++j)
+                for (int k = 0; k < N; ++k)
+                    for (int l = 0; l <= k; ++l) {
+                        const int ai = atom_of[i], aj = atom_of[j], ak = atom_of[k], al = atom_of[l];
+                        if (ai == aj && ak == al && ai == ak) continue;
+                        const int m = j + i * (i + 1) / 2, n = l + k * (k + 1) / 2;
+                        if (m > n) continue;
+                        const auto g = electronRepulsionGradient(g_aos[i], g_aos[k], g_aos[j], g_aos[l]);
+                        double cd = Pt[pk(j, i)] * Pt[pk(l, k)];
+                        double xd = 2.0 * Pa[pk(j, k)] * Pa[pk(i, l)];
+                        if (i != j || k != l) {
+                            xd += 2.0 * Pa[pk(i, k)] * Pa[pk(j, l)];
+                            xd *= 0.5;
+                        }
+                        if (i != j) {
+                            cd *= 2.0;
+                            xd *= 2.0;
+                        }
+                        if (k != l) {
+                            cd *= 2.0;
+                            xd *= 2.0;
+                        }
+                        if (m != n) {
+                            cd *= 2.0;
+                            xd *= 2.0;
+                        }
+                        const double f = 0.5 * (cd - xd);
+                        add(ai, g[0], f);
+                        add(ak, g[1], f);
+                        add(aj, g[2], f);
+                        add(al, g[3], f);
+                    }
+    }
+    if (part & 4) {
+        for (int i = 0; i < natom; ++i)
+            for (int j = i + 1; j < natom; ++j) {
+                // Atom::getRepulsionGradient (Molecule.cpp:20-26) times Z_j
+                const vec3 x = pos(j) - pos(i);
+                const double den = std::pow(length2(x), -1.5);
+                const vec3 g = (double)g_Z[j] * (-((double)g_Z[i]) * x * den);
+                add(i, g, -1.0);
+                add(j, g, 1.0);
+            }
+    }
+}
+
+}  // extern "C"

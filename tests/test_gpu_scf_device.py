@@ -109,7 +109,7 @@ def test_molden_of_solved_job(device_step, tmp_path):
         elif line.startswith("Ene="):
             assert abs(float(line.split()[1]) - eps[mo]) <= 1e-5 * max(1.0, abs(eps[mo]))
         elif line.startswith("Occup="):
-            assert int(line.split()[1]) == (2 if mo < 20 else 0)
+            assert int(line.split()[1]) == (2 if mo < 10 else 0)  # 20 electrons
         elif line.strip() and not line.startswith("Spin"):
             idx, val = line.split()
             assert int(idx) == row + 1

@@ -172,6 +172,60 @@ def bench_scf(a):
     print(json.dumps(out))
 
 
+def bench_gradient(a):
+    """RHF nuclear gradient of the 8-water/6-31G lattice (104 AOs, s/p shells)
+    at its converged density: the two-electron part on the device (every
+    surviving shell quartet, Schwarz 1e-12), the one-electron and
+    nuclear-repulsion parts on the host.  CPU baseline: the reference's
+    electronRepulsionGradient (oracle/_ref, 1 core) on a sample of unique AO
+    quartets, extrapolated to all of them."""
+    import dollarstorescf_amd as ds
+    from dollarstorescf_amd import geometry
+
+    Z, xyz = geometry.water_lattice(1)
+    xyz = np.asarray(xyz, float).reshape(-1, 3)
+    job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-10, max_iterations=60, schwarz_threshold=1e-12)
+    job.set_basis("6-31G").set_molecule(Z, xyz).construct()
+    energy, iters = job.solve()
+    n = job.n_ao
+    ao_atom = np.array([int(np.argmin(np.linalg.norm(xyz - job.ao(i)["center"], axis=1))) for i in range(n)], np.int32)
+    Pa = job.get("Pa")
+    ctx = job.context
+    for _ in range(a.warmup):
+        ctx.eri_gradient(Pa, ao_atom, len(Z))
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.eri_gradient(Pa, ao_atom, len(Z))
+    ms_eri = (time.perf_counter() - t0) / a.steps * 1e3
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        g = job.gradient()
+    ms_total = (time.perf_counter() - t0) / a.steps * 1e3
+    cpu = None
+    if not a.no_cpu:
+        from oracle import ref
+        if ref.available():
+            rm = ref.RefMolecule(ds.basis_path("6-31G"), Z, xyz.reshape(-1))
+            rng = np.random.default_rng(5)
+            sample = max(1000, a.cpu_sample // 10)
+            q = rng.integers(0, n, size=(sample, 4)).astype(np.int32)
+            t = rm.time_eri_gradient(q)
+            n_unique = (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2
+            cpu = {"value": 1.0 / (t / sample * n_unique), "unit": "ERI gradients/s", "cores": 1, "kind": "reference",
+                   "sample": f"{sample} random AO quartets of the same system through the reference's "
+                             f"electronRepulsionGradient (GaussianIntegral.cpp:1052), {t:.1f} s on 1 core, "
+                             f"extrapolated to {n_unique} unique AO quartets"}
+    out = {"metric": METRIC, "value": 1e3 / ms_eri, "unit": "ERI gradients/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms_eri, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded water lattice geometry)",
+           "config": {"workload": "RHF/6-31G nuclear gradient, 8-water 2x2x2 lattice, 104 AOs, Schwarz 1e-12, "
+                                  "converged density", "parallelism": "single GPU"},
+           "ms_full_gradient": ms_total, "scf_energy": energy, "scf_iterations": iters,
+           "max_abs_gradient": float(np.max(np.abs(g))), "sum_over_atoms": np.abs(g.sum(axis=0)).max(),
+           "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
 def batch_specs(n_jobs: int, seed: int = 2024):
     """n_jobs molecules of 1-6 rigid waters (uniform), each placed by rejection
     sampling in a 6 A box with minimum O-O distance 2.6 A and random
@@ -344,11 +398,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=5_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--boys", choices=["accurate", "reference"], default="accurate")
-    ap.add_argument("--workload", choices=["water8", "alkane", "batch", "serve", "scf"], default="water8",
+    ap.add_argument("--workload", choices=["water8", "alkane", "batch", "serve", "scf", "gradient"], default="water8",
                     help="water8: the Fock-build benchmark (default); alkane: C20H42/def2-SVP, FP64 vs "
                          "FP32 below Schwarz 1e-5; batch: 512 RHF/6-31G* jobs of 1-6 waters, fused vs one "
                          "job at a time; serve: Poisson stream of SCF requests, mixed precision vs FP64; scf: whole "
-                         "SCF solves with the iteration step on the device vs the host")
+                         "SCF solves with the iteration step on the device vs the host; gradient: RHF nuclear "
+                         "gradient of the water8 system")
     ap.add_argument("--jobs", type=int, default=512, help="jobs of the batch workload")
     ap.add_argument("--requests", type=int, default=200, help="requests of the serve workload")
     ap.add_argument("--rate", type=float, default=200.0, help="mean arrival rate of the serve workload (1/s)")
@@ -374,6 +429,9 @@ def main():
         return
     if a.workload == "scf":
         bench_scf(a)
+        return
+    if a.workload == "gradient":
+        bench_gradient(a)
         return
 
     import dollarstorescf_amd as ds

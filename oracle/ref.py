@@ -38,6 +38,8 @@ def lib():
         L.ref_one_electron.argtypes = [p, p, p]
         L.ref_gradient.argtypes = [p, p, p, i, p]
         L.ref_eri_gradient.argtypes = [i, i, i, i, p]
+        L.ref_time_eri_gradient.restype = d
+        L.ref_time_eri_gradient.argtypes = [p, ctypes.c_long, p]
         L.ref_molden.restype = i
         L.ref_molden.argtypes = [ctypes.c_char_p, p, p, i, i]
         _LIB = L
@@ -110,6 +112,12 @@ class RefMolecule:
         out = np.empty(12)
         lib().ref_eri_gradient(i, j, k, l, _p(out))
         return out.reshape(4, 3)
+
+    def time_eri_gradient(self, quartets) -> float:
+        q = np.ascontiguousarray(quartets, dtype=np.int32).reshape(-1, 4)
+        sink = np.zeros(1)
+        self.rebuild()
+        return lib().ref_time_eri_gradient(_p(q), len(q), _p(sink))
 
     def molden(self, path: str, C, eps, nocc: int, write_virtual: bool):
         """The reference's writeOrbitalsMolden text for orbitals C (n x n,

@@ -351,3 +351,24 @@ void ref_gradient(const double* Pt, const double* Pa, const double* W, int part,
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Wall-clock seconds of electronRepulsionGradient over the given AO quartets
+// (i j k l as (ij|kl), the call of ExactCoulomb.cpp:141): CPU baseline of the
+// gradient; the 12 values of each are summed into *sink.
+double ref_time_eri_gradient(const int* q, long n, double* sink)
+{
+    double s = 0.0;
+    auto t0 = std::chrono::steady_clock::now();
+    for (long t = 0; t < n; ++t) {
+        const auto g = flo::electronRepulsionGradient(g_aos[q[4 * t]], g_aos[q[4 * t + 2]], g_aos[q[4 * t + 1]],
+                                                      g_aos[q[4 * t + 3]]);
+        for (int k = 0; k < 4; ++k) s += g[k].x + g[k].y + g[k].z;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    *sink = s;
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"

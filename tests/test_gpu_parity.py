@@ -356,3 +356,25 @@ def test_device_boys_interval_matches_taylor_table():
         a = ctx.boys(T, L, 1)
         b = ctx.boys(T, L, 2)
         assert np.all(np.abs(a - b) <= 4e-15 * np.abs(a)), L
+
+
+@pytest.mark.parametrize("boys,device_step", [(ds.BOYS_REFERENCE, False), (ds.BOYS_ACCURATE, False),
+                                              (ds.BOYS_ACCURATE, True)])
+def test_scf_config1_water_lattice_vs_oracle(boys, device_step):
+    """Config 1 (8-water 2x2x2 lattice, RHF/6-31G, 104 AOs, Schwarz 1e-12,
+    converged to 1e-10): total energy within 1e-8 Eh of the oracle's RHF over
+    the full stored tensor in the same Boys mode (the reference's series in
+    reference mode), and each energy term within 1e-7 Eh."""
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "energies.json")))
+    ref = g["cases"]["water_lattice8_631g"]["reference" if boys == ds.BOYS_REFERENCE else "accurate"]
+    Z, xyz = G.water_lattice(1)
+    job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-10, max_iterations=80, boys_mode=boys,
+                                schwarz_threshold=1e-12, device_step=device_step)
+    job.set_basis("6-31G").set_molecule(Z, xyz).construct()
+    assert job.n_ao == 104
+    e, _ = job.solve()
+    assert abs(e - ref["energy"]) < 1e-8, (e, ref["energy"])
+    got = job.energies()
+    for k, name in (("kinetic", "kinetic"), ("nuclear_attraction", "nuclear_attraction"), ("coulomb", "coulomb"),
+                    ("exchange", "exchange"), ("nuclear", "nuclear")):
+        assert abs(got[k] - ref["components"][name]) < 1e-7, k

@@ -88,6 +88,10 @@ def build(maxl: int = DEFAULT_MAXL, jobs: int = 8, verbose: bool = False) -> str
         o = os.path.join(OBJ, os.path.basename(s) + ".o")
         cmds.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
+    # the AVX2/FMA build of the host eigensolver and products (chosen at run time)
+    o = os.path.join(OBJ, "linalg_avx2.cpp.o")
+    cmds.append(["g++", *CXX_FLAGS, "-DLA_AVX2", "-mavx2", "-mfma", "-c", os.path.join(CSRC, "host/linalg.cpp"), "-o", o])
+    objs.append(o)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for r in ex.map(_run, cmds):
             if verbose and (r.stdout or r.stderr):

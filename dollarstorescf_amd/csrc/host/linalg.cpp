@@ -1,27 +1,21 @@
 // linalg.cpp -- see linalg.hpp.
+//
+// Compiled twice: as is (any x86-64) and with -DLA_AVX2 -mavx2 -mfma; the
+// gemm and syev of la:: pick the AVX2/FMA build at run time where the host
+// CPU has it.  The other functions exist once.
 #include "linalg.hpp"
 
 #include <algorithm>
 #include <cmath>
 
-namespace la {
+#ifdef LA_AVX2
+#define LA_KERNELS la_avx2
+#else
+#define LA_KERNELS la_base
+#endif
 
-void unpack(const double* pk, int n, Mat& M)
-{
-    M.assign((size_t)n * n, 0.0);
-    for (int x = 0; x < n; ++x)
-        for (int y = 0; y <= x; ++y) {
-            const double v = pk[(size_t)x * (x + 1) / 2 + y];
-            M[(size_t)y * n + x] = v;
-            M[(size_t)x * n + y] = v;
-        }
-}
-
-void pack(const Mat& M, int n, double* pk)
-{
-    for (int x = 0; x < n; ++x)
-        for (int y = 0; y <= x; ++y) pk[(size_t)x * (x + 1) / 2 + y] = M[(size_t)y * n + x];
-}
+namespace LA_KERNELS {
+using la::Mat;
 
 void gemm(const Mat& A, const Mat& B, Mat& C, int n)
 {
@@ -39,13 +33,6 @@ void gemm(const Mat& A, const Mat& B, Mat& C, int n)
                 }
 }
 
-void transpose(const Mat& A, Mat& C, int n)
-{
-    C.resize((size_t)n * n);
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) C[(size_t)j * n + i] = A[(size_t)i * n + j];
-}
-
 // Symmetric eigensolver in two stages (Golub & Van Loan, "Matrix
 // Computations", 8.3):
 //  1. Householder tridiagonalization T = Q^T A Q: for column k a reflector
@@ -58,7 +45,10 @@ void transpose(const Mat& A, Mat& C, int n)
 //     applied to the eigenvector rows Z (row j = eigenvector j), which start
 //     as the columns of Q.  Off-diagonals with |e_k| <= eps (|d_k|+|d_k+1|)
 //     are set to zero (deflation).
-bool syev(const Mat& A0, int n, std::vector<double>& w, Mat& V)
+// Built twice, for AVX2/FMA hosts and for any x86-64, the loader picking one
+// at run time.
+bool syev(const Mat& A0, int n, std::vector<double>& w,
+                                                                       Mat& V)
 {
     w.assign(n, 0.0);
     V.assign((size_t)n * n, 0.0);
@@ -154,10 +144,12 @@ bool syev(const Mat& A0, int n, std::vector<double>& w, Mat& V)
         }
         // Wilkinson shift from the trailing 2x2 of the block [lo, hi]
         const double dl = (d[hi - 1] - d[hi]) * 0.5, eh = e[hi - 1];
-        const double mu = d[hi] - eh * eh / (dl + (dl >= 0.0 ? 1.0 : -1.0) * std::hypot(dl, eh));
+        // (plain square roots: the entries are Fock-matrix sized, far from
+        // overflow, and hypot's care costs more than the rest of the chase)
+        const double mu = d[hi] - eh * eh / (dl + (dl >= 0.0 ? 1.0 : -1.0) * std::sqrt(dl * dl + eh * eh));
         double x = d[lo] - mu, z = e[lo], bulge = 0.0;
         for (int k = lo; k < hi; ++k) {
-            const double r = std::hypot(x, z);
+            const double r = std::sqrt(x * x + z * z);
             const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : z / r;
             // T <- G T G^T on rows/cols (k, k+1), G = [[c, s], [-s, c]]
             if (k > lo) e[k - 1] = r;  // the rotation folded the bulge into e[k-1]
@@ -188,6 +180,57 @@ bool syev(const Mat& A0, int n, std::vector<double>& w, Mat& V)
     return ok;
 }
 
+}  // namespace LA_KERNELS
+
+#ifndef LA_AVX2
+namespace la_avx2 {
+void gemm(const la::Mat& A, const la::Mat& B, la::Mat& C, int n);
+bool syev(const la::Mat& A0, int n, std::vector<double>& w, la::Mat& V);
+}  // namespace la_avx2
+
+namespace la {
+
+static bool host_has_avx2()
+{
+    static const bool v = __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+    return v;
+}
+
+void gemm(const Mat& A, const Mat& B, Mat& C, int n)
+{
+    if (host_has_avx2()) la_avx2::gemm(A, B, C, n);
+    else la_base::gemm(A, B, C, n);
+}
+
+bool syev(const Mat& A0, int n, std::vector<double>& w, Mat& V)
+{
+    return host_has_avx2() ? la_avx2::syev(A0, n, w, V) : la_base::syev(A0, n, w, V);
+}
+
+void unpack(const double* pk, int n, Mat& M)
+{
+    M.assign((size_t)n * n, 0.0);
+    for (int x = 0; x < n; ++x)
+        for (int y = 0; y <= x; ++y) {
+            const double v = pk[(size_t)x * (x + 1) / 2 + y];
+            M[(size_t)y * n + x] = v;
+            M[(size_t)x * n + y] = v;
+        }
+}
+
+void pack(const Mat& M, int n, double* pk)
+{
+    for (int x = 0; x < n; ++x)
+        for (int y = 0; y <= x; ++y) pk[(size_t)x * (x + 1) / 2 + y] = M[(size_t)y * n + x];
+}
+
+void transpose(const Mat& A, Mat& C, int n)
+{
+    C.resize((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) C[(size_t)j * n + i] = A[(size_t)i * n + j];
+}
+
 bool solve(std::vector<double> A, std::vector<double>& b, int n)
 {
     for (int c = 0; c < n; ++c) {
@@ -215,3 +258,4 @@ bool solve(std::vector<double> A, std::vector<double>& b, int n)
 }
 
 }  // namespace la
+#endif

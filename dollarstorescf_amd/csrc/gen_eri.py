@@ -540,7 +540,11 @@ def gen_wave_class(c, spherical=True):
     na = [nao(l, spherical) for l in c]
     NAO = na[0] * na[1] * na[2] * na[3]
     nx = zero + 1
-    lds = max(nx, ny + NAO)
+    # the staged AO transform writes its 1st and 3rd stages after the HRR
+    # outputs (the 2nd and 4th go to the start of the table)
+    st1 = ncart(la) * ncart(lb) * ncart(lc) * na[3]
+    st3 = ncart(la) * na[1] * na[2] * na[3]
+    lds = max(nx, ny + max(st1, st3), ny + NAO)
     o = []
     o.append("// ---- class (%s) : wave-cooperative, %d VRR entries in %d levels, %d HRR entries, LDS %d doubles"
              % (name, len(keys), nlev, len(hent), lds))

@@ -185,32 +185,80 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             }
             __syncthreads();
         }
-        // Cartesian -> AO
-        for (int o = lg; o < W::NAO; o += LG) {
-            const int md = o % ND, mc = (o / ND) % NC, mb = (o / (ND * NC)) % NB, ma = o / (ND * NC * NB);
-            const int pa = la * la + ma, pb = lb * lb + mb, pc = lc * lc + mc, pd = ld * ld + md;
-            double sum = 0.0;
-            for (int ta = 0; ta < ao_pat_n[pa]; ++ta) {
-                const int ca = ao_pat_c[3 * pa + ta];
-                const double wa = c_ao_xform[xform_off(la) + ma * ncA + ca];
-                for (int tb = 0; tb < ao_pat_n[pb]; ++tb) {
-                    const int cb = ao_pat_c[3 * pb + tb];
-                    const double wb = wa * c_ao_xform[xform_off(lb) + mb * ncB + cb];
-                    for (int tc = 0; tc < ao_pat_n[pc]; ++tc) {
-                        const int cc = ao_pat_c[3 * pc + tc];
-                        const double wc = wb * c_ao_xform[xform_off(lc) + mc * ncC + cc];
-                        for (int td = 0; td < ao_pat_n[pd]; ++td) {
-                            const int cd = ao_pat_c[3 * pd + td];
-                            const double wd = wc * c_ao_xform[xform_off(ld) + md * ncD + cd];
-                            sum += wd * sm[carty[((ca * ncB + cb) * ncC + cc) * ncD + cd]];
-                        }
-                    }
+        // Cartesian -> AO, one index at a time (d, c, b, a): each stage output
+        // sums the at most three Cartesian components of one AO.  The AO index
+        // m is the outer, wavefront-uniform loop, so its terms (weight and
+        // Cartesian position) come through scalar loads; the lanes split the
+        // remaining indices.  Stages 1 and 3 write after the HRR outputs,
+        // stages 2 and 4 from the table's start (the HRR outputs are dead once
+        // stage 1 has read them).
+        {
+            double* __restrict__ R1 = sm + W::NY;
+            double* __restrict__ R2 = sm;
+            for (int md = 0; md < ND; ++md) {  // T1[ca][cb][cc][md]
+                const int pat = ld * ld + md, nt = ao_pat_n[pat];
+                const int c0 = ao_pat_c[3 * pat], c1 = ao_pat_c[3 * pat + 1], c2 = ao_pat_c[3 * pat + 2];
+                const double w0 = c_ao_xform[xform_off(ld) + md * ncD + c0];
+                const double w1 = nt > 1 ? c_ao_xform[xform_off(ld) + md * ncD + c1] : 0.0;
+                const double w2 = nt > 2 ? c_ao_xform[xform_off(ld) + md * ncD + c2] : 0.0;
+                for (int r = lg; r < ncA * ncB * ncC; r += LG) {
+                    const short* cy = carty + r * ncD;
+                    double sum = w0 * sm[cy[c0]];
+                    if (nt > 1) sum += w1 * sm[cy[c1]];
+                    if (nt > 2) sum += w2 * sm[cy[c2]];
+                    R1[r * ND + md] = sum;
                 }
             }
-            sm[W::NY + o] = scale * sum;
+            __syncthreads();
+            for (int mc = 0; mc < NC; ++mc) {  // T2[ca][cb][mc][md]
+                const int pat = lc * lc + mc, nt = ao_pat_n[pat];
+                const int c0 = ao_pat_c[3 * pat], c1 = ao_pat_c[3 * pat + 1], c2 = ao_pat_c[3 * pat + 2];
+                const double w0 = c_ao_xform[xform_off(lc) + mc * ncC + c0];
+                const double w1 = nt > 1 ? c_ao_xform[xform_off(lc) + mc * ncC + c1] : 0.0;
+                const double w2 = nt > 2 ? c_ao_xform[xform_off(lc) + mc * ncC + c2] : 0.0;
+                for (int o = lg; o < ncA * ncB * ND; o += LG) {
+                    const int md = o % ND, r = o / ND;
+                    const double* in = R1 + r * ncC * ND + md;
+                    double sum = w0 * in[c0 * ND];
+                    if (nt > 1) sum += w1 * in[c1 * ND];
+                    if (nt > 2) sum += w2 * in[c2 * ND];
+                    R2[(r * NC + mc) * ND + md] = sum;
+                }
+            }
+            __syncthreads();
+            for (int mb = 0; mb < NB; ++mb) {  // T3[ca][mb][mc][md]
+                const int pat = lb * lb + mb, nt = ao_pat_n[pat];
+                const int c0 = ao_pat_c[3 * pat], c1 = ao_pat_c[3 * pat + 1], c2 = ao_pat_c[3 * pat + 2];
+                const double w0 = c_ao_xform[xform_off(lb) + mb * ncB + c0];
+                const double w1 = nt > 1 ? c_ao_xform[xform_off(lb) + mb * ncB + c1] : 0.0;
+                const double w2 = nt > 2 ? c_ao_xform[xform_off(lb) + mb * ncB + c2] : 0.0;
+                for (int o = lg; o < ncA * NC * ND; o += LG) {
+                    const int q = o % (NC * ND), ca = o / (NC * ND);
+                    const double* in = R2 + ca * ncB * NC * ND + q;
+                    double sum = w0 * in[c0 * NC * ND];
+                    if (nt > 1) sum += w1 * in[c1 * NC * ND];
+                    if (nt > 2) sum += w2 * in[c2 * NC * ND];
+                    R1[(ca * NB + mb) * NC * ND + q] = sum;
+                }
+            }
+            __syncthreads();
+            for (int ma = 0; ma < NA; ++ma) {  // V[ma][mb][mc][md]
+                const int pat = la * la + ma, nt = ao_pat_n[pat];
+                const int c0 = ao_pat_c[3 * pat], c1 = ao_pat_c[3 * pat + 1], c2 = ao_pat_c[3 * pat + 2];
+                const double w0 = c_ao_xform[xform_off(la) + ma * ncA + c0];
+                const double w1 = nt > 1 ? c_ao_xform[xform_off(la) + ma * ncA + c1] : 0.0;
+                const double w2 = nt > 2 ? c_ao_xform[xform_off(la) + ma * ncA + c2] : 0.0;
+                for (int q = lg; q < NB * NC * ND; q += LG) {
+                    const double* in = R1 + q;
+                    double sum = w0 * in[c0 * NB * NC * ND];
+                    if (nt > 1) sum += w1 * in[c1 * NB * NC * ND];
+                    if (nt > 2) sum += w2 * in[c2 * NB * NC * ND];
+                    R2[ma * NB * NC * ND + q] = scale * sum;
+                }
+            }
         }
         __syncthreads();
-        const double* V = sm + W::NY;
+        const double* V = sm;
 #define VX(i, j, k, l) V[(((i) * NB + (j)) * NC + (k)) * ND + (l)]
         if (!valid) {
         } else if (MODE == WAVE_BLOCK) {

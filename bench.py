@@ -315,7 +315,7 @@ def serve_requests(n_req: int, rate: float, seed: int = 7):
     return arrivals, out
 
 
-def serve_stream(ds, arrivals, reqs, fp32: float, max_active: int, budget: float):
+def serve_stream(ds, arrivals, reqs, fp32: float, max_active: int, budget: float, device_steps: bool = False):
     """Replay the stream in real time against one server; returns per-request
     latency (finish - scheduled arrival), energies, status and the wall time
     from the first arrival to the last finish."""
@@ -324,7 +324,7 @@ def serve_stream(ds, arrivals, reqs, fp32: float, max_active: int, budget: float
         job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12,
                                     fp32_threshold=fp32)
         jobs.append(job.set_basis(basis).set_molecule(Z, xyz))
-    srv = ds.Server(0, budget_bytes=budget, max_active=max_active, workers=12)
+    srv = ds.Server(0, budget_bytes=budget, max_active=max_active, workers=12, device_steps=device_steps)
     n = len(reqs)
     t0 = srv.clock()
     i, done = 0, {}
@@ -357,13 +357,17 @@ def bench_serve(a):
     (jobs constructed by worker threads, admitted by device-memory budget,
     one fused Fock build per round over every job on the device).  The
     stream runs with FP32 below Schwarz 1e-5 (value) and again with every
-    quartet in FP64; reported: p50/p99 latency, solves/s, |dE| per request."""
+    quartet in FP64; reported: p50/p99 latency, solves/s, |dE| per request.
+    --device-steps: every job's densities, J/K, Fock matrix, DIIS history and
+    eigenproblem stay in HBM (device SCF step); otherwise the iterations'
+    linear algebra runs on host worker threads."""
     import dollarstorescf_amd as ds
 
     arrivals, reqs = serve_requests(a.requests, a.rate)
     runs = {}
     for mode, thr in (("mixed", 1e-5), ("fp64", 0.0)):
-        lat, e, st, wall, stats = serve_stream(ds, arrivals, reqs, thr, max_active=64, budget=32e9)
+        lat, e, st, wall, stats = serve_stream(ds, arrivals, reqs, thr, max_active=64, budget=32e9,
+                                               device_steps=a.device_steps)
         runs[mode] = {"p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                       "mean_latency_s": float(lat.mean()), "solves_per_s": len(reqs) / wall, "wall_s": wall,
                       "failed": int(np.count_nonzero(st)), "server": stats, "_e": e}
@@ -382,6 +386,8 @@ def bench_serve(a):
         "config": {"workload": f"{len(reqs)} RHF requests, Poisson arrivals at {a.rate:g}/s (seed 7), STO-3G / "
                                "6-31G / def2-SVP, 1-12 waters or C1-C12 alkanes; server: 64 slots, 32 GB budget, "
                                "12 host workers",
+                   "iterations": "device-resident (DIIS, diagonalization on the device)" if a.device_steps
+                   else "host steps on the workers",
                    "n_ao_min": int(min(n_ao)), "n_ao_max": int(max(n_ao)), "parallelism": "single GPU"},
         "mixed": runs["mixed"], "fp64": runs["fp64"],
         "p50_latency_s": runs["mixed"]["p50_latency_s"], "p99_latency_s": runs["mixed"]["p99_latency_s"],
@@ -407,6 +413,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=512, help="jobs of the batch workload")
     ap.add_argument("--requests", type=int, default=200, help="requests of the serve workload")
     ap.add_argument("--rate", type=float, default=200.0, help="mean arrival rate of the serve workload (1/s)")
+    ap.add_argument("--device-steps", action="store_true",
+                    help="serve workload: keep each job's iterations (DIIS, diagonalization) on the device")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     a = ap.parse_args()

@@ -83,7 +83,7 @@ def lib():
             "dsscf_job_solve": [p, p, p], "dsscf_job_energies": [p, p], "dsscf_job_history": [p, p, i],
             "dsscf_syev": [i, p, p, p], "dsscf_jobs_construct": [p, i], "dsscf_jobs_solve": [p, i, i, i, p, p, p],
             "dsscf_server_create": [p, i, d, i, i], "dsscf_server_destroy": [p], "dsscf_server_submit": [p, p, ctypes.c_longlong],
-            "dsscf_server_step": [p, p, p], "dsscf_server_poll": [p, i, p, p, p, p, p, p, p, p],
+            "dsscf_server_step": [p, p, p], "dsscf_server_set_device_steps": [p, i], "dsscf_server_poll": [p, i, p, p, p, p, p, p, p, p],
             "dsscf_server_idle": [p], "dsscf_server_stats": [p, p], "dsscf_one_electron": [p, i, p, p, p, p, p],
             "dsscf_boys_eval": [p, i, i, i, p, p],
         }
@@ -93,6 +93,7 @@ def lib():
             f.restype = i
         for name, args in {"dsscf_batch_create": [p, i], "dsscf_batch_destroy": [p], "dsscf_batch_set": [p, p, i],
                            "dsscf_batch_set_graph": [p, i], "dsscf_batch_jk": [p, p, p, p, p],
+                           "dsscf_batch_set_device": [p, p, i, p, p, p, p], "dsscf_batch_jk_device": [p],
                            "dsscf_batch_time": [p, i, i, p], "dsscf_batch_autotune": [p, p]}.items():
             f = getattr(L, name)
             f.argtypes = args
@@ -149,10 +150,12 @@ class Server:
     dicts (tag, job, status, energy, iterations, t_submit, t_admit, t_done on
     the server clock)."""
 
-    def __init__(self, device=0, budget_bytes=8e9, max_active=64, workers=4):
+    def __init__(self, device=0, budget_bytes=8e9, max_active=64, workers=4, device_steps=False):
         self._h = ctypes.c_void_p()
         _check(lib().dsscf_server_create(ctypes.byref(self._h), device, float(budget_bytes), max_active, workers))
         self._jobs = {}
+        if device_steps:
+            _check(lib().dsscf_server_set_device_steps(self._h, 1))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -376,6 +379,19 @@ class Batch:
         mk = lambda xs: (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
         _check(lib().dsscf_batch_jk(self._h, mk(Pt), mk(Pa), mk(J), mk(K)))
         return J, K
+
+    def set_device(self, contexts, d_Ptot, d_Pa, d_J, d_K, graph=True):
+        """Members whose packed-U densities and J/K live in caller-owned device
+        buffers, given as device addresses (one per member)."""
+        self._members = list(contexts)
+        n = len(contexts)
+        arr = (ctypes.c_void_p * n)(*[c._h.value for c in contexts])
+        mk = lambda xs: (ctypes.c_void_p * n)(*[int(x) for x in xs])
+        _check(lib().dsscf_batch_set_device(self._h, arr, n, mk(d_Ptot), mk(d_Pa), mk(d_J), mk(d_K)))
+        _check(lib().dsscf_batch_set_graph(self._h, 1 if graph else 0))
+
+    def jk_device(self):
+        _check(lib().dsscf_batch_jk_device(self._h))
 
     def time(self, warmup=2, iters=10):
         ms = ctypes.c_double()

@@ -1522,6 +1522,10 @@ struct dsscf_batch {
     Plan plan;
     int use_graph = 1;
     std::vector<int> block_size;
+    // dsscf_batch_set_device: the members' own device buffers
+    bool device_io = false;
+    std::vector<const double*> io_pt, io_pa;
+    std::vector<double*> io_j, io_k;
 
     void release_buffers()
     {
@@ -1566,6 +1570,9 @@ namespace {
 int batch_build_plan(dsscf_batch* b)
 {
     const int n = (int)b->ctxs.size();
+    if (b->device_io)
+        return plan_build(b->plan, b->ctxs.data(), n, b->io_pt.data(), b->io_pa.data(), b->io_j.data(),
+                          b->io_k.data(), b->block_size.data(), b->stream);
     std::vector<const double*> pt(n), pa(n);
     std::vector<double*> j(n), k(n);
     for (int i = 0; i < n; ++i) {
@@ -1579,20 +1586,29 @@ int batch_build_plan(dsscf_batch* b)
 }
 }  // namespace
 
+namespace {
+int batch_check_members(dsscf_batch* b, dsscf_ctx* const* ctxs, int n, const char* who)
+{
+    if (!b || n <= 0 || !ctxs) return fail(DSSCF_ERR_ARG, std::string(who) + ": bad argument");
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i] || !ctxs[i]->prepared) return fail(DSSCF_ERR_STATE, std::string(who) + ": context not prepared");
+        if (ctxs[i]->device != b->device) return fail(DSSCF_ERR_ARG, std::string(who) + ": contexts on another device");
+        if (ctxs[i]->opt.boys_mode != ctxs[0]->opt.boys_mode)
+            return fail(DSSCF_ERR_ARG, std::string(who) + ": mixed Boys modes in one batch");
+        for (int k = 0; k < i; ++k)
+            if (ctxs[k] == ctxs[i]) return fail(DSSCF_ERR_ARG, std::string(who) + ": context listed twice");
+    }
+    return DSSCF_OK;
+}
+}  // namespace
+
 int dsscf_batch_set(dsscf_batch* b, dsscf_ctx* const* ctxs, int n)
 {
-    if (!b || n <= 0 || !ctxs) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: bad argument");
-    for (int i = 0; i < n; ++i) {
-        if (!ctxs[i] || !ctxs[i]->prepared) return fail(DSSCF_ERR_STATE, "dsscf_batch_set: context not prepared");
-        if (ctxs[i]->device != b->device) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: contexts on another device");
-        if (ctxs[i]->opt.boys_mode != ctxs[0]->opt.boys_mode)
-            return fail(DSSCF_ERR_ARG, "dsscf_batch_set: mixed Boys modes in one batch");
-        for (int k = 0; k < i; ++k)
-            if (ctxs[k] == ctxs[i]) return fail(DSSCF_ERR_ARG, "dsscf_batch_set: context listed twice");
-    }
+    if (int rc = batch_check_members(b, ctxs, n, "dsscf_batch_set")) return rc;
     HIP_TRY(hipSetDevice(b->device));
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->plan.reset();  // tables, streams and (if large enough) buffers are reused
+    b->device_io = false;
     b->ctxs.assign(ctxs, ctxs + n);
     b->off.resize(n);
     b->total = 0;
@@ -1615,6 +1631,40 @@ int dsscf_batch_set(dsscf_batch* b, dsscf_ctx* const* ctxs, int n)
     return batch_build_plan(b);
 }
 
+int dsscf_batch_set_device(dsscf_batch* b, dsscf_ctx* const* ctxs, int n, const double* const* d_Ptot,
+                           const double* const* d_Palpha, double* const* d_J, double* const* d_K)
+{
+    if (int rc = batch_check_members(b, ctxs, n, "dsscf_batch_set_device")) return rc;
+    if (!d_Ptot || !d_Palpha || !d_J || !d_K) return fail(DSSCF_ERR_ARG, "dsscf_batch_set_device: null argument");
+    for (int i = 0; i < n; ++i)
+        if (!d_Ptot[i] || !d_Palpha[i] || !d_J[i] || !d_K[i])
+            return fail(DSSCF_ERR_ARG, "dsscf_batch_set_device: null matrix");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->plan.reset();
+    b->device_io = true;
+    b->ctxs.assign(ctxs, ctxs + n);
+    b->io_pt.assign(d_Ptot, d_Ptot + n);
+    b->io_pa.assign(d_Palpha, d_Palpha + n);
+    b->io_j.assign(d_J, d_J + n);
+    b->io_k.assign(d_K, d_K + n);
+    b->off.assign(n, 0);
+    b->total = 0;
+    for (dsscf_ctx* c : b->ctxs) HIP_TRY(hipStreamSynchronize(c->stream));
+    return batch_build_plan(b);
+}
+
+int dsscf_batch_jk_device(dsscf_batch* b)
+{
+    if (!b) return fail(DSSCF_ERR_ARG, "dsscf_batch_jk_device: null batch");
+    if (!b->plan.built || !b->device_io)
+        return fail(DSSCF_ERR_STATE, "dsscf_batch_jk_device: call dsscf_batch_set_device first");
+    HIP_TRY(hipSetDevice(b->device));
+    if (int rc = plan_run(b->plan, b->stream, b->use_graph != 0, nullptr)) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return DSSCF_OK;
+}
+
 int dsscf_batch_set_graph(dsscf_batch* b, int use_graph)
 {
     if (!b) return fail(DSSCF_ERR_ARG, "dsscf_batch_set_graph: null batch");
@@ -1626,7 +1676,7 @@ int dsscf_batch_jk(dsscf_batch* b, const double* const* Ptot, const double* cons
                    double* const* K)
 {
     if (!b || !Ptot || !Palpha || !J || !K) return fail(DSSCF_ERR_ARG, "dsscf_batch_jk: null argument");
-    if (!b->plan.built) return fail(DSSCF_ERR_STATE, "dsscf_batch_jk: call dsscf_batch_set first");
+    if (!b->plan.built || b->device_io) return fail(DSSCF_ERR_STATE, "dsscf_batch_jk: call dsscf_batch_set first");
     HIP_TRY(hipSetDevice(b->device));
     const int n = (int)b->ctxs.size();
     for (int i = 0; i < n; ++i) {

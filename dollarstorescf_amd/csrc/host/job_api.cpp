@@ -360,6 +360,12 @@ int dsscf_server_submit(dsscf_server* s, dsscf_job* j, long long tag)
     return s->s->submit(j, &j->job, tag);
 }
 
+int dsscf_server_set_device_steps(dsscf_server* s, int on)
+{
+    if (!s) return DSSCF_ERR_ARG;
+    return s->s->set_device_steps(on != 0);
+}
+
 int dsscf_server_step(dsscf_server* s, int* n_active, int* n_waiting)
 {
     if (!s) return DSSCF_ERR_ARG;

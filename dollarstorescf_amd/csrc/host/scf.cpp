@@ -399,58 +399,52 @@ bool converged(const Job& job)
            std::fabs(job.total_electronic_energy - job.previous_electronic_energy) < job.opt.energy_threshold;
 }
 
-// solveMOs with the iteration's host part on the device: the densities,
-// J/K, the DIIS history and the eigenproblem stay in HBM; per iteration the
-// host sees <e_new, e_i> (its B matrix and the bordered solve), the orbital
-// energies and tr(Pa G).  Same stopping test, DIIS rules and energy as the
-// host loop below.
-static int solveMOsDevice(Job& job)
+// The iteration's host part on the device: the densities, J/K, the DIIS
+// history and the eigenproblem stay in HBM; per iteration the host sees
+// <e_new, e_i> (its B matrix and the bordered solve), the orbital energies
+// and tr(Pa G).  Same stopping test, DIIS rules and energy as the host loop
+// of solveMOs.
+int beginDeviceSteps(Job& job)
 {
-    const int n = job.n;
-    const unsigned nocc = job.electron_count[0];
     if (job.dstep) {
         scfdev::destroy(job.dstep);
         job.dstep = nullptr;
     }
-    if (int rc = scfdev::create(&job.dstep, job.opt.device, n, (int)nocc, job.opt.diis_size, job.H.data(),
-                                job.Sd.data(), job.X.data(), &job.error))
+    if (int rc = scfdev::create(&job.dstep, job.opt.device, job.n, (int)job.electron_count[0], job.opt.diis_size,
+                                job.H.data(), job.Sd.data(), job.X.data(), &job.error))
         return rc;
-    if (int rc = scfdev::set_density(job.dstep, job.Pa.data(), &job.error)) return rc;
-    std::vector<double> dots(job.opt.diis_size);
-    while (!converged(job)) {
-        if ((int)job.iteration >= job.opt.max_iterations) {
-            job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
-            return DSSCF_ERR_NOCONV;
-        }
-        ++job.iteration;
-        if (int rc = scfdev::fock_build(job.dstep, job.ctx, &job.error)) return rc;
-        double trPG = 0.0;
-        if (job.iteration == 1) {
-            if (int rc = scfdev::assemble(job.dstep, -1, 0, nullptr, nullptr, &job.error)) return rc;
-            if (int rc = scfdev::diagonalize_direct(job.dstep, job.eps, &trPG, &job.error)) return rc;
-        } else {
-            const int slot = job.diis.reserveSlot();
-            const int count = job.diis.count();
-            double rms = 0.0;
-            if (int rc = scfdev::assemble(job.dstep, slot, count, dots.data(), &rms, &job.error)) return rc;
-            job.diis.setRow(slot, dots.data());
-            job.diis.solve();
-            if (int rc = scfdev::extrapolate(job.dstep, job.diis.coef.data(), count, job.eps, &trPG, &job.error))
-                return rc;
-        }
-        double E = 0.0;
-        for (unsigned i = 0; i < nocc; ++i) E += 2.0 * job.eps[i];
-        E -= trPG;
-        job.previous_electronic_energy = job.total_electronic_energy;
-        job.total_electronic_energy = E;
-        job.history.push_back(E);
-        if (job.opt.verbose)
-            std::printf("%9u %22.12f %22.12f %14.6e\n", job.iteration, job.total_electronic_energy,
-                        job.total_electronic_energy - job.previous_electronic_energy, job.diis.rms);
+    return scfdev::set_density(job.dstep, job.Pa.data(), &job.error);
+}
+
+int deviceIteration(Job& job)
+{
+    double trPG = 0.0;
+    if (job.iteration == 1) {
+        if (int rc = scfdev::assemble(job.dstep, -1, 0, nullptr, nullptr, &job.error)) return rc;
+        if (int rc = scfdev::diagonalize_direct(job.dstep, job.eps, &trPG, &job.error)) return rc;
+    } else {
+        std::vector<double> dots(job.opt.diis_size);
+        const int slot = job.diis.reserveSlot();
+        const int count = job.diis.count();
+        double rms = 0.0;
+        if (int rc = scfdev::assemble(job.dstep, slot, count, dots.data(), &rms, &job.error)) return rc;
+        job.diis.setRow(slot, dots.data());
+        job.diis.solve();
+        if (int rc = scfdev::extrapolate(job.dstep, job.diis.coef.data(), count, job.eps, &trPG, &job.error))
+            return rc;
     }
-    // J and K at the final density for the energies, then everything the
-    // host API reads back
-    if (int rc = scfdev::fock_build(job.dstep, job.ctx, &job.error)) return rc;
+    double E = 0.0;
+    for (unsigned i = 0; i < job.electron_count[0]; ++i) E += 2.0 * job.eps[i];
+    E -= trPG;
+    job.previous_electronic_energy = job.total_electronic_energy;
+    job.total_electronic_energy = E;
+    job.history.push_back(E);
+    return DSSCF_OK;
+}
+
+int finishDeviceSteps(Job& job)
+{
+    const int n = job.n;
     job.C.resize((size_t)n * n);
     if (int rc = scfdev::download(job.dstep, job.Pa.data(), job.Ptot.data(), job.J.data(), job.K.data(), nullptr,
                                   job.C.data(), &job.error))
@@ -458,6 +452,27 @@ static int solveMOsDevice(Job& job)
     for (size_t i = 0; i < job.G.size(); ++i) job.G[i] = job.J[i] - job.K[i];
     job.jk_valid = true;
     return calculateEnergies(job);
+}
+
+static int solveMOsDevice(Job& job)
+{
+    if (int rc = beginDeviceSteps(job)) return rc;
+    while (!converged(job)) {
+        if ((int)job.iteration >= job.opt.max_iterations) {
+            job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
+            return DSSCF_ERR_NOCONV;
+        }
+        ++job.iteration;
+        if (int rc = scfdev::fock_build(job.dstep, job.ctx, &job.error)) return rc;
+        if (int rc = deviceIteration(job)) return rc;
+        if (job.opt.verbose)
+            std::printf("%9u %22.12f %22.12f %14.6e\n", job.iteration, job.total_electronic_energy,
+                        job.total_electronic_energy - job.previous_electronic_energy, job.diis.rms);
+    }
+    // J and K at the final density for the energies, then everything the
+    // host API reads back
+    if (int rc = scfdev::fock_build(job.dstep, job.ctx, &job.error)) return rc;
+    return finishDeviceSteps(job);
 }
 
 int solveMOs(Job& job)

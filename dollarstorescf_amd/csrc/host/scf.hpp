@@ -116,6 +116,14 @@ int fockBuild(Job& job);  // J[Ptot], K[Pa] on the device
 int runSCFIteration(Job& job);  // fockBuild + scfStep
 int scfStep(Job& job);          // G, DIIS, F' = X F X, diagonalization, density, E_elec
 bool converged(const Job& job);  // the stopping test of solveMOs (SCFSolver.cpp:174-177)
+// The same iteration with its host part on the device (job.dstep):
+// beginDeviceSteps creates the step state at the job's density;
+// deviceIteration is scfStep with J/K already in the step's buffers;
+// finishDeviceSteps, with J/K at the final density there, downloads the
+// densities, J, K and C and computes the energies.
+int beginDeviceSteps(Job& job);
+int deviceIteration(Job& job);
+int finishDeviceSteps(Job& job);
 int solveMOs(Job& job);
 int calculateEnergies(Job& job);
 // scf::getEnergyGradient (Gradient.cpp:14-133), restricted: dE/dR of a solved

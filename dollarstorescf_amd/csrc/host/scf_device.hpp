@@ -47,5 +47,11 @@ int download(Step* s, double* Pa, double* Ptot, double* J, double* K, double* G,
 
 // Device bytes held by the step buffers.
 long long device_bytes(const Step* s);
+// The same figure before a step exists (n AOs, DIIS history of kmax).
+long long device_bytes_for(int n, int kmax);
+
+// The packed-U Fock-build buffers of the step (densities in, J and K out),
+// for fused builds that read and write them in place.
+void buffers(Step* s, const double** Pt, const double** Pa, double** J, double** K);
 
 }  // namespace scfdev

@@ -12,6 +12,10 @@
 //     diagonalization, density; or, after convergence, the energies of
 //     Energy.cpp:21-50), and returns: the next round fuses whichever jobs are
 //     ready then, so a large job's host step holds back no one else.
+// With device steps (set_device_steps) a job's densities, J/K, DIIS history
+// and eigenproblem stay in HBM: the fused build reads and writes the job's
+// step buffers in place, and the worker runs the step on the job's own
+// stream (rocBLAS, rocSOLVER), with only DIIS bookkeeping on the host.
 // A finished job's worker frees its device memory and queues it for poll().
 // The fused set changes from round to round, so launches are made directly
 // (no graph); the batch keeps its buffers and tables between rounds.
@@ -39,13 +43,30 @@ Server::~Server()
     for (auto& t : workers_) t.join();
     if (batch_) dsscf_batch_destroy(batch_);
     for (Req* r : active_) {
-        if (r->job->ctx) {
-            dsscf_ctx_destroy(r->job->ctx);
-            r->job->ctx = nullptr;
-        }
+        free_device(*r->job);
         delete r;
     }
     for (Req* r : waiting_) delete r;
+}
+
+void Server::free_device(Job& job)
+{
+    if (job.dstep) {
+        scfdev::destroy(job.dstep);
+        job.dstep = nullptr;
+    }
+    if (job.ctx) {
+        dsscf_ctx_destroy(job.ctx);
+        job.ctx = nullptr;
+    }
+}
+
+int Server::set_device_steps(bool on)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    if (submitted_) return DSSCF_ERR_STATE;
+    device_steps_ = on;
+    return DSSCF_OK;
 }
 
 double Server::clock() const
@@ -98,8 +119,9 @@ double Server::estimate_bytes(const Job& job) const
         }
     const double pairs = shells * (shells + 1.0) / 2.0;
     const double n = job.n;
+    const double step = device_steps_ ? (double)scfdev::device_bytes_for(job.n, job.opt.diis_size) : 0.0;
     return 6.0 * pairs * (pairs + 1.0) / 2.0 + 8.0 * n * n * 34.0 + 32.0 * n * (n + 1.0) / 2.0 + 76.0 * pairs +
-           64.0 * prims * prims / 2.0 + 256.0 * 1024.0;
+           64.0 * prims * prims / 2.0 + 256.0 * 1024.0 + step;
 }
 
 int Server::submit(void* handle, Job* job, long long tag)
@@ -108,6 +130,8 @@ int Server::submit(void* handle, Job* job, long long tag)
     Req* r = new Req{handle, job, tag, HOST, 0, 0.0, 0.0, clock(), 0.0, 0.0};
     {
         std::lock_guard<std::mutex> lk(mu_);
+        submitted_ = true;
+        job->opt.device_step = device_steps_ ? 1 : 0;
         waiting_.push_back(r);
     }
     enqueue([this, r] {
@@ -128,10 +152,7 @@ int Server::submit(void* handle, Job* job, long long tag)
 // Called with mu_ held: hand r back (frees its device memory).
 void Server::retire_locked(Req* r, int status)
 {
-    if (r->job->ctx) {
-        dsscf_ctx_destroy(r->job->ctx);
-        r->job->ctx = nullptr;
-    }
+    free_device(*r->job);
     r->st = RETIRED;
     batch_dirty_ = true;  // the fused plan may point at r's buffers: rebuild before the next launch
     r->status = status;
@@ -146,10 +167,7 @@ void Server::retire_locked(Req* r, int status)
 // any round since its last build completed).
 void Server::retire(Req* r, int status)
 {
-    if (r->job->ctx) {
-        dsscf_ctx_destroy(r->job->ctx);
-        r->job->ctx = nullptr;
-    }
+    free_device(*r->job);
     std::lock_guard<std::mutex> lk(mu_);
     used_ -= r->est_bytes;
     active_.erase(std::find(active_.begin(), active_.end(), r));
@@ -163,14 +181,15 @@ void Server::host_step(Req* r, bool final)
     Job& job = *r->job;
     job.jk_valid = true;
     int status = -1;  // -1: keep iterating
+    const bool dev = job.dstep != nullptr;
     if (final) {
-        status = calculateEnergies(job);
+        status = dev ? finishDeviceSteps(job) : calculateEnergies(job);
     } else if ((int)job.iteration >= job.opt.max_iterations) {
         job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
         status = DSSCF_ERR_NOCONV;
     } else {
         ++job.iteration;
-        const int rc = scfStep(job);
+        const int rc = dev ? deviceIteration(job) : scfStep(job);
         if (rc) status = rc;
     }
     const double dt = clock() - t_start;
@@ -214,17 +233,20 @@ int Server::step(int* n_active, int* n_waiting)
             r->t_admit = clock();
             active_.push_back(r);
             tasks_.push_back([this, r] {  // mu_ is held: queue directly, wake a worker below
-                const int rc = constructDevice(*r->job);
+                Job& j = *r->job;
+                int rc = constructDevice(j);
+                if (!rc) {
+                    useCoreGuess(j);
+                    if (j.opt.device_step) rc = beginDeviceSteps(j);
+                }
                 std::lock_guard<std::mutex> lk2(mu_);
                 if (!rc) {
-                    useCoreGuess(*r->job);
-                    Job& j = *r->job;
                     j.iteration = 0;
                     j.total_electronic_energy = j.previous_electronic_energy = 0.0;
                     j.history.clear();
                     j.diis.resize(j.opt.diis_size);
                     // the budget holds the measured footprint from here on
-                    r->bytes = (double)dsscf_ctx_device_bytes(j.ctx);
+                    r->bytes = (double)dsscf_ctx_device_bytes(j.ctx) + (double)scfdev::device_bytes(j.dstep);
                     used_ += r->bytes - r->est_bytes;
                     r->est_bytes = r->bytes;
                     r->st = ITERATING;
@@ -261,22 +283,37 @@ int Server::step(int* n_active, int* n_waiting)
             dsscf_batch_set_graph(batch_, 0);
         }
         const double ta = clock();
+        const bool dev = device_steps_;
         if (dirty || members != members_) {
             std::vector<dsscf_ctx*> ctxs;
             for (Req* r : members) ctxs.push_back(r->job->ctx);
-            if ((rc = dsscf_batch_set(batch_, ctxs.data(), (int)ctxs.size()))) return rc;
+            if (dev) {
+                // the members' step buffers, in place
+                const int m = (int)members.size();
+                std::vector<const double*> pt(m), pa(m);
+                std::vector<double*> jj(m), kk(m);
+                for (int i = 0; i < m; ++i) scfdev::buffers(members[i]->job->dstep, &pt[i], &pa[i], &jj[i], &kk[i]);
+                rc = dsscf_batch_set_device(batch_, ctxs.data(), m, pt.data(), pa.data(), jj.data(), kk.data());
+            } else {
+                rc = dsscf_batch_set(batch_, ctxs.data(), (int)ctxs.size());
+            }
+            if (rc) return rc;
             members_ = members;
         }
         const double tb = clock();
-        std::vector<const double*> pt, pa;
-        std::vector<double*> jj, kk;
-        for (Req* r : members) {
-            pt.push_back(r->job->Ptot.data());
-            pa.push_back(r->job->Pa.data());
-            jj.push_back(r->job->J.data());
-            kk.push_back(r->job->K.data());
+        if (dev) {
+            if ((rc = dsscf_batch_jk_device(batch_))) return rc;
+        } else {
+            std::vector<const double*> pt, pa;
+            std::vector<double*> jj, kk;
+            for (Req* r : members) {
+                pt.push_back(r->job->Ptot.data());
+                pa.push_back(r->job->Pa.data());
+                jj.push_back(r->job->J.data());
+                kk.push_back(r->job->K.data());
+            }
+            if ((rc = dsscf_batch_jk(batch_, pt.data(), pa.data(), jj.data(), kk.data()))) return rc;
         }
-        if ((rc = dsscf_batch_jk(batch_, pt.data(), pa.data(), jj.data(), kk.data()))) return rc;
         const double tc = clock();
         {
             std::lock_guard<std::mutex> lk(mu_);

@@ -35,6 +35,11 @@ public:
     // constructs it in the background and admits it when its device
     // footprint fits; handle and tag come back from poll().
     int submit(void* handle, Job* job, long long tag);
+    // Device-resident iterations: every job submitted afterwards keeps its
+    // DIIS history, Fock matrix and eigenproblem in HBM as well (the device
+    // SCF step), and the fused builds read and write its device buffers in
+    // place.  Only before the first submit.
+    int set_device_steps(bool on);
     // One round: admit ready jobs, one fused Fock build over the jobs whose
     // next density is ready, then hand each of them to a worker for its host
     // step (the next round does not wait for those).
@@ -50,7 +55,7 @@ public:
 
 private:
     // ITERATING / FINAL: waiting for a Fock build (for the next iteration /
-    // for the energies); STEPPING: its host step runs on a worker.
+    // for the energies); STEPPING: its host (or device) step runs on a worker.
     enum State { HOST, READY, PREPARING, ITERATING, FINAL, STEPPING, FAILED, RETIRED };
     struct Req {
         void* handle;
@@ -68,7 +73,11 @@ private:
     void host_step(Req* r, bool final);
     double estimate_bytes(const Job& job) const;
 
+    void free_device(Job& job);  // context and device step state
+
     int device_;
+    bool device_steps_ = false;
+    bool submitted_ = false;
     double budget_, used_ = 0.0;
     int max_active_;
     std::vector<std::thread> workers_;

@@ -374,10 +374,24 @@ int download(Step* s, double* Pa, double* Pt, double* J, double* K, double* G, d
     return DSSCF_OK;
 }
 
+long long device_bytes_for(int n, int kmax)
+{
+    const size_t np = (size_t)n * (n + 1) / 2, nn = (size_t)n * n;
+    return (long long)(sizeof(double) * (6 * np + (8 + 2 * (size_t)kmax) * nn + 3 * (size_t)n + kmax));
+}
+
 long long device_bytes(const Step* s)
 {
     if (!s) return 0;
-    return (long long)(sizeof(double) * (6 * s->np + (8 + 2 * (size_t)s->kmax) * s->nn + 3 * (size_t)s->n + s->kmax));
+    return device_bytes_for(s->n, s->kmax);
+}
+
+void buffers(Step* s, const double** Pt, const double** Pa, double** J, double** K)
+{
+    *Pt = s->Pt;
+    *Pa = s->Pa;
+    *J = s->J;
+    *K = s->K;
 }
 
 }  // namespace scfdev

@@ -204,6 +204,15 @@ int dsscf_batch_set_graph(dsscf_batch* batch, int use_graph);
 /* J[Ptot_i], K[Palpha_i] for every member i (packed-U host arrays). */
 int dsscf_batch_jk(dsscf_batch* batch, const double* const* Ptot, const double* const* Palpha, double* const* J,
                    double* const* K);
+/* Member contexts whose densities and J/K stay in caller-owned device
+ * buffers (packed-U, one set per member, alive until the next set call):
+ * the fused plan reads d_Ptot[i], d_Palpha[i] and writes d_J[i], d_K[i]
+ * directly, with no host copies.  dsscf_batch_jk then refuses; call
+ * dsscf_batch_jk_device, which returns when the build has finished.  The
+ * caller must have finished its own writes to the densities. */
+int dsscf_batch_set_device(dsscf_batch* batch, dsscf_ctx* const* ctxs, int n, const double* const* d_Ptot,
+                           const double* const* d_Palpha, double* const* d_J, double* const* d_K);
+int dsscf_batch_jk_device(dsscf_batch* batch);
 /* Average device time of `iters` fused builds on the resident densities. */
 int dsscf_batch_time(dsscf_batch* batch, int warmup, int iters, double* ms_per_build);
 /* Choose threads per block for each thread-per-quartet class by timing

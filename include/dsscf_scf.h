@@ -131,6 +131,13 @@ typedef struct dsscf_server dsscf_server;
 int dsscf_server_create(dsscf_server** server, int device, double budget_bytes, int max_active, int workers);
 int dsscf_server_destroy(dsscf_server* server);
 int dsscf_server_submit(dsscf_server* server, dsscf_job* job, long long tag);
+/* on = 1: every job submitted afterwards runs its iterations with the device
+ * SCF step (as dsscf_job_set_step(job, 1)): its densities, J/K, Fock
+ * matrix, DIIS history and eigenproblem stay in HBM from admission to
+ * retirement, and the fused builds read and write them in place
+ * (dsscf_batch_set_device).  Only before the first submit
+ * (DSSCF_ERR_STATE after it). */
+int dsscf_server_set_device_steps(dsscf_server* server, int on);
 /* One round; n_active / n_waiting (may be null): jobs on the device and
  * jobs queued or constructing after it. */
 int dsscf_server_step(dsscf_server* server, int* n_active, int* n_waiting);

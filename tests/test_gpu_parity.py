@@ -156,6 +156,48 @@ def test_batch_matches_single_context(graph):
         assert np.max(np.abs(Jb - Js)) < 1e-11
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_device_buffers_match_host_io(graph):
+    """dsscf_batch_set_device / dsscf_batch_jk_device: the fused plan reads
+    the members' densities from caller-owned device buffers and writes J/K
+    there in place; equal (1e-11) to each context's own Fock build, over
+    several replays with the densities changed between them on the device."""
+    import torch
+
+    specs = [("STO-3G", G.water()), ("6-31G", G.water_box(2, seed=3)), ("def2-SVP", G.alkane(2))]
+    jobs, dens = [], []
+    rng = np.random.default_rng(11)
+    for basis, (Z, xyz) in specs:
+        j = _job(basis, Z, xyz, ds.BOYS_ACCURATE, schwarz=1e-12)
+        C = rng.standard_normal((j.n_ao, 4))
+        jobs.append(j)
+        dens.append(O.pack(C @ C.T * 0.1))
+    dev = torch.device("cuda", 0)
+    d_pa = [torch.tensor(p, dtype=torch.float64, device=dev) for p in dens]
+    d_pt = [2 * x for x in d_pa]
+    d_j = [torch.zeros_like(x) for x in d_pa]
+    d_k = [torch.zeros_like(x) for x in d_pa]
+    torch.cuda.synchronize()
+    b = ds.Batch(0)
+    b.set_device([j.context for j in jobs], [x.data_ptr() for x in d_pt], [x.data_ptr() for x in d_pa],
+                 [x.data_ptr() for x in d_j], [x.data_ptr() for x in d_k], graph=graph)
+    with pytest.raises(ds.DsscfError):
+        b.jk([2 * p for p in dens], dens)  # host I/O refused in device mode
+    for rep in range(3):
+        scale = 1.0 + 0.5 * rep
+        for x, y, p in zip(d_pa, d_pt, dens):
+            x.copy_(torch.tensor(scale * p, dtype=torch.float64, device=dev))
+            y.copy_(2 * x)
+        torch.cuda.synchronize()
+        b.jk_device()
+        for j, p, Jd, Kd in zip(jobs, dens, d_j, d_k):
+            j.set_density(scale * p)
+            Js, Ks = j.fock_build()
+            assert np.max(np.abs(Jd.cpu().numpy() - Js)) < 1e-11
+            assert np.max(np.abs(Kd.cpu().numpy() - Ks)) < 1e-11
+    b.close()
+
+
 def test_jk_primitive_screening_accurate():
     """Default accurate mode screens primitive quartets (bound 1e-17): J/K stay
     within 1e-11 (relative to max|J|) of the unscreened accurate-mode oracle."""
@@ -267,23 +309,26 @@ def _serve_job(basis, Z, xyz, fp32=0.0):
     return job.set_basis(basis).set_molecule(Z, xyz)
 
 
-@pytest.mark.parametrize("fp32", [0.0, 1e-5])
-def test_server_result_same_alone_and_in_mix(fp32):
+@pytest.mark.parametrize("fp32,device_steps", [(0.0, False), (1e-5, False), (0.0, True)])
+def test_server_result_same_alone_and_in_mix(fp32, device_steps):
     """A job served alone and the same job served inside a mix of other
     molecules and bases (arriving while others iterate, with a footprint
     budget that forces queueing) converge to the same energy, within 1e-9 Eh;
     every job in the mix equals its own Job.solve energy within 1e-9 Eh
-    (FP64) and, with FP32 below Schwarz 1e-5, within 1e-6 Eh of it."""
+    (FP64) and, with FP32 below Schwarz 1e-5, within 1e-6 Eh of it.  With
+    device steps the densities, J/K, Fock matrices and DIIS histories stay
+    in HBM and the fused builds read and write them in place; the reference
+    energies are still the host-step Job.solve ones."""
     probe = ("6-31G", G.water_box(3, seed=4))
     others = [("STO-3G", G.water()), ("def2-SVP", G.water()), ("6-31G", G.alkane(3)),
               ("STO-3G", G.water_box(5, seed=2)), ("6-31GP", G.water_box(2, seed=7)), ("def2-SVP", G.alkane(1))]
-    srv = ds.Server(0, budget_bytes=8e9, max_active=8, workers=3)
+    srv = ds.Server(0, budget_bytes=8e9, max_active=8, workers=3, device_steps=device_steps)
     srv.submit(_serve_job(probe[0], *probe[1], fp32=fp32), 0)
     alone = srv.run_until_idle()
     assert len(alone) == 1 and alone[0]["status"] == 0
     srv.close()
     # the mix: a small budget and four slots force admission in several waves
-    srv = ds.Server(0, budget_bytes=2.5e8, max_active=4, workers=3)
+    srv = ds.Server(0, budget_bytes=2.5e8, max_active=4, workers=3, device_steps=device_steps)
     specs = others[:3] + [probe] + others[3:]
     srv.submit(_serve_job(specs[0][0], *specs[0][1], fp32=fp32), 0)
     srv.submit(_serve_job(specs[1][0], *specs[1][1], fp32=fp32), 1)

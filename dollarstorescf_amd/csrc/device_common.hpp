@@ -22,7 +22,10 @@
 // Boys-function modes: the reference's truncated series, or the accurate
 // table/asymptotic evaluation.  BOYS_ACCURATE_LDS is the accurate mode
 // evaluated from a block's LDS copy of the interval table below (device
-// internal; the results agree with BOYS_ACCURATE to ~1e-16 relative).
+// internal; the results agree with BOYS_ACCURATE to ~1e-16 relative).  In
+// prim_setup it also means that the bra primitive record holds P - A in
+// place of P (the J/K kernels stage their bra's records that way, which
+// takes the P - A subtraction out of the primitive-quartet loop).
 enum { BOYS_REFERENCE = 0, BOYS_ACCURATE = 1, BOYS_ACCURATE_LDS = 2 };
 
 // Interval table (after the 801-row table in the same buffer): for each
@@ -76,6 +79,28 @@ struct JKSeg {
     int N;
     int blk0;
     int nslice;                // copies of Jh/Kh (power of two, <= kJKSlicesMax)
+    // J_cd slot records (thread-per-quartet classes, or null: atomics into
+    // Jh): quartet t of work unit u is slot us[u] + t, and stores its ket
+    // block's nc * nd J_cd terms at js + jd[slot].  The records are laid out
+    // by ket AO block, so k_jcd sums each block's records from one
+    // contiguous range afterwards.
+    double* js;
+    const int* us;
+    const int* jd;
+};
+
+// One context's J_cd reduction (k_jcd): one row per ket AO block; ri =
+// (c0, d0, nc, nd), rr = (first double in js, records); a record is nc * nd
+// doubles.
+struct JcdSeg {
+    const int2* rr;
+    const int4* ri;
+    const double* js;
+    double* Jh;  // copy 0 of the half-accumulated J
+    int N;
+    int nrow;
+    int blk0;
+    int pad;
 };
 
 // One context's matrices for the fused prepare/fold kernels; row0 is its
@@ -124,6 +149,19 @@ template <typename T>
 __device__ __forceinline__ GPtr<T> gbl(const T* p)
 {
     return (GPtr<T>)p;
+}
+// Writable global-memory view (stores compile to global_store).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using GOut = __attribute__((address_space(1))) T*;
+#else
+template <typename T>
+using GOut = T*;
+#endif
+template <typename T>
+__device__ __forceinline__ GOut<T> gout(T* p)
+{
+    return (GOut<T>)p;
 }
 // Whole record (a multiple of 8 bytes) through global loads.
 template <typename T>
@@ -403,7 +441,11 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
     const double p = P.p, q = Q.p;
     const double rs = rsqrt_nr(p + q);
     const double inv_pq = rs * rs;
-    const double PQx = P.P[0] - Q.P[0], PQy = P.P[1] - Q.P[1], PQz = P.P[2] - Q.P[2];
+    constexpr bool kPA = BM == BOYS_ACCURATE_LDS;  // P.P holds P - A
+    // P - Q = (P - A) - (Q - A): Q - A is invariant in the bra-primitive loop
+    const double PQx = kPA ? P.P[0] - (Q.P[0] - bra.A[0]) : P.P[0] - Q.P[0];
+    const double PQy = kPA ? P.P[1] - (Q.P[1] - bra.A[1]) : P.P[1] - Q.P[1];
+    const double PQz = kPA ? P.P[2] - (Q.P[2] - bra.A[2]) : P.P[2] - Q.P[2];
     const double rho = p * q * inv_pq;
     const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
     const double pref = P.Kp * Q.Kp * rs;
@@ -412,9 +454,9 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
 #pragma unroll
     for (int m = 0; m <= L; ++m) Fm[m] = (R)(F[m] * pref);
     const double qf = q * inv_pq, pf = p * inv_pq;
-    g.PA[0] = (R)(P.P[0] - bra.A[0]);
-    g.PA[1] = (R)(P.P[1] - bra.A[1]);
-    g.PA[2] = (R)(P.P[2] - bra.A[2]);
+    g.PA[0] = (R)(kPA ? P.P[0] : P.P[0] - bra.A[0]);
+    g.PA[1] = (R)(kPA ? P.P[1] : P.P[1] - bra.A[1]);
+    g.PA[2] = (R)(kPA ? P.P[2] : P.P[2] - bra.A[2]);
     g.QC[0] = (R)(Q.P[0] - ket.A[0]);
     g.QC[1] = (R)(Q.P[1] - ket.A[1]);
     g.QC[2] = (R)(Q.P[2] - ket.A[2]);

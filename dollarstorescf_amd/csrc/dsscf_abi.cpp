@@ -132,6 +132,13 @@ struct Plan {
     std::vector<hipEvent_t> slot_ev;  // start/end of each slot's kernel (timed builds)
 
     size_t cap_segs = 0, cap_blk0 = 0, cap_mat = 0;  // device table capacities (grow only)
+    // J_cd slot reduction (k_jcd) over the contexts that have slot records
+    std::vector<JcdSeg> h_jcd;
+    std::vector<int> h_jblk0;
+    JcdSeg* d_jcd = nullptr;
+    int* d_jblk0 = nullptr;
+    size_t cap_jcd = 0, cap_jblk0 = 0;
+    int jcd_blocks = 0;
 
     // Forget the launch (graph, built flag); keep tables and streams for the next build.
     void reset()
@@ -159,7 +166,9 @@ struct Plan {
         dev_free(d_segs);
         dev_free(d_blk0);
         dev_free(d_mat);
-        cap_segs = cap_blk0 = cap_mat = 0;
+        dev_free(d_jcd);
+        dev_free(d_jblk0);
+        cap_segs = cap_blk0 = cap_mat = cap_jcd = cap_jblk0 = 0;
     }
 };
 // Fisher-Yates shuffle driven by splitmix64 (deterministic for a seed).
@@ -219,6 +228,15 @@ struct dsscf_ctx {
     double *d_Dt = nullptr, *d_Da = nullptr, *d_Jh = nullptr, *d_Kh = nullptr;
     double *d_Pt_pk = nullptr, *d_Pa_pk = nullptr, *d_J_pk = nullptr, *d_K_pk = nullptr;
     double* d_xform = nullptr;                   // AO transform (one-electron kernel)
+    // J_cd slot records of the thread-per-quartet classes (null: atomics):
+    // per unit its first double in d_js; per ket AO block (row) its records
+    double* d_js = nullptr;
+    int* d_us = nullptr;
+    int* d_jd = nullptr;
+    int2* d_jrr = nullptr;
+    int4* d_jri = nullptr;
+    int jcd_rows = 0;
+    long long jcd_bytes = 0;
     std::vector<int> pc_start, pc_count;         // pair classes in d_plist
 
     std::vector<int> block_size;  // per class, thread-per-quartet kernels (autotuned)
@@ -244,6 +262,13 @@ struct dsscf_ctx {
         dev_free(d_J_pk);
         dev_free(d_K_pk);
         dev_free(d_xform);
+        dev_free(d_js);
+        dev_free(d_us);
+        dev_free(d_jd);
+        dev_free(d_jrr);
+        dev_free(d_jri);
+        jcd_rows = 0;
+        jcd_bytes = 0;
         prepared = false;
     }
 };
@@ -318,7 +343,8 @@ int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: FP32 quartets need the accurate Boys mode");
     const bool changed = o->boys_mode != c->opt.boys_mode || o->schwarz_threshold != c->opt.schwarz_threshold ||
                          o->fp32_threshold != c->opt.fp32_threshold || o->prim_threshold != c->opt.prim_threshold ||
-                         o->work_item_flops != c->opt.work_item_flops || o->group_prims != c->opt.group_prims;
+                         o->work_item_flops != c->opt.work_item_flops || o->group_prims != c->opt.group_prims ||
+                         o->jcd_max_bytes != c->opt.jcd_max_bytes;
     c->opt = *o;
     if (changed) c->prepared = false;
     return DSSCF_OK;
@@ -811,6 +837,83 @@ int dsscf_prepare(dsscf_ctx* c)
         HIP_TRY(hipMemcpyAsync(c->d_units, units.data(), sizeof(int4) * units.size(), hipMemcpyHostToDevice, c->stream));
     if (!kets.empty())
         HIP_TRY(hipMemcpyAsync(c->d_kets, kets.data(), sizeof(unsigned) * kets.size(), hipMemcpyHostToDevice, c->stream));
+    {
+        // J_cd slot records: a context whose records fit jcd_max_bytes stores
+        // each quartet's J_cd terms (k_jk) into a record laid out by ket AO
+        // block and sums each block's records per element (k_jcd); a larger
+        // one keeps the atomic adds.
+        dev_free(c->d_js);
+        dev_free(c->d_us);
+        dev_free(c->d_jd);
+        dev_free(c->d_jrr);
+        dev_free(c->d_jri);
+        c->jcd_rows = 0;
+        c->jcd_bytes = 0;
+        std::vector<int> us(units.size(), 0);
+        std::vector<int> cls_ncnd(ncls, 0);
+        long long nd = 0, ne = 0;
+        for (int cls = 0; cls < ncls; ++cls) {
+            if (dsscf_class_is_wave(cls)) continue;
+            int l4[4];
+            dk_class_l(cls, l4);
+            cls_ncnd[cls] = (2 * l4[2] + 1) * (2 * l4[3] + 1);
+            for (long long ui = c->u_off[cls]; ui < c->u_off32[cls] + c->u_cnt32[cls]; ++ui) {
+                us[ui] = (int)std::min<long long>(ne, 0x7fffffff);
+                nd += (long long)units[ui].z * cls_ncnd[cls];
+                ne += units[ui].z;
+            }
+        }
+        const double bytes = 8.0 * (double)nd + 4.0 * (double)ne + 4.0 * (double)units.size();
+        const double cap = c->opt.jcd_max_bytes == 0.0 ? DSSCF_JCD_MAX_BYTES : c->opt.jcd_max_bytes;
+        if (ne > 0 && bytes <= cap && nd < 0x7fffffffLL) {
+            const int N = c->n_ao;
+            std::vector<int> row_of((size_t)N * N, -1), slot_row((size_t)ne), jd((size_t)ne);
+            std::vector<int4> ri;
+            std::vector<int2> rr;
+            size_t slot = 0;
+            for (int cls = 0; cls < ncls; ++cls) {  // rows and their record counts
+                if (!cls_ncnd[cls]) continue;
+                int l4[4];
+                dk_class_l(cls, l4);
+                for (long long ui = c->u_off[cls]; ui < c->u_off32[cls] + c->u_cnt32[cls]; ++ui) {
+                    const int4 u = units[ui];
+                    for (int t = 0; t < u.z; ++t, ++slot) {
+                        const ShellPair& kp = work[kets[(size_t)u.y + t]];
+                        int& r = row_of[(size_t)kp.ao_a * N + kp.ao_b];
+                        if (r < 0) {
+                            r = (int)ri.size();
+                            ri.push_back(make_int4(kp.ao_a, kp.ao_b, 2 * l4[2] + 1, 2 * l4[3] + 1));
+                            rr.push_back(make_int2(0, 0));
+                        }
+                        ++rr[r].y;
+                        slot_row[slot] = r;
+                    }
+                }
+            }
+            int first = 0;
+            for (size_t r = 0; r < rr.size(); ++r) {
+                rr[r].x = first;
+                first += rr[r].y * ri[r].z * ri[r].w;
+            }
+            std::vector<int> fill(rr.size(), 0);
+            for (size_t q = 0; q < (size_t)ne; ++q) {
+                const int r = slot_row[q];
+                jd[q] = rr[r].x + fill[r]++ * ri[r].z * ri[r].w;
+            }
+            c->jcd_rows = (int)ri.size();
+            if (int rc = dev_alloc(&c->d_js, (size_t)nd)) return rc;
+            if (int rc = dev_alloc(&c->d_us, us.size())) return rc;
+            if (int rc = dev_alloc(&c->d_jd, jd.size())) return rc;
+            if (int rc = dev_alloc(&c->d_jrr, rr.size())) return rc;
+            if (int rc = dev_alloc(&c->d_jri, ri.size())) return rc;
+            HIP_TRY(hipMemcpyAsync(c->d_us, us.data(), sizeof(int) * us.size(), hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(c->d_jd, jd.data(), sizeof(int) * jd.size(), hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(c->d_jrr, rr.data(), sizeof(int2) * rr.size(), hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(c->d_jri, ri.data(), sizeof(int4) * ri.size(), hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+            c->jcd_bytes = (long long)bytes + (long long)((sizeof(int2) + sizeof(int4)) * rr.size());
+        }
+    }
     if (int rc = dev_alloc(&c->d_q, all.size())) return rc;
     if (!all.empty())
         HIP_TRY(hipMemcpyAsync(c->d_q, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice, c->stream));
@@ -819,7 +922,7 @@ int dsscf_prepare(dsscf_ctx* c)
     c->device_bytes = (long long)(sizeof(ShellPair) * work.size() + sizeof(PrimPair) * c->prims.size() +
                                   (sizeof(double) + sizeof(int)) * npair + sizeof(double) * nn * (2 + 2 * c->n_slice) +
                                   sizeof(double) * np * 4 + sizeof(int4) * units.size() +
-                                  sizeof(unsigned) * kets.size() + sizeof(uint2) * all.size());
+                                  sizeof(unsigned) * kets.size() + sizeof(uint2) * all.size() + c->jcd_bytes);
     c->prepared = true;
     return DSSCF_OK;
 }
@@ -878,6 +981,11 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
             sg.Jh = c->d_Jh;
             sg.Kh = c->d_Kh;
             sg.nslice = c->n_slice;
+            if (!wave && c->d_js) {
+                sg.js = c->d_js;
+                sg.us = c->d_us + (prec32 ? c->u_off32[cls] : c->u_off[cls]);
+                sg.jd = c->d_jd;
+            }
             sg.n = (int)cnt;
             sg.N = c->n_ao;
             sg.blk0 = (int)blocks;
@@ -939,6 +1047,24 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     p.nmat = n;
     p.total_rows = row;
     p.maxN = maxN;
+    p.h_jcd.clear();
+    p.h_jblk0.clear();
+    p.jcd_blocks = 0;
+    for (int k = 0; k < n; ++k) {
+        dsscf_ctx* c = ctxs[k];
+        if (!c->d_js || c->jcd_rows == 0) continue;
+        JcdSeg js{};
+        js.rr = c->d_jrr;
+        js.ri = c->d_jri;
+        js.js = c->d_js;
+        js.Jh = c->d_Jh;
+        js.N = c->n_ao;
+        js.nrow = c->jcd_rows;
+        js.blk0 = p.jcd_blocks;
+        p.jcd_blocks += c->jcd_rows;
+        p.h_jcd.push_back(js);
+        p.h_jblk0.push_back(js.blk0);
+    }
     auto grow = [](auto** ptr, size_t& cap, size_t need) -> int {
         if (need <= cap && *ptr) return DSSCF_OK;
         dev_free(*ptr);
@@ -951,6 +1077,13 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     if (int rc = grow(&p.d_segs, p.cap_segs, p.h_segs.size())) return rc;
     if (int rc = grow(&p.d_blk0, p.cap_blk0, p.h_blk0.size())) return rc;
     if (int rc = grow(&p.d_mat, p.cap_mat, p.h_mat.size())) return rc;
+    if (!p.h_jcd.empty()) {
+        if (int rc = grow(&p.d_jcd, p.cap_jcd, p.h_jcd.size())) return rc;
+        if (int rc = grow(&p.d_jblk0, p.cap_jblk0, p.h_jblk0.size())) return rc;
+        HIP_TRY(hipMemcpyAsync(p.d_jcd, p.h_jcd.data(), sizeof(JcdSeg) * p.h_jcd.size(), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(p.d_jblk0, p.h_jblk0.data(), sizeof(int) * p.h_jblk0.size(), hipMemcpyHostToDevice,
+                               stream));
+    }
     if (!p.h_segs.empty()) {
         HIP_TRY(hipMemcpyAsync(p.d_segs, p.h_segs.data(), sizeof(JKSeg) * p.h_segs.size(), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(p.d_blk0, p.h_blk0.data(), sizeof(int) * p.h_blk0.size(), hipMemcpyHostToDevice, stream));
@@ -1027,6 +1160,8 @@ int plan_enqueue(Plan& p, hipStream_t s, std::vector<hipEvent_t>* ev, bool timed
                 HIP_TRY(hipStreamWaitEvent(s, p.join[i - 1], 0));
             }
     }
+    if (p.jcd_blocks > 0)
+        HIP_TRY(dsscf_launch_jcd(p.d_jcd, p.d_jblk0, (int)p.h_jcd.size(), p.jcd_blocks, s));
     HIP_TRY(dsscf_launch_fold(p.d_mat, p.nmat, p.total_rows, p.maxN, s));
     return DSSCF_OK;
 }

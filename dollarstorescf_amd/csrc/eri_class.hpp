@@ -53,8 +53,11 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // Work unit = one bra shell pair and a chunk of its ket pairs (bra-major):
 // the bra record and its primitive pairs are uniform across the block; the
 // Jh[a][b] sums are reduced across the block, the Kh rows of shells a and b
-// are accumulated in LDS and flushed once per block; only Jh[c][d] goes
-// straight to global memory per quartet.  A unit's fourth field gives the
+// are accumulated in LDS and flushed once per block; only Jh[c][d] leaves
+// per quartet: as a plain store into the quartet's slot record, which k_jcd
+// sums per element after all classes (an atomic add per quartet and element
+// serializes at the few hundred addresses a small molecule's J has), or,
+// for contexts too large for the slot buffer, as an atomic add.  A unit's fourth field gives the
 // lanes per quartet, gb (bra primitives) | gk (ket primitives) << 8: a group
 // of gb*gk lanes splits a primitive-heavy quartet's primitive loops, and the
 // group's first lane digests the summed integrals.
@@ -78,6 +81,8 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     double* __restrict__ Jh = S.Jh + slice;
     double* __restrict__ Kh = S.Kh + slice;
     const GPtr<unsigned> kets = gbl(S.k);
+    // J_cd: per-quartet slot records (summed by k_jcd) or atomics
+    const int slot0 = S.js ? gbl(S.us)[(int)blockIdx.x - S.blk0] : 0;
     const ShellPair bra = ldg(pairs + u.x);
     const int a0 = bra.ao_a, b0 = bra.ao_b;
     // FP64 accurate mode: the Boys function from the block's LDS copy of the
@@ -92,7 +97,12 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     }
     const double* __restrict__ tq = kLds ? bpoly : btab;
     for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;
-    for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) braP[i] = ldg(pp + bra.prim_off + i);
+    for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) {
+        PrimPair r = ldg(pp + bra.prim_off + i);
+        if (kLds)  // the LDS Boys mode's prim_setup takes P - A
+            for (int k = 0; k < 3; ++k) r.P[k] -= bra.A[k];
+        braP[i] = r;
+    }
     double dab[NA * NB];
 #pragma unroll
     for (int i = 0; i < NA; ++i)
@@ -119,6 +129,7 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
         const double g = gb * (c0 == d0 ? 0.5 : 1.0) * (same ? 0.5 : 1.0);
 #define VIDX(i, j, k, l) ((((i) * NB + (j)) * NC + (k)) * ND + (l))
         // Coulomb
+        const GOut<double> js = S.js ? gout(S.js) + gbl(S.jd)[slot0 + t] : nullptr;
 #pragma unroll
         for (int k = 0; k < NC; ++k)
 #pragma unroll
@@ -131,7 +142,10 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
                     jab[ij] += g * x * dcd;
                     s += x * dab[ij];
                 }
-                atomic_add_f64(&Jh[(size_t)(c0 + k) * N + d0 + l], 2.0 * g * s);
+                if (js)
+                    js[k * ND + l] = 2.0 * g * s;
+                else
+                    atomic_add_f64(&Jh[(size_t)(c0 + k) * N + d0 + l], 2.0 * g * s);
             }
         // Exchange into the LDS rows
 #pragma unroll

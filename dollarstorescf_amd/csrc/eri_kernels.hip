@@ -66,6 +66,38 @@ __global__ void k_fold(const MatSeg* __restrict__ segs, int nseg)
     }
 }
 
+// J_cd terms of the thread-per-quartet classes, stored per quartet in slot
+// records by k_jk and laid out by ket AO block, summed per element: one
+// 256-thread block per block (row) reads the row's records as one contiguous
+// range, thread t on component t % (nc nd); every element belongs to one
+// row and the class kernels have finished, so the add into copy 0 of Jh is a
+// plain read-modify-write.
+__global__ void __launch_bounds__(256) k_jcd(const JcdSeg* __restrict__ segs, const int* __restrict__ blk0, int nseg)
+{
+    __shared__ double part[256];
+    const JcdSeg& S = segs[find_segment_by_block(blk0, nseg, (int)blockIdx.x)];
+    const int r = (int)blockIdx.x - S.blk0;
+    const int4 ri = S.ri[r];  // c0, d0, nc, nd
+    const int2 rr = S.rr[r];  // first double, records
+    const int ncnd = ri.z * ri.w;
+    const int per = 256 / ncnd;  // records in flight per pass
+    const int tid = (int)threadIdx.x;
+    const int comp = tid % ncnd, e0 = tid / ncnd;
+    double x = 0.0;
+    if (e0 < per) {
+        const GPtr<double> js = gbl(S.js) + rr.x + comp;
+        for (int e = e0; e < rr.y; e += per) x += js[(size_t)e * ncnd];
+    }
+    part[tid] = x;
+    __syncthreads();
+    if (tid < ncnd) {
+        double s = 0.0;
+        for (int j = 0; j < per; ++j) s += part[tid + j * ncnd];
+        const int k = tid / ri.w, l = tid - k * ri.w;
+        S.Jh[(size_t)(ri.x + k) * S.N + ri.y + l] += s;
+    }
+}
+
 // ------------------------------------------------- one-electron terms ---
 // S, T and V over the AOs of one shell pair per thread (the pair's first shell
 // has the higher l), the roles of OSOverlap / OSKineticEnergy /
@@ -360,6 +392,12 @@ hipError_t dsscf_launch_fill(double* p, size_t n, double v, hipStream_t s)
 hipError_t dsscf_launch_prep(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s)
 {
     hipLaunchKernelGGL(k_prep, dim3((maxN + 255) / 256, total_rows), dim3(256), 0, s, segs, nseg);
+    return hipGetLastError();
+}
+
+hipError_t dsscf_launch_jcd(const JcdSeg* segs, const int* blk0, int nseg, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_jcd, dim3(nblocks), dim3(256), 0, s, segs, blk0, nseg);
     return hipGetLastError();
 }
 

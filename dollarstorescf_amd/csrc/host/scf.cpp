@@ -289,6 +289,7 @@ int constructDevice(Job& job)
         }
     }
     dsscf_options o{job.opt.boys_mode, job.opt.schwarz_threshold, job.opt.fp32_threshold, job.opt.prim_threshold};
+    o.jcd_max_bytes = job.opt.jcd_max_bytes;
     int rc = dsscf_set_options(job.ctx, &o);
     if (!rc)
         rc = dsscf_set_basis(job.ctx, (int)sd.l.size(), sd.l.data(), sd.nprim.data(), sd.center.data(), sd.exps.data(),

@@ -28,6 +28,7 @@ struct Options {
     double prim_threshold = 0.0;       // primitive-quartet screening bound (0 = none)
     int device = 0;
     int device_step = 0;               // 1: DIIS, X F X, diagonalization and density on the device
+    double jcd_max_bytes = 0.0;        // dsscf_options::jcd_max_bytes of the job's context
     int verbose = 0;                   // print the iteration table like SCFSolver.cpp:198-225
 };
 

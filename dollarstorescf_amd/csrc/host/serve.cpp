@@ -108,20 +108,27 @@ void Server::enqueue(std::function<void()> f)
 
 // Upper bound of a job's device memory before its quartet lists exist: the
 // unscreened quartets at 6 B each (ket indices, work units, sub-pairs), the
-// matrices with the largest J/K slice count, pair and primitive records.
+// matrices with the largest J/K slice count, pair and primitive records, the
+// J_cd slot records (at most the job's cap; one record of at most
+// (2 l_max + 1)^2 doubles and two indices per quartet) and the device step.
 double Server::estimate_bytes(const Job& job) const
 {
     double shells = 0.0, prims = 0.0;
+    int lmax = 0;
     for (const auto& ab : job.atom_bases)
         for (const auto& sh : ab) {
             shells += 1.0;
             prims += (double)sh.primitives.size();
+            lmax = std::max<int>(lmax, (int)sh.l);
         }
     const double pairs = shells * (shells + 1.0) / 2.0;
+    const double quartets = pairs * (pairs + 1.0) / 2.0;
     const double n = job.n;
     const double step = device_steps_ ? (double)scfdev::device_bytes_for(job.n, job.opt.diis_size) : 0.0;
-    return 6.0 * pairs * (pairs + 1.0) / 2.0 + 8.0 * n * n * 34.0 + 32.0 * n * (n + 1.0) / 2.0 + 76.0 * pairs +
-           64.0 * prims * prims / 2.0 + 256.0 * 1024.0 + step;
+    const double rec = 8.0 * (2 * lmax + 1) * (2 * lmax + 1) + 8.0;
+    const double slots = job.opt.jcd_max_bytes > 0.0 ? std::min(job.opt.jcd_max_bytes, rec * quartets) : 0.0;
+    return 6.0 * quartets + 8.0 * n * n * 34.0 + 32.0 * n * (n + 1.0) / 2.0 + 76.0 * pairs +
+           64.0 * prims * prims / 2.0 + 256.0 * 1024.0 + step + slots;
 }
 
 int Server::submit(void* handle, Job* job, long long tag)
@@ -132,6 +139,8 @@ int Server::submit(void* handle, Job* job, long long tag)
         std::lock_guard<std::mutex> lk(mu_);
         submitted_ = true;
         job->opt.device_step = device_steps_ ? 1 : 0;
+        // served jobs share the budget: a modest J_cd slot cap unless the job sets one
+        if (job->opt.jcd_max_bytes == 0.0) job->opt.jcd_max_bytes = kServeJcdMaxBytes;
         waiting_.push_back(r);
     }
     enqueue([this, r] {

@@ -34,6 +34,8 @@ hipError_t dsscf_launch_one_electron(int la, int lb, int boys_mode, const int* p
                                      const double* xform, double* S, double* T, double* V, hipStream_t s);
 hipError_t dsscf_launch_prep(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);
 hipError_t dsscf_launch_fold(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);
+// Sum the J_cd slot records into copy 0 of Jh: nblocks = rows over all segments.
+hipError_t dsscf_launch_jcd(const JcdSeg* segs, const int* blk0, int nseg, int nblocks, hipStream_t s);
 // F_0..F_L (L <= 8) at n arguments in device Boys mode `mode` (BOYS_*):
 // out[k*(L+1)+m].
 hipError_t dsscf_launch_boys_eval(int mode, int L, const double* T, int n, const double* btab, double* out,

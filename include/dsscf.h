@@ -60,10 +60,17 @@ typedef struct {
                                  keeps about this many primitive quartets; its first lane digests
                                  the summed integrals into J and K; 0 = default
                                  (DSSCF_GROUP_PRIMS), < 0 = one lane per quartet */
+    double jcd_max_bytes;     /* device memory the context may spend on J_cd slot records: each
+                                 quartet then stores its ket block's Coulomb terms, and a reduction
+                                 kernel sums them per element, in place of one atomic add per
+                                 quartet and element (which serialize where many quartets share a
+                                 ket block); a context whose records would exceed it keeps the
+                                 atomic adds; 0 = default (DSSCF_JCD_MAX_BYTES), < 0 = never */
 } dsscf_options;
 
 #define DSSCF_WORK_ITEM_FLOPS 16384.0
 #define DSSCF_GROUP_PRIMS 8.0
+#define DSSCF_JCD_MAX_BYTES 8589934592.0
 
 typedef struct {
     long long n_pairs;             /* unique shell pairs */

@@ -230,6 +230,28 @@ def test_batch_device_buffers_match_host_io(graph):
     mem.close()
 
 
+@pytest.mark.parametrize("basis,mol", [("6-31G", "water8"), ("def2-SVP", "butane")])
+def test_jcd_slot_records_match_atomics(basis, mol):
+    """J_cd through slot records and the k_jcd reduction (the default) equals
+    J_cd through one atomic add per quartet and element (jcd_max_bytes < 0):
+    J within 1e-12 of max|J|, K unchanged, on the 104-AO benchmark system and
+    butane/def2-SVP (register and wave-cooperative d classes)."""
+    Z, xyz = G.water_lattice(1) if mol == "water8" else G.alkane(4)
+    job = _job(basis, Z, xyz, ds.BOYS_ACCURATE, schwarz=1e-12)
+    ctx = job.context
+    n = job.n_ao
+    C = np.random.default_rng(21).standard_normal((n, 8))
+    Pa = O.pack(C @ C.T * 0.05)
+    out = []
+    for cap in (0.0, -1.0):
+        ctx.set_options(boys_mode=ds.BOYS_ACCURATE, schwarz_threshold=1e-12, jcd_max_bytes=cap)
+        ctx.prepare()
+        out.append(ctx.jk(2 * Pa, Pa))
+    (J0, K0), (J1, K1) = out
+    assert np.max(np.abs(J0 - J1)) < 1e-12 * np.max(np.abs(J1))
+    assert np.max(np.abs(K0 - K1)) < 1e-12 * np.max(np.abs(K1))
+
+
 def test_jk_primitive_screening_accurate():
     """Default accurate mode screens primitive quartets (bound 1e-17): J/K stay
     within 1e-11 (relative to max|J|) of the unscreened accurate-mode oracle."""

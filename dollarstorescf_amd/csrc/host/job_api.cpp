@@ -322,7 +322,7 @@ int dsscf_jobs_solve(dsscf_job* const* jobs, int n, int use_graph, int autotune,
     if (int rc = dsscf_batch_create(&b, js[0]->opt.device)) return rc;
     dsscf_batch_set_graph(b, use_graph);
     std::vector<int> st;
-    const int rc = scf::solveTogether(js, b, autotune != 0, &st);
+    const int rc = scf::solveTogether(js, b, autotune != 0, &st, use_graph != 0);
     dsscf_batch_destroy(b);
     for (int i = 0; i < n; ++i) {
         if (energies) energies[i] = st[i] ? 0.0 : js[i]->energies.total();

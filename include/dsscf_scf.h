@@ -106,8 +106,11 @@ dsscf_ctx* dsscf_job_context(dsscf_job* job);
  * iterating run as one fused launch per quartet class over all of them
  * (dsscf_batch_*; with use_graph the fused build is a captured hipGraph
  * replayed until a job leaves the set), the host steps (DIIS,
- * diagonalization, density) in parallel over jobs.  With autotune the
- * block-size autotuner runs on the first fused set.  energies[i] and
+ * diagonalization, density) in parallel over jobs.  From 16 jobs on they
+ * form two groups whose rounds alternate: one group's fused build runs on
+ * the device while the other group's host steps run on the CPU.  With
+ * autotune the block-size autotuner first times each group's classes on a
+ * subset of at most 64 of its jobs.  energies[i] and
  * iterations[i] (either may be null) get each job's results; status[i]
  * (may be null) 0 or the job's error code (dsscf_job_error has the
  * message).  Returns 0 if every job converged, else the first failure's code. */

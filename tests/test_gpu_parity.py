@@ -332,11 +332,13 @@ def test_mixed_precision_alkane_def2svp():
     assert abs(e32 - e64) < 1e-6
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_jobs_solved_together_match_alone(use_graph):
+@pytest.mark.parametrize("use_graph,copies", [(False, 1), (True, 1), (True, 3)])
+def test_jobs_solved_together_match_alone(use_graph, copies):
     """Uneven jobs (several bases and sizes) solved in lock step with fused
     Fock builds (and the block-size autotuner) converge to the energies each
-    reaches alone, in the same number of iterations."""
+    reaches alone, in the same number of iterations.  Three copies of the six
+    jobs (18 jobs) run as two alternating groups (one group's fused build
+    beside the other's host steps)."""
     specs = [("STO-3G", G.water()), ("6-31G", G.water_box(2, seed=3)), ("6-31GP", G.water_box(3, seed=4)),
              ("def2-SVP", G.water()), ("6-31G", G.alkane(2)), ("STO-3G", G.water_box(4, seed=9))]
 
@@ -349,10 +351,10 @@ def test_jobs_solved_together_match_alone(use_graph):
         j = make(basis, Z, xyz)
         j.construct()
         alone.append(j.solve())
-    jobs = ds.construct_jobs([make(b, Z, xyz) for b, (Z, xyz) in specs])
+    jobs = ds.construct_jobs([make(b, Z, xyz) for b, (Z, xyz) in specs * copies])
     e, it, st = ds.solve_jobs(jobs, use_graph=use_graph, autotune=True)
-    assert list(st) == [0] * len(specs)
-    for (ea, ia), eb, ib in zip(alone, e, it):
+    assert list(st) == [0] * len(specs) * copies
+    for (ea, ia), eb, ib in zip(alone * copies, e, it):
         assert abs(ea - eb) < 1e-9
         assert abs(ia - ib) <= 1
 

@@ -102,6 +102,7 @@ def bench_alkane(a):
         t_scf = time.perf_counter() - t0
         ctx = job.context
         st = ctx.stats()
+        ctx.autotune()  # block sizes for the repeated builds below (not timed)
         Pa = job.get("Pa")
         ctx.set_density(2.0 * Pa, Pa)
         for _ in range(a.warmup):
@@ -465,6 +466,11 @@ def main():
     n = job.n_ao
     ctx = job.context
     st = ctx.stats()
+    # threads per block of each class from the project's autotuner, timed on
+    # this context's own quartet lists (set-up of the repeated builds, not timed)
+    t0 = time.perf_counter()
+    tuned = ctx.autotune()
+    t_tune = time.perf_counter() - t0
     Pa = job.get("Pa")
     ctx.set_density(2.0 * Pa, Pa)
 
@@ -528,7 +534,10 @@ def main():
                                "104 AOs, Schwarz 1e-12, FP64",
                    "boys": a.boys, "n_ao": n, "shell_quartets": int(st.n_quartets),
                    "shell_quartets_unscreened": int(st.n_quartets_unscreened),
-                   "prim_quartets": int(st.n_prim_quartets), "parallelism": "single GPU"},
+                   "prim_quartets": int(st.n_prim_quartets), "parallelism": "single GPU",
+                   "block_sizes": {"".join("spd"[l] for l in c): int(b) for c, b, q in zip(classes, tuned, cq)
+                                   if q > 0},
+                   "autotune_seconds": t_tune},
         "eri_shell_quartets_per_s": st.n_quartets * builds_per_s,
         "eri_prim_quartets_per_s": st.n_prim_quartets * builds_per_s,
         "fock_build_tflops": total_flops / (ms * 1e-3) / 1e12,

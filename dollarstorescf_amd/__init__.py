@@ -241,12 +241,14 @@ class Context:
         self._h = ctypes.c_void_p()
         _check(lib().dsscf_ctx_create(ctypes.byref(self._h), device))
         self._own = True
+        self._device = device
 
     @classmethod
-    def borrow(cls, handle):
+    def borrow(cls, handle, device: int = 0):
         obj = cls.__new__(cls)
         obj._h = ctypes.c_void_p(handle)
         obj._own = False
+        obj._device = device
         return obj
 
     def close(self):
@@ -339,6 +341,18 @@ class Context:
         _check(lib().dsscf_schwarz(self._h, _p(Q)))
         return Q
 
+    def autotune(self):
+        """Threads per block of each thread-per-quartet class, chosen by timing
+        64/128/256 on this context's own quartet lists (dsscf_batch_autotune
+        over a batch of this context alone, which keeps the choice in the
+        context).  Pays off for many builds of one geometry."""
+        b = Batch(self._device)
+        try:
+            b.set([self], graph=False)
+            return b.autotune()
+        finally:
+            b.close()
+
     def time_classes(self, iters=10):
         out = np.zeros(lib().dsscf_class_count())
         _check(lib().dsscf_time_jk_classes(self._h, iters, _p(out)))
@@ -412,6 +426,7 @@ class Job:
     def __init__(self, device: int = 0):
         self._h = ctypes.c_void_p()
         _check(lib().dsscf_job_create(ctypes.byref(self._h), device))
+        self._device = device
 
     def _err(self):
         return lib().dsscf_job_error(self._h).decode()
@@ -544,7 +559,7 @@ class Job:
 
     @property
     def context(self) -> Context:
-        return Context.borrow(lib().dsscf_job_context(self._h))
+        return Context.borrow(lib().dsscf_job_context(self._h), self._device)
 
 
 def single_point(Z, xyz_bohr, basis: str, charge=0, multiplicity=1, device=0, **opts):

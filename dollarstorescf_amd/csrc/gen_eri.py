@@ -306,6 +306,9 @@ def classes(maxl=MAXL):
     return pairs, out
 
 
+NO_BRA_UNROLL = {"ssss", "psss", "pppp"}
+
+
 def cls_name(c):
     return "".join("spdf"[l] for l in c)
 
@@ -337,7 +340,10 @@ def gen_class(c, spherical=True):
         o.append("  double %s = 0.0;" % acc_name(e, f))
     o.append("  for (int ik = rk; ik < ket.nprim; ik += gk) {")
     o.append("    const PrimPair Q = ldg(ket_pp + ik);")
-    o.append("#pragma unroll 2")
+    # two bra primitives per trip overlap their Boys-table reads, at the cost
+    # of registers; measured per class (8 waters/6-31G and C20H42/def2-SVP
+    # builds): (ss|ss), (ps|ss) and (pp|pp) run faster with one per trip
+    o.append("#pragma unroll %d" % (1 if name in NO_BRA_UNROLL else 2))
     o.append("    for (int ib = rb; ib < bra.nprim; ib += gb) {")
     o.append("      const PrimPair P = bra_pp[ib];")
     o.append("      R Fm[%d];" % (L + 1))

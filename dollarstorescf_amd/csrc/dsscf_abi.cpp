@@ -127,6 +127,10 @@ struct Plan {
     // caller's), balanced by cls_cost; side streams and fork/join events.
     std::vector<double> cls_cost;
     std::vector<int> cls_stream;
+    // launch order of the slots on the forked build: highest L first, so that
+    // the classes with the longest per-quartet chains start early and their
+    // tails overlap the rest (then heaviest first)
+    std::vector<int> launch_order;
     hipStream_t side[kStreams - 1] = {};
     hipEvent_t fork = nullptr, join[kStreams - 1] = {};
     std::vector<hipEvent_t> slot_ev;  // start/end of each slot's kernel (timed builds)
@@ -1015,6 +1019,14 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
         std::vector<int> order(nslot);
         for (int i = 0; i < nslot; ++i) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return p.cls_cost[x] > p.cls_cost[y]; });
+        std::vector<int> lsum(nslot);
+        for (int slot = 0; slot < nslot; ++slot) {
+            int l4[4];
+            dk_class_l(slot / 2, l4);
+            lsum[slot] = l4[0] + l4[1] + l4[2] + l4[3];
+        }
+        p.launch_order = order;
+        std::stable_sort(p.launch_order.begin(), p.launch_order.end(), [&](int x, int y) { return lsum[x] > lsum[y]; });
         double load[kStreams] = {};
         for (int slot : order) {
             if (p.cls_nseg[slot] == 0) continue;
@@ -1147,7 +1159,7 @@ int plan_enqueue(Plan& p, hipStream_t s, std::vector<hipEvent_t>* ev, bool timed
             p.slot_ev.assign(4 * p.ncls, nullptr);
             for (hipEvent_t& e : p.slot_ev) HIP_TRY(hipEventCreate(&e));
         }
-        for (int slot = 0; slot < 2 * p.ncls; ++slot) {
+        for (int slot : p.launch_order) {
             const int k = p.cls_stream[slot];
             hipStream_t st = k ? p.side[k - 1] : s;
             if (timed && p.cls_nseg[slot]) HIP_TRY(hipEventRecord(p.slot_ev[2 * slot], st));

@@ -823,14 +823,16 @@ int dsscf_prepare(dsscf_ctx* c)
     }
     c->n_work_pairs = (long long)work.size();
     {
-        // J/K copies: about one per 32 atomic adds that each element of J would
-        // otherwise take in a build, as a power of two up to kJKSlicesMax
+        // J/K copies: about one per 64 quartets that meet each element of J in a
+        // build, as a power of two up to kJKSlicesMax (with the J_cd slot
+        // records only the J_ab and exchange-row adds remain atomic: one per
+        // 32 had been the better choice when J_cd was atomic too)
         long long items = c->n_work_items;
         for (int k = 0; k < ncls; ++k)
             if (dsscf_class_is_wave(k)) items += c->q_cnt[k];
         const double per_elem = (double)items / std::max<double>(1.0, (double)N * (N + 1) / 2);
         c->n_slice = 1;
-        while (c->n_slice < kJKSlicesMax && c->n_slice * 32.0 < per_elem) c->n_slice *= 2;
+        while (c->n_slice < kJKSlicesMax && c->n_slice * 64.0 < per_elem) c->n_slice *= 2;
     }
     for (double** p : {&c->d_Jh, &c->d_Kh})
         if (int rc = dev_alloc(p, nn * c->n_slice)) return rc;

@@ -308,6 +308,12 @@ int Server::step(int* n_active, int* n_waiting)
             }
             if (rc) return rc;
             members_ = members;
+            if (!tuned_ && (int)members.size() >= kTuneMembers) {
+                // threads per block of each class, timed once on the first fused
+                // set of a few jobs; the batch keeps the choice for later sets
+                if ((rc = dsscf_batch_autotune(batch_, nullptr))) return rc;
+                tuned_ = true;
+            }
         }
         const double tb = clock();
         if (dev) {

@@ -98,6 +98,8 @@ private:
     dsscf_batch* batch_ = nullptr;
     std::vector<Req*> members_;  // the fused set of the last round
     bool batch_dirty_ = true;    // a job retired since the fused set was built
+    bool tuned_ = false;         // block sizes chosen by the autotuner (once)
+    static constexpr int kTuneMembers = 4;
     double st_rounds_ = 0, st_set_ = 0, st_jk_ = 0, st_host_ = 0, st_members_ = 0;
     std::chrono::steady_clock::time_point t0_;
 };

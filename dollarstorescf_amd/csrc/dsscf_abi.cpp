@@ -452,6 +452,10 @@ void build_pairs(dsscf_ctx* c)
 void shuffle_run(unsigned* v, size_t n, unsigned long long seed) { shuffle_seq(v, n, seed); }
 void shuffle_units(int4* v, size_t n, unsigned long long seed) { shuffle_seq(v, n, seed); }
 
+// Unscreened shell quartets from which the kets of a bra are ordered by
+// distance (see dsscf_prepare).
+constexpr long long kDistSortQuartets = 100000000LL;
+
 int pair_index(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
 
 // Primitive screening: a primitive pair whose e times the largest e of any
@@ -681,6 +685,7 @@ int dsscf_prepare(dsscf_ctx* c)
         const std::vector<int>& SY = by_class[j.Y];
         const std::vector<int> GY = groups(SY);
         std::vector<unsigned> k64, k32, bsub, ksub;
+        std::vector<std::pair<double, unsigned>> byd;
         for (int i = t.i0; i < t.i1; ++i) {
             const int bi = SX[i];
             k64.clear();
@@ -715,18 +720,39 @@ int dsscf_prepare(dsscf_ctx* c)
                 std::vector<int4>& uu = pass ? t.units32 : t.units;
                 ksub.clear();
                 for (unsigned kj : kk) sub_lookup((int)kj, j.P, ksub);
-                // equal primitive counts side by side: uniform loop trips in a wavefront
-                std::stable_sort(ksub.begin(), ksub.end(),
-                                 [&](unsigned x, unsigned y) { return work[x].nprim > work[y].nprim; });
-                // Every bra's kets would otherwise come in nearly the same order, so the
-                // blocks running at one time would add into the same J_cd elements at
-                // the same moment; a per-bra shuffle inside each run of equal primitive
-                // counts spreads those atomics over the matrix.
-                for (size_t r0 = 0; r0 < ksub.size();) {
-                    size_t r1 = r0 + 1;
-                    while (r1 < ksub.size() && work[ksub[r1]].nprim == work[ksub[r0]].nprim) ++r1;
-                    shuffle_run(ksub.data() + r0, r1 - r0, 0x9e3779b97f4a7c15ULL * (unsigned long long)(bi + 1) + pass);
-                    r0 = r1;
+                // Equal primitive counts side by side: uniform loop trips in a
+                // wavefront.  In a large context, inside each run of equal counts
+                // by distance from the bra: neighbouring lanes then see similar
+                // Boys arguments T, so a wavefront more often takes one branch
+                // (T < 40 or the asymptotic form) instead of both (C20H42/def2-SVP
+                // build -5%).  Below kDistSortQuartets the sort costs more host
+                // time at preparation than it saves on the device, and a per-bra
+                // shuffle inside each run spreads the J_cd atomics of contexts
+                // without slot records instead.
+                if (c->n_unscreened < kDistSortQuartets) {
+                    std::stable_sort(ksub.begin(), ksub.end(),
+                                     [&](unsigned x, unsigned y) { return work[x].nprim > work[y].nprim; });
+                    for (size_t r0 = 0; r0 < ksub.size();) {
+                        size_t r1 = r0 + 1;
+                        while (r1 < ksub.size() && work[ksub[r1]].nprim == work[ksub[r0]].nprim) ++r1;
+                        shuffle_run(ksub.data() + r0, r1 - r0,
+                                    0x9e3779b97f4a7c15ULL * (unsigned long long)(bi + 1) + pass);
+                        r0 = r1;
+                    }
+                } else {
+                    const ShellPair& bp = work[bi];
+                    const double bx = bp.A[0] - 0.5 * bp.AB[0], by = bp.A[1] - 0.5 * bp.AB[1],
+                                 bz = bp.A[2] - 0.5 * bp.AB[2];
+                    byd.clear();
+                    for (unsigned k : ksub) {
+                        const ShellPair& kp = work[k];
+                        const double x = kp.A[0] - 0.5 * kp.AB[0] - bx, y = kp.A[1] - 0.5 * kp.AB[1] - by,
+                                     z = kp.A[2] - 0.5 * kp.AB[2] - bz;
+                        // more primitives first; the distance is below 2^40 bohr^2
+                        byd.emplace_back((double)(1024 - kp.nprim) * 1099511627776.0 + (x * x + y * y + z * z), k);
+                    }
+                    std::sort(byd.begin(), byd.end());
+                    for (size_t q = 0; q < ksub.size(); ++q) ksub[q] = byd[q].second;
                 }
                 const int k0 = (int)t.kets.size();
                 t.kets.insert(t.kets.end(), ksub.begin(), ksub.end());

@@ -35,7 +35,7 @@ class Options(ctypes.Structure):
     _fields_ = [("boys_mode", ctypes.c_int), ("schwarz_threshold", ctypes.c_double),
                 ("fp32_threshold", ctypes.c_double), ("prim_threshold", ctypes.c_double),
                 ("work_item_flops", ctypes.c_double), ("group_prims", ctypes.c_double),
-                ("jcd_max_bytes", ctypes.c_double)]
+                ("jcd_max_bytes", ctypes.c_double), ("ket_order", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -259,9 +259,9 @@ class Context:
     __del__ = close
 
     def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12, fp32_threshold=0.0, prim_threshold=0.0,
-                    work_item_flops=0.0, group_prims=0.0, jcd_max_bytes=0.0):
+                    work_item_flops=0.0, group_prims=0.0, jcd_max_bytes=0.0, ket_order=0):
         o = Options(boys_mode, schwarz_threshold, fp32_threshold, prim_threshold, work_item_flops, group_prims,
-                    jcd_max_bytes)
+                    jcd_max_bytes, ket_order)
         _check(lib().dsscf_set_options(self._h, ctypes.byref(o)))
 
     def set_basis(self, l, nprim, centers, exps, coefs, xform):

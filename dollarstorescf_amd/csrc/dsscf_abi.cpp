@@ -341,6 +341,7 @@ int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
     if (!c || !o) return fail(DSSCF_ERR_ARG, "dsscf_set_options: null argument");
     if (o->boys_mode != DSSCF_BOYS_REFERENCE && o->boys_mode != DSSCF_BOYS_ACCURATE)
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: unknown boys_mode");
+    if (o->ket_order < 0 || o->ket_order > 2) return fail(DSSCF_ERR_ARG, "dsscf_set_options: ket_order 0..2");
     if (!(o->schwarz_threshold >= 0.0) || !(o->fp32_threshold >= 0.0) || !(o->prim_threshold >= 0.0))
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: negative threshold");
     if (o->fp32_threshold > 0.0 && o->boys_mode != DSSCF_BOYS_ACCURATE)
@@ -348,7 +349,7 @@ int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
     const bool changed = o->boys_mode != c->opt.boys_mode || o->schwarz_threshold != c->opt.schwarz_threshold ||
                          o->fp32_threshold != c->opt.fp32_threshold || o->prim_threshold != c->opt.prim_threshold ||
                          o->work_item_flops != c->opt.work_item_flops || o->group_prims != c->opt.group_prims ||
-                         o->jcd_max_bytes != c->opt.jcd_max_bytes;
+                         o->jcd_max_bytes != c->opt.jcd_max_bytes || o->ket_order != c->opt.ket_order;
     c->opt = *o;
     if (changed) c->prepared = false;
     return DSSCF_OK;
@@ -451,10 +452,6 @@ void build_pairs(dsscf_ctx* c)
 
 void shuffle_run(unsigned* v, size_t n, unsigned long long seed) { shuffle_seq(v, n, seed); }
 void shuffle_units(int4* v, size_t n, unsigned long long seed) { shuffle_seq(v, n, seed); }
-
-// Unscreened shell quartets from which the kets of a bra are ordered by
-// distance (see dsscf_prepare).
-constexpr long long kDistSortQuartets = 100000000LL;
 
 int pair_index(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
 
@@ -679,6 +676,8 @@ int dsscf_prepare(dsscf_ctx* c)
             }
         }
     }
+    const bool by_distance = c->opt.ket_order == 2 ||
+                             (c->opt.ket_order == 0 && (double)c->n_unscreened >= DSSCF_KET_DISTANCE_QUARTETS);
     auto run_task = [&](Task& t) {
         const ClassJob& j = cj[t.job];
         const std::vector<int>& SX = by_class[j.X];
@@ -725,11 +724,11 @@ int dsscf_prepare(dsscf_ctx* c)
                 // by distance from the bra: neighbouring lanes then see similar
                 // Boys arguments T, so a wavefront more often takes one branch
                 // (T < 40 or the asymptotic form) instead of both (C20H42/def2-SVP
-                // build -5%).  Below kDistSortQuartets the sort costs more host
-                // time at preparation than it saves on the device, and a per-bra
-                // shuffle inside each run spreads the J_cd atomics of contexts
-                // without slot records instead.
-                if (c->n_unscreened < kDistSortQuartets) {
+                // build -5%).  Below DSSCF_KET_DISTANCE_QUARTETS the sort costs
+                // more host time at preparation than it saves on the device, and a
+                // per-bra shuffle inside each run spreads the J_cd atomics of
+                // contexts without slot records instead (ket_order chooses).
+                if (!by_distance) {
                     std::stable_sort(ksub.begin(), ksub.end(),
                                      [&](unsigned x, unsigned y) { return work[x].nprim > work[y].nprim; });
                     for (size_t r0 = 0; r0 < ksub.size();) {

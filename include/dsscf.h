@@ -66,11 +66,16 @@ typedef struct {
                                  quartet and element (which serialize where many quartets share a
                                  ket block); a context whose records would exceed it keeps the
                                  atomic adds; 0 = default (DSSCF_JCD_MAX_BYTES), < 0 = never */
+    int ket_order;            /* order of a bra's kets of equal primitive count: 0 = by context size
+                                 (distance from the bra from DSSCF_KET_DISTANCE_QUARTETS unscreened
+                                 quartets on, else shuffled), 1 = shuffled, 2 = by distance; the
+                                 results agree to rounding */
 } dsscf_options;
 
 #define DSSCF_WORK_ITEM_FLOPS 16384.0
 #define DSSCF_GROUP_PRIMS 8.0
 #define DSSCF_JCD_MAX_BYTES 8589934592.0
+#define DSSCF_KET_DISTANCE_QUARTETS 100000000.0
 
 typedef struct {
     long long n_pairs;             /* unique shell pairs */

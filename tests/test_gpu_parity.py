@@ -252,6 +252,28 @@ def test_jcd_slot_records_match_atomics(basis, mol):
     assert np.max(np.abs(K0 - K1)) < 1e-12 * np.max(np.abs(K1))
 
 
+@pytest.mark.parametrize("basis,mol", [("6-31G", "water8"), ("def2-SVP", "octane")])
+def test_ket_order_by_distance_matches_shuffled(basis, mol):
+    """Kets ordered by distance from the bra (the default from 1e8 unscreened
+    quartets on) and shuffled (the default below) give the same J and K to
+    rounding (1e-12 of max |J|, |K|), also through the sub-pairs and wave
+    classes of octane/def2-SVP."""
+    Z, xyz = G.water_lattice(1) if mol == "water8" else G.alkane(8)
+    job = _job(basis, Z, xyz, ds.BOYS_ACCURATE, schwarz=1e-12)
+    ctx = job.context
+    n = job.n_ao
+    C = np.random.default_rng(31).standard_normal((n, 10))
+    Pa = O.pack(C @ C.T * 0.05)
+    out = []
+    for order in (1, 2):
+        ctx.set_options(boys_mode=ds.BOYS_ACCURATE, schwarz_threshold=1e-12, ket_order=order)
+        ctx.prepare()
+        out.append(ctx.jk(2 * Pa, Pa))
+    (J1, K1), (J2, K2) = out
+    assert np.max(np.abs(J1 - J2)) < 1e-12 * np.max(np.abs(J1))
+    assert np.max(np.abs(K1 - K2)) < 1e-12 * np.max(np.abs(K1))
+
+
 def test_jk_primitive_screening_accurate():
     """Default accurate mode screens primitive quartets (bound 1e-17): J/K stay
     within 1e-11 (relative to max|J|) of the unscreened accurate-mode oracle."""

@@ -366,18 +366,53 @@ struct PrimQuartet {
 // FP32 form for low-bound quartets: setup, small-T Boys (same table, Taylor
 // and downward recursion in float) and the values handed to the VRR in FP32;
 // the large-T Boys branch stays FP64.
-template <int L>
+// FP32 interval table in LDS (k_jk stages it from the FP64 interval table of
+// m = L): per interval 8 floats, e^-c and the Taylor coefficients a_0..a_6
+// of F_L about c; |u| <= 1/4 leaves a truncation below 1.2e-8 relative,
+// under FP32 rounding.
+#define DSSCF_BPOLY32_COEF 8
+
+template <int L, bool kLds>
 __device__ __forceinline__ void prim_setup_f32(const PrimPair& P, const PrimPair& Q, const ShellPair& bra,
                                                const ShellPair& ket, const double* __restrict__ btab,
                                                PrimQuartet<float>& g, float* Fm)
 {
+    // kLds: btab points at the block's FP32 interval table and P.P holds P - A
     const float p = (float)P.p, q = (float)Q.p;
     const float inv_pq = __builtin_amdgcn_rcpf(p + q);
-    const float PQx = (float)(P.P[0] - Q.P[0]), PQy = (float)(P.P[1] - Q.P[1]), PQz = (float)(P.P[2] - Q.P[2]);
+    const float PQx = kLds ? (float)(P.P[0] - (Q.P[0] - bra.A[0])) : (float)(P.P[0] - Q.P[0]);
+    const float PQy = kLds ? (float)(P.P[1] - (Q.P[1] - bra.A[1])) : (float)(P.P[1] - Q.P[1]);
+    const float PQz = kLds ? (float)(P.P[2] - (Q.P[2] - bra.A[2])) : (float)(P.P[2] - Q.P[2]);
     const float rho = p * q * inv_pq;
     const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
     const float pref = (float)P.Kp * (float)Q.Kp * __builtin_sqrtf(inv_pq);
-    if (T < (float)DSSCF_BOYS_TMAX) {
+    if (kLds && T < (float)DSSCF_BOYS_TMAX) {
+        const int i = (int)(T * 2.0f);
+        const float u = T - (0.5f * (float)i + 0.25f);
+        const float4* c = reinterpret_cast<const float4*>(btab) + 2 * i;
+        const float4 c0 = c[0], c1 = c[1];  // e^-c, a_0, a_1, a_2 | a_3 .. a_6
+        float s = c1.w;
+        s = fmaf(s, u, c1.z);
+        s = fmaf(s, u, c1.y);
+        s = fmaf(s, u, c1.x);
+        s = fmaf(s, u, c0.w);
+        s = fmaf(s, u, c0.z);
+        s = fmaf(s, u, c0.y);
+        Fm[L] = s;
+        if (L > 0) {
+            float e = 1.0f / 720.0f;  // sum_k (-u)^k / k!, k <= 6
+            e = fmaf(e, -u, 1.0f / 120.0f);
+            e = fmaf(e, -u, 1.0f / 24.0f);
+            e = fmaf(e, -u, 1.0f / 6.0f);
+            e = fmaf(e, -u, 0.5f);
+            e = fmaf(e, -u, 1.0f);
+            e = fmaf(e, -u, 1.0f);
+            const float eT = c0.x * e;
+            const float T2 = 2.0f * T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) Fm[m] = fmaf(T2, Fm[m + 1], eT) * (1.0f / (2 * m + 1));
+        }
+    } else if (!kLds && T < (float)DSSCF_BOYS_TMAX) {
         const int i = (int)(T * (float)DSSCF_BOYS_INV_H + 0.5f);
         const float nd = (float)i * (float)DSSCF_BOYS_H - T;
         const double* row = btab + i * DSSCF_BOYS_STRIDE;
@@ -410,9 +445,9 @@ __device__ __forceinline__ void prim_setup_f32(const PrimPair& P, const PrimPair
 #pragma unroll
     for (int m = 0; m <= L; ++m) Fm[m] *= pref;
     const float qf = q * inv_pq, pf = p * inv_pq;
-    g.PA[0] = (float)(P.P[0] - bra.A[0]);
-    g.PA[1] = (float)(P.P[1] - bra.A[1]);
-    g.PA[2] = (float)(P.P[2] - bra.A[2]);
+    g.PA[0] = kLds ? (float)P.P[0] : (float)(P.P[0] - bra.A[0]);
+    g.PA[1] = kLds ? (float)P.P[1] : (float)(P.P[1] - bra.A[1]);
+    g.PA[2] = kLds ? (float)P.P[2] : (float)(P.P[2] - bra.A[2]);
     g.QC[0] = (float)(Q.P[0] - ket.A[0]);
     g.QC[1] = (float)(Q.P[1] - ket.A[1]);
     g.QC[2] = (float)(Q.P[2] - ket.A[2]);
@@ -435,7 +470,7 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
                                            PrimQuartet<R>& g, R* Fm)
 {
     if constexpr (sizeof(R) == 4) {
-        prim_setup_f32<L>(P, Q, bra, ket, btab, g, Fm);
+        prim_setup_f32<L, BM == BOYS_ACCURATE_LDS>(P, Q, bra, ket, btab, g, Fm);
         return;
     } else {
     const double p = P.p, q = Q.p;

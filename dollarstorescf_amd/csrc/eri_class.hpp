@@ -88,12 +88,23 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     // FP64 accurate mode: the Boys function from the block's LDS copy of the
     // interval table of m = L (a per-lane gather from the 801-row table
     // touches up to 64 cache lines per load)
-    constexpr bool kLds = BM == BOYS_ACCURATE && sizeof(R) == 8;
+    // (FP32 lists: a float copy of its e^-c and first seven coefficients)
+    constexpr bool kLds = BM == BOYS_ACCURATE;
+    constexpr bool k64 = sizeof(R) == 8;
     constexpr int BMq = kLds ? BOYS_ACCURATE_LDS : BM;
-    __shared__ __align__(16) double bpoly[kLds ? DSSCF_BPOLY_M_STRIDE : 1];
+    __shared__ __align__(16) double bpoly[kLds ? (k64 ? DSSCF_BPOLY_M_STRIDE : DSSCF_BPOLY_ROWS * DSSCF_BPOLY32_COEF / 2) : 1];
     if (kLds) {
         const GPtr<double> src = gbl(btab + DSSCF_BPOLY_OFFSET + CI::L * DSSCF_BPOLY_M_STRIDE);
-        for (int i = threadIdx.x; i < DSSCF_BPOLY_M_STRIDE; i += blockDim.x) bpoly[i] = src[i];
+        if (k64) {
+            for (int i = threadIdx.x; i < DSSCF_BPOLY_M_STRIDE; i += blockDim.x) bpoly[i] = src[i];
+        } else {
+            // coefficient q of interval r sits at double 2 ((q / 2) ROWS + r) + q % 2
+            float* f = reinterpret_cast<float*>(bpoly);
+            for (int i = threadIdx.x; i < DSSCF_BPOLY_ROWS * DSSCF_BPOLY32_COEF; i += blockDim.x) {
+                const int r = i / DSSCF_BPOLY32_COEF, q = i % DSSCF_BPOLY32_COEF;
+                f[i] = (float)src[2 * ((q >> 1) * DSSCF_BPOLY_ROWS + r) + (q & 1)];
+            }
+        }
     }
     const double* __restrict__ tq = kLds ? bpoly : btab;
     for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;

@@ -62,6 +62,16 @@ struct __align__(16) PrimPair {
 };
 static_assert(sizeof(PrimPair) == 64, "PrimPair must be one 64 B record");
 
+// A bra primitive pair as the FP32 J/K lists stage it in LDS: converted once
+// per block, with P - A in place of P.
+struct __align__(16) PrimPairF {
+    float p, inv_p;
+    float PA[3];
+    float Kp;
+    float pad[2];
+};
+static_assert(sizeof(PrimPairF) == 32, "PrimPairF must be one 32 B record");
+
 // One context's share of a fused Fock-build launch (one class): its quartet
 // list and the buffers the quartets read and accumulate into.  blk0 is the
 // first block of the launch that belongs to this segment.
@@ -458,6 +468,80 @@ __device__ __forceinline__ void prim_setup_f32(const PrimPair& P, const PrimPair
     g.WQ[1] = pf * PQy;
     g.WQ[2] = pf * PQz;
     g.o2p = 0.5f * (float)P.inv_p;
+    g.o2q = 0.5f * (float)Q.inv_p;
+    g.o2pq = 0.5f * inv_pq;
+    g.rp = qf;
+    g.rq = pf;
+}
+
+// FP32 lists with their bra records staged as PrimPairF (P - A, converted
+// once per block) and the interval table as floats in LDS: as
+// prim_setup_f32<L, true>, without the per-quartet conversions of the bra
+// record (the ket record's are invariant in the bra loop).
+template <int BM, typename R, int L>
+__device__ __forceinline__ void prim_setup(const PrimPairF& P, const PrimPair& Q, const ShellPair& bra,
+                                           const ShellPair& ket, const double* __restrict__ btab,
+                                           PrimQuartet<float>& g, float* Fm)
+{
+    static_assert(BM == BOYS_ACCURATE_LDS && sizeof(R) == 4, "PrimPairF records are staged for FP32 LDS lists");
+    const float p = P.p, q = (float)Q.p;
+    const float inv_pq = __builtin_amdgcn_rcpf(p + q);
+    const float QAx = (float)(Q.P[0] - bra.A[0]), QAy = (float)(Q.P[1] - bra.A[1]), QAz = (float)(Q.P[2] - bra.A[2]);
+    const float PQx = P.PA[0] - QAx, PQy = P.PA[1] - QAy, PQz = P.PA[2] - QAz;
+    const float rho = p * q * inv_pq;
+    const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+    const float pref = P.Kp * (float)Q.Kp * __builtin_sqrtf(inv_pq);
+    if (T < (float)DSSCF_BOYS_TMAX) {
+        const int i = (int)(T * 2.0f);
+        const float u = T - (0.5f * (float)i + 0.25f);
+        const float4* c = reinterpret_cast<const float4*>(btab) + 2 * i;
+        const float4 c0 = c[0], c1 = c[1];  // e^-c, a_0, a_1, a_2 | a_3 .. a_6
+        float s = c1.w;
+        s = fmaf(s, u, c1.z);
+        s = fmaf(s, u, c1.y);
+        s = fmaf(s, u, c1.x);
+        s = fmaf(s, u, c0.w);
+        s = fmaf(s, u, c0.z);
+        s = fmaf(s, u, c0.y);
+        Fm[L] = s;
+        if (L > 0) {
+            float e = 1.0f / 720.0f;  // sum_k (-u)^k / k!, k <= 6
+            e = fmaf(e, -u, 1.0f / 120.0f);
+            e = fmaf(e, -u, 1.0f / 24.0f);
+            e = fmaf(e, -u, 1.0f / 6.0f);
+            e = fmaf(e, -u, 0.5f);
+            e = fmaf(e, -u, 1.0f);
+            e = fmaf(e, -u, 1.0f);
+            const float eT = c0.x * e;
+            const float T2 = 2.0f * T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) Fm[m] = fmaf(T2, Fm[m + 1], eT) * (1.0f / (2 * m + 1));
+        }
+    } else {
+        double F[L + 1];
+        const double inv_T = 1.0 / (double)T;
+        F[0] = 0.88622692545275801365 * sqrt(inv_T);
+#pragma unroll
+        for (int m = 1; m <= L; ++m) F[m] = F[m - 1] * ((m - 0.5) * inv_T);
+#pragma unroll
+        for (int m = 0; m <= L; ++m) Fm[m] = (float)F[m];
+    }
+#pragma unroll
+    for (int m = 0; m <= L; ++m) Fm[m] *= pref;
+    const float qf = q * inv_pq, pf = p * inv_pq;
+    g.PA[0] = P.PA[0];
+    g.PA[1] = P.PA[1];
+    g.PA[2] = P.PA[2];
+    g.QC[0] = (float)(Q.P[0] - ket.A[0]);
+    g.QC[1] = (float)(Q.P[1] - ket.A[1]);
+    g.QC[2] = (float)(Q.P[2] - ket.A[2]);
+    g.WP[0] = -qf * PQx;
+    g.WP[1] = -qf * PQy;
+    g.WP[2] = -qf * PQz;
+    g.WQ[0] = pf * PQx;
+    g.WQ[1] = pf * PQy;
+    g.WQ[2] = pf * PQz;
+    g.o2p = 0.5f * P.inv_p;
     g.o2q = 0.5f * (float)Q.inv_p;
     g.o2pq = 0.5f * inv_pq;
     g.rp = qf;

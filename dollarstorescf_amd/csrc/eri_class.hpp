@@ -14,6 +14,8 @@
 #include "device_common.hpp"
 #include "launch.hpp"
 
+#include <type_traits>
+
 #include "eri_classes.inc"
 
 template <int la, int lb, int lc, int ld, int BM, typename R>
@@ -22,13 +24,14 @@ struct Quartet;
 #define DSSCF_QSPEC_0(a, b, c, d, name)                                                               \
     template <int BM, typename R>                                                                     \
     struct Quartet<a, b, c, d, BM, R> {                                                               \
+        template <typename BP>                                                                        \
         static __device__ __forceinline__ void run(const ShellPair& bra, const ShellPair& ket,        \
-                                                   const PrimPair* __restrict__ bra_pp,               \
+                                                   const BP* __restrict__ bra_pp,                     \
                                                    const PrimPair* __restrict__ ket_pp,               \
                                                    const double* __restrict__ btab, double* out,      \
                                                    int gb = 1, int rb = 0, int gk = 1, int rk = 0)    \
         {                                                                                             \
-            shell_quartet_##name<BM, R>(bra, ket, bra_pp, ket_pp, btab, out, gb, rb, gk, rk);         \
+            shell_quartet_##name<BM, R, BP>(bra, ket, bra_pp, ket_pp, btab, out, gb, rb, gk, rk);     \
         }                                                                                             \
     };
 #define DSSCF_QUARTET_SPECIALIZE(a, b, c, d, name, wave) DSSCF_QSPEC_##wave(a, b, c, d, name)
@@ -68,7 +71,10 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     typedef ClassInfo<la, lb, lc, ld> CI;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
     extern __shared__ double krow[];  // [(NA + NB) * N]: Kh rows of shell a, then shell b
-    __shared__ PrimPair braP[kBraStage];
+    // bra primitive records: FP64 lists PrimPair, FP32 lists (accurate mode) PrimPairF
+    constexpr bool kF32Rec = sizeof(R) == 4 && BM == BOYS_ACCURATE;
+    typedef typename std::conditional<kF32Rec, PrimPairF, PrimPair>::type BPT;
+    __shared__ BPT braP[kBraStage];
     const int si = find_segment_by_block(seg_blk0, nseg, (int)blockIdx.x);
     const JKSeg& S = segs[si];
     const int4 u = ldg(S.u + ((int)blockIdx.x - S.blk0));  // bra pair, first ket, ket count
@@ -112,7 +118,17 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
         PrimPair r = ldg(pp + bra.prim_off + i);
         if (kLds)  // the LDS Boys mode's prim_setup takes P - A
             for (int k = 0; k < 3; ++k) r.P[k] -= bra.A[k];
-        braP[i] = r;
+        if constexpr (kF32Rec) {
+            PrimPairF f;
+            f.p = (float)r.p;
+            f.inv_p = (float)r.inv_p;
+            for (int k = 0; k < 3; ++k) f.PA[k] = (float)r.P[k];
+            f.Kp = (float)r.Kp;
+            f.pad[0] = f.pad[1] = 0.0f;
+            braP[i] = f;
+        } else {
+            braP[i] = r;
+        }
     }
     double dab[NA * NB];
 #pragma unroll

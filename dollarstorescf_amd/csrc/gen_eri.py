@@ -331,9 +331,10 @@ def gen_class(c, spherical=True):
     o.append("// lane (rb, rk) takes bra primitives rb, rb+gb, .. and ket primitives rk,")
     o.append("// rk+gk, ..; the [e0|f0] sums are then added across the group, so every")
     o.append("// lane of it ends with the whole quartet.  gb = gk = 1: one lane.")
-    o.append("template <int BM, typename R>")
+    o.append("// BP: the bra primitive record (PrimPair, or PrimPairF staged for FP32 lists).")
+    o.append("template <int BM, typename R, typename BP = PrimPair>")
     o.append("__device__ __forceinline__ void shell_quartet_%s(const ShellPair& bra, const ShellPair& ket,"
-             " const PrimPair* __restrict__ bra_pp, const PrimPair* __restrict__ ket_pp,"
+             " const BP* __restrict__ bra_pp, const PrimPair* __restrict__ ket_pp,"
              " const double* __restrict__ btab, double* __restrict__ out,"
              " int gb = 1, int rb = 0, int gk = 1, int rk = 0) {" % name)
     for (e, f) in accs:
@@ -345,7 +346,7 @@ def gen_class(c, spherical=True):
     # builds): (ss|ss), (ps|ss) and (pp|pp) run faster with one per trip
     o.append("#pragma unroll %d" % (1 if name in NO_BRA_UNROLL else 2))
     o.append("    for (int ib = rb; ib < bra.nprim; ib += gb) {")
-    o.append("      const PrimPair P = bra_pp[ib];")
+    o.append("      const BP P = bra_pp[ib];")
     o.append("      R Fm[%d];" % (L + 1))
     o.append("      PrimQuartet<R> g; prim_setup<BM, R, %d>(P, Q, bra, ket, btab, g, Fm);" % L)
     o.append("      const R PAx = g.PA[0], PAy = g.PA[1], PAz = g.PA[2];")

@@ -200,7 +200,7 @@ def gen_class(c):
         o.append("    for (int ik = 0; ik < ket.nprim; ++ik) {")
         o.append("      const PrimPair Q = ldg(ket_pp + ik);")
         o.append("      for (int ib = 0; ib < bra.nprim; ++ib) {")
-        o.append("        const PrimPair P = ldg(bra_pp + ib);")
+        o.append("        const PrimPair P = bra_pp[ib];  // the block's LDS copy (accurate mode) or global")
         o.append("        double Fm[%d];" % (L + 1))
         o.append("        PrimQuartet<double> g; prim_setup<BM, double, %d>(P, Q, bra, ket, btab, g, Fm);" % L)
         o.append("        const double PAx = g.PA[0], PAy = g.PA[1], PAz = g.PA[2];")

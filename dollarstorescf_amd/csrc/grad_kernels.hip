@@ -51,9 +51,28 @@ __global__ void __launch_bounds__(128) k_grad(const JKSeg* __restrict__ segs, co
     const int si = find_segment_by_block(seg_blk0, nseg, (int)blockIdx.x);
     const JKSeg& S = segs[si];
     const int4 u = ldg(S.u + ((int)blockIdx.x - S.blk0));
+    const ShellPair bra = ldg(S.pairs + u.x);
+    // accurate mode: as the J/K kernels, the Boys function from the block's LDS
+    // copy of the interval table of m = L + 1 and the bra's primitive records
+    // in LDS with P - A in place of P
+    constexpr bool kLds = BM == BOYS_ACCURATE;
+    constexpr int BMq = kLds ? BOYS_ACCURATE_LDS : BM;
+    constexpr int LG = la + lb + lc + ld + 1;
+    __shared__ __align__(16) double bpoly[kLds ? DSSCF_BPOLY_M_STRIDE : 1];
+    __shared__ PrimPair braP[kLds ? kBraStage : 1];
+    if (kLds) {
+        const GPtr<double> src = gbl(btab + DSSCF_BPOLY_OFFSET + LG * DSSCF_BPOLY_M_STRIDE);
+        for (int i = threadIdx.x; i < DSSCF_BPOLY_M_STRIDE; i += blockDim.x) bpoly[i] = src[i];
+        for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) {
+            PrimPair r = ldg(S.pp + bra.prim_off + i);
+            for (int k = 0; k < 3; ++k) r.P[k] -= bra.A[k];
+            braP[i] = r;
+        }
+    }
+    const double* __restrict__ tq = kLds ? bpoly : btab;
+    const PrimPair* __restrict__ bp = kLds ? braP : S.pp + bra.prim_off;
     for (int i = threadIdx.x; i < 3 * natom; i += blockDim.x) gsh[i] = 0.0;
     __syncthreads();
-    const ShellPair bra = ldg(S.pairs + u.x);
     const int a0 = bra.ao_a, b0 = bra.ao_b, N = S.N;
     const GPtr<double> D = gbl(S.Da);
     const int atA = ao_atom[a0], atB = ao_atom[b0];
@@ -78,9 +97,9 @@ __global__ void __launch_bounds__(128) k_grad(const JKSeg* __restrict__ segs, co
                                                                              D[a * N + d] * D[b * N + c]));
                     }
         double Gc[NCART];
-        GradQuartet<la, lb, lc, ld, BM>::density(xf, Gs, Gc);
+        GradQuartet<la, lb, lc, ld, BMq>::density(xf, Gs, Gc);
         double g9[9];
-        GradQuartet<la, lb, lc, ld, BM>::run(bra, ket, S.pp + bra.prim_off, S.pp + ket.prim_off, btab, Gc, g9);
+        GradQuartet<la, lb, lc, ld, BMq>::run(bra, ket, bp, S.pp + ket.prim_off, tq, Gc, g9);
         const int atC = ao_atom[c0], atD = ao_atom[d0];
 #pragma unroll
         for (int x = 0; x < 3; ++x) {

@@ -60,10 +60,11 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // per quartet: as a plain store into the quartet's slot record, which k_jcd
 // sums per element after all classes (an atomic add per quartet and element
 // serializes at the few hundred addresses a small molecule's J has), or,
-// for contexts too large for the slot buffer, as an atomic add.  A unit's fourth field gives the
-// lanes per quartet, gb (bra primitives) | gk (ket primitives) << 8: a group
-// of gb*gk lanes splits a primitive-heavy quartet's primitive loops, and the
-// group's first lane digests the summed integrals.
+// for contexts too large for the slot buffer, as an atomic add.  A unit's
+// fourth field gives the lanes per quartet, gb (bra primitives) | gk (ket
+// primitives) << 8: a group of gb*gk lanes splits a primitive-heavy
+// quartet's primitive loops, and the group's first lane digests the summed
+// integrals.
 template <int la, int lb, int lc, int ld, int BM, typename R>
 __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
                                             int nseg, const double* __restrict__ btab, double scale)

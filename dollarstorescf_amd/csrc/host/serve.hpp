@@ -16,7 +16,7 @@ namespace scf {
 
 // J_cd slot-record cap (dsscf_options::jcd_max_bytes) of served jobs that
 // set none: their footprints share one device-memory budget.
-constexpr double kServeJcdMaxBytes = 256.0 * 1024 * 1024;
+constexpr double kServeJcdMaxBytes = 1024.0 * 1024 * 1024;
 
 class Server {
 public:

@@ -86,7 +86,18 @@ __global__ void __launch_bounds__(256) k_jcd(const JcdSeg* __restrict__ segs, co
     double x = 0.0;
     if (e0 < per) {
         const GPtr<double> js = gbl(S.js) + rr.x + comp;
-        for (int e = e0; e < rr.y; e += per) x += js[(size_t)e * ncnd];
+        // four independent loads in flight per thread (the row sums are
+        // bound by load latency, not bandwidth)
+        double x1 = 0.0, x2 = 0.0, x3 = 0.0;
+        int e = e0;
+        for (; e + 3 * per < rr.y; e += 4 * per) {
+            x += js[(size_t)e * ncnd];
+            x1 += js[(size_t)(e + per) * ncnd];
+            x2 += js[(size_t)(e + 2 * per) * ncnd];
+            x3 += js[(size_t)(e + 3 * per) * ncnd];
+        }
+        for (; e < rr.y; e += per) x += js[(size_t)e * ncnd];
+        x += (x1 + x2) + x3;
     }
     part[tid] = x;
     __syncthreads();

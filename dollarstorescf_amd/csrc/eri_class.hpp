@@ -102,11 +102,15 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     __shared__ __align__(16) double bpoly[kLds ? (k64 ? DSSCF_BPOLY_M_STRIDE : DSSCF_BPOLY_ROWS * DSSCF_BPOLY32_COEF / 2) : 1];
     if (kLds) {
         const GPtr<double> src = gbl(btab + DSSCF_BPOLY_OFFSET + CI::L * DSSCF_BPOLY_M_STRIDE);
+        // four loads in flight per thread: the copy is on every block's way
+        // to its first quartet
         if (k64) {
+#pragma unroll 4
             for (int i = threadIdx.x; i < DSSCF_BPOLY_M_STRIDE; i += blockDim.x) bpoly[i] = src[i];
         } else {
             // coefficient q of interval r sits at double 2 ((q / 2) ROWS + r) + q % 2
             float* f = reinterpret_cast<float*>(bpoly);
+#pragma unroll 4
             for (int i = threadIdx.x; i < DSSCF_BPOLY_ROWS * DSSCF_BPOLY32_COEF; i += blockDim.x) {
                 const int r = i / DSSCF_BPOLY32_COEF, q = i % DSSCF_BPOLY32_COEF;
                 f[i] = (float)src[2 * ((q >> 1) * DSSCF_BPOLY_ROWS + r) + (q & 1)];

@@ -62,6 +62,7 @@ __global__ void __launch_bounds__(128) k_grad(const JKSeg* __restrict__ segs, co
     __shared__ PrimPair braP[kLds ? kBraStage : 1];
     if (kLds) {
         const GPtr<double> src = gbl(btab + DSSCF_BPOLY_OFFSET + LG * DSSCF_BPOLY_M_STRIDE);
+#pragma unroll 4
         for (int i = threadIdx.x; i < DSSCF_BPOLY_M_STRIDE; i += blockDim.x) bpoly[i] = src[i];
         for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) {
             PrimPair r = ldg(S.pp + bra.prim_off + i);

@@ -27,28 +27,102 @@ METRIC = "Fock builds/s, ERI quartets/s; SCF/s + p99 vs FP64 plain at |dE|<=1e-6
 FP64_PEAK_TFLOPS = 78.6
 
 
-def cpu_baseline(Z, xyz, n_ao, n_unique_ao_quartets, sample, seed=7):
-    """The reference's electronRepulsionIntegral (oracle/_ref, one core) on a
-    random sample of unique AO quartets of the same system, extrapolated to a
-    full Fock build's unique AO quartets (the stored-tensor J/K lookups are
-    not included)."""
+def cpu_model() -> str:
+    """The host CPU's model name (as lscpu prints it), from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def packed_quartets(n: int) -> np.ndarray:
+    """Every unique AO quartet (ij|kl) in the order of the reference's packed
+    tensor (ExactCoulomb.cpp:58-61): pair index m = j + i(i+1)/2 <= n = l + k(k+1)/2,
+    m-major."""
+    ii, jj = np.tril_indices(n)
+    order = np.argsort(jj + ii * (ii + 1) // 2, kind="stable")
+    pi, pj = ii[order].astype(np.int32), jj[order].astype(np.int32)
+    D = len(pi)
+    m = np.repeat(np.arange(D, dtype=np.int64), D - np.arange(D))
+    n_idx = np.concatenate([np.arange(k, D, dtype=np.int64) for k in range(D)])
+    return np.stack([pi[m], pj[m], pi[n_idx], pj[n_idx]], axis=1)
+
+
+def cpu_baseline(ctx, Z, xyz, n_ao, Pa, J_gpu, K_gpu, sample, jk_iters=3, seed=7):
+    """The reference's own Fock build on the host, in its two parts.
+
+    * Per iteration (the reported value): the stored-tensor J and K loops of
+      getExactCoulombMatrix / getExactExchangeMatrix (ExactCoulomb.cpp:103-123,
+      as computeCoulombTerms / computeExchangeTerms call them every iteration,
+      SCFSolver.cpp:29-43), restated in oracle/oracle.c, on 1 core over the real
+      packed tensor of this 104-AO system (14.9 M doubles, filled with this
+      engine's integrals: the loops' time does not depend on the values, and
+      their J/K are checked against the GPU's).  Fock builds/s = 1 / (J + K).
+    * One-off: the tensor build (assignExactRepulsionTensor, ExactCoulomb.cpp:
+      41-72), i.e. the reference's electronRepulsionIntegral (oracle/_ref, built
+      from its sources) over every unique AO quartet, extrapolated from a
+      random sample of them."""
+    from oracle import oracle as O
     from oracle import ref
 
-    if not ref.available():
-        return None
-    import dollarstorescf_amd as ds
-
-    rm = ref.RefMolecule(ds.basis_path("6-31G"), Z, xyz)
-    rng = np.random.default_rng(seed)
-    q = rng.integers(0, n_ao, size=(sample, 4)).astype(np.int32)
-    t, _ = rm.time_eri_list(q)
-    per_q = t / sample
-    return {"value": 1.0 / (per_q * n_unique_ao_quartets), "unit": "Fock builds/s", "cores": 1,
-            "kind": "reference",
-            "sample": f"{sample} random AO quartets of the same 104-AO system through the reference's "
-                      f"electronRepulsionIntegral (GaussianIntegral.cpp:561), {t:.1f} s on 1 core, "
-                      f"extrapolated to {n_unique_ao_quartets} unique AO quartets per Fock build; "
-                      f"J/K tensor lookups excluded"}
+    q = packed_quartets(n_ao)
+    t0 = time.perf_counter()
+    tensor = ctx.eri(q)
+    t_fill = time.perf_counter() - t0
+    del q
+    O.set_threads(1)
+    Ptot = 2.0 * Pa
+    t_j = t_k = 0.0
+    for _ in range(jk_iters):
+        t0 = time.perf_counter()
+        J = O.coulomb(tensor, n_ao, Ptot)
+        t1 = time.perf_counter()
+        K = O.exchange(tensor, n_ao, Pa)
+        t_j += t1 - t0
+        t_k += time.perf_counter() - t1
+    t_j /= jk_iters
+    t_k /= jk_iters
+    P = host_cores()
+    O.set_threads(P)
+    t0 = time.perf_counter()
+    for _ in range(jk_iters):
+        O.coulomb(tensor, n_ao, Ptot)
+        O.exchange(tensor, n_ao, Pa)
+    t_jk_p = (time.perf_counter() - t0) / jk_iters
+    O.set_threads(1)
+    n_unique = len(tensor)
+    tensor_s = None
+    if ref.available():
+        rm = ref.RefMolecule(os.path.join(ROOT, "dollarstorescf_amd", "basis", "6-31G"), Z, xyz)
+        rng = np.random.default_rng(seed)
+        qs = packed_quartets(n_ao)[rng.integers(0, n_unique, size=sample)]
+        t, _ = rm.time_eri_list(np.ascontiguousarray(qs, np.int32))
+        tensor_s = t / sample * n_unique
+    scale = max(1.0, float(np.max(np.abs(J_gpu))))
+    return {"value": 1.0 / (t_j + t_k), "unit": "Fock builds/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(), "host_cores": P,
+            "per_iteration_s": {"J": t_j, "K": t_k, "J+K": t_j + t_k},
+            "per_iteration_s_all_cores": t_jk_p,
+            "tensor_build_s_extrapolated": tensor_s,
+            "jk_vs_gpu_max_abs": {"J": float(np.max(np.abs(J - J_gpu))), "K": float(np.max(np.abs(K - K_gpu))),
+                                  "scale": scale},
+            "sample": f"per-iteration J+K: the reference's stored-tensor loops (ExactCoulomb.cpp:103-123, restated "
+                      f"in oracle/oracle.c) over the {n_unique}-element packed tensor of this system at the "
+                      f"converged density, mean of {jk_iters} on 1 core ({P}-thread figure beside it); one-off "
+                      f"tensor build: the reference's electronRepulsionIntegral (GaussianIntegral.cpp:561, "
+                      f"oracle/_ref) on {sample} random unique AO quartets, extrapolated to all {n_unique} "
+                      f"(labelled extrapolated); tensor filled from this engine's ERIs in {t_fill:.1f} s"}
 
 
 def traffic_of(kernel: str):
@@ -90,6 +164,7 @@ def bench_alkane(a):
     Z, xyz = geometry.alkane(20)
     classes = ds.class_list()
     runs = {}
+    Pa_ref = None
     for mode, thr in (("fp64", 0.0), ("mixed", 1e-5)):
         job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=60, schwarz_threshold=1e-12,
                                     fp32_threshold=thr)
@@ -111,15 +186,33 @@ def bench_alkane(a):
         for _ in range(a.steps):
             ctx.jk_resident()
         ms = (time.perf_counter() - t0) / a.steps * 1e3
+        # per class, its kernels one after another on one stream (FP64 and FP32 lists summed)
+        per_cls = ctx.time_classes(iters=max(3, a.steps // 4))[:len(classes)]
+        # J/K of both precisions at one density: the all-FP64 run's converged one
+        if mode == "fp64":
+            Pa_ref = Pa
+        ctx.set_density(2.0 * Pa_ref, Pa_ref)
+        ctx.jk_resident()
+        J, K = ctx.get_jk(job.n_ao)
         cq = list(st.class_quartets)[:len(classes)]
         cp = list(st.class_prim_quartets)[:len(classes)]
+        cq32 = list(st.class_quartets_fp32)[:len(classes)]
         runs[mode] = {"fock_builds_per_s": 1e3 / ms, "ms_per_build": ms, "energy": energy, "iterations": iters,
                       "scf_seconds": t_scf, "construct_seconds": t_construct, "n_ao": job.n_ao,
                       "shell_quartets": int(st.n_quartets), "fp32_shell_quartets": int(st.n_quartets_fp32),
                       "prim_quartets": int(st.n_prim_quartets),
                       "fock_build_tflops": flops.fock_build_flops(classes, cq, cp) / (ms * 1e-3) / 1e12,
-                      "history": [float(x) for x in job.history()]}
+                      "history": [float(x) for x in job.history()],
+                      "_cls": per_cls, "_q": cq, "_q32": cq32, "_jk": (J, K)}
     h0, h1 = runs["fp64"].pop("history"), runs["mixed"].pop("history")
+    c64, c32 = runs["fp64"].pop("_cls"), runs["mixed"].pop("_cls")
+    q64, q32 = runs["fp64"].pop("_q"), runs["mixed"].pop("_q32")
+    runs["mixed"].pop("_q")
+    runs["fp64"].pop("_q32")
+    (J64, K64), (J32, K32) = runs["fp64"].pop("_jk"), runs["mixed"].pop("_jk")
+    per_class = [{"class": "".join("spdf"[l] for l in c), "quartets": int(q), "fp32_quartets": int(f),
+                  "fp64_ms": float(m64), "mixed_ms": float(m32)}
+                 for c, q, f, m64, m32 in zip(classes, q64, q32, c64, c32) if q > 0]
     out = {
         "metric": METRIC, "value": runs["mixed"]["fock_builds_per_s"], "unit": "Fock builds/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": runs["mixed"]["ms_per_build"],
@@ -130,6 +223,10 @@ def bench_alkane(a):
         "fp64": runs["fp64"], "mixed": runs["mixed"],
         "final_dE": runs["mixed"]["energy"] - runs["fp64"]["energy"],
         "per_iteration_dE": [b - c for b, c in zip(h1, h0)],
+        "jk_max_abs_diff_at_fp64_density": {"J": float(np.max(np.abs(J32 - J64))),
+                                            "K": float(np.max(np.abs(K32 - K64))),
+                                            "max_abs_J": float(np.max(np.abs(J64)))},
+        "classes": per_class,
     }
     print(json.dumps(out))
 
@@ -316,13 +413,15 @@ def serve_requests(n_req: int, rate: float, seed: int = 7):
     return arrivals, out
 
 
-def serve_stream(ds, arrivals, reqs, fp32: float, max_active: int, budget: float, device_steps: bool = False):
+def serve_stream(ds, arrivals, reqs, fp32: float, max_active: int, budget: float, device_steps: bool = False,
+                 conv: float = 1e-8):
     """Replay the stream in real time against one server; returns per-request
     latency (finish - scheduled arrival), energies, status and the wall time
-    from the first arrival to the last finish."""
+    from the first arrival to the last finish.  conv: the jobs' |dE_elec|
+    stopping threshold."""
     jobs = []
     for basis, Z, xyz in reqs:
-        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12,
+        job = ds.Job(0).set_options(diis_size=8, energy_threshold=conv, max_iterations=100, schwarz_threshold=1e-12,
                                     fp32_threshold=fp32)
         jobs.append(job.set_basis(basis).set_molecule(Z, xyz))
     srv = ds.Server(0, budget_bytes=budget, max_active=max_active, workers=12, device_steps=device_steps)
@@ -402,7 +501,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-sample", type=int, default=5_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--boys", choices=["accurate", "reference"], default="accurate")
     ap.add_argument("--workload", choices=["water8", "alkane", "batch", "serve", "scf", "gradient"], default="water8",
@@ -513,8 +612,10 @@ def main():
 
     per_class = class_rows(kernel_ms)
     per_class_serial = class_rows(serial_ms)
-    n_unique_ao = (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2
-    cpu = None if a.no_cpu else cpu_baseline(Z, xyz, n, n_unique_ao, a.cpu_sample)
+    cpu = None
+    if not a.no_cpu:
+        J_gpu, K_gpu = ctx.get_jk(n)
+        cpu = cpu_baseline(ctx, Z, xyz, n, Pa, J_gpu, K_gpu, a.cpu_sample)
 
     out = {
         "metric": METRIC,
@@ -531,8 +632,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded water lattice geometry)",
         "config": {"workload": "RHF/6-31G Fock build, 8-water 2x2x2 lattice (3.0 A, +-0.3 A jitter, seed 1), "
-                               "104 AOs, Schwarz 1e-12, FP64",
-                   "boys": a.boys, "n_ao": n, "shell_quartets": int(st.n_quartets),
+                               "104 AOs, Schwarz 1e-12, primitive-pair screening 1e-17 (accurate Boys), FP64",
+                   "boys": a.boys, "prim_threshold": ds.DEFAULT_PRIM_THRESHOLD if a.boys == "accurate" else 0.0,
+                   "n_ao": n, "shell_quartets": int(st.n_quartets),
                    "shell_quartets_unscreened": int(st.n_quartets_unscreened),
                    "prim_quartets": int(st.n_prim_quartets), "parallelism": "single GPU",
                    "block_sizes": {"".join("spd"[l] for l in c): int(b) for c, b, q in zip(classes, tuned, cq)
@@ -541,7 +643,7 @@ def main():
         "eri_shell_quartets_per_s": st.n_quartets * builds_per_s,
         "eri_prim_quartets_per_s": st.n_prim_quartets * builds_per_s,
         "fock_build_tflops": total_flops / (ms * 1e-3) / 1e12,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_of(f"k_jk<{name}>"),
                      "kernel": f"k_jk<{name}>", "kernel_ms": ms_dom,
                      "note": "FP64 vector-ALU bound; peak is MI355X FP64 78.6 TFLOP/s; achieved = algorithmic "

@@ -166,6 +166,11 @@ class Server:
     __del__ = close
 
     def submit(self, job, tag: int):
+        """Queue ``job`` under ``tag``.  A tag still in flight is refused (the
+        server keeps the job alive through it), and so is a job the server
+        already holds (DSSCF_ERR_STATE from the library)."""
+        if tag in self._jobs:
+            raise DsscfError(1, f"Server.submit: tag {tag} is still in flight")
         self._jobs[tag] = job
         rc = lib().dsscf_server_submit(self._h, job._h, tag)
         if rc:

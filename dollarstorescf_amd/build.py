@@ -71,27 +71,54 @@ def class_units(maxl: int):
     return units
 
 
-def build(maxl: int = DEFAULT_MAXL, jobs: int = 8, verbose: bool = False) -> str:
+def _newest(paths):
+    return max((os.path.getmtime(p) for p in paths if os.path.exists(p)), default=0.0)
+
+
+def _deps(src: str):
+    """Files a unit may include: the headers and generated tables of csrc/
+    (and csrc/host/ for host units) and include/."""
+    d = os.path.dirname(src)
+    out = [src] + [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    for folder in {d, CSRC, GEN}:
+        out += [os.path.join(folder, f) for f in os.listdir(folder) if f.endswith((".hpp", ".inc", ".h"))]
+    return out
+
+
+def build(maxl: int = DEFAULT_MAXL, jobs: int = 8, verbose: bool = False, incremental: bool = False) -> str:
     os.makedirs(GEN, exist_ok=True)
     os.makedirs(OBJ, exist_ok=True)
-    _run([sys.executable, os.path.join(CSRC, "gen_eri.py"), os.path.join(GEN, "eri_classes.inc"), str(maxl)])
-    _run([sys.executable, os.path.join(CSRC, "gen_grad.py"), os.path.join(GEN, "grad_classes.inc")])
+    for gen, inc, extra in (("gen_eri.py", "eri_classes.inc", [str(maxl)]), ("gen_grad.py", "grad_classes.inc", [])):
+        out = os.path.join(GEN, inc)
+        _run([sys.executable, os.path.join(CSRC, gen), out + ".new", *extra])
+        # replace only on change, so that unchanged tables keep their time stamp
+        if os.path.exists(out) and open(out).read() == open(out + ".new").read():
+            os.remove(out + ".new")
+        else:
+            os.replace(out + ".new", out)
     cmds = []
     objs = []
     # the largest classes (last in class order) first, so that they start at once
     hip_units = class_units(maxl)[::-1] + [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    def add(src, o, cmd):
+        objs.append(o)
+        # incremental (a development aid): skip units whose object is newer
+        # than every file they may include
+        if incremental and os.path.exists(o) and os.path.getmtime(o) > _newest(_deps(src)):
+            return
+        cmds.append(cmd)
+
     for src in hip_units:
         o = os.path.join(OBJ, os.path.basename(src) + ".o")
-        cmds.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", src, "-o", o])
-        objs.append(o)
+        add(src, o, [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", src, "-o", o])
     for s in HOST_SOURCES:
         o = os.path.join(OBJ, os.path.basename(s) + ".o")
-        cmds.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
-        objs.append(o)
+        src = os.path.join(CSRC, s)
+        add(src, o, ["g++", *CXX_FLAGS, "-c", src, "-o", o])
     # the AVX2/FMA build of the host eigensolver and products (chosen at run time)
     o = os.path.join(OBJ, "linalg_avx2.cpp.o")
-    cmds.append(["g++", *CXX_FLAGS, "-DLA_AVX2", "-mavx2", "-mfma", "-c", os.path.join(CSRC, "host/linalg.cpp"), "-o", o])
-    objs.append(o)
+    src = os.path.join(CSRC, "host/linalg.cpp")
+    add(src, o, ["g++", *CXX_FLAGS, "-DLA_AVX2", "-mavx2", "-mfma", "-c", src, "-o", o])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for r in ex.map(_run, cmds):
             if verbose and (r.stdout or r.stderr):
@@ -107,8 +134,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--maxl", type=int, default=DEFAULT_MAXL)
     ap.add_argument("--jobs", type=int, default=8)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only units older than a file they may include")
     a = ap.parse_args()
-    print(build(a.maxl, a.jobs, verbose=True))
+    print(build(a.maxl, a.jobs, verbose=True, incremental=a.incremental))
 
 
 if __name__ == "__main__":

@@ -240,14 +240,16 @@ __device__ __forceinline__ double boys_reference_m(double x, unsigned m)
     return exp(-x) / (double)(2 * m + 1) * F;
 }
 
-// 1/sqrt(x) for x > 0: hardware estimate (v_rsq_f64) refined by two Newton
-// steps r <- r (3 - x r^2)/2, each of which doubles the correct bits.
+// 1/sqrt(x) for x > 0: the hardware estimate (v_rsq_f64, relative error
+// below 2^-24 on gfx950, measured over 2^20 arguments in [2^-20, 2^20])
+// refined by one third-order (Halley) step: with e = 1 - x r^2,
+// r <- r (1 + e/2 + 3e^2/8), which leaves about e^3 ~ 2^-72 before rounding
+// (two Newton steps r <- r (3 - x r^2)/2 cost two more instructions).
 __device__ __forceinline__ double rsqrt_nr(double x)
 {
-    double r = __builtin_amdgcn_rsq(x);
-    r = r * fma(-0.5 * x * r, r, 1.5);
-    r = r * fma(-0.5 * x * r, r, 1.5);
-    return r;
+    const double r = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * r, r, 1.0);
+    return fma(r * e, fma(e, 0.375, 0.5), r);
 }
 
 // Accurate F_0..F_L: 7-term Taylor expansion of F_L about the nearest grid
@@ -373,14 +375,30 @@ struct PrimQuartet {
 // Per-primitive-quartet quantities (the per-primitive block of
 // GaussianIntegral.cpp:577-595 in HGP form): W-P, W-Q, 1/2p, 1/2q,
 // 1/2(p+q), rho/p, rho/q and prefactor-scaled Boys values.
-// FP32 form for low-bound quartets: setup, small-T Boys (same table, Taylor
-// and downward recursion in float) and the values handed to the VRR in FP32;
-// the large-T Boys branch stays FP64.
+// FP32 form for low-bound quartets: setup, the small-T Boys interpolation
+// (Taylor polynomial of F_L and e^-T in float) and the values handed to the
+// VRR in FP32; the downward recursion and the large-T Boys branch run in
+// FP64.
 // FP32 interval table in LDS (k_jk stages it from the FP64 interval table of
 // m = L): per interval 8 floats, e^-c and the Taylor coefficients a_0..a_6
 // of F_L about c; |u| <= 1/4 leaves a truncation below 1.2e-8 relative,
 // under FP32 rounding.
 #define DSSCF_BPOLY32_COEF 8
+
+// FP32 lists: the downward recursion F_m = (2T F_{m+1} + e^-T)/(2m+1) from
+// the FP32 F_L and e^-T runs in FP64 (as the large-T branch does), and only
+// its results are rounded to FP32 for the VRR.
+template <int L>
+__device__ __forceinline__ void boys_down_f64(double T, double eT, float* Fm)
+{
+    double F = (double)Fm[L];
+    const double T2 = 2.0 * T;
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) {
+        F = fma(T2, F, eT) * (1.0 / (2 * m + 1));
+        Fm[m] = (float)F;
+    }
+}
 
 template <int L, bool kLds>
 __device__ __forceinline__ void prim_setup_f32(const PrimPair& P, const PrimPair& Q, const ShellPair& bra,
@@ -417,10 +435,7 @@ __device__ __forceinline__ void prim_setup_f32(const PrimPair& P, const PrimPair
             e = fmaf(e, -u, 0.5f);
             e = fmaf(e, -u, 1.0f);
             e = fmaf(e, -u, 1.0f);
-            const float eT = c0.x * e;
-            const float T2 = 2.0f * T;
-#pragma unroll
-            for (int m = L - 1; m >= 0; --m) Fm[m] = fmaf(T2, Fm[m + 1], eT) * (1.0f / (2 * m + 1));
+            boys_down_f64<L>((double)T, (double)(c0.x * e), Fm);
         }
     } else if (!kLds && T < (float)DSSCF_BOYS_TMAX) {
         const int i = (int)(T * (float)DSSCF_BOYS_INV_H + 0.5f);
@@ -438,10 +453,7 @@ __device__ __forceinline__ void prim_setup_f32(const PrimPair& P, const PrimPair
             e = fmaf(e, nd, 0.5f);
             e = fmaf(e, nd, 1.0f);
             e = fmaf(e, nd, 1.0f);
-            const float eT = (float)row[0] * e;
-            const float T2 = 2.0f * T;
-#pragma unroll
-            for (int m = L - 1; m >= 0; --m) Fm[m] = fmaf(T2, Fm[m + 1], eT) * (1.0f / (2 * m + 1));
+            boys_down_f64<L>((double)T, row[0] * (double)e, Fm);
         }
     } else {
         double F[L + 1];
@@ -512,10 +524,7 @@ __device__ __forceinline__ void prim_setup(const PrimPairF& P, const PrimPair& Q
             e = fmaf(e, -u, 0.5f);
             e = fmaf(e, -u, 1.0f);
             e = fmaf(e, -u, 1.0f);
-            const float eT = c0.x * e;
-            const float T2 = 2.0f * T;
-#pragma unroll
-            for (int m = L - 1; m >= 0; --m) Fm[m] = fmaf(T2, Fm[m + 1], eT) * (1.0f / (2 * m + 1));
+            boys_down_f64<L>((double)T, (double)(c0.x * e), Fm);
         }
     } else {
         double F[L + 1];

@@ -110,7 +110,7 @@ struct Plan {
     int boys_mode = DSSCF_BOYS_ACCURATE;
     const double* btab = nullptr;
     int ncls = 0;
-    std::vector<int> cls_nseg, cls_nblocks, cls_off, cls_bs, cls_maxN;
+    std::vector<int> cls_nseg, cls_nblocks, cls_off, cls_bs, cls_maxN, cls_maxbra;
     int nmat = 0, total_rows = 0, maxN = 0;
     std::vector<JKSeg> h_segs;
     std::vector<int> h_blk0;
@@ -214,6 +214,7 @@ struct dsscf_ctx {
     // thread-per-quartet classes: work units (bra pair, first ket, count) into `kets`,
     // per class [u_off, u_off+u_cnt) FP64 and [u_off32, u_off32+u_cnt32) FP32
     std::vector<long long> u_off, u_cnt, u_off32, u_cnt32;
+    std::vector<int> u_maxbra, u_maxbra32;  // per class: most primitive pairs of a unit's bra
     long long n_unscreened = 0;
     long long n_work_pairs = 0, n_work_items = 0;  // shell pairs + sub-pairs; (sub-)quartets
     int n_slice = 1;                                // copies of Jh/Kh
@@ -591,6 +592,8 @@ int dsscf_prepare(dsscf_ctx* c)
     };
 
     c->chunk.assign(ncls, 512);
+    c->u_maxbra.assign(ncls, 0);
+    c->u_maxbra32.assign(ncls, 0);
     for (auto* v : {&c->q_off, &c->q_cnt, &c->q_prim, &c->q_off32, &c->q_cnt32, &c->q_prim32, &c->u_off, &c->u_cnt,
                     &c->u_off32, &c->u_cnt32})
         v->assign(ncls, 0);
@@ -833,6 +836,10 @@ int dsscf_prepare(dsscf_ctx* c)
         };
         for (size_t k = (size_t)u0; k < units.size(); ++k) set_group(units[k]);
         for (int4& u : units32) set_group(u);
+        c->u_maxbra[cls] = c->u_maxbra32[cls] = 0;
+        for (size_t k = (size_t)u0; k < units.size(); ++k)
+            c->u_maxbra[cls] = std::max(c->u_maxbra[cls], work[units[k].x].nprim);
+        for (const int4& u : units32) c->u_maxbra32[cls] = std::max(c->u_maxbra32[cls], work[u.x].nprim);
         c->u_off[cls] = u0;
         c->u_cnt[cls] = (long long)units.size() - u0;
         c->u_off32[cls] = (long long)units.size();
@@ -977,6 +984,7 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     p.cls_off.assign(nslot, 0);
     p.cls_bs.assign(nslot, 256);
     p.cls_maxN.assign(nslot, 0);
+    p.cls_maxbra.assign(nslot, 0);
     p.cls_cost.assign(nslot, 0.0);
     p.cls_stream.assign(nslot, 0);
     p.h_segs.clear();
@@ -1005,6 +1013,7 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
             sg.u = c->d_units + (prec32 ? c->u_off32[cls] : c->u_off[cls]);
             sg.k = c->d_kets;
             slot_maxN = std::max(slot_maxN, c->n_ao);
+            if (!wave) p.cls_maxbra[slot] = std::max(p.cls_maxbra[slot], prec32 ? c->u_maxbra32[cls] : c->u_maxbra[cls]);
             sg.pairs = c->d_pairs;
             sg.pp = c->d_pp;
             sg.Dt = c->d_Dt;
@@ -1038,7 +1047,8 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
         if (!wave && p.cls_nseg[slot]) {
             int l4[4];
             dk_class_l(cls, l4);
-            const size_t lds = sizeof(double) * (size_t)(2 * l4[0] + 2 * l4[1] + 2) * slot_maxN;
+            const size_t lds = sizeof(double) * ((size_t)(2 * l4[0] + 2 * l4[1] + 2) * slot_maxN + 1) +
+                               sizeof(PrimPair) * (size_t)p.cls_maxbra[slot];
             if (lds > 64 * 1024) return fail(DSSCF_ERR_UNSUPPORTED, "basis too large for the LDS exchange rows");
         }
     }
@@ -1137,8 +1147,8 @@ int plan_launch_slot(Plan& p, int slot, hipStream_t s)
 {
     if (p.cls_nseg[slot] == 0) return DSSCF_OK;
     HIP_TRY(dsscf_launch_jk(slot / 2, p.boys_mode, slot % 2, p.d_segs + p.cls_off[slot], p.d_blk0 + p.cls_off[slot],
-                            p.cls_nseg[slot], p.cls_nblocks[slot], p.cls_bs[slot], p.cls_maxN[slot], p.btab,
-                            kEriScale, s));
+                            p.cls_nseg[slot], p.cls_nblocks[slot], p.cls_bs[slot], p.cls_maxN[slot],
+                            p.cls_maxbra[slot], p.btab, kEriScale, s));
     return DSSCF_OK;
 }
 
@@ -1210,15 +1220,20 @@ int plan_enqueue(Plan& p, hipStream_t s, std::vector<hipEvent_t>* ev, bool timed
 int plan_run(Plan& p, hipStream_t s, bool use_graph, std::vector<float>* cls_ms)
 {
     if (cls_ms) {
-        std::vector<hipEvent_t> ev(p.ncls + 1);
-        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-        int rc = plan_enqueue(p, s, &ev);
-        if (!rc) {
-            HIP_TRY(hipEventSynchronize(ev[p.ncls]));
+        // every event created here is destroyed on every path
+        std::vector<hipEvent_t> ev(p.ncls + 1, nullptr);
+        hipError_t e = hipSuccess;
+        for (auto& x : ev)
+            if (e == hipSuccess) e = hipEventCreate(&x);
+        int rc = e == hipSuccess ? plan_enqueue(p, s, &ev) : DSSCF_OK;
+        if (e == hipSuccess && !rc) {
+            e = hipEventSynchronize(ev[p.ncls]);
             cls_ms->assign(p.ncls, 0.0f);
-            for (int k = 0; k < p.ncls; ++k) HIP_TRY(hipEventElapsedTime(&(*cls_ms)[k], ev[k], ev[k + 1]));
+            for (int k = 0; k < p.ncls && e == hipSuccess; ++k) e = hipEventElapsedTime(&(*cls_ms)[k], ev[k], ev[k + 1]);
         }
-        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (auto& x : ev)
+            if (x) (void)hipEventDestroy(x);
+        if (e != hipSuccess) return fail(DSSCF_ERR_HIP, std::string("plan_run (timed): ") + hipGetErrorString(e));
         return rc;
     }
     if (!use_graph) return plan_enqueue(p, s, nullptr);
@@ -1590,24 +1605,29 @@ int dsscf_get_stats(dsscf_ctx* c, dsscf_stats* st)
     return DSSCF_OK;
 }
 
-int dsscf_eri_batch(dsscf_ctx* c, const int* ijkl, long long n, double* out)
+}  // extern "C"
+
+namespace {
+
+// dsscf_eri_batch and its no-alloc form: map every request onto (bra pair,
+// ket pair, AO component of the block), evaluate each class's unique shell
+// quartets as whole AO blocks, pick the requested components.  staging null:
+// the class's item list and block outputs go into buffers allocated per
+// class; otherwise into staging (at most staging_bytes, checked first).
+int eri_batch_impl(dsscf_ctx* c, const int* ijkl, long long n, double* out, void* staging, size_t staging_bytes,
+                   const char* who)
 {
-    if (!c || (n > 0 && (!ijkl || !out)) || n < 0) return fail(DSSCF_ERR_ARG, "dsscf_eri_batch: bad argument");
-    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_eri_batch: call dsscf_prepare first");
-    if (n == 0) return DSSCF_OK;
-    HIP_TRY(hipSetDevice(c->device));
     const int ncls = dk_n_class();
     struct Req {
         long long t;
         int bra, ket, comp;
     };
-    // Map every request onto (bra pair, ket pair, AO component in the block).
     std::vector<std::vector<Req>> per_cls(ncls);
     for (long long t = 0; t < n; ++t) {
         int ao[4];
         for (int k = 0; k < 4; ++k) {
             ao[k] = ijkl[4 * t + k];
-            if (ao[k] < 0 || ao[k] >= c->n_ao) return fail(DSSCF_ERR_ARG, "dsscf_eri_batch: AO index out of range");
+            if (ao[k] < 0 || ao[k] >= c->n_ao) return fail(DSSCF_ERR_ARG, std::string(who) + ": AO index out of range");
         }
         int pidx[2], ca[2], cb[2], nb[2];
         for (int h = 0; h < 2; ++h) {
@@ -1639,30 +1659,55 @@ int dsscf_eri_batch(dsscf_ctx* c, const int* ijkl, long long n, double* out)
         const int comp = (ca[bra] * nb[bra] + cb[bra]) * nket + ca[ket] * nb[ket] + cb[ket];
         per_cls[cls].push_back(Req{t, pidx[bra], pidx[ket], comp});
     }
+    std::vector<std::vector<long long>> keys(ncls), uniq(ncls);
+    std::vector<int> nblock(ncls, 0);
     for (int cls = 0; cls < ncls; ++cls) {
         auto& rq = per_cls[cls];
         if (rq.empty()) continue;
         int l4[4];
         dk_class_l(cls, l4);
-        const int nblock = (2 * l4[0] + 1) * (2 * l4[1] + 1) * (2 * l4[2] + 1) * (2 * l4[3] + 1);
+        nblock[cls] = (2 * l4[0] + 1) * (2 * l4[1] + 1) * (2 * l4[2] + 1) * (2 * l4[3] + 1);
         // unique shell quartets of this class
-        std::vector<long long> key(rq.size());
+        auto& key = keys[cls];
+        key.resize(rq.size());
         for (size_t i = 0; i < rq.size(); ++i) key[i] = (long long)rq[i].bra * (long long)c->pairs.size() + rq[i].ket;
-        std::vector<long long> uk = key;
+        auto& uk = uniq[cls];
+        uk = key;
         std::sort(uk.begin(), uk.end());
         uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
+        if (staging) {
+            const size_t need = staging_offset_out(uk.size()) + sizeof(double) * uk.size() * nblock[cls];
+            if (need > staging_bytes)
+                return fail(DSSCF_ERR_ARG, std::string(who) + ": class " + std::to_string(cls) + " needs " +
+                                               std::to_string(need) + " bytes of staging for " +
+                                               std::to_string(uk.size()) + " shell quartets, more than the " +
+                                               std::to_string(staging_bytes) +
+                                               "-byte scratch: split the request (or call dsscf_eri_batch)");
+        }
+    }
+    for (int cls = 0; cls < ncls; ++cls) {
+        auto& rq = per_cls[cls];
+        if (rq.empty()) continue;
+        const auto& uk = uniq[cls];
+        const int nb = nblock[cls];
         std::vector<int4> items(uk.size());
         for (size_t u = 0; u < uk.size(); ++u)
             items[u] = make_int4((int)(uk[u] / (long long)c->pairs.size()), (int)(uk[u] % (long long)c->pairs.size()),
-                                 (int)(u * nblock), 0);
+                                 (int)(u * nb), 0);
         int4* d_items = nullptr;
         double* d_out = nullptr;
-        if (int rc = dev_alloc(&d_items, items.size())) return rc;
-        if (int rc = dev_alloc(&d_out, items.size() * nblock)) {
-            dev_free(d_items);
-            return rc;
+        if (staging) {
+            d_items = (int4*)staging;
+            d_out = (double*)((char*)staging + staging_offset_out(items.size()));
+        } else {
+            if (int rc = dev_alloc(&d_items, items.size())) return rc;
+            if (int rc = dev_alloc(&d_out, items.size() * nb)) {
+                dev_free(d_items);
+                return rc;
+            }
         }
-        std::vector<double> blk(items.size() * nblock);
+        std::vector<double> blk(items.size() * nb);
+        // every block's nb components are written by the kernel: nothing to clear
         hipError_t e = hipMemcpyAsync(d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess)
             e = dsscf_launch_eri_block(cls, c->opt.boys_mode, d_items, (int)items.size(), c->d_pairs, c->d_pp, c->d_btab,
@@ -1670,15 +1715,30 @@ int dsscf_eri_batch(dsscf_ctx* c, const int* ijkl, long long n, double* out)
         if (e == hipSuccess)
             e = hipMemcpyAsync(blk.data(), d_out, sizeof(double) * blk.size(), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        dev_free(d_items);
-        dev_free(d_out);
-        if (e != hipSuccess) return fail(DSSCF_ERR_HIP, std::string("dsscf_eri_batch: ") + hipGetErrorString(e));
+        if (!staging) {
+            dev_free(d_items);
+            dev_free(d_out);
+        }
+        if (e != hipSuccess) return fail(DSSCF_ERR_HIP, std::string(who) + ": " + hipGetErrorString(e));
         for (size_t i = 0; i < rq.size(); ++i) {
-            const size_t u = std::lower_bound(uk.begin(), uk.end(), key[i]) - uk.begin();
-            out[rq[i].t] = blk[u * nblock + rq[i].comp];
+            const size_t u = std::lower_bound(uk.begin(), uk.end(), keys[cls][i]) - uk.begin();
+            out[rq[i].t] = blk[u * nb + rq[i].comp];
         }
     }
     return DSSCF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dsscf_eri_batch(dsscf_ctx* c, const int* ijkl, long long n, double* out)
+{
+    if (!c || (n > 0 && (!ijkl || !out)) || n < 0) return fail(DSSCF_ERR_ARG, "dsscf_eri_batch: bad argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_eri_batch: call dsscf_prepare first");
+    if (n == 0) return DSSCF_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    return eri_batch_impl(c, ijkl, n, out, nullptr, 0, "dsscf_eri_batch");
 }
 
 }  // extern "C"

@@ -18,6 +18,7 @@
 
 #include "eri_classes.inc"
 
+
 template <int la, int lb, int lc, int ld, int BM, typename R>
 struct Quartet;
 #define DSSCF_QSPEC_1(a, b, c, d, name)
@@ -56,7 +57,7 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // Work unit = one bra shell pair and a chunk of its ket pairs (bra-major):
 // the bra record and its primitive pairs are uniform across the block; the
 // Jh[a][b] sums are reduced across the block, the Kh rows of shells a and b
-// are accumulated in LDS and flushed once per block; only Jh[c][d] leaves
+// are accumulated in LDS and flushed once per unit; only Jh[c][d] leaves
 // per quartet: as a plain store into the quartet's slot record, which k_jcd
 // sums per element after all classes (an atomic add per quartet and element
 // serializes at the few hundred addresses a small molecule's J has), or,
@@ -65,33 +66,29 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // primitives) << 8: a group of gb*gk lanes splits a primitive-heavy
 // quartet's primitive loops, and the group's first lane digests the summed
 // integrals.
+//
+// Block b takes units b, b + gridDim, b + 2 gridDim, ... (the list is
+// heaviest first; launches use one block per unit, measured as fast as a
+// grid of two resident blocks per CU that loop, and faster than blocks
+// taking units from an atomic counter, whose same-address adds serialize).
+//
+// Dynamic LDS: the Kh rows [(NA + NB) * maxN doubles, rounded to an even
+// count], then the bra's primitive records (at most maxBra of them, sized by
+// the launch rather than for the largest possible shell pair, so that more
+// blocks fit on a CU).
 template <int la, int lb, int lc, int ld, int BM, typename R>
 __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
-                                            int nseg, const double* __restrict__ btab, double scale)
+                                            int nseg, int nunits, int maxN,
+                                            const double* __restrict__ btab, double scale)
 {
     typedef ClassInfo<la, lb, lc, ld> CI;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
-    extern __shared__ double krow[];  // [(NA + NB) * N]: Kh rows of shell a, then shell b
     // bra primitive records: FP64 lists PrimPair, FP32 lists (accurate mode) PrimPairF
     constexpr bool kF32Rec = sizeof(R) == 4 && BM == BOYS_ACCURATE;
     typedef typename std::conditional<kF32Rec, PrimPairF, PrimPair>::type BPT;
-    __shared__ BPT braP[kBraStage];
-    const int si = find_segment_by_block(seg_blk0, nseg, (int)blockIdx.x);
-    const JKSeg& S = segs[si];
-    const int4 u = ldg(S.u + ((int)blockIdx.x - S.blk0));  // bra pair, first ket, ket count
-    const ShellPair* __restrict__ pairs = S.pairs;
-    const PrimPair* __restrict__ pp = S.pp;
-    const GPtr<double> Dt = gbl(S.Dt);
-    const GPtr<double> Da = gbl(S.Da);  // read-only in this kernel
-    const int N = S.N;
-    const size_t slice = (size_t)jk_slice(S.nslice) * N * N;
-    double* __restrict__ Jh = S.Jh + slice;
-    double* __restrict__ Kh = S.Kh + slice;
-    const GPtr<unsigned> kets = gbl(S.k);
-    // J_cd: per-quartet slot records (summed by k_jcd) or atomics
-    const int slot0 = S.js ? gbl(S.us)[(int)blockIdx.x - S.blk0] : 0;
-    const ShellPair bra = ldg(pairs + u.x);
-    const int a0 = bra.ao_a, b0 = bra.ao_b;
+    extern __shared__ __align__(16) double lds_dyn[];
+    double* krow = lds_dyn;  // Kh rows of shell a, then shell b
+    BPT* braP = reinterpret_cast<BPT*>(lds_dyn + (((NA + NB) * maxN + 1) & ~1));
     // FP64 accurate mode: the Boys function from the block's LDS copy of the
     // interval table of m = L (a per-lane gather from the 801-row table
     // touches up to 64 cache lines per load)
@@ -102,8 +99,7 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     __shared__ __align__(16) double bpoly[kLds ? (k64 ? DSSCF_BPOLY_M_STRIDE : DSSCF_BPOLY_ROWS * DSSCF_BPOLY32_COEF / 2) : 1];
     if (kLds) {
         const GPtr<double> src = gbl(btab + DSSCF_BPOLY_OFFSET + CI::L * DSSCF_BPOLY_M_STRIDE);
-        // four loads in flight per thread: the copy is on every block's way
-        // to its first quartet
+        // four loads in flight per thread
         if (k64) {
 #pragma unroll 4
             for (int i = threadIdx.x; i < DSSCF_BPOLY_M_STRIDE; i += blockDim.x) bpoly[i] = src[i];
@@ -118,128 +114,171 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
         }
     }
     const double* __restrict__ tq = kLds ? bpoly : btab;
-    for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;
-    for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) {
-        PrimPair r = ldg(pp + bra.prim_off + i);
-        if (kLds)  // the LDS Boys mode's prim_setup takes P - A
-            for (int k = 0; k < 3; ++k) r.P[k] -= bra.A[k];
-        if constexpr (kF32Rec) {
-            PrimPairF f;
-            f.p = (float)r.p;
-            f.inv_p = (float)r.inv_p;
-            for (int k = 0; k < 3; ++k) f.PA[k] = (float)r.P[k];
-            f.Kp = (float)r.Kp;
-            f.pad[0] = f.pad[1] = 0.0f;
-            braP[i] = f;
-        } else {
-            braP[i] = r;
-        }
-    }
-    double dab[NA * NB];
-#pragma unroll
-    for (int i = 0; i < NA; ++i)
-#pragma unroll
-        for (int j = 0; j < NB; ++j) dab[i * NB + j] = Dt[(size_t)(a0 + i) * N + b0 + j];
-    double jab[NA * NB];
-#pragma unroll
-    for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
-    const double gb = scale * (a0 == b0 ? 0.5 : 1.0);
-    __syncthreads();
 
-    const int lgb = u.w & 0xff, lgk = (u.w >> 8) & 0xff, gl = lgb * lgk;  // lanes per quartet (power of two)
-    const int lane = (int)threadIdx.x & (gl - 1);
-    const int rb = lane & (lgb - 1), rk = lane / lgb;
-    for (int t = (int)threadIdx.x / gl; t < u.z; t += (int)blockDim.x / gl) {
-        const int ki = (int)kets[u.y + t];
-        const ShellPair ket = ldg(pairs + ki);
-        double v[CI::NAO];
-        Quartet<la, lb, lc, ld, BMq, R>::run(bra, ket, braP, S.pp + ket.prim_off, tq, v, lgb, rb, lgk, rk);
-        if (lane != 0) continue;
-        const int c0 = ket.ao_a, d0 = ket.ao_b;
-        // (ab|ab): the bra and ket (sub-)pairs belong to one shell pair
-        const bool same = ket.ao_a == a0 && ket.ao_b == b0;
-        const double g = gb * (c0 == d0 ? 0.5 : 1.0) * (same ? 0.5 : 1.0);
-#define VIDX(i, j, k, l) ((((i) * NB + (j)) * NC + (k)) * ND + (l))
-        // Coulomb
-        const GOut<double> js = S.js ? gout(S.js) + gbl(S.jd)[slot0 + t] : nullptr;
+    for (int vb = (int)blockIdx.x; vb < nunits; vb += (int)gridDim.x) {
+        const int si = find_segment_by_block(seg_blk0, nseg, vb);
+        const JKSeg& S = segs[si];
+        const int4 u = ldg(S.u + (vb - S.blk0));  // bra pair, first ket, ket count
+        const ShellPair* __restrict__ pairs = S.pairs;
+        const PrimPair* __restrict__ pp = S.pp;
+        const GPtr<double> Dt = gbl(S.Dt);
+        const GPtr<double> Da = gbl(S.Da);  // read-only in this kernel
+        const int N = S.N;
+        const size_t slice = (size_t)jk_slice(S.nslice) * N * N;
+        double* __restrict__ Jh = S.Jh + slice;
+        double* __restrict__ Kh = S.Kh + slice;
+        const GPtr<unsigned> kets = gbl(S.k);
+        // J_cd: per-quartet slot records (summed by k_jcd) or atomics
+        const int slot0 = S.js ? gbl(S.us)[vb - S.blk0] : 0;
+        const ShellPair bra = ldg(pairs + u.x);
+        const int a0 = bra.ao_a, b0 = bra.ao_b;
+        for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;
+        for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) {
+            PrimPair r = ldg(pp + bra.prim_off + i);
+            if (kLds)  // the LDS Boys mode's prim_setup takes P - A
+                for (int k = 0; k < 3; ++k) r.P[k] -= bra.A[k];
+            if constexpr (kF32Rec) {
+                PrimPairF f;
+                f.p = (float)r.p;
+                f.inv_p = (float)r.inv_p;
+                for (int k = 0; k < 3; ++k) f.PA[k] = (float)r.P[k];
+                f.Kp = (float)r.Kp;
+                f.pad[0] = f.pad[1] = 0.0f;
+                braP[i] = f;
+            } else {
+                braP[i] = r;
+            }
+        }
+        double dab[NA * NB];
 #pragma unroll
-        for (int k = 0; k < NC; ++k)
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) dab[i * NB + j] = Dt[(size_t)(a0 + i) * N + b0 + j];
+        double jab[NA * NB];
+#pragma unroll
+        for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
+        const double gb = scale * (a0 == b0 ? 0.5 : 1.0);
+        __syncthreads();
+
+        const int lgb = u.w & 0xff, lgk = (u.w >> 8) & 0xff, gl = lgb * lgk;  // lanes per quartet (power of two)
+        const int lane = (int)threadIdx.x & (gl - 1);
+        const int rb = lane & (lgb - 1), rk = lane / lgb;
+        for (int t = (int)threadIdx.x / gl; t < u.z; t += (int)blockDim.x / gl) {
+            const int ki = (int)kets[u.y + t];
+            const ShellPair ket = ldg(pairs + ki);
+            const int c0 = ket.ao_a, d0 = ket.ao_b;
+            double v[CI::NAO];
+            Quartet<la, lb, lc, ld, BMq, R>::run(bra, ket, braP, S.pp + ket.prim_off, tq, v, lgb, rb, lgk, rk);
+            if (lane != 0) continue;
+            // density elements of the digestion: Dt[c][d], Da[b][d], Da[a][d], Da[b][c], Da[a][c]
+            // (loading them before the primitive loops, to overlap their latency,
+            // was measured slower: the registers held across the loops cost occupancy)
+            double dcd[NC * ND], dbd[NB * ND], dad[NA * ND], dbc[NB * NC], dac[NA * NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+#pragma unroll
+                for (int l = 0; l < ND; ++l) dcd[k * ND + l] = Dt[(size_t)(c0 + k) * N + d0 + l];
 #pragma unroll
             for (int l = 0; l < ND; ++l) {
-                const double dcd = Dt[(size_t)(c0 + k) * N + d0 + l];
-                double s = 0.0;
 #pragma unroll
-                for (int ij = 0; ij < NA * NB; ++ij) {
-                    const double x = v[ij * NC * ND + k * ND + l];
-                    jab[ij] += g * x * dcd;
-                    s += x * dab[ij];
+                for (int j = 0; j < NB; ++j) dbd[j * ND + l] = Da[(size_t)(b0 + j) * N + d0 + l];
+#pragma unroll
+                for (int i = 0; i < NA; ++i) dad[i * ND + l] = Da[(size_t)(a0 + i) * N + d0 + l];
+            }
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+#pragma unroll
+                for (int j = 0; j < NB; ++j) dbc[j * NC + k] = Da[(size_t)(b0 + j) * N + c0 + k];
+#pragma unroll
+                for (int i = 0; i < NA; ++i) dac[i * NC + k] = Da[(size_t)(a0 + i) * N + c0 + k];
+            }
+            // (ab|ab): the bra and ket (sub-)pairs belong to one shell pair
+            const bool same = ket.ao_a == a0 && ket.ao_b == b0;
+            const double g = gb * (c0 == d0 ? 0.5 : 1.0) * (same ? 0.5 : 1.0);
+#define VIDX(i, j, k, l) ((((i) * NB + (j)) * NC + (k)) * ND + (l))
+            // Coulomb
+            const GOut<double> js = S.js ? gout(S.js) + gbl(S.jd)[slot0 + t] : nullptr;
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+#pragma unroll
+                for (int l = 0; l < ND; ++l) {
+                    const double dk = dcd[k * ND + l];
+                    double s = 0.0;
+#pragma unroll
+                    for (int ij = 0; ij < NA * NB; ++ij) {
+                        const double x = v[ij * NC * ND + k * ND + l];
+                        jab[ij] += g * x * dk;
+                        s += x * dab[ij];
+                    }
+                    if (js)
+                        js[k * ND + l] = 2.0 * g * s;
+                    else
+                        atomic_add_f64(&Jh[(size_t)(c0 + k) * N + d0 + l], 2.0 * g * s);
                 }
-                if (js)
-                    js[k * ND + l] = 2.0 * g * s;
-                else
-                    atomic_add_f64(&Jh[(size_t)(c0 + k) * N + d0 + l], 2.0 * g * s);
-            }
-        // Exchange into the LDS rows
+            // Exchange into the LDS rows
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
+            for (int k = 0; k < NC; ++k) {
 #pragma unroll
-            for (int i = 0; i < NA; ++i) {  // Kh[a][c] += g sum_bd v D_bd
-                double s = 0.0;
+                for (int i = 0; i < NA; ++i) {  // Kh[a][c] += g sum_bd v D_bd
+                    double s = 0.0;
 #pragma unroll
-                for (int j = 0; j < NB; ++j)
+                    for (int j = 0; j < NB; ++j)
 #pragma unroll
-                    for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * Da[(size_t)(b0 + j) * N + d0 + l];
-                atomicAdd(&krow[i * N + c0 + k], g * s);
-            }
+                        for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * dbd[j * ND + l];
+                    atomicAdd(&krow[i * N + c0 + k], g * s);
+                }
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {  // Kh[b][c] += g sum_ad v D_ad
-                double s = 0.0;
+                for (int j = 0; j < NB; ++j) {  // Kh[b][c] += g sum_ad v D_ad
+                    double s = 0.0;
 #pragma unroll
-                for (int i = 0; i < NA; ++i)
+                    for (int i = 0; i < NA; ++i)
 #pragma unroll
-                    for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * Da[(size_t)(a0 + i) * N + d0 + l];
-                atomicAdd(&krow[(NA + j) * N + c0 + k], g * s);
-            }
-        }
-#pragma unroll
-        for (int l = 0; l < ND; ++l) {
-#pragma unroll
-            for (int i = 0; i < NA; ++i) {  // Kh[a][d] += g sum_bc v D_bc
-                double s = 0.0;
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-#pragma unroll
-                    for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * Da[(size_t)(b0 + j) * N + c0 + k];
-                atomicAdd(&krow[i * N + d0 + l], g * s);
+                        for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * dad[i * ND + l];
+                    atomicAdd(&krow[(NA + j) * N + c0 + k], g * s);
+                }
             }
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {  // Kh[b][d] += g sum_ac v D_ac
-                double s = 0.0;
+            for (int l = 0; l < ND; ++l) {
 #pragma unroll
-                for (int i = 0; i < NA; ++i)
+                for (int i = 0; i < NA; ++i) {  // Kh[a][d] += g sum_bc v D_bc
+                    double s = 0.0;
 #pragma unroll
-                    for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * Da[(size_t)(a0 + i) * N + c0 + k];
-                atomicAdd(&krow[(NA + j) * N + d0 + l], g * s);
+                    for (int j = 0; j < NB; ++j)
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * dbc[j * NC + k];
+                    atomicAdd(&krow[i * N + d0 + l], g * s);
+                }
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {  // Kh[b][d] += g sum_ac v D_ac
+                    double s = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NA; ++i)
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * dac[i * NC + k];
+                    atomicAdd(&krow[(NA + j) * N + d0 + l], g * s);
+                }
             }
-        }
 #undef VIDX
-    }
-    // Jh[a][b]: wavefront reduction, one atomic per wavefront and element
-#pragma unroll
-    for (int k = 0; k < NA * NB; ++k) {
-        double x = jab[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-        if ((threadIdx.x & 63) == 0 && x != 0.0) atomic_add_f64(&Jh[(size_t)(a0 + k / NB) * N + b0 + k % NB], 2.0 * x);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) {
-        const double x = krow[i];
-        if (x != 0.0) {
-            const int r = i / N, col = i % N;
-            const int row = r < NA ? a0 + r : b0 + (r - NA);
-            atomic_add_f64(&Kh[(size_t)row * N + col], x);
         }
+        // Jh[a][b]: wavefront reduction, one atomic per wavefront and element
+#pragma unroll
+        for (int k = 0; k < NA * NB; ++k) {
+            double x = jab[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+            if ((threadIdx.x & 63) == 0 && x != 0.0)
+                atomic_add_f64(&Jh[(size_t)(a0 + k / NB) * N + b0 + k % NB], 2.0 * x);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) {
+            const double x = krow[i];
+            if (x != 0.0) {
+                const int r = i / N, col = i % N;
+                const int row = r < NA ? a0 + r : b0 + (r - NA);
+                atomic_add_f64(&Kh[(size_t)row * N + col], x);
+            }
+        }
+        __syncthreads();  // the flush has read krow before the next unit clears it
     }
 }
 
@@ -312,8 +351,8 @@ void ClassLaunch<a, b, c, d, wave>::wave_go(const void* items, int n, const Shel
 // wave-cooperative classes one 64-lane block per WaveGroup::G quartets.
 template <int a, int b, int c, int d, int wave>
 hipError_t ClassLaunch<a, b, c, d, wave>::jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                                             int nblocks, int bs, int maxN, const double* btab, double scale,
-                                             hipStream_t s)
+                                             int nblocks, int bs, int maxN, int maxBra, const double* btab,
+                                             double scale, hipStream_t s)
 {
     if constexpr (wave) {
         if (bm == BOYS_ACCURATE)
@@ -323,17 +362,19 @@ hipError_t ClassLaunch<a, b, c, d, wave>::jk(int bm, int fp32, const JKSeg* segs
             hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_REFERENCE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
                                0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
     } else {
-        const dim3 blk(bs), grd(nblocks);
-        const size_t lds = sizeof(double) * (size_t)(ClassInfo<a, b, c, d>::NA + ClassInfo<a, b, c, d>::NB) * maxN;
+        typedef ClassInfo<a, b, c, d> CI;
+        const size_t rows = ((size_t)(CI::NA + CI::NB) * maxN + 1) & ~(size_t)1;
+        const size_t lds = sizeof(double) * rows + (fp32 ? sizeof(PrimPairF) : sizeof(PrimPair)) * (size_t)maxBra;
+        const dim3 blk(bs), grd(nblocks);  // one block per work unit
         if (fp32)
-            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, float>), grd, blk, lds, s, segs, blk0, nseg, btab,
-                               scale);
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, float>), grd, blk, lds, s, segs, blk0, nseg, nblocks,
+                               maxN, btab, scale);
         else if (bm == BOYS_ACCURATE)
-            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
-                               scale);
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, double>), grd, blk, lds, s, segs, blk0, nseg, nblocks,
+                               maxN, btab, scale);
         else
-            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE, double>), grd, blk, lds, s, segs, blk0, nseg, btab,
-                               scale);
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE, double>), grd, blk, lds, s, segs, blk0, nseg,
+                               nblocks, maxN, btab, scale);
     }
     return hipGetLastError();
 }

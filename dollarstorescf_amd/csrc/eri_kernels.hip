@@ -335,14 +335,15 @@ void dk_class_l(int c, int* l)
 }
 
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                           int nblocks, int block_size, int maxN, const double* btab, double scale, hipStream_t s)
+                           int nblocks, int block_size, int maxN, int maxBra, const double* btab, double scale,
+                           hipStream_t s)
 {
     if (nseg <= 0 || nblocks <= 0) return hipSuccess;
     int c = 0;
 #define DSSCF_LAUNCH_JK(a, b, cc, d, name, wave)                                                                 \
     if (c++ == cls)                                                                                              \
-        return ClassLaunch<a, b, cc, d, wave>::jk(boys_mode, fp32, segs, blk0, nseg, nblocks, block_size, maxN, btab, \
-                                                  scale, s);
+        return ClassLaunch<a, b, cc, d, wave>::jk(boys_mode, fp32, segs, blk0, nseg, nblocks, block_size, maxN,   \
+                                                  maxBra, btab, scale, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_JK)
 #undef DSSCF_LAUNCH_JK
     return hipErrorInvalidValue;

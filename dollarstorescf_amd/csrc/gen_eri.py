@@ -164,6 +164,64 @@ def gen_vrr(la, lb, lc, ld):
     return lines, targets
 
 
+def stmts_of(la, lb, lc, ld):
+    """VRR statements of gen_vrr as {key: [(coefficient, source key), ...]}
+    (None for the Boys base values)."""
+    E, Fk = la + lb, lc + ld
+    targets = [(e, f, 0) for le in range(la, E + 1) for e in cart(le)
+               for lf in range(lc, Fk + 1) for f in cart(lf)]
+    stmts = {}
+
+    def need(key):
+        if key in stmts:
+            return
+        e, f, m = key
+        if sum(f) > 0:
+            i = pick_dir(f)
+            f1 = sub1(f, i)
+            terms = [("QC%s" % XYZ[i], (e, f1, m)), ("WQ%s" % XYZ[i], (e, f1, m + 1))]
+            if f1[i] > 0:
+                f2 = sub1(f1, i)
+                terms.append(("%d.0*o2q" % f1[i], (e, f2, m)))
+                terms.append(("-%d.0*o2q*rq" % f1[i], (e, f2, m + 1)))
+            if e[i] > 0:
+                terms.append(("%d.0*o2pq" % e[i], (sub1(e, i), f1, m + 1)))
+        elif sum(e) > 0:
+            i = pick_dir(e)
+            e1 = sub1(e, i)
+            terms = [("PA%s" % XYZ[i], (e1, f, m)), ("WP%s" % XYZ[i], (e1, f, m + 1))]
+            if e1[i] > 0:
+                e2 = sub1(e1, i)
+                terms.append(("%d.0*o2p" % e1[i], (e2, f, m)))
+                terms.append(("-%d.0*o2p*rp" % e1[i], (e2, f, m + 1)))
+        else:
+            stmts[key] = None
+            return
+        stmts[key] = terms
+        for _, k in terms:
+            need(k)
+
+    for t in targets:
+        need(t)
+    return stmts
+
+
+def sources_of(la, lb, lc, ld):
+    """VRR entries that some other entry reads."""
+    out = set()
+    for t in stmts_of(la, lb, lc, ld).values():
+        if t:
+            out.update(k for _, k in t)
+    return out
+
+
+def _vrr_key(line):
+    """The (e, f, m) key a gen_vrr line defines."""
+    name = line.split("=")[0].split()[-1]  # v<e>_<f>_<m>
+    a, b, m = name[1:].split("_")
+    return (tuple(int(x) for x in a), tuple(int(x) for x in b), int(m))
+
+
 def acc_name(e, f):
     return "a%d%d%d_%d%d%d" % (e + f)
 
@@ -356,10 +414,27 @@ def gen_class(c, spherical=True):
     o.append("      const R o2p = g.o2p, o2q = g.o2q, o2pq = g.o2pq, rp = g.rp, rq = g.rq;")
     o.append("      (void)PAx; (void)PAy; (void)PAz; (void)WPx; (void)WPy; (void)WPz; (void)QCx; (void)QCy; (void)QCz;")
     o.append("      (void)WQx; (void)WQy; (void)WQz; (void)o2p; (void)o2q; (void)o2pq; (void)rp; (void)rq;")
+    # A target that no other VRR entry reads goes straight into its FP64
+    # accumulator as a chain of FMAs (one instruction per term, no separate
+    # sum and add); in the FP32 lists it is summed in R and converted.
+    fused = {k: t for k, t in stmts_of(la, lb, lc, ld).items() if k not in sources_of(la, lb, lc, ld) and t}
     for ln in vrr:
+        key = _vrr_key(ln)
+        if key in fused:
+            continue
         o.append("      " + ln)
     for (e, f) in accs:
-        o.append("      %s += (double)%s;" % (acc_name(e, f), vname(e, f, 0)))
+        key = (e, f, 0)
+        if key in fused:
+            terms = fused[key]
+            chain = acc_name(e, f)
+            for c, k in reversed(terms):
+                chain = "fma((double)(%s), (double)%s, %s)" % (c, vname(*k), chain)
+            expr = " + ".join("(%s)*%s" % (c, vname(*k)) for c, k in terms)
+            o.append("      if constexpr (sizeof(R) == 8) %s = %s;" % (acc_name(e, f), chain))
+            o.append("      else %s += (double)(%s);" % (acc_name(e, f), expr))
+        else:
+            o.append("      %s += (double)%s;" % (acc_name(e, f), vname(e, f, 0)))
     o.append("    }")
     o.append("  }")
     o.append("  for (int off = 1; off < gb * gk; off <<= 1) {")

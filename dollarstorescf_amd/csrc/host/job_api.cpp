@@ -35,6 +35,8 @@ int dsscf_job_create(dsscf_job** out, int device)
 
 int dsscf_job_destroy(dsscf_job* j)
 {
+    if (j && j->job.in_server.load())
+        return set_err(j, DSSCF_ERR_STATE, "dsscf_job_destroy: a server still holds the job (poll it first)");
     delete j;
     return DSSCF_OK;
 }
@@ -355,6 +357,7 @@ int dsscf_server_submit(dsscf_server* s, dsscf_job* j, long long tag)
 {
     if (!s || !j) return DSSCF_ERR_ARG;
     if (!j->have_basis || !j->job.molecule.size()) return set_err(j, DSSCF_ERR_STATE, "molecule and basis must be set first");
+    if (j->job.in_server.load()) return set_err(j, DSSCF_ERR_STATE, "dsscf_server_submit: the job is already held by a server");
     if (j->job.ctx) return set_err(j, DSSCF_ERR_STATE, "dsscf_server_submit: job already constructed");
     if (j->job.opt.device != s->s->device()) return set_err(j, DSSCF_ERR_ARG, "dsscf_server_submit: job on another device");
     return s->s->submit(j, &j->job, tag);

@@ -5,6 +5,7 @@
 // (SCFCommon.hpp:6-37); here it lives in a Job so several molecules can be
 // in flight, and the free functions act on the current job.
 #pragma once
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -84,6 +85,9 @@ struct Job {
     dsscf_ctx* ctx = nullptr;
     scfdev::Step* dstep = nullptr;  // device SCF step state (opt.device_step)
     std::string error;
+    // set while a Server holds the job (from submit until it is retired):
+    // a second submit and a destroy are refused meanwhile
+    std::atomic<bool> in_server{false};
 
     ~Job();
 };

@@ -44,9 +44,13 @@ Server::~Server()
     if (batch_) dsscf_batch_destroy(batch_);
     for (Req* r : active_) {
         free_device(*r->job);
+        r->job->in_server.store(false);
         delete r;
     }
-    for (Req* r : waiting_) delete r;
+    for (Req* r : waiting_) {
+        r->job->in_server.store(false);
+        delete r;
+    }
 }
 
 void Server::free_device(Job& job)
@@ -134,6 +138,11 @@ double Server::estimate_bytes(const Job& job) const
 int Server::submit(void* handle, Job* job, long long tag)
 {
     if (!job) return DSSCF_ERR_ARG;
+    bool expected = false;
+    if (!job->in_server.compare_exchange_strong(expected, true)) {
+        job->error = "dsscf_server_submit: the job is already held by a server";
+        return DSSCF_ERR_STATE;
+    }
     Req* r = new Req{handle, job, tag, HOST, 0, 0.0, 0.0, clock(), 0.0, 0.0};
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -168,6 +177,7 @@ void Server::retire_locked(Req* r, int status)
     r->t_done = clock();
     done_.push_back(Done{r->handle, r->tag, status, status ? 0.0 : r->job->energies.total(), (int)r->job->iteration,
                          r->t_submit, r->t_admit, r->t_done});
+    r->job->in_server.store(false);  // the caller may destroy or resubmit it from now on
     delete r;
 }
 

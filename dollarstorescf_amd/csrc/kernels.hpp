@@ -15,9 +15,13 @@ void dsscf_upload_xform(const double* x, int n, hipStream_t s);
 // ignored by the wave-cooperative classes, which use one 64-lane block per
 // quartet).
 // fp32: the FP32 form of a thread-per-quartet class (always accurate Boys).
-// maxN: largest AO count among the segments (sizes the LDS exchange rows).
+// maxN: largest AO count among the segments (sizes the LDS exchange rows);
+// maxBra: most primitive pairs of any unit's bra (sizes the LDS bra records);
+// nblocks: work units (thread-per-quartet classes) or blocks
+// (wave-cooperative classes, which ignore maxBra).
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                           int nblocks, int block_size, int maxN, const double* btab, double scale, hipStream_t s);
+                           int nblocks, int block_size, int maxN, int maxBra, const double* btab, double scale,
+                           hipStream_t s);
 int dsscf_class_is_wave(int cls);
 // Shell quartets per wavefront of a wave-cooperative class (1 for the others).
 int dsscf_wave_group(int cls);

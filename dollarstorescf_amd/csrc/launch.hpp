@@ -8,7 +8,7 @@
 template <int a, int b, int c, int d, int wave>
 struct ClassLaunch {
     static hipError_t jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int bs,
-                         int maxN, const double* btab, double scale, hipStream_t s);
+                         int maxN, int maxBra, const double* btab, double scale, hipStream_t s);
     static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
                             const double* btab, double* out, double scale, hipStream_t s);
     static hipError_t schwarz(int bm, const int* plist, int n, const ShellPair* pairs, const PrimPair* pp,

@@ -48,6 +48,7 @@ struct Step {
     double *scal = nullptr;                             // [0]: tr(Pa G), [1..kmax]: dots
     rocblas_int* info = nullptr;
     double* h_scal = nullptr;                           // pinned host mirror of scal
+    size_t blas_bytes = 0;                              // the handle's workspace (sized at creation)
 };
 
 namespace {
@@ -238,6 +239,22 @@ int create(Step** out, int device, int n, int nocc, int kmax, const double* Hp, 
         if (err) *err = "device SCF step: rocblas_create_handle failed";
         return fail(DSSCF_ERR_HIP);
     }
+    // size the handle's workspace for this n now (dsyevd is the only call
+    // that takes one), so that the step's device footprint is fixed and counted
+    {
+        size_t ws = 0;
+        bool ok = rocblas_start_device_memory_size_query(s->blas) == rocblas_status_success;
+        // in a size query the call only reports what it would need (its status says so)
+        if (ok) (void)rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n, nullptr, n, nullptr,
+                                       nullptr, nullptr);
+        ok = ok && rocblas_stop_device_memory_size_query(s->blas, &ws) == rocblas_status_success &&
+             rocblas_set_device_memory_size(s->blas, ws) == rocblas_status_success;
+        if (!ok) {
+            if (err) *err = "device SCF step: cannot size the rocBLAS workspace";
+            return fail(DSSCF_ERR_NOMEM);
+        }
+        s->blas_bytes = ws;
+    }
     for (auto pr : {std::make_pair(&s->H, s->np), std::make_pair(&s->S, s->nn), std::make_pair(&s->X, s->nn),
                     std::make_pair(&s->Pa, s->np), std::make_pair(&s->Pt, s->np), std::make_pair(&s->J, s->np),
                     std::make_pair(&s->K, s->np), std::make_pair(&s->G, s->np), std::make_pair(&s->F0, s->nn),
@@ -374,16 +391,25 @@ int download(Step* s, double* Pa, double* Pt, double* J, double* K, double* G, d
     return DSSCF_OK;
 }
 
-long long device_bytes_for(int n, int kmax)
+// The step's own matrices.
+static long long matrix_bytes(int n, int kmax)
 {
     const size_t np = (size_t)n * (n + 1) / 2, nn = (size_t)n * n;
     return (long long)(sizeof(double) * (6 * np + (8 + 2 * (size_t)kmax) * nn + 3 * (size_t)n + kmax));
 }
 
+long long device_bytes_for(int n, int kmax)
+{
+    // plus an upper estimate of dsyevd's workspace in the rocBLAS handle
+    // (divide-and-conquer: about 2n^2 + 6n doubles; the handle's own 1 MiB)
+    const size_t nn = (size_t)n * n;
+    return matrix_bytes(n, kmax) + (long long)(sizeof(double) * (3 * nn + 16 * (size_t)n)) + (1 << 20);
+}
+
 long long device_bytes(const Step* s)
 {
     if (!s) return 0;
-    return device_bytes_for(s->n, s->kmax);
+    return matrix_bytes(s->n, s->kmax) + (long long)s->blas_bytes;
 }
 
 void buffers(Step* s, const double** Pt, const double** Pa, double** J, double** K)

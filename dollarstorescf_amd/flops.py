@@ -9,8 +9,11 @@ nc(l) = (l+1)(l+2)/2:
 
 (the Head-Gordon-Pople VRR count at <= 5 FMA per entry).  Per surviving
 shell quartet: F_hrr = 2 (bra-HRR entries x ket components + ket-HRR entries
-x nc(la) nc(lb)) and F_digest = 14 n_ao(a) n_ao(b) n_ao(c) n_ao(d) (J: 2 FMA,
-K: 4 FMA per integral).  No hardware counter enters these numbers.
+x nc(la) nc(lb)) and F_digest = 14 n_ao(a) n_ao(b) n_ao(c) n_ao(d): per AO
+integral J takes 2 FMA (the bra sum and the ket sum) and K 4 FMA (one per
+exchange image), 12 FLOPs, and the permutational weight is applied to the
+integral for the J and for the K terms, 2 more.  No hardware counter enters
+these numbers.
 """
 from __future__ import annotations
 

@@ -22,6 +22,7 @@
  *   orc_coulomb/exchange vs direct contraction of the full tensor in numpy
  */
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 #include <stdint.h>
@@ -243,6 +244,11 @@ static inline size_t tensor_index(size_t D, size_t T, size_t m, size_t n)
 {
     return T - (D - m) * (D - m + 1) / 2 + n - m;
 }
+
+/* Threads of the OpenMP loops below (the reference itself is single-threaded:
+ * its CPU baseline runs them with one). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads(void) { return omp_get_max_threads(); }
 
 void orc_build_tensor(const orc_basis* B, int mode, double* tensor)
 {

@@ -212,8 +212,28 @@ class AO:
     terms: list               # [[w, x, y, z]]
 
 
-def build_aos(basis: dict, Z, xyz):
-    """AO list in atom -> shell (file order) -> m=-l..l order (SCFBasis.cpp:120-140)."""
+def cartesian_components(l: int):
+    """(a, b, c) with a + b + c = l in x-major order (xx, xy, xz, yy, yz, zz for d)."""
+    return [(a, b, l - a - b) for a in range(l, -1, -1) for b in range(l - a, -1, -1)]
+
+
+def cartesian_norm(l: int, a: int, b: int, c: int) -> float:
+    """Weight of x^a y^b z^c that normalizes one primitive carrying the
+    reference's primitive weight (GTO.cpp:90-93, which normalizes r^l times a
+    unit real solid harmonic): sqrt((2l+1)!! / (4 pi (2a-1)!! (2b-1)!! (2c-1)!!)).
+    For l <= 1 it is the real solid harmonic's own weight."""
+    den = _double_factorial_f(2.0 * a - 1.0) * _double_factorial_f(2.0 * b - 1.0) * _double_factorial_f(2.0 * c - 1.0)
+    return math.sqrt(_double_factorial_f(2.0 * l + 1.0) / (4.0 * PI * den))
+
+
+def build_aos(basis: dict, Z, xyz, cartesian: bool = False):
+    """AO list in atom -> shell (file order) -> m=-l..l order (SCFBasis.cpp:120-140).
+
+    cartesian: shells with l >= 2 become ncart(l) Cartesian functions x^a y^b z^c
+    (x-major order, m = component index), each primitive normalized
+    (cartesian_norm), the 6D convention of 6-31G*; s and p shells are the
+    reference's own.  No reference counterpart (the reference builds real
+    solid harmonics only)."""
     aos = []
     for a in range(len(Z)):
         shells = basis.get(int(Z[a]))
@@ -221,8 +241,13 @@ def build_aos(basis: dict, Z, xyz):
             raise ValueError(f"no basis for Z={Z[a]}")
         for l, prims in shells:
             np_ = normalize_primitives(l, prims)
+            c = tuple(float(v) for v in xyz[a])
+            if cartesian and l >= 2:
+                for k, (x, y, z) in enumerate(cartesian_components(l)):
+                    aos.append(AO(l, k, c, np_, [[cartesian_norm(l, x, y, z), x, y, z]]))
+                continue
             for m in range(-l, l + 1):
-                aos.append(AO(l, m, tuple(float(v) for v in xyz[a]), np_, solid_harmonic_terms(l, m)))
+                aos.append(AO(l, m, c, np_, solid_harmonic_terms(l, m)))
     return aos
 
 
@@ -258,6 +283,9 @@ def lib():
         L.orc_coulomb.argtypes = [p, i, p, p]
         L.orc_exchange.argtypes = [p, i, p, p]
         L.orc_one_electron.argtypes = [p, i, p, p, i, p, p, p]
+        L.orc_set_threads.argtypes = [i]
+        L.orc_set_threads.restype = None
+        L.orc_max_threads.restype = i
         _LIB = L
     return _LIB
 
@@ -313,6 +341,16 @@ class OracleBasis:
         S, T, V = (np.empty(D) for _ in range(3))
         lib().orc_one_electron(self.ref, len(self.Z), _ptr(self.Z), _ptr(self.xyz), mode, _ptr(S), _ptr(T), _ptr(V))
         return S, T, V
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the tensor fill and the J/K loops (1 = the reference's
+    single-threaded path)."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return lib().orc_max_threads()
 
 
 def coulomb(tensor, n, Ptot_packed):
@@ -479,10 +517,10 @@ def tensor_jk(tensor, n):
     return jk
 
 
-def scf_energy(basis_path: str, Z, xyz, charge=0, mult=1, mode=0, diis_size=8, conv=1e-8):
+def scf_energy(basis_path: str, Z, xyz, charge=0, mult=1, mode=0, diis_size=8, conv=1e-8, cartesian=False):
     """Whole oracle single point: basis, AOs, 1e, packed tensor, RHF."""
     bas = read_basis(basis_path)
-    aos = build_aos(bas, Z, xyz)
+    aos = build_aos(bas, Z, xyz, cartesian)
     ob = OracleBasis(aos, Z, xyz)
     S, T, V = (unpack(x, ob.n) for x in ob.one_electron(mode))
     t = ob.tensor(mode)

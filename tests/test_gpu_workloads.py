@@ -1,0 +1,175 @@
+"""GPU parity at the benchmark workloads themselves: the d-class paths of a
+multi-centre def2-SVP alkane against the oracle's stored-tensor J/K, the
+C20H42/def2-SVP mixed-precision build, the 512-job batch and the seeded
+200-request Poisson stream (each result against the same job solved alone)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import dollarstorescf_amd as ds
+from dollarstorescf_amd import geometry as G
+from oracle import oracle as O
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import bench  # noqa: E402  (the workload generators the benchmark lines use)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def propane_tensors():
+    """The oracle's packed tensors of propane/def2-SVP (82 AOs, 5.8 M unique AO
+    quartets) in both Boys modes."""
+    Z, xyz = G.alkane(3)
+    ob = O.OracleBasis(O.build_aos(O.read_basis(ds.basis_path("def2-SVP")), Z, xyz), Z, xyz)
+    O.set_threads(O.max_threads())
+    return Z, xyz, {mode: ob.tensor(mode) for mode in (0, 1)}
+
+
+@pytest.mark.parametrize("boys", [ds.BOYS_REFERENCE, ds.BOYS_ACCURATE])
+@pytest.mark.parametrize("slots", [True, False])
+def test_jk_propane_def2svp_forced_paths_vs_tensor(propane_tensors, boys, slots):
+    """Multi-centre d classes through every thread-per-quartet device path at
+    once: kets ordered by distance (ket_order 2), sub-pairs of at most 5
+    primitive pairs (work_item_flops 1), lane groups of up to 16 lanes for
+    every quartet with 2+ primitive quartets (group_prims 1), the J_cd slot
+    records (or, slots False, the atomic J_cd adds), plus the wave-cooperative
+    (dd|dp), (dd|dd).  J[Ptot] and K[Pa] of a random density against the
+    reference's stored-tensor loops (ExactCoulomb.cpp:103-123, oracle) over the
+    unscreened tensor in the same Boys mode: 1e-11 of max |J|."""
+    Z, xyz, tensors = propane_tensors
+    job = ds.Job(0).set_options(boys_mode=boys, schwarz_threshold=0.0)
+    job.set_basis("def2-SVP").set_molecule(Z, xyz).construct()
+    n = job.n_ao
+    assert n == 82
+    ctx = job.context
+    ctx.set_options(boys_mode=boys, schwarz_threshold=0.0, work_item_flops=1.0, group_prims=1.0,
+                    jcd_max_bytes=0.0 if slots else -1.0, ket_order=2)
+    ctx.prepare()
+    st = ctx.stats()
+    assert st.n_work_pairs > st.n_pairs  # sub-pairs exist
+    C = np.random.default_rng(41).standard_normal((n, 13)) * 0.2
+    Pa = O.pack(C @ C.T)
+    J, K = ctx.jk(2 * Pa, Pa)
+    t = tensors[boys]
+    Jr, Kr = O.coulomb(t, n, 2 * Pa), O.exchange(t, n, Pa)
+    scale = max(1.0, np.max(np.abs(Jr)))
+    assert np.max(np.abs(J - Jr)) < 1e-11 * scale
+    assert np.max(np.abs(K - Kr)) < 1e-11 * scale
+
+
+def test_mixed_precision_c20h42_def2svp():
+    """Config 2 at its own size: all-trans C20H42/def2-SVP (490 AOs), quartets
+    with Schwarz bound below 1e-5 in FP32.  J and K at a fixed density within
+    1e-7 of the all-FP64 GPU build (the oracle for this size: the reference's
+    tensor would hold 7.2e9 doubles), and the converged RHF energy within
+    1e-6 Eh of the all-FP64 one."""
+    Z, xyz = G.alkane(20)
+    res = {}
+    for thr in (0.0, 1e-5):
+        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=60, schwarz_threshold=1e-12,
+                                    fp32_threshold=thr, device_step=True)
+        job.set_basis("def2-SVP").set_molecule(Z, xyz).construct()
+        n = job.n_ao
+        st = job.context.stats()
+        C = np.random.default_rng(23).standard_normal((n, 81)) * 0.1
+        Pa = O.pack(C @ C.T)
+        J, K = job.context.jk(2 * Pa, Pa)
+        e, it = job.solve()
+        res[thr] = (J, K, e, st.n_quartets_fp32, st.n_quartets)
+        print(f"C20H42 fp32_threshold={thr}: n_ao={n} E={e:.10f} iterations={it}", flush=True)
+        job.close()
+    J64, K64, e64, nf64, nq = res[0.0]
+    J32, K32, e32, nf32, _ = res[1e-5]
+    assert n == 490 and nf64 == 0
+    assert 0.3 * nq < nf32 < nq
+    assert np.max(np.abs(J32 - J64)) < 1e-7
+    assert np.max(np.abs(K32 - K64)) < 1e-7
+    assert abs(e32 - e64) < 1e-6
+
+
+def test_batch_512_jobs_fused_equal_alone():
+    """Config 3's 512 seeded jobs (1-6 waters, 6-31G*), solved together with
+    fused, graph-replayed Fock builds and autotuned block sizes: every energy
+    within 1e-9 Eh of the same job solved alone, iterations within one."""
+    specs = bench.batch_specs(512)
+
+    def make(Z, xyz):
+        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12)
+        return job.set_basis("6-31GP").set_molecule(Z, xyz)
+
+    jobs = ds.construct_jobs([make(Z, xyz) for Z, xyz in specs])
+    e, it, st = ds.solve_jobs(jobs, use_graph=True, autotune=True)
+    for j in jobs:
+        j.close()
+    assert list(st) == [0] * len(specs)
+    print(f"batch: fused {len(specs)} jobs, {int(it.sum())} iterations", flush=True)
+    worst = 0.0
+    for k, (Z, xyz) in enumerate(specs):
+        j = make(Z, xyz)
+        j.construct()
+        ea, ia = j.solve()
+        j.close()
+        worst = max(worst, abs(ea - e[k]))
+        assert abs(ea - e[k]) < 1e-9, k
+        assert abs(ia - it[k]) <= 1, k
+    print(f"batch: max |dE| fused vs alone {worst:.2e}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def stream_alone():
+    """Config 4's seeded stream (200 requests, Poisson at 200/s, seed 7) and
+    each request's energy solved alone on its own context (all FP64),
+    converged to |dE_elec| < 1e-10."""
+    arrivals, reqs = bench.serve_requests(200, 200.0)
+    alone = []
+    for basis, Z, xyz in reqs:
+        j = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-10, max_iterations=100, schwarz_threshold=1e-12)
+        j.set_basis(basis).set_molecule(Z, xyz).construct()
+        alone.append(j.solve()[0])
+        j.close()
+    return arrivals, reqs, np.array(alone)
+
+
+@pytest.mark.parametrize("fp32", [0.0, 1e-5])
+def test_served_stream_equals_alone(stream_alone, fp32):
+    """The stream replayed in real time against the continuous-batching
+    server (64 slots, 32 GB budget, 12 workers): every request converges,
+    and each served energy is within 5e-9 Eh (all FP64) or 1e-6 Eh (FP32
+    below Schwarz 1e-5) of the same request solved alone (both converged to
+    |dE_elec| < 1e-10).  5e-9 and not 1e-9: the FP64 path's J and K sum
+    with atomic adds in no fixed order, and its energy terms of ~1e4 Eh then
+    vary by ~1e-13 relative from run to run, so four alone solves of one
+    38-atom alkane (6-31G, 160 AOs) spread over 1.3e-9 Eh (6.8e-10 and
+    1.2e-9 at 1e-11 and 1e-12 convergence: not a stopping-test effect);
+    served and alone energies of that request agree within the spread."""
+    arrivals, reqs, alone = stream_alone
+    lat, e, st, wall, _ = bench.serve_stream(ds, arrivals, reqs, fp32, max_active=64, budget=32e9, conv=1e-10)
+    assert np.all(st == 0)
+    de = np.abs(e - alone)
+    print(f"stream fp32_threshold={fp32}: {len(reqs) / wall:.1f} solves/s, p99 {np.percentile(lat, 99):.2f} s, "
+          f"max |dE| vs alone {de.max():.2e}", flush=True)
+    for k in np.argsort(-de)[:3]:
+        print(f"  request {k}: {reqs[k][0]}, {len(reqs[k][1])} atoms, alone {alone[k]:.12f}, served {e[k]:.12f}",
+              flush=True)
+    assert de.max() < (5e-9 if fp32 == 0.0 else 1e-6)
+
+
+def test_iteration_cap_reports_noconv_alone_and_served():
+    """Forced failure: an iteration cap of 1 ends a solve with
+    DSSCF_ERR_NOCONV (6), both from Job.solve and as a served job's status,
+    while a job beside it in the same server still converges."""
+    Z, xyz = G.water()
+    job = ds.Job(0).set_options(max_iterations=1).set_basis("STO-3G").set_molecule(Z, xyz).construct()
+    with pytest.raises(ds.DsscfError) as ei:
+        job.solve()
+    assert ei.value.code == 6
+    srv = ds.Server(0, budget_bytes=1e9, max_active=4, workers=2)
+    srv.submit(ds.Job(0).set_options(max_iterations=1).set_basis("STO-3G").set_molecule(Z, xyz), 0)
+    srv.submit(ds.Job(0).set_options(max_iterations=60).set_basis("STO-3G").set_molecule(Z, xyz), 1)
+    done = {d["tag"]: d for d in srv.run_until_idle()}
+    srv.close()
+    assert done[0]["status"] == 6
+    assert done[1]["status"] == 0

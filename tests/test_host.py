@@ -180,6 +180,45 @@ def test_server_rounds_return_every_job():
         assert d["t_submit"] <= d["t_done"]
 
 
+def test_server_refuses_duplicate_tag_second_submit_and_early_destroy():
+    """A job the server still holds can be neither submitted again (a second
+    request would construct the same Job on two workers) nor destroyed (its
+    context may be in a fused build), and Python refuses a tag still in
+    flight (it keeps the job alive through that tag).  Once polled, the job
+    may be destroyed."""
+    srv = ds.Server(0, budget_bytes=1e9, max_active=1, workers=1)
+    Z, xyz = G.water()
+    a = ds.Job(0).set_options(max_iterations=50).set_basis("STO-3G").set_molecule(Z, xyz)
+    b = ds.Job(0).set_options(max_iterations=50).set_basis("STO-3G").set_molecule(Z, xyz)
+    srv.submit(a, 0)
+    with pytest.raises(ds.DsscfError):
+        srv.submit(b, 0)  # tag 0 in flight
+    rc = ds.lib().dsscf_server_submit(srv._h, a._h, 1)
+    assert rc == 3  # DSSCF_ERR_STATE: already held
+    assert ds.lib().dsscf_job_destroy(a._h) == 3
+    done = srv.run_until_idle()
+    assert [d["tag"] for d in done] == [0]
+    assert ds.lib().dsscf_job_destroy(a._h) == 0
+    a._h = None
+    srv.close()
+    b.close()
+
+
+def test_server_refuses_job_that_can_never_fit():
+    """Forced failure: a job whose device-footprint estimate exceeds the whole
+    budget is retired at admission with DSSCF_ERR_NOMEM (5) and a message,
+    before any device work, instead of waiting for memory forever."""
+    srv = ds.Server(0, budget_bytes=1e4, max_active=4, workers=1)
+    Z, xyz = G.water()
+    job = ds.Job(0).set_basis("STO-3G").set_molecule(Z, xyz)
+    srv.submit(job, 7)
+    done = srv.run_until_idle()
+    srv.close()
+    assert [d["tag"] for d in done] == [7]
+    assert done[0]["status"] == 5
+    assert "budget" in job._err()
+
+
 def _molden_cases():
     import json
     with open(os.path.join(ROOT, "tests", "golden", "molden.json")) as f:

@@ -182,3 +182,68 @@ def test_scf_invariants_and_fixture():
     # idempotency P S P = P and electron count tr(Ptot S) = 10
     assert np.allclose(r.Pa @ S @ r.Pa, r.Pa, atol=1e-7)
     assert np.trace(2 * r.Pa @ S) == pytest.approx(10.0, abs=1e-9)
+
+
+# ------------------------------------------------- Cartesian d (6D) ----
+def test_cartesian_d_normalization_and_overlap_closed_forms():
+    """One-primitive Cartesian d shell: every component has unit self-overlap
+    (the normalization closed form), <xx|yy> = 1/3 and <xx|xy> = 0 on one
+    centre, and s/p AOs equal the spherical build's."""
+    basis = {8: [(0, [(1.3, 1.0)]), (1, [(0.9, 1.0)]), (2, [(0.8, 1.0)])]}
+    Z, xyz = np.array([8], np.int32), np.zeros((1, 3))
+    aos = O.build_aos(basis, Z, xyz, cartesian=True)
+    assert [(a.l, a.m) for a in aos] == [(0, 0), (1, -1), (1, 0), (1, 1)] + [(2, k) for k in range(6)]
+    assert [tuple(a.terms[0][1:]) for a in aos[4:]] == [(2, 0, 0), (1, 1, 0), (1, 0, 1), (0, 2, 0), (0, 1, 1), (0, 0, 2)]
+    S = O.unpack(O.OracleBasis(aos, Z, xyz).one_electron(0)[0], len(aos))
+    assert np.max(np.abs(np.diag(S) - 1.0)) < 1e-14
+    xx, xy, yy = 4, 5, 7
+    assert abs(S[xx, yy] - 1.0 / 3.0) < 1e-14
+    assert abs(S[xx, xy]) < 1e-15
+    sph = O.build_aos(basis, Z, xyz)
+    for a, b in zip(aos[:4], sph[:4]):
+        assert (a.l, a.m, a.terms) == (b.l, b.m, b.terms)
+
+
+def test_cartesian_d_spans_spherical_eris():
+    """Every spherical d AO is a fixed combination of the Cartesian ones,
+    Y_2m = sum_t w_t x^a y^b z^c = sum_t (w_t / N_abc) cart_abc, so each
+    spherical ERI (water 6-31G*, d AOs in all four positions) equals the
+    same combination of Cartesian ERIs: a wrong normalization, component
+    order or transposed index fails it."""
+    Z, xyz = G.water_box(2, seed=5)
+    bas = O.read_basis(os.path.join(BASIS, "6-31GP"))
+    sph = O.build_aos(bas, Z, xyz)
+    car = O.build_aos(bas, Z, xyz, cartesian=True)
+    obs, obc = O.OracleBasis(sph, Z, xyz), O.OracleBasis(car, Z, xyz)
+    # each spherical AO as {cartesian AO index: coefficient}
+    comb, ic = [], 0
+    i = 0
+    while i < len(sph):
+        a = sph[i]
+        if a.l < 2:
+            comb.append({ic: 1.0})
+            i += 1
+            ic += 1
+            continue
+        comps = O.cartesian_components(2)
+        for k in range(5):
+            t = sph[i + k].terms
+            comb.append({ic + comps.index(tuple(x[1:])): x[0] / O.cartesian_norm(2, *x[1:]) for x in t})
+        i += 5
+        ic += 6
+    assert ic == len(car)
+    rng = np.random.default_rng(2)
+    d_sph = [k for k, a in enumerate(sph) if a.l == 2]
+    q = rng.integers(0, len(sph), size=(60, 4))
+    q[:20, 0] = rng.choice(d_sph, 20)
+    q[20:40, 1:3] = rng.choice(d_sph, (20, 2))
+    q[40:, :] = rng.choice(d_sph, (20, 4))
+    for i, j, k, l in q:
+        want = obs.eri(int(i), int(j), int(k), int(l), 1)
+        got = 0.0
+        for a, ca in comb[i].items():
+            for b, cb in comb[j].items():
+                for c, cc in comb[k].items():
+                    for d, cd in comb[l].items():
+                        got += ca * cb * cc * cd * obc.eri(a, b, c, d, 1)
+        assert abs(got - want) < 1e-12 * max(1.0, abs(want))

@@ -18,9 +18,52 @@
 #include "boys_table.hpp"
 #include "kernels.hpp"
 
+#include <mutex>
+
+// ------------------------------------------------------- no-alloc scratch --
+// Device memory of dsscf_eri_batch_noalloc (its shell-quartet item list and
+// AO-block outputs): a file-scope array, so that the call needs no device
+// allocation.  Each GPU the library runs on holds its own copy, at its own
+// address, until the process ends.  Host code must not pass `scratch` itself
+// to a kernel (on the host the name is only a stand-in): it uses the address
+// from hipGetSymbolAddress, taken once per device at context creation.
+#define DSSCF_SCRATCH_BYTES ((size_t)16 << 20)
+static __device__ __align__(256) unsigned char scratch[DSSCF_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_dev;  // per device: the device address of `scratch`
+
+// Look up `scratch` on `device` (which must be current) once.
+int scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_dev.size() <= device) g_scratch_dev.resize(device + 1, nullptr);
+    if (g_scratch_dev[device]) return DSSCF_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return DSSCF_ERR_HIP;
+    }
+    g_scratch_dev[device] = p;
+    return DSSCF_OK;
+}
+
+void* scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < (int)g_scratch_dev.size() ? g_scratch_dev[device] : nullptr;
+}
+
+// Byte offset of the block outputs behind n int4 items (256-byte aligned).
+size_t staging_offset_out(size_t n_items) { return (sizeof(int4) * n_items + 255) & ~(size_t)255; }
+
+}  // namespace
+
+namespace {
 
 int fail(int code, const std::string& msg)
 {
@@ -300,6 +343,8 @@ int dsscf_ctx_create(dsscf_ctx** out, int device)
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(DSSCF_ERR_ARG, "dsscf_ctx_create: no such device");
     HIP_TRY(hipSetDevice(device));
+    // the no-alloc entry points' scratch address on this device (once per device)
+    if (int rc = scratch_setup(device)) return rc;
     dsscf_ctx* c = new dsscf_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1739,6 +1784,17 @@ int dsscf_eri_batch(dsscf_ctx* c, const int* ijkl, long long n, double* out)
     if (n == 0) return DSSCF_OK;
     HIP_TRY(hipSetDevice(c->device));
     return eri_batch_impl(c, ijkl, n, out, nullptr, 0, "dsscf_eri_batch");
+}
+
+int dsscf_eri_batch_noalloc(dsscf_ctx* c, const int* ijkl, long long n, double* out)
+{
+    if (!c || (n > 0 && (!ijkl || !out)) || n < 0) return fail(DSSCF_ERR_ARG, "dsscf_eri_batch_noalloc: bad argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_eri_batch_noalloc: call dsscf_prepare first");
+    if (n == 0) return DSSCF_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    void* st = scratch_ptr(c->device);
+    if (!st) return fail(DSSCF_ERR_STATE, "dsscf_eri_batch_noalloc: no scratch address for this device");
+    return eri_batch_impl(c, ijkl, n, out, st, DSSCF_SCRATCH_BYTES, "dsscf_eri_batch_noalloc");
 }
 
 }  // extern "C"

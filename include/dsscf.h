@@ -168,6 +168,17 @@ int dsscf_one_electron(dsscf_ctx* ctx, int n_atom, const int* Z, const double* x
  * ijkl = {mu, nu, sigma, tau}.  No screening is applied. */
 int dsscf_eri_batch(dsscf_ctx* ctx, const int* ijkl, long long n, double* out);
 
+/* dsscf_eri_batch without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments
+ * and results as dsscf_eri_batch; the item list and AO-block outputs of each
+ * class are staged in a fixed 16 MiB device array per GPU instead of
+ * per-call allocations.  A call whose largest class needs more staging
+ * (16 B per unique shell quartet plus 8 B per AO integral of its blocks)
+ * fails with DSSCF_ERR_ARG before any work.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (from several threads) race. */
+int dsscf_eri_batch_noalloc(dsscf_ctx* ctx, const int* ijkl, long long n, double* out);
+
 /* F_0..F_L (L <= 8) of the Boys function at n arguments T[], evaluated on the
  * device exactly as the ERI kernels do: mode 0 = the reference series
  * (boysFunctionExact, GaussianIntegral.cpp:209-223), 1 = accurate from the

@@ -337,18 +337,20 @@ def batch_specs(n_jobs: int, seed: int = 2024):
 
 
 def bench_batch(a):
-    """512 independent RHF/6-31G* jobs (spherical d; 18-108 AOs).  Fused: the
-    jobs are constructed together (host parts in parallel) and solved in lock
-    step, each round's Fock builds of all unconverged jobs one launch per
-    quartet class (hipGraph replays, block sizes from the autotuner).  Loop:
-    the same jobs solved one after another, each with its own context.  Both
-    time the SCF solves (construction reported apart); value = fused SCF/s."""
+    """512 independent RHF/6-31G* jobs with Cartesian d (6D; 19-114 AOs).
+    Fused: the jobs are constructed together (host parts in parallel) and
+    solved in lock step, each round's Fock builds of all unconverged jobs one
+    launch per quartet class (hipGraph replays, block sizes from the
+    autotuner).  Loop: the same jobs solved one after another, each with its
+    own context.  Both time the SCF solves (construction reported apart);
+    value = fused SCF/s."""
     import dollarstorescf_amd as ds
 
     specs = batch_specs(a.jobs)
 
     def make(Z, xyz):
-        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12)
+        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12,
+                                    cartesian=True)
         return job.set_basis("6-31GP").set_molecule(Z, xyz)
 
     t0 = time.perf_counter()
@@ -379,7 +381,7 @@ def bench_batch(a):
         "metric": METRIC, "value": len(specs) / t_fused, "unit": "SCF solves/s", "n_gpus": 1,
         "steps": len(specs), "warmup": 0, "ms_per_step": t_fused / len(specs) * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded water boxes)",
-        "config": {"workload": f"{len(specs)} RHF/6-31G* jobs (spherical d), 1-6 waters in a 6 A box, min O-O "
+        "config": {"workload": f"{len(specs)} RHF/6-31G* jobs (Cartesian d, 6D), 1-6 waters in a 6 A box, min O-O "
                                "2.6 A, seed 2024; Schwarz 1e-12; converge |dE| < 1e-8",
                    "n_ao_min": int(min(n_ao)), "n_ao_max": int(max(n_ao)), "parallelism": "single GPU"},
         "fused": {"solves_per_s": len(specs) / t_fused, "seconds": t_fused, "construct_seconds": t_construct_fused,

@@ -35,7 +35,7 @@ class Options(ctypes.Structure):
     _fields_ = [("boys_mode", ctypes.c_int), ("schwarz_threshold", ctypes.c_double),
                 ("fp32_threshold", ctypes.c_double), ("prim_threshold", ctypes.c_double),
                 ("work_item_flops", ctypes.c_double), ("group_prims", ctypes.c_double),
-                ("jcd_max_bytes", ctypes.c_double), ("ket_order", ctypes.c_int)]
+                ("jcd_max_bytes", ctypes.c_double), ("ket_order", ctypes.c_int), ("cartesian", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -74,7 +74,7 @@ def lib():
             "dsscf_get_jk": [p, p, p], "dsscf_time_jk_classes": [p, i, p], "dsscf_time_jk": [p, i, i, p, p, p], "dsscf_class_l": [i, p],
             "dsscf_job_create": [p, i], "dsscf_job_destroy": [p],
             "dsscf_job_set_options": [p, i, d, i, i, d, i], "dsscf_job_set_precision": [p, d, d],
-            "dsscf_job_set_step": [p, i], "dsscf_job_set_orbitals": [p, p, p], "dsscf_job_write_molden": [p, c, i],
+            "dsscf_job_set_step": [p, i], "dsscf_job_set_cartesian": [p, i], "dsscf_eri_batch_noalloc": [p, p, ctypes.c_longlong, p], "dsscf_job_set_orbitals": [p, p, p], "dsscf_job_write_molden": [p, c, i],
             "dsscf_job_gradient": [p, p, p], "dsscf_eri_gradient": [p, p, i, p, p], "dsscf_job_n_atom": [p],
             "dsscf_job_gradient_terms": [p, p, p, p, p],
             "dsscf_job_set_molecule": [p, i, p, p, i, i], "dsscf_job_load_xyz": [p, c, i, i],
@@ -264,9 +264,9 @@ class Context:
     __del__ = close
 
     def set_options(self, boys_mode=BOYS_ACCURATE, schwarz_threshold=1e-12, fp32_threshold=0.0, prim_threshold=0.0,
-                    work_item_flops=0.0, group_prims=0.0, jcd_max_bytes=0.0, ket_order=0):
+                    work_item_flops=0.0, group_prims=0.0, jcd_max_bytes=0.0, ket_order=0, cartesian=0):
         o = Options(boys_mode, schwarz_threshold, fp32_threshold, prim_threshold, work_item_flops, group_prims,
-                    jcd_max_bytes, ket_order)
+                    jcd_max_bytes, ket_order, int(cartesian))
         _check(lib().dsscf_set_options(self._h, ctypes.byref(o)))
 
     def set_basis(self, l, nprim, centers, exps, coefs, xform):
@@ -444,16 +444,20 @@ class Job:
     __del__ = close
 
     def set_options(self, diis_size=8, energy_threshold=1e-8, max_iterations=500, boys_mode=BOYS_ACCURATE,
-                    schwarz_threshold=1e-12, verbose=0, fp32_threshold=0.0, prim_threshold=None, device_step=False):
+                    schwarz_threshold=1e-12, verbose=0, fp32_threshold=0.0, prim_threshold=None, device_step=False,
+                    cartesian=False):
         """prim_threshold defaults to 1e-17 in the accurate Boys mode and to 0 (every
         primitive quartet, as the reference) in the reference mode.  device_step runs
-        each iteration's DIIS, X F X, diagonalization and density on the device."""
+        each iteration's DIIS, X F X, diagonalization and density on the device.
+        cartesian: d shells as six Cartesian functions (6D) instead of the
+        reference's five real solid harmonics."""
         if prim_threshold is None:
             prim_threshold = DEFAULT_PRIM_THRESHOLD if boys_mode == BOYS_ACCURATE else 0.0
         _check(lib().dsscf_job_set_options(self._h, diis_size, energy_threshold, max_iterations, boys_mode,
                                            schwarz_threshold, verbose), self._err)
         _check(lib().dsscf_job_set_precision(self._h, fp32_threshold, prim_threshold), self._err)
         _check(lib().dsscf_job_set_step(self._h, 1 if device_step else 0), self._err)
+        _check(lib().dsscf_job_set_cartesian(self._h, 1 if cartesian else 0), self._err)
         return self
 
     def set_molecule(self, Z, xyz_bohr, charge=0, multiplicity=1):

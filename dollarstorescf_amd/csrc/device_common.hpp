@@ -602,7 +602,9 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
     }
 }
 
-template <int la, int lb, int lc, int ld>
+// C = 1: the class's d shells are Cartesian functions (6D) instead of real
+// solid harmonics; only classes with a d shell have that form.
+template <int la, int lb, int lc, int ld, int C = 0>
 struct ClassInfo;
-template <int la, int lb, int lc, int ld>
+template <int la, int lb, int lc, int ld, int C = 0>
 struct WaveTab;

@@ -109,6 +109,9 @@ int quartet_group(int nb, int nk, double target)
 }
 
 int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+// AOs of an l-shell: 2l+1 real solid harmonics, or with cartesian d (l >= 2)
+// the ncart(l) Cartesian functions
+int nfun(int l, int cart) { return cart && l >= 2 ? ncart(l) : 2 * l + 1; }
 
 // Algorithmic FLOPs of one primitive quartet of class (la lb|lc ld): setup,
 // Boys, Head-Gordon-Pople VRR at 5 FLOPs per entry, contraction into the
@@ -151,6 +154,7 @@ constexpr int kStreams = 4;  // streams per fused Fock build (GPU_MAX_HW_QUEUES 
 
 struct Plan {
     int boys_mode = DSSCF_BOYS_ACCURATE;
+    int cart = 0;  // Cartesian d shells (every member context agrees)
     const double* btab = nullptr;
     int ncls = 0;
     std::vector<int> cls_nseg, cls_nblocks, cls_off, cls_bs, cls_maxN, cls_maxbra;
@@ -321,6 +325,23 @@ struct dsscf_ctx {
     }
 };
 
+namespace {
+// First AO of each shell and the shell of each AO, from the shells' l and
+// the cartesian option.
+void layout_aos(dsscf_ctx* c)
+{
+    int nao = 0;
+    for (int s = 0; s < c->n_shell; ++s) {
+        c->sh_ao[s] = nao;
+        nao += nfun(c->sh_l[s], c->opt.cartesian);
+    }
+    c->n_ao = nao;
+    c->ao_shell.resize(nao);
+    for (int s = 0; s < c->n_shell; ++s)
+        for (int m = 0; m < nfun(c->sh_l[s], c->opt.cartesian); ++m) c->ao_shell[c->sh_ao[s] + m] = s;
+}
+}  // namespace
+
 extern "C" {
 
 const char* dsscf_last_error(void) { return g_err.c_str(); }
@@ -388,6 +409,7 @@ int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
     if (o->boys_mode != DSSCF_BOYS_REFERENCE && o->boys_mode != DSSCF_BOYS_ACCURATE)
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: unknown boys_mode");
     if (o->ket_order < 0 || o->ket_order > 2) return fail(DSSCF_ERR_ARG, "dsscf_set_options: ket_order 0..2");
+    if (o->cartesian != 0 && o->cartesian != 1) return fail(DSSCF_ERR_ARG, "dsscf_set_options: cartesian 0 or 1");
     if (!(o->schwarz_threshold >= 0.0) || !(o->fp32_threshold >= 0.0) || !(o->prim_threshold >= 0.0))
         return fail(DSSCF_ERR_ARG, "dsscf_set_options: negative threshold");
     if (o->fp32_threshold > 0.0 && o->boys_mode != DSSCF_BOYS_ACCURATE)
@@ -395,9 +417,12 @@ int dsscf_set_options(dsscf_ctx* c, const dsscf_options* o)
     const bool changed = o->boys_mode != c->opt.boys_mode || o->schwarz_threshold != c->opt.schwarz_threshold ||
                          o->fp32_threshold != c->opt.fp32_threshold || o->prim_threshold != c->opt.prim_threshold ||
                          o->work_item_flops != c->opt.work_item_flops || o->group_prims != c->opt.group_prims ||
-                         o->jcd_max_bytes != c->opt.jcd_max_bytes || o->ket_order != c->opt.ket_order;
+                         o->jcd_max_bytes != c->opt.jcd_max_bytes || o->ket_order != c->opt.ket_order ||
+                         o->cartesian != c->opt.cartesian;
+    const bool relayout = o->cartesian != c->opt.cartesian && c->have_basis;
     c->opt = *o;
     if (changed) c->prepared = false;
+    if (relayout) layout_aos(c);
     return DSSCF_OK;
 }
 
@@ -413,7 +438,7 @@ int dsscf_set_basis(dsscf_ctx* c, int n_shell, const int* shell_l, const int* sh
     c->sh_center.assign(shell_center, shell_center + 3 * n_shell);
     c->sh_prim_off.resize(n_shell);
     c->sh_ao.resize(n_shell);
-    int np = 0, nao = 0;
+    int np = 0;
     for (int s = 0; s < n_shell; ++s) {
         if (shell_l[s] < 0 || shell_l[s] > maxl)
             return fail(DSSCF_ERR_UNSUPPORTED, "dsscf_set_basis: shell angular momentum " + std::to_string(shell_l[s]) +
@@ -421,9 +446,7 @@ int dsscf_set_basis(dsscf_ctx* c, int n_shell, const int* shell_l, const int* sh
         if (shell_nprim[s] <= 0 || shell_nprim[s] > 100)
             return fail(DSSCF_ERR_ARG, "dsscf_set_basis: primitive count outside 1-100");
         c->sh_prim_off[s] = np;
-        c->sh_ao[s] = nao;
         np += shell_nprim[s];
-        nao += 2 * shell_l[s] + 1;
     }
     for (int i = 0; i < np; ++i)
         if (!(prim_exp[i] > 0.0)) return fail(DSSCF_ERR_ARG, "dsscf_set_basis: non-positive exponent");
@@ -436,10 +459,7 @@ int dsscf_set_basis(dsscf_ctx* c, int n_shell, const int* shell_l, const int* sh
     c->prim_coef.assign(prim_coef, prim_coef + np);
     c->xform.assign(ao_xform, ao_xform + dk_xform_size());
     c->n_shell = n_shell;
-    c->n_ao = nao;
-    c->ao_shell.resize(nao);
-    for (int s = 0; s < n_shell; ++s)
-        for (int m = 0; m < 2 * shell_l[s] + 1; ++m) c->ao_shell[c->sh_ao[s] + m] = s;
+    layout_aos(c);
     c->have_basis = true;
     c->prepared = false;
     return DSSCF_OK;
@@ -575,7 +595,7 @@ int dsscf_prepare(dsscf_ctx* c)
         for (int l1 = 0; l1 <= maxl; ++l1)
             for (int l2 = 0; l2 <= l1; ++l2) {
                 const int pc = pair_class_of(l1, l2);
-                HIP_TRY(dsscf_launch_schwarz(l1, l2, c->opt.boys_mode, c->d_plist + start[pc],
+                HIP_TRY(dsscf_launch_schwarz(l1, l2, c->opt.boys_mode, c->opt.cartesian, c->d_plist + start[pc],
                                              (int)by_class[pc].size(), c->d_pairs, c->d_pp, c->d_btab, c->d_Q,
                                              kEriScale, c->stream));
             }
@@ -939,7 +959,7 @@ int dsscf_prepare(dsscf_ctx* c)
             if (dsscf_class_is_wave(cls)) continue;
             int l4[4];
             dk_class_l(cls, l4);
-            cls_ncnd[cls] = (2 * l4[2] + 1) * (2 * l4[3] + 1);
+            cls_ncnd[cls] = nfun(l4[2], c->opt.cartesian) * nfun(l4[3], c->opt.cartesian);
             for (long long ui = c->u_off[cls]; ui < c->u_off32[cls] + c->u_cnt32[cls]; ++ui) {
                 us[ui] = (int)std::min<long long>(ne, 0x7fffffff);
                 nd += (long long)units[ui].z * cls_ncnd[cls];
@@ -965,7 +985,8 @@ int dsscf_prepare(dsscf_ctx* c)
                         int& r = row_of[(size_t)kp.ao_a * N + kp.ao_b];
                         if (r < 0) {
                             r = (int)ri.size();
-                            ri.push_back(make_int4(kp.ao_a, kp.ao_b, 2 * l4[2] + 1, 2 * l4[3] + 1));
+                            ri.push_back(make_int4(kp.ao_a, kp.ao_b, nfun(l4[2], c->opt.cartesian),
+                                                   nfun(l4[3], c->opt.cartesian)));
                             rr.push_back(make_int2(0, 0));
                         }
                         ++rr[r].y;
@@ -1022,6 +1043,7 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
     const int ncls = dk_n_class();
     p.ncls = ncls;
     p.boys_mode = ctxs[0]->opt.boys_mode;
+    p.cart = ctxs[0]->opt.cartesian;
     p.btab = ctxs[0]->d_btab;
     const int nslot = 2 * ncls;  // slot 2*cls: FP64 list, 2*cls+1: FP32 list
     p.cls_nseg.assign(nslot, 0);
@@ -1076,7 +1098,7 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
             sg.blk0 = (int)blocks;
             p.cls_cost[slot] += (double)(prec32 ? c->q_prim32[cls] : c->q_prim[cls]);
             // one block per work unit, or per dsscf_wave_group quartets (wave classes)
-            blocks += wave ? (cnt + dsscf_wave_group(cls) - 1) / dsscf_wave_group(cls) : cnt;
+            blocks += wave ? (cnt + dsscf_wave_group(cls, p.cart) - 1) / dsscf_wave_group(cls, p.cart) : cnt;
             if (blocks > 0x7fffffffLL) return fail(DSSCF_ERR_UNSUPPORTED, "launch grid too large");
             p.h_segs.push_back(sg);
             p.h_blk0.push_back(sg.blk0);
@@ -1092,7 +1114,7 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
         if (!wave && p.cls_nseg[slot]) {
             int l4[4];
             dk_class_l(cls, l4);
-            const size_t lds = sizeof(double) * ((size_t)(2 * l4[0] + 2 * l4[1] + 2) * slot_maxN + 1) +
+            const size_t lds = sizeof(double) * ((size_t)(nfun(l4[0], p.cart) + nfun(l4[1], p.cart)) * slot_maxN + 1) +
                                sizeof(PrimPair) * (size_t)p.cls_maxbra[slot];
             if (lds > 64 * 1024) return fail(DSSCF_ERR_UNSUPPORTED, "basis too large for the LDS exchange rows");
         }
@@ -1193,7 +1215,7 @@ int plan_launch_slot(Plan& p, int slot, hipStream_t s)
     if (p.cls_nseg[slot] == 0) return DSSCF_OK;
     HIP_TRY(dsscf_launch_jk(slot / 2, p.boys_mode, slot % 2, p.d_segs + p.cls_off[slot], p.d_blk0 + p.cls_off[slot],
                             p.cls_nseg[slot], p.cls_nblocks[slot], p.cls_bs[slot], p.cls_maxN[slot],
-                            p.cls_maxbra[slot], p.btab, kEriScale, s));
+                            p.cls_maxbra[slot], p.cart, p.btab, kEriScale, s));
     return DSSCF_OK;
 }
 
@@ -1600,8 +1622,8 @@ int dsscf_one_electron(dsscf_ctx* c, int n_atom, const int* Z, const double* xyz
         for (int l2 = 0; l2 <= l1 && e == hipSuccess; ++l2) {
             const int pc = pair_class_of(l1, l2);
             e = dsscf_launch_one_electron(l1, l2, c->opt.boys_mode, c->d_plist + c->pc_start[pc], c->pc_count[pc],
-                                          c->d_pairs, c->d_pp, d_at, n_atom, c->d_btab, c->d_xform, d_out,
-                                          d_out + np, d_out + 2 * np, c->stream);
+                                          c->d_pairs, c->d_pp, d_at, n_atom, c->d_btab, c->d_xform, c->opt.cartesian,
+                                          d_out, d_out + np, d_out + 2 * np, c->stream);
         }
     if (e == hipSuccess) e = hipMemcpyAsync(S, d_out, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(T, d_out + np, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream);
@@ -1689,7 +1711,7 @@ int eri_batch_impl(dsscf_ctx* c, const int* ijkl, long long n, double* out, void
                 cb[h] = m1;
             }
             const int s_second = f == s1 ? s2 : s1;
-            nb[h] = 2 * c->sh_l[s_second] + 1;
+            nb[h] = nfun(c->sh_l[s_second], c->opt.cartesian);
             pidx[h] = p;
         }
         int X = c->pair_class[pidx[0]], Y = c->pair_class[pidx[1]];
@@ -1700,7 +1722,7 @@ int eri_batch_impl(dsscf_ctx* c, const int* ijkl, long long n, double* out, void
             ket = 0;
         }
         const int cls = X * (X + 1) / 2 + Y;
-        const int nket = nb[ket] * (2 * c->sh_l[c->pair_first[pidx[ket]]] + 1);
+        const int nket = nb[ket] * nfun(c->sh_l[c->pair_first[pidx[ket]]], c->opt.cartesian);
         const int comp = (ca[bra] * nb[bra] + cb[bra]) * nket + ca[ket] * nb[ket] + cb[ket];
         per_cls[cls].push_back(Req{t, pidx[bra], pidx[ket], comp});
     }
@@ -1711,7 +1733,8 @@ int eri_batch_impl(dsscf_ctx* c, const int* ijkl, long long n, double* out, void
         if (rq.empty()) continue;
         int l4[4];
         dk_class_l(cls, l4);
-        nblock[cls] = (2 * l4[0] + 1) * (2 * l4[1] + 1) * (2 * l4[2] + 1) * (2 * l4[3] + 1);
+        const int ct = c->opt.cartesian;
+        nblock[cls] = nfun(l4[0], ct) * nfun(l4[1], ct) * nfun(l4[2], ct) * nfun(l4[3], ct);
         // unique shell quartets of this class
         auto& key = keys[cls];
         key.resize(rq.size());
@@ -1755,7 +1778,8 @@ int eri_batch_impl(dsscf_ctx* c, const int* ijkl, long long n, double* out, void
         // every block's nb components are written by the kernel: nothing to clear
         hipError_t e = hipMemcpyAsync(d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess)
-            e = dsscf_launch_eri_block(cls, c->opt.boys_mode, d_items, (int)items.size(), c->d_pairs, c->d_pp, c->d_btab,
+            e = dsscf_launch_eri_block(cls, c->opt.boys_mode, c->opt.cartesian, d_items, (int)items.size(), c->d_pairs,
+                                       c->d_pp, c->d_btab,
                                        d_out, kEriScale, c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(blk.data(), d_out, sizeof(double) * blk.size(), hipMemcpyDeviceToHost, c->stream);
@@ -1885,6 +1909,8 @@ int batch_check_members(dsscf_batch* b, dsscf_ctx* const* ctxs, int n, const cha
         if (ctxs[i]->device != b->device) return fail(DSSCF_ERR_ARG, std::string(who) + ": contexts on another device");
         if (ctxs[i]->opt.boys_mode != ctxs[0]->opt.boys_mode)
             return fail(DSSCF_ERR_ARG, std::string(who) + ": mixed Boys modes in one batch");
+        if (ctxs[i]->opt.cartesian != ctxs[0]->opt.cartesian)
+            return fail(DSSCF_ERR_ARG, std::string(who) + ": spherical and Cartesian-d contexts in one batch");
         for (int k = 0; k < i; ++k)
             if (ctxs[k] == ctxs[i]) return fail(DSSCF_ERR_ARG, std::string(who) + ": context listed twice");
     }

@@ -19,12 +19,12 @@
 #include "eri_classes.inc"
 
 
-template <int la, int lb, int lc, int ld, int BM, typename R>
+template <int la, int lb, int lc, int ld, int BM, typename R, int C = 0>
 struct Quartet;
-#define DSSCF_QSPEC_1(a, b, c, d, name)
-#define DSSCF_QSPEC_0(a, b, c, d, name)                                                               \
+#define DSSCF_QSPEC_1(a, b, c, d, name, C, fn)
+#define DSSCF_QSPEC_0(a, b, c, d, name, C, fn)                                                        \
     template <int BM, typename R>                                                                     \
-    struct Quartet<a, b, c, d, BM, R> {                                                               \
+    struct Quartet<a, b, c, d, BM, R, C> {                                                            \
         template <typename BP>                                                                        \
         static __device__ __forceinline__ void run(const ShellPair& bra, const ShellPair& ket,        \
                                                    const BP* __restrict__ bra_pp,                     \
@@ -32,11 +32,15 @@ struct Quartet;
                                                    const double* __restrict__ btab, double* out,      \
                                                    int gb = 1, int rb = 0, int gk = 1, int rk = 0)    \
         {                                                                                             \
-            shell_quartet_##name<BM, R, BP>(bra, ket, bra_pp, ket_pp, btab, out, gb, rb, gk, rk);     \
+            fn<BM, R, BP>(bra, ket, bra_pp, ket_pp, btab, out, gb, rb, gk, rk);                       \
         }                                                                                             \
     };
-#define DSSCF_QUARTET_SPECIALIZE(a, b, c, d, name, wave) DSSCF_QSPEC_##wave(a, b, c, d, name)
+#define DSSCF_QUARTET_SPECIALIZE(a, b, c, d, name, wave) \
+    DSSCF_QSPEC_##wave(a, b, c, d, name, 0, shell_quartet_##name)
 DSSCF_FOR_EACH_CLASS(DSSCF_QUARTET_SPECIALIZE)
+#define DSSCF_QUARTET_SPECIALIZE_CART(a, b, c, d, name, wave) \
+    DSSCF_QSPEC_##wave(a, b, c, d, name, 1, shell_quartet_##name##_cart)
+DSSCF_FOR_EACH_D_CLASS(DSSCF_QUARTET_SPECIALIZE_CART)
 
 #include "wave_kernel.hpp"
 
@@ -76,12 +80,12 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // count], then the bra's primitive records (at most maxBra of them, sized by
 // the launch rather than for the largest possible shell pair, so that more
 // blocks fit on a CU).
-template <int la, int lb, int lc, int ld, int BM, typename R>
+template <int la, int lb, int lc, int ld, int BM, typename R, int C>
 __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
                                             int nseg, int nunits, int maxN,
                                             const double* __restrict__ btab, double scale)
 {
-    typedef ClassInfo<la, lb, lc, ld> CI;
+    typedef ClassInfo<la, lb, lc, ld, C> CI;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
     // bra primitive records: FP64 lists PrimPair, FP32 lists (accurate mode) PrimPairF
     constexpr bool kF32Rec = sizeof(R) == 4 && BM == BOYS_ACCURATE;
@@ -168,7 +172,7 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
             const ShellPair ket = ldg(pairs + ki);
             const int c0 = ket.ao_a, d0 = ket.ao_b;
             double v[CI::NAO];
-            Quartet<la, lb, lc, ld, BMq, R>::run(bra, ket, braP, S.pp + ket.prim_off, tq, v, lgb, rb, lgk, rk);
+            Quartet<la, lb, lc, ld, BMq, R, C>::run(bra, ket, braP, S.pp + ket.prim_off, tq, v, lgb, rb, lgk, rk);
             if (lane != 0) continue;
             // density elements of the digestion: Dt[c][d], Da[b][d], Da[a][d], Da[b][c], Da[a][c]
             // (loading them before the primitive loops, to overlap their latency,
@@ -285,21 +289,21 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
 // --------------------------------------------------------- ERI batch ---
 // Whole AO blocks of listed shell quartets (parity path for the unit API
 // flo::electronRepulsionIntegral, GaussianIntegral.cpp:561).
-template <int la, int lb, int lc, int ld, int BM>
+template <int la, int lb, int lc, int ld, int BM, int C>
 __global__ void __launch_bounds__(256) k_eri_block(const int4* __restrict__ items, int n,
                                                    const ShellPair* __restrict__ pairs,
                                                    const PrimPair* __restrict__ pp,
                                                    const double* __restrict__ btab, double* __restrict__ out,
                                                    double scale)
 {
-    typedef ClassInfo<la, lb, lc, ld> CI;
+    typedef ClassInfo<la, lb, lc, ld, C> CI;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int4 it = items[t];
     const ShellPair bra = pairs[it.x];
     const ShellPair ket = pairs[it.y];
     double v[CI::NAO];
-    Quartet<la, lb, lc, ld, BM, double>::run(bra, ket, pp + bra.prim_off, pp + ket.prim_off, btab, v);
+    Quartet<la, lb, lc, ld, BM, double, C>::run(bra, ket, pp + bra.prim_off, pp + ket.prim_off, btab, v);
     double* o = out + (size_t)it.z;
 #pragma unroll
     for (int i = 0; i < CI::NAO; ++i) o[i] = scale * v[i];
@@ -307,21 +311,21 @@ __global__ void __launch_bounds__(256) k_eri_block(const int4* __restrict__ item
 
 // --------------------------------------------------------- Schwarz -----
 // Q_ab = sqrt(max_{mu in a, nu in b} |(mu nu|mu nu)|) for each listed pair.
-template <int la, int lb, int BM>
+template <int la, int lb, int BM, int C>
 __global__ void __launch_bounds__(256) k_schwarz(const int* __restrict__ plist, int n,
                                                  const ShellPair* __restrict__ pairs,
                                                  const PrimPair* __restrict__ pp,
                                                  const double* __restrict__ btab, double* __restrict__ Q,
                                                  double scale)
 {
-    typedef ClassInfo<la, lb, la, lb> CI;
+    typedef ClassInfo<la, lb, la, lb, C> CI;
     constexpr int NA = CI::NA, NB = CI::NB;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int pidx = plist[t];
     const ShellPair pr = pairs[pidx];
     double v[CI::NAO];
-    Quartet<la, lb, la, lb, BM, double>::run(pr, pr, pp + pr.prim_off, pp + pr.prim_off, btab, v);
+    Quartet<la, lb, la, lb, BM, double, C>::run(pr, pr, pp + pr.prim_off, pp + pr.prim_off, btab, v);
     double mx = 0.0;
 #pragma unroll
     for (int i = 0; i < NA; ++i)
@@ -332,97 +336,138 @@ __global__ void __launch_bounds__(256) k_schwarz(const int* __restrict__ plist, 
 
 
 // ------------------------------------------------------------ launches --
+// cart (run time) selects a class's Cartesian-d form (template C = 1), which
+// exists for classes with a d shell; the others have one form.
 template <int a, int b, int c, int d, int wave>
-template <int BM, int MODE>
+template <int BM, int MODE, int C>
 void ClassLaunch<a, b, c, d, wave>::wave_go(const void* items, int n, const ShellPair* pairs, const PrimPair* pp,
                                             const double* btab, double* out, double scale, hipStream_t s)
 {
     if constexpr (wave) {
-        constexpr int G = WaveGroup<a, b, c, d>::G;
+        constexpr int G = WaveGroup<a, b, c, d, C>::G;
         const int ng = (n + G - 1) / G;
         const int grd = ng < (1 << 20) ? ng : (1 << 20);
-        hipLaunchKernelGGL((k_wave<a, b, c, d, BM, MODE>), dim3(grd), dim3(64), 0, s, items, n, pairs, pp, btab,
+        hipLaunchKernelGGL((k_wave<a, b, c, d, BM, MODE, C>), dim3(grd), dim3(64), 0, s, items, n, pairs, pp, btab,
                            nullptr, nullptr, 0, out, scale);
     }
 }
 
-// J/K over nseg context segments: thread-per-quartet classes one block of bs
-// threads per work unit (LDS: the Kh rows of the bra's two shells),
-// wave-cooperative classes one 64-lane block per WaveGroup::G quartets.
 template <int a, int b, int c, int d, int wave>
-hipError_t ClassLaunch<a, b, c, d, wave>::jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                                             int nblocks, int bs, int maxN, int maxBra, const double* btab,
-                                             double scale, hipStream_t s)
+template <int C>
+hipError_t ClassLaunch<a, b, c, d, wave>::jk_c(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg,
+                                               int nblocks, int bs, int maxN, int maxBra, const double* btab,
+                                               double scale, hipStream_t s)
 {
     if constexpr (wave) {
         if (bm == BOYS_ACCURATE)
-            hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_ACCURATE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
+            hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_ACCURATE, WAVE_JK, C>), dim3(nblocks), dim3(64), 0, s, nullptr,
                                0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
         else
-            hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_REFERENCE, WAVE_JK>), dim3(nblocks), dim3(64), 0, s, nullptr,
-                               0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
+            hipLaunchKernelGGL((k_wave<a, b, c, d, BOYS_REFERENCE, WAVE_JK, C>), dim3(nblocks), dim3(64), 0, s,
+                               nullptr, 0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
     } else {
-        typedef ClassInfo<a, b, c, d> CI;
+        typedef ClassInfo<a, b, c, d, C> CI;
         const size_t rows = ((size_t)(CI::NA + CI::NB) * maxN + 1) & ~(size_t)1;
         const size_t lds = sizeof(double) * rows + (fp32 ? sizeof(PrimPairF) : sizeof(PrimPair)) * (size_t)maxBra;
         const dim3 blk(bs), grd(nblocks);  // one block per work unit
         if (fp32)
-            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, float>), grd, blk, lds, s, segs, blk0, nseg, nblocks,
-                               maxN, btab, scale);
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, float, C>), grd, blk, lds, s, segs, blk0, nseg,
+                               nblocks, maxN, btab, scale);
         else if (bm == BOYS_ACCURATE)
-            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, double>), grd, blk, lds, s, segs, blk0, nseg, nblocks,
-                               maxN, btab, scale);
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, double, C>), grd, blk, lds, s, segs, blk0, nseg,
+                               nblocks, maxN, btab, scale);
         else
-            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE, double>), grd, blk, lds, s, segs, blk0, nseg,
+            hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE, double, C>), grd, blk, lds, s, segs, blk0, nseg,
                                nblocks, maxN, btab, scale);
     }
     return hipGetLastError();
 }
 
+// J/K over nseg context segments: thread-per-quartet classes one block of bs
+// threads per work unit (LDS: the Kh rows of the bra's two shells and its
+// primitive records), wave-cooperative classes one 64-lane block per
+// WaveGroup::G quartets.
 template <int a, int b, int c, int d, int wave>
-hipError_t ClassLaunch<a, b, c, d, wave>::block(int bm, const int4* items, int n, const ShellPair* pairs,
-                                                const PrimPair* pp, const double* btab, double* out, double scale,
-                                                hipStream_t s)
+hipError_t ClassLaunch<a, b, c, d, wave>::jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg,
+                                             int nblocks, int bs, int maxN, int maxBra, int cart,
+                                             const double* btab, double scale, hipStream_t s)
+{
+    if constexpr (kHasD) {
+        if (cart) return jk_c<1>(bm, fp32, segs, blk0, nseg, nblocks, bs, maxN, maxBra, btab, scale, s);
+    }
+    return jk_c<0>(bm, fp32, segs, blk0, nseg, nblocks, bs, maxN, maxBra, btab, scale, s);
+}
+
+template <int a, int b, int c, int d, int wave>
+template <int C>
+hipError_t ClassLaunch<a, b, c, d, wave>::block_c(int bm, const int4* items, int n, const ShellPair* pairs,
+                                                  const PrimPair* pp, const double* btab, double* out, double scale,
+                                                  hipStream_t s)
 {
     if constexpr (wave) {
         if (bm == BOYS_ACCURATE)
-            wave_go<BOYS_ACCURATE, WAVE_BLOCK>(items, n, pairs, pp, btab, out, scale, s);
+            wave_go<BOYS_ACCURATE, WAVE_BLOCK, C>(items, n, pairs, pp, btab, out, scale, s);
         else
-            wave_go<BOYS_REFERENCE, WAVE_BLOCK>(items, n, pairs, pp, btab, out, scale, s);
+            wave_go<BOYS_REFERENCE, WAVE_BLOCK, C>(items, n, pairs, pp, btab, out, scale, s);
     } else {
         const dim3 blk(256), grd((n + 255) / 256);
         if (bm == BOYS_ACCURATE)
-            hipLaunchKernelGGL((k_eri_block<a, b, c, d, BOYS_ACCURATE>), grd, blk, 0, s, items, n, pairs, pp, btab,
+            hipLaunchKernelGGL((k_eri_block<a, b, c, d, BOYS_ACCURATE, C>), grd, blk, 0, s, items, n, pairs, pp, btab,
                                out, scale);
         else
-            hipLaunchKernelGGL((k_eri_block<a, b, c, d, BOYS_REFERENCE>), grd, blk, 0, s, items, n, pairs, pp, btab,
-                               out, scale);
+            hipLaunchKernelGGL((k_eri_block<a, b, c, d, BOYS_REFERENCE, C>), grd, blk, 0, s, items, n, pairs, pp,
+                               btab, out, scale);
     }
     return hipGetLastError();
 }
 
+template <int a, int b, int c, int d, int wave>
+hipError_t ClassLaunch<a, b, c, d, wave>::block(int bm, int cart, const int4* items, int n, const ShellPair* pairs,
+                                                const PrimPair* pp, const double* btab, double* out, double scale,
+                                                hipStream_t s)
+{
+    if constexpr (kHasD) {
+        if (cart) return block_c<1>(bm, items, n, pairs, pp, btab, out, scale, s);
+    }
+    return block_c<0>(bm, items, n, pairs, pp, btab, out, scale, s);
+}
+
 // Schwarz bounds exist for the diagonal classes (ab|ab) only.
 template <int a, int b, int c, int d, int wave>
-hipError_t ClassLaunch<a, b, c, d, wave>::schwarz(int bm, const int* plist, int n, const ShellPair* pairs,
-                                                  const PrimPair* pp, const double* btab, double* Q, double scale,
-                                                  hipStream_t s)
+template <int C>
+hipError_t ClassLaunch<a, b, c, d, wave>::schwarz_c(int bm, const int* plist, int n, const ShellPair* pairs,
+                                                    const PrimPair* pp, const double* btab, double* Q, double scale,
+                                                    hipStream_t s)
 {
     if constexpr (a != c || b != d) {
         return hipErrorInvalidValue;
     } else if constexpr (wave) {
         if (bm == BOYS_ACCURATE)
-            wave_go<BOYS_ACCURATE, WAVE_SCHWARZ>(plist, n, pairs, pp, btab, Q, scale, s);
+            wave_go<BOYS_ACCURATE, WAVE_SCHWARZ, C>(plist, n, pairs, pp, btab, Q, scale, s);
         else
-            wave_go<BOYS_REFERENCE, WAVE_SCHWARZ>(plist, n, pairs, pp, btab, Q, scale, s);
+            wave_go<BOYS_REFERENCE, WAVE_SCHWARZ, C>(plist, n, pairs, pp, btab, Q, scale, s);
         return hipGetLastError();
     } else {
         const dim3 blk(256), grd((n + 255) / 256);
         if (bm == BOYS_ACCURATE)
-            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_ACCURATE>), grd, blk, 0, s, plist, n, pairs, pp, btab, Q, scale);
+            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_ACCURATE, C>), grd, blk, 0, s, plist, n, pairs, pp, btab, Q,
+                               scale);
         else
-            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_REFERENCE>), grd, blk, 0, s, plist, n, pairs, pp, btab, Q, scale);
+            hipLaunchKernelGGL((k_schwarz<a, b, BOYS_REFERENCE, C>), grd, blk, 0, s, plist, n, pairs, pp, btab, Q,
+                               scale);
         return hipGetLastError();
     }
+}
+
+template <int a, int b, int c, int d, int wave>
+hipError_t ClassLaunch<a, b, c, d, wave>::schwarz(int bm, int cart, const int* plist, int n, const ShellPair* pairs,
+                                                  const PrimPair* pp, const double* btab, double* Q, double scale,
+                                                  hipStream_t s)
+{
+    if constexpr (kHasD) {
+        if (cart) return schwarz_c<1>(bm, plist, n, pairs, pp, btab, Q, scale, s);
+    }
+    return schwarz_c<0>(bm, plist, n, pairs, pp, btab, Q, scale, s);
 }
 
 // This translation unit's copy of the AO transform (each unit has its own
@@ -434,8 +479,14 @@ hipError_t ClassLaunch<a, b, c, d, wave>::upload_xform(const double* x, int n, h
 }
 
 template <int a, int b, int c, int d, int wave>
-int ClassLaunch<a, b, c, d, wave>::wave_group()
+int ClassLaunch<a, b, c, d, wave>::wave_group(int cart)
 {
-    if constexpr (wave) return WaveGroup<a, b, c, d>::G;
-    else return 1;
+    if constexpr (wave) {
+        if constexpr (kHasD) {
+            if (cart) return WaveGroup<a, b, c, d, 1>::G;
+        }
+        return WaveGroup<a, b, c, d, 0>::G;
+    } else {
+        return 1;
+    }
 }

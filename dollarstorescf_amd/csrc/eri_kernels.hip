@@ -148,8 +148,9 @@ __global__ void __launch_bounds__(64) k_one_electron(const int* __restrict__ pli
                                                      const PrimPair* __restrict__ pp,
                                                      const double* __restrict__ atoms, int natom,
                                                      const double* __restrict__ btab,
-                                                     const double* __restrict__ xform, double* __restrict__ S,
-                                                     double* __restrict__ T, double* __restrict__ V)
+                                                     const double* __restrict__ xform, int cart,
+                                                     double* __restrict__ S, double* __restrict__ T,
+                                                     double* __restrict__ V)
 {
     using namespace one_e;
     constexpr double kPi = 3.14159265358979;
@@ -252,11 +253,12 @@ __global__ void __launch_bounds__(64) k_one_electron(const int* __restrict__ pli
                     }
             v[ia * NB + ib] = w;
         }
-    // Cartesian -> AO and packed-U stores
-    const double* xa = xform + xform_off(la);
-    const double* xb = xform + xform_off(lb);
-    for (int ma = 0; ma < 2 * la + 1; ++ma)
-        for (int mb = 0; mb < 2 * lb + 1; ++mb) {
+    // Cartesian -> AO and packed-U stores (cart: d shells as Cartesian functions)
+    const double* xa = xform + xform_off(la, cart);
+    const double* xb = xform + xform_off(lb, cart);
+    const int nfa = cart && la >= 2 ? NA : 2 * la + 1, nfb = cart && lb >= 2 ? NB : 2 * lb + 1;
+    for (int ma = 0; ma < nfa; ++ma)
+        for (int mb = 0; mb < nfb; ++mb) {
             double ss = 0.0, tt = 0.0, vv = 0.0;
             for (int ia = 0; ia < NA; ++ia) {
                 const double wa = xa[ma * NA + ia];
@@ -278,7 +280,7 @@ __global__ void __launch_bounds__(64) k_one_electron(const int* __restrict__ pli
 
 hipError_t dsscf_launch_one_electron(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
                                      const PrimPair* pp, const double* atoms, int natom, const double* btab,
-                                     const double* xform, double* S, double* T, double* V, hipStream_t s)
+                                     const double* xform, int cart, double* S, double* T, double* V, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const dim3 blk(64), grd((n + 63) / 64);
@@ -286,10 +288,10 @@ hipError_t dsscf_launch_one_electron(int la, int lb, int boys_mode, const int* p
     if (la == A_ && lb == B_) {                                                                                  \
         if (boys_mode == BOYS_ACCURATE)                                                                          \
             hipLaunchKernelGGL((k_one_electron<A_, B_, BOYS_ACCURATE>), grd, blk, 0, s, plist, n, pairs, pp, atoms, \
-                               natom, btab, xform, S, T, V);                                                     \
+                               natom, btab, xform, cart, S, T, V);                                               \
         else                                                                                                     \
             hipLaunchKernelGGL((k_one_electron<A_, B_, BOYS_REFERENCE>), grd, blk, 0, s, plist, n, pairs, pp,     \
-                               atoms, natom, btab, xform, S, T, V);                                              \
+                               atoms, natom, btab, xform, cart, S, T, V);                                        \
         return hipGetLastError();                                                                                \
     }
     DSSCF_ONE_E(0, 0)
@@ -335,25 +337,25 @@ void dk_class_l(int c, int* l)
 }
 
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                           int nblocks, int block_size, int maxN, int maxBra, const double* btab, double scale,
-                           hipStream_t s)
+                           int nblocks, int block_size, int maxN, int maxBra, int cart, const double* btab,
+                           double scale, hipStream_t s)
 {
     if (nseg <= 0 || nblocks <= 0) return hipSuccess;
     int c = 0;
 #define DSSCF_LAUNCH_JK(a, b, cc, d, name, wave)                                                                 \
     if (c++ == cls)                                                                                              \
         return ClassLaunch<a, b, cc, d, wave>::jk(boys_mode, fp32, segs, blk0, nseg, nblocks, block_size, maxN,   \
-                                                  maxBra, btab, scale, s);
+                                                  maxBra, cart, btab, scale, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_JK)
 #undef DSSCF_LAUNCH_JK
     return hipErrorInvalidValue;
 }
 
-int dsscf_wave_group(int cls)
+int dsscf_wave_group(int cls, int cart)
 {
     int c = 0;
 #define DSSCF_WAVE_G(a, b, cc, d, name, wave) \
-    if (c++ == cls) return ClassLaunch<a, b, cc, d, wave>::wave_group();
+    if (c++ == cls) return ClassLaunch<a, b, cc, d, wave>::wave_group(cart);
     DSSCF_FOR_EACH_CLASS(DSSCF_WAVE_G)
 #undef DSSCF_WAVE_G
     return 1;
@@ -369,26 +371,28 @@ int dsscf_class_is_wave(int cls)
     return 0;
 }
 
-hipError_t dsscf_launch_eri_block(int cls, int boys_mode, const int4* items, int n, const ShellPair* pairs,
+hipError_t dsscf_launch_eri_block(int cls, int boys_mode, int cart, const int4* items, int n, const ShellPair* pairs,
                                   const PrimPair* pp, const double* btab, double* out, double scale,
                                   hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     int c = 0;
-#define DSSCF_LAUNCH_BLK(a, b, cc, d, name, wave) \
-    if (c++ == cls) return ClassLaunch<a, b, cc, d, wave>::block(boys_mode, items, n, pairs, pp, btab, out, scale, s);
+#define DSSCF_LAUNCH_BLK(a, b, cc, d, name, wave)                                                                    \
+    if (c++ == cls)                                                                                                 \
+        return ClassLaunch<a, b, cc, d, wave>::block(boys_mode, cart, items, n, pairs, pp, btab, out, scale, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_BLK)
 #undef DSSCF_LAUNCH_BLK
     return hipErrorInvalidValue;
 }
 
-hipError_t dsscf_launch_schwarz(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
-                                const PrimPair* pp, const double* btab, double* Q, double scale, hipStream_t s)
+hipError_t dsscf_launch_schwarz(int la, int lb, int boys_mode, int cart, const int* plist, int n,
+                                const ShellPair* pairs, const PrimPair* pp, const double* btab, double* Q, double scale,
+                                hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-#define DSSCF_LAUNCH_SCH(a, b, cc, d, name, wave)                                                       \
-    if (a == cc && b == d && la == a && lb == b)                                                          \
-        return ClassLaunch<a, b, cc, d, wave>::schwarz(boys_mode, plist, n, pairs, pp, btab, Q, scale, s);
+#define DSSCF_LAUNCH_SCH(a, b, cc, d, name, wave)                                                             \
+    if (a == cc && b == d && la == a && lb == b)                                                                \
+        return ClassLaunch<a, b, cc, d, wave>::schwarz(boys_mode, cart, plist, n, pairs, pp, btab, Q, scale, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_SCH)
 #undef DSSCF_LAUNCH_SCH
     return hipErrorInvalidValue;

@@ -301,9 +301,10 @@ def gen_hrr(la, lb, lc, ld):
     return lines, kn
 
 
-def gen_xform(la, lb, lc, ld, kn, spherical=True):
-    """Cartesian -> AO, out[((ia*NB+ib)*NC+ic)*ND+id]."""
-    pats = [ao_index_pattern(l, spherical) for l in (la, lb, lc, ld)]
+def gen_xform(la, lb, lc, ld, kn, cartd=False):
+    """Cartesian -> AO, out[((ia*NB+ib)*NC+ic)*ND+id].  cartd: shells with
+    l >= 2 are Cartesian functions (their block of c_ao_xform is diagonal)."""
+    pats = [ao_index_pattern(l, not (cartd and l >= 2)) for l in (la, lb, lc, ld)]
     cs = [cart(l) for l in (la, lb, lc, ld)]
     nb_, nc_, nd_ = (len(p) for p in pats[1:])
     lines = []
@@ -319,7 +320,8 @@ def gen_xform(la, lb, lc, ld, kn, spherical=True):
         return "t3_%d_%d_%d_%d" % (a, mb, mc, md)
 
     def coef(l, m, ci):
-        return "c_ao_xform[%d]" % (xform_offset(l) + m * ncart(l) + ci)
+        off = xform_offset_cart(l) if cartd and l >= 2 else xform_offset(l)
+        return "c_ao_xform[%d]" % (off + m * ncart(l) + ci)
 
     for a in range(len(cs[0])):
         for b in range(len(cs[1])):
@@ -355,6 +357,20 @@ def xform_offset(l):
     return off
 
 
+def xform_offset_cart(l, maxl=MAXL):
+    """Offset of the ncart(l) x ncart(l) Cartesian block of l >= 2, after
+    the spherical blocks of l = 0..maxl."""
+    return xform_offset(maxl + 1) + sum(ncart(k) ** 2 for k in range(2, l))
+
+
+def xform_size(maxl=MAXL):
+    return xform_offset_cart(maxl + 1, maxl)
+
+
+def has_d(c):
+    return max(c) >= 2
+
+
 def classes(maxl=MAXL):
     pairs = [(a, b) for a in range(maxl + 1) for b in range(a + 1)]
     out = []
@@ -371,20 +387,20 @@ def cls_name(c):
     return "".join("spdf"[l] for l in c)
 
 
-def gen_class(c, spherical=True):
+def gen_class(c, cartd=False):
     la, lb, lc, ld = c
     L = sum(c)
     vrr, targets = gen_vrr(la, lb, lc, ld)
     hrr, kn = gen_hrr(la, lb, lc, ld)
-    xf = gen_xform(la, lb, lc, ld, kn, spherical)
-    na = [nao(l, spherical) for l in c]
+    xf = gen_xform(la, lb, lc, ld, kn, cartd)
+    na = [nao(l, not (cartd and l >= 2)) for l in c]
     NAO = na[0] * na[1] * na[2] * na[3]
-    name = cls_name(c)
+    name = cls_name(c) + ("_cart" if cartd else "")
     accs = sorted({(e, f) for (e, f, _) in targets})
     o = []
     o.append("// ---- class (%s) : L=%d, %d VRR terms, %d AO integrals" % (name, L, len(vrr), NAO))
-    o.append("template <> struct ClassInfo<%d,%d,%d,%d> { static constexpr int L = %d, NAO = %d, NA = %d, NB = %d, NC = %d, ND = %d; };"
-             % (la, lb, lc, ld, L, NAO, *na))
+    o.append("template <> struct ClassInfo<%d,%d,%d,%d,%d> { static constexpr int L = %d, NAO = %d, NA = %d, NB = %d, NC = %d, ND = %d; };"
+             % (la, lb, lc, ld, 1 if cartd else 0, L, NAO, *na))
     o.append("// A group of gb*gk lanes (aligned, power of two) may share one quartet:")
     o.append("// lane (rb, rk) takes bra primitives rb, rb+gb, .. and ket primitives rk,")
     o.append("// rk+gk, ..; the [e0|f0] sums are then added across the group, so every")
@@ -592,7 +608,9 @@ def hrr_structure(la, lb, lc, ld, n_targets, tidx):
     return entries, levels, cart_y, nxt
 
 
-def gen_wave_class(c, spherical=True):
+def gen_wave_class(c, cartd=False):
+    """cartd: the Cartesian-d form of the class, which shares the spherical
+    form's VRR/HRR tables (generated before it) and differs in its AO count."""
     la, lb, lc, ld = c
     L = sum(c)
     name = cls_name(c)
@@ -619,7 +637,7 @@ def gen_wave_class(c, spherical=True):
     tidx = {(e, f): n for n, (e, f) in enumerate(tlist)}
     tgt = [xi[(e, f, 0)] for (e, f) in tlist]
     hent, hlev, cart_y, ny = hrr_structure(la, lb, lc, ld, len(tlist), tidx)
-    na = [nao(l, spherical) for l in c]
+    na = [nao(l, not (cartd and l >= 2)) for l in c]
     NAO = na[0] * na[1] * na[2] * na[3]
     nx = zero + 1
     # the staged AO transform writes its 1st and 3rd stages after the HRR
@@ -628,21 +646,22 @@ def gen_wave_class(c, spherical=True):
     st3 = ncart(la) * na[1] * na[2] * na[3]
     lds = max(nx, ny + max(st1, st3), ny + NAO)
     o = []
-    o.append("// ---- class (%s) : wave-cooperative, %d VRR entries in %d levels, %d HRR entries, LDS %d doubles"
-             % (name, len(keys), nlev, len(hent), lds))
-    o.append("template <> struct ClassInfo<%d,%d,%d,%d> { static constexpr int L = %d, NAO = %d, NA = %d, NB = %d, NC = %d, ND = %d; };"
-             % (la, lb, lc, ld, L, NAO, *na))
+    o.append("// ---- class (%s%s) : wave-cooperative, %d VRR entries in %d levels, %d HRR entries, LDS %d doubles"
+             % (name, ", Cartesian d" if cartd else "", len(keys), nlev, len(hent), lds))
+    o.append("template <> struct ClassInfo<%d,%d,%d,%d,%d> { static constexpr int L = %d, NAO = %d, NA = %d, NB = %d, NC = %d, ND = %d; };"
+             % (la, lb, lc, ld, 1 if cartd else 0, L, NAO, *na))
     def arr(t, nm, vals):
         return "__device__ const %s %s[%d] = {%s};" % (t, nm, max(len(vals), 1), ",".join(vals) if vals else "0")
-    o.append(arr("int", "wv_%s_vrr" % name, ["%d" % v for r in vrr_rows for v in r]))
-    o.append(arr("int", "wv_%s_vlev" % name, ["%d" % v for v in lev_off]))
-    o.append(arr("short", "wv_%s_base" % name, ["%d" % v for v in base]))
-    o.append(arr("short", "wv_%s_tgt" % name, ["%d" % v for v in tgt]))
-    o.append(arr("int", "wv_%s_hrr" % name, ["%d" % (s1 | (s2 << 16)) for (s1, s2, i, st) in hent] ))
-    o.append(arr("char", "wv_%s_hdir" % name, ["%d" % (i | (st << 2)) for (s1, s2, i, st) in hent]))
-    o.append(arr("int", "wv_%s_hlev" % name, ["%d" % v for lv in hlev for v in lv]))
-    o.append(arr("short", "wv_%s_cart" % name, ["%d" % v for v in cart_y]))
-    o.append("template <> struct WaveTab<%d,%d,%d,%d> {" % c)
+    if not cartd:
+        o.append(arr("int", "wv_%s_vrr" % name, ["%d" % v for r in vrr_rows for v in r]))
+        o.append(arr("int", "wv_%s_vlev" % name, ["%d" % v for v in lev_off]))
+        o.append(arr("short", "wv_%s_base" % name, ["%d" % v for v in base]))
+        o.append(arr("short", "wv_%s_tgt" % name, ["%d" % v for v in tgt]))
+        o.append(arr("int", "wv_%s_hrr" % name, ["%d" % (s1 | (s2 << 16)) for (s1, s2, i, st) in hent] ))
+        o.append(arr("char", "wv_%s_hdir" % name, ["%d" % (i | (st << 2)) for (s1, s2, i, st) in hent]))
+        o.append(arr("int", "wv_%s_hlev" % name, ["%d" % v for lv in hlev for v in lv]))
+        o.append(arr("short", "wv_%s_cart" % name, ["%d" % v for v in cart_y]))
+    o.append("template <> struct WaveTab<%d,%d,%d,%d,%d> {" % (*c, 1 if cartd else 0))
     o.append("  static constexpr int L = %d, NBASE = %d, NVE = %d, ZERO = %d, NLEV = %d, NT = %d, NHE = %d, NHLEV = %d,"
              " NY = %d, NCART = %d, NAO = %d, LDS = %d;" % (L, nbase, len(vrr_rows), zero, nlev, len(tlist), len(hent),
                                                            len(hlev), ny, len(cart_y), NAO, lds))
@@ -653,21 +672,46 @@ def gen_wave_class(c, spherical=True):
     return "\n".join(o)
 
 
-def gen_ao_patterns(maxl, spherical=True):
+def gen_ao_patterns(maxl):
     cnt, idx = [], []
     for l in range(maxl + 1):
-        for pat in ao_index_pattern(l, spherical):
+        for pat in ao_index_pattern(l, True):
             cnt.append(len(pat))
             idx.extend(pat + [0] * (3 - len(pat)))
-    return ["// Cartesian components used by AO m of an l-shell (AO order l=0..maxl, m=-l..l).",
+    for l in range(2, maxl + 1):
+        for pat in ao_index_pattern(l, False):
+            cnt.append(len(pat))
+            idx.extend(pat + [0] * (3 - len(pat)))
+    cart_base = [(maxl + 1) ** 2 + sum(ncart(k) for k in range(2, l)) for l in range(maxl + 1)]
+    xo = [xform_offset(l) for l in range(maxl + 1)]
+    xc = [xform_offset_cart(l, maxl) if l >= 2 else xform_offset(l) for l in range(maxl + 1)]
+    return ["// Cartesian components used by AO m of an l-shell: the real solid harmonics",
+            "// (l = 0..maxl, m = -l..l), then the Cartesian functions of l = 2..maxl.",
             "__device__ const char ao_pat_n[%d] = {%s};" % (len(cnt), ",".join(map(str, cnt))),
-            "__device__ const char ao_pat_c[%d] = {%s};" % (len(idx), ",".join(map(str, idx)))]
+            "__device__ const char ao_pat_c[%d] = {%s};" % (len(idx), ",".join(map(str, idx))),
+            "// Pattern row and c_ao_xform block of AO m of an l-shell; C = 1: shells",
+            "// with l >= 2 are Cartesian (x-major components, diagonal transform).",
+            "__device__ __forceinline__ constexpr int ao_pat_row(int l, int m, int C) {",
+            "  constexpr int cb[%d] = {%s};" % (maxl + 1, ",".join(map(str, cart_base))),
+            "  return C && l >= 2 ? cb[l] + m : l * l + m;",
+            "}",
+            "__device__ __forceinline__ constexpr int xform_off(int l, int C = 0) {",
+            "  constexpr int xs[%d] = {%s}, xc[%d] = {%s};" % (maxl + 1, ",".join(map(str, xo)), maxl + 1,
+                                                              ",".join(map(str, xc))),
+            "  return C ? xc[l] : xs[l];",
+            "}"]
 
 
 def gen_dispatch(cls_list):
     o = ["#define DSSCF_FOR_EACH_CLASS(X) \\"]
     for c in cls_list:
         o.append("  X(%d, %d, %d, %d, %s, %d) \\" % (*c, cls_name(c), 1 if is_wave(c) else 0))
+    o.append("")
+    o.append("// the classes with a d shell, which also have a Cartesian-d form")
+    o.append("#define DSSCF_FOR_EACH_D_CLASS(X) \\")
+    for c in cls_list:
+        if has_d(c):
+            o.append("  X(%d, %d, %d, %d, %s, %d) \\" % (*c, cls_name(c), 1 if is_wave(c) else 0))
     o.append("")
     return "\n".join(o)
 
@@ -676,18 +720,20 @@ def main(out_path, maxl=MAXL):
     pairs, cls_list = classes(maxl)
     text = ["// Generated by gen_eri.py -- do not edit.", "#pragma once", ""]
     text.append("#define DSSCF_MAXL %d" % maxl)
-    text.append("#define DSSCF_XFORM_SIZE %d" % xform_offset(maxl + 1))
+    text.append("#define DSSCF_XFORM_SIZE %d" % xform_size(maxl))
     text.append("#define DSSCF_NCLASS %d" % len(cls_list))
     text.append(gen_dispatch(cls_list))
-    text.append("// Cartesian -> reference-AO coefficients, per l: (2l+1) x ncart(l), row-major.")
+    text.append("// Cartesian -> AO coefficients: per l = 0..maxl a (2l+1) x ncart(l) row-major block")
+    text.append("// (the reference's real solid harmonics), then per l = 2..maxl an ncart(l) x ncart(l)")
+    text.append("// block (Cartesian functions, diagonal).")
     text.append("static __constant__ double c_ao_xform[DSSCF_XFORM_SIZE];  // one copy per translation unit")
     text.append("")
     text.extend(gen_ao_patterns(maxl))
-    text.append("__device__ __forceinline__ constexpr int xform_off(int l) { return l == 0 ? 0 : (l == 1 ? 1 : (l == 2 ? 10 : 40)); }")
     text.append("")
     for c in cls_list:
-        text.append(gen_wave_class(c) if is_wave(c) else gen_class(c))
-        text.append("")
+        for cartd in ((False, True) if has_d(c) else (False,)):
+            text.append(gen_wave_class(c, cartd) if is_wave(c) else gen_class(c, cartd))
+            text.append("")
     open(out_path, "w").write("\n".join(text))
 
 

@@ -109,6 +109,34 @@ ContractedGaussian::ContractedGaussian(const std::vector<GTOPrimitive>& prims, u
     }
 }
 
+ContractedGaussian ContractedGaussian::cartesianFunction(const std::vector<GTOPrimitive>& prims, unsigned l_, int k,
+                                                         const vec3& c)
+{
+    ContractedGaussian g(prims, l_, 0, c);  // primitive weights as the reference's
+    g.m = k;
+    g.cartesian = true;
+    // component k in x-major order: (l,0,0), (l-1,1,0), (l-1,0,1), ...
+    int idx = 0;
+    for (int a = (int)l_; a >= 0; --a)
+        for (int b = (int)l_ - a; b >= 0; --b, ++idx)
+            if (idx == k) {
+                CartesianPolynomialTerm t;
+                t.x = (unsigned)a;
+                t.y = (unsigned)b;
+                t.z = l_ - (unsigned)a - (unsigned)b;
+                t.weight = cartesianWeight(l_, t.x, t.y, t.z);
+                g.spherical_harmonic.assign(1, t);
+            }
+    return g;
+}
+
+double ContractedGaussian::cartesianWeight(unsigned l_, unsigned a, unsigned b, unsigned c)
+{
+    const double den = double_factorial_d(2.0 * a - 1.0) * double_factorial_d(2.0 * b - 1.0) *
+                       double_factorial_d(2.0 * c - 1.0);
+    return std::sqrt(double_factorial_d(2.0 * l_ + 1.0) / (4.0 * PI * den));
+}
+
 // ------------------------------------------------------------- parsing --
 static std::vector<std::string> split_ws(const std::string& s)
 {
@@ -331,7 +359,7 @@ void oneElectronMatrices(const std::vector<ContractedGaussian>& basis, const Mol
     const int N = (int)basis.size();
     std::vector<Shell> shells;
     for (int i = 0; i < N; ++i)
-        if (basis[i].m == -(int)basis[i].l) shells.push_back({i, (int)basis[i].l, &basis[i]});
+        if (basis[i].m == (basis[i].cartesian ? 0 : -(int)basis[i].l)) shells.push_back({i, (int)basis[i].l, &basis[i]});
     const size_t np = (size_t)N * (N + 1) / 2;
     S.assign(np, 0.0);
     T.assign(np, 0.0);
@@ -464,9 +492,10 @@ void oneElectronMatrices(const std::vector<ContractedGaussian>& basis, const Mol
                                                                cb.xyz[ib][0], cb.xyz[ib][1], cb.xyz[ib][2], 0);
                     }
                 }
-            // Cartesian -> AO with each AO's term list
-            for (int ma = 0; ma < 2 * la + 1; ++ma)
-                for (int mb = 0; mb < 2 * lb + 1; ++mb) {
+            // Cartesian -> AO with each AO's term list (a Cartesian d shell has 6 AOs)
+            const int nfa = A.ao->cartesian ? ca.n : 2 * la + 1, nfb = B.ao->cartesian ? cb.n : 2 * lb + 1;
+            for (int ma = 0; ma < nfa; ++ma)
+                for (int mb = 0; mb < nfb; ++mb) {
                     const ContractedGaussian& ga = basis[A.first_ao + ma];
                     const ContractedGaussian& gb = basis[B.first_ao + mb];
                     double ss = 0, tt = 0, vv = 0;

@@ -36,8 +36,19 @@ struct ContractedGaussian {
     unsigned l = 0;
     int m = 0;
 
+    bool cartesian = false;  // a Cartesian function of the shell (m = component index)
+
     ContractedGaussian() = default;
     ContractedGaussian(const std::vector<GTOPrimitive>& prims, unsigned l, int m, const vec3& center);
+    // Cartesian function x^a y^b z^c, component k of the l-shell in x-major
+    // order, with the weight sqrt((2l+1)!! / (4 pi (2a-1)!! (2b-1)!! (2c-1)!!))
+    // that normalizes each of its primitives (their weights as the
+    // reference's, GTO.cpp:90-93).  No reference counterpart: the reference
+    // builds real solid harmonics only.
+    static ContractedGaussian cartesianFunction(const std::vector<GTOPrimitive>& prims, unsigned l, int k,
+                                                const vec3& center);
+    // The weight of x^a y^b z^c above.
+    static double cartesianWeight(unsigned l, unsigned a, unsigned b, unsigned c);
 };
 
 struct GTOShell {  // GTO.hpp:44-51

@@ -78,6 +78,16 @@ int dsscf_job_set_step(dsscf_job* j, int on_device)
     return DSSCF_OK;
 }
 
+int dsscf_job_set_cartesian(dsscf_job* j, int on)
+{
+    if (!j || (on != 0 && on != 1)) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_cartesian: bad argument");
+    if (j->job.opt.cartesian == on) return DSSCF_OK;
+    if (j->job.ctx || !j->job.basis.empty())
+        return set_err(j, DSSCF_ERR_STATE, "dsscf_job_set_cartesian: set before constructing the job");
+    j->job.opt.cartesian = on;
+    return DSSCF_OK;
+}
+
 int dsscf_job_set_molecule(dsscf_job* j, int natom, const int* Z, const double* xyz, int charge, int mult)
 {
     if (!j || natom <= 0 || !Z || !xyz || mult < 1) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_molecule: bad argument");

@@ -267,6 +267,11 @@ void row(std::ofstream& f, const Cell* cells, std::string* text, int n)
 
 int writeOrbitalsMolden(const Job& job, const std::string& path, bool write_virtual, std::string* error)
 {
+    for (const auto& g : job.basis)
+        if (g.cartesian) {  // the reference's writer knows the 5D order only
+            if (error) *error = "writeOrbitalsMolden: Cartesian d functions are not written (the reference's [GTO]/[MO] layout is 5D)";
+            return DSSCF_ERR_UNSUPPORTED;
+        }
     if (job.basis.empty() || job.C.size() != (size_t)job.n * job.n || job.eps.size() != (size_t)job.n) {
         if (error) *error = "writeOrbitalsMolden: no orbitals (solve the job or set them first)";
         return DSSCF_ERR_STATE;

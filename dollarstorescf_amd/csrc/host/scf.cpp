@@ -158,7 +158,7 @@ int shellData(const Job& job, ShellData& out, std::string* error)
     out = ShellData();
     const int maxl = dsscf_max_l();
     for (const auto& ao : job.basis) {
-        if (ao.m != -(int)ao.l) continue;
+        if (ao.m != (ao.cartesian ? 0 : -(int)ao.l)) continue;  // first AO of each shell
         if ((int)ao.l > maxl) {
             if (error) *error = "angular momentum " + std::to_string(ao.l) + " not built (max " + std::to_string(maxl) + ")";
             return DSSCF_ERR_UNSUPPORTED;
@@ -185,6 +185,19 @@ int shellData(const Job& job, ShellData& out, std::string* error)
             }
         off += (size_t)(2 * l + 1) * nc;
     }
+    for (int l = 2; l <= maxl; ++l) {  // Cartesian functions of l >= 2: diagonal blocks
+        const int nc = (l + 1) * (l + 2) / 2;
+        int k = 0;
+        for (int a = l; a >= 0; --a)
+            for (int b = l - a; b >= 0; --b, ++k)
+                out.xform[off + (size_t)k * nc + k] =
+                    flo::ContractedGaussian::cartesianWeight((unsigned)l, (unsigned)a, (unsigned)b, (unsigned)(l - a - b));
+        off += (size_t)nc * nc;
+    }
+    if (off != out.xform.size()) {
+        if (error) *error = "AO transform size differs from the library's";
+        return DSSCF_ERR_STATE;
+    }
     return DSSCF_OK;
 }
 
@@ -204,11 +217,20 @@ static int buildAOs(Job& job)
     job.basis_atoms.clear();
     job.atom_basis.assign(1, 0);
     for (size_t i = 0; i < job.molecule.size(); ++i) {
-        for (const auto& sh : job.atom_bases[i])
+        for (const auto& sh : job.atom_bases[i]) {
+            if (job.opt.cartesian && sh.l >= 2) {  // x-major Cartesian functions
+                for (int k = 0; k < (int)((sh.l + 1) * (sh.l + 2) / 2); ++k) {
+                    job.basis.push_back(
+                        flo::ContractedGaussian::cartesianFunction(sh.primitives, sh.l, k, job.molecule[i].position));
+                    job.basis_atoms.push_back((unsigned)i);
+                }
+                continue;
+            }
             for (int m = -(int)sh.l; m <= (int)sh.l; ++m) {
                 job.basis.emplace_back(sh.primitives, sh.l, m, job.molecule[i].position);
                 job.basis_atoms.push_back((unsigned)i);
             }
+        }
         job.atom_basis.push_back((unsigned)job.basis.size());
     }
     job.n = (int)job.basis.size();
@@ -290,6 +312,7 @@ int constructDevice(Job& job)
     }
     dsscf_options o{job.opt.boys_mode, job.opt.schwarz_threshold, job.opt.fp32_threshold, job.opt.prim_threshold};
     o.jcd_max_bytes = job.opt.jcd_max_bytes;
+    o.cartesian = job.opt.cartesian;
     int rc = dsscf_set_options(job.ctx, &o);
     if (!rc)
         rc = dsscf_set_basis(job.ctx, (int)sd.l.size(), sd.l.data(), sd.nprim.data(), sd.center.data(), sd.exps.data(),

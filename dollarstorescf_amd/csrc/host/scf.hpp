@@ -30,6 +30,7 @@ struct Options {
     int device = 0;
     int device_step = 0;               // 1: DIIS, X F X, diagonalization and density on the device
     double jcd_max_bytes = 0.0;        // dsscf_options::jcd_max_bytes of the job's context
+    int cartesian = 0;                 // d shells as 6 Cartesian functions (dsscf_options::cartesian)
     int verbose = 0;                   // print the iteration table like SCFSolver.cpp:198-225
 };
 

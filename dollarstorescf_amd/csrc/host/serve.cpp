@@ -129,7 +129,8 @@ double Server::estimate_bytes(const Job& job) const
     const double quartets = pairs * (pairs + 1.0) / 2.0;
     const double n = job.n;
     const double step = device_steps_ ? (double)scfdev::device_bytes_for(job.n, job.opt.diis_size) : 0.0;
-    const double rec = 8.0 * (2 * lmax + 1) * (2 * lmax + 1) + 8.0;
+    const int nf = job.opt.cartesian && lmax >= 2 ? (lmax + 1) * (lmax + 2) / 2 : 2 * lmax + 1;
+    const double rec = 8.0 * nf * nf + 8.0;
     const double slots = job.opt.jcd_max_bytes > 0.0 ? std::min(job.opt.jcd_max_bytes, rec * quartets) : 0.0;
     return 6.0 * quartets + 8.0 * n * n * 34.0 + 32.0 * n * (n + 1.0) / 2.0 + 76.0 * pairs +
            64.0 * prims * prims / 2.0 + 256.0 * 1024.0 + step + slots;

@@ -19,23 +19,25 @@ void dsscf_upload_xform(const double* x, int n, hipStream_t s);
 // maxBra: most primitive pairs of any unit's bra (sizes the LDS bra records);
 // nblocks: work units (thread-per-quartet classes) or blocks
 // (wave-cooperative classes, which ignore maxBra).
+// cart: d shells as Cartesian functions (the classes' Cartesian-d form).
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                           int nblocks, int block_size, int maxN, int maxBra, const double* btab, double scale,
-                           hipStream_t s);
+                           int nblocks, int block_size, int maxN, int maxBra, int cart, const double* btab,
+                           double scale, hipStream_t s);
 int dsscf_class_is_wave(int cls);
 // Shell quartets per wavefront of a wave-cooperative class (1 for the others).
-int dsscf_wave_group(int cls);
-hipError_t dsscf_launch_eri_block(int cls, int boys_mode, const int4* items, int n, const ShellPair* pairs,
+int dsscf_wave_group(int cls, int cart);
+hipError_t dsscf_launch_eri_block(int cls, int boys_mode, int cart, const int4* items, int n, const ShellPair* pairs,
                                   const PrimPair* pp, const double* btab, double* out, double scale,
                                   hipStream_t s);
-hipError_t dsscf_launch_schwarz(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
-                                const PrimPair* pp, const double* btab, double* Q, double scale, hipStream_t s);
+hipError_t dsscf_launch_schwarz(int la, int lb, int boys_mode, int cart, const int* plist, int n,
+                                const ShellPair* pairs, const PrimPair* pp, const double* btab, double* Q, double scale,
+                                hipStream_t s);
 hipError_t dsscf_launch_fill(double* p, size_t n, double v, hipStream_t s);
 // S, T, V (packed-U) of the pairs plist[0..n) of pair class (la, lb), la >= lb;
 // atoms: x, y, z, Z per nucleus; xform: the AO transform (device copy).
 hipError_t dsscf_launch_one_electron(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
                                      const PrimPair* pp, const double* atoms, int natom, const double* btab,
-                                     const double* xform, double* S, double* T, double* V, hipStream_t s);
+                                     const double* xform, int cart, double* S, double* T, double* V, hipStream_t s);
 hipError_t dsscf_launch_prep(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);
 hipError_t dsscf_launch_fold(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);
 // Sum the J_cd slot records into copy 0 of Jh: nblocks = rows over all segments.

@@ -7,17 +7,28 @@
 
 template <int a, int b, int c, int d, int wave>
 struct ClassLaunch {
+    // cart: the class's Cartesian-d form (ignored by classes without a d shell)
     static hipError_t jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int bs,
-                         int maxN, int maxBra, const double* btab, double scale, hipStream_t s);
-    static hipError_t block(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
+                         int maxN, int maxBra, int cart, const double* btab, double scale, hipStream_t s);
+    static hipError_t block(int bm, int cart, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
                             const double* btab, double* out, double scale, hipStream_t s);
-    static hipError_t schwarz(int bm, const int* plist, int n, const ShellPair* pairs, const PrimPair* pp,
+    static hipError_t schwarz(int bm, int cart, const int* plist, int n, const ShellPair* pairs, const PrimPair* pp,
                               const double* btab, double* Q, double scale, hipStream_t s);
     static hipError_t upload_xform(const double* x, int n, hipStream_t s);
-    static int wave_group();
+    static int wave_group(int cart);
 
 private:
-    template <int BM, int MODE>
+    static constexpr bool kHasD = a >= 2 || b >= 2 || c >= 2 || d >= 2;
+    template <int C>
+    static hipError_t jk_c(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg, int nblocks, int bs,
+                           int maxN, int maxBra, const double* btab, double scale, hipStream_t s);
+    template <int C>
+    static hipError_t block_c(int bm, const int4* items, int n, const ShellPair* pairs, const PrimPair* pp,
+                              const double* btab, double* out, double scale, hipStream_t s);
+    template <int C>
+    static hipError_t schwarz_c(int bm, const int* plist, int n, const ShellPair* pairs, const PrimPair* pp,
+                                const double* btab, double* Q, double scale, hipStream_t s);
+    template <int BM, int MODE, int C>
     static void wave_go(const void* items, int n, const ShellPair* pairs, const PrimPair* pp, const double* btab,
                         double* out, double scale, hipStream_t s);
 };

@@ -25,9 +25,9 @@ __device__ __forceinline__ double pick3(int d, double x, double y, double z) { r
 #ifndef DSSCF_WAVE_LDS_BUDGET
 #define DSSCF_WAVE_LDS_BUDGET 40960
 #endif
-template <int la, int lb, int lc, int ld>
+template <int la, int lb, int lc, int ld, int C = 0>
 struct WaveGroup {
-    static constexpr int bytes = (int)sizeof(double) * WaveTab<la, lb, lc, ld>::LDS;
+    static constexpr int bytes = (int)sizeof(double) * WaveTab<la, lb, lc, ld, C>::LDS;
     static constexpr int G = 8 * bytes <= DSSCF_WAVE_LDS_BUDGET   ? 8
                              : 4 * bytes <= DSSCF_WAVE_LDS_BUDGET ? 4
                              : 2 * bytes <= DSSCF_WAVE_LDS_BUDGET ? 2
@@ -44,16 +44,16 @@ __device__ __forceinline__ int wave_max(int v)
 // MODE WAVE_JK: block b covers quartets G*(b - blk0) .. +G-1 of the segment
 // its index falls in (items, n, pairs, pp unused).  WAVE_BLOCK / WAVE_SCHWARZ:
 // items over one context, grid-stride over groups of G.
-template <int la, int lb, int lc, int ld, int BM, int MODE>
+template <int la, int lb, int lc, int ld, int BM, int MODE, int C>
 __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int n,
                                              const ShellPair* __restrict__ pairs_in,
                                              const PrimPair* __restrict__ pp_in, const double* __restrict__ btab,
                                              const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
                                              int nseg, double* __restrict__ out, double scale)
 {
-    typedef WaveTab<la, lb, lc, ld> W;
-    typedef ClassInfo<la, lb, lc, ld> CI;
-    constexpr int G = WaveGroup<la, lb, lc, ld>::G;
+    typedef WaveTab<la, lb, lc, ld, C> W;
+    typedef ClassInfo<la, lb, lc, ld, C> CI;
+    constexpr int G = WaveGroup<la, lb, lc, ld, C>::G;
     constexpr int LG = 64 / G;
     constexpr int L = W::L;
     constexpr int SLOTS = (W::NT + LG - 1) / LG;
@@ -196,11 +196,11 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             double* __restrict__ R1 = sm + W::NY;
             double* __restrict__ R2 = sm;
             for (int md = 0; md < ND; ++md) {  // T1[ca][cb][cc][md]
-                const int pat = ld * ld + md, nt = ao_pat_n[pat];
+                const int pat = ao_pat_row(ld, md, C), nt = ao_pat_n[pat];
                 const int c0 = ao_pat_c[3 * pat], c1 = ao_pat_c[3 * pat + 1], c2 = ao_pat_c[3 * pat + 2];
-                const double w0 = c_ao_xform[xform_off(ld) + md * ncD + c0];
-                const double w1 = nt > 1 ? c_ao_xform[xform_off(ld) + md * ncD + c1] : 0.0;
-                const double w2 = nt > 2 ? c_ao_xform[xform_off(ld) + md * ncD + c2] : 0.0;
+                const double w0 = c_ao_xform[xform_off(ld, C) + md * ncD + c0];
+                const double w1 = nt > 1 ? c_ao_xform[xform_off(ld, C) + md * ncD + c1] : 0.0;
+                const double w2 = nt > 2 ? c_ao_xform[xform_off(ld, C) + md * ncD + c2] : 0.0;
                 for (int r = lg; r < ncA * ncB * ncC; r += LG) {
                     const short* cy = carty + r * ncD;
                     double sum = w0 * sm[cy[c0]];
@@ -211,11 +211,11 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             }
             __syncthreads();
             for (int mc = 0; mc < NC; ++mc) {  // T2[ca][cb][mc][md]
-                const int pat = lc * lc + mc, nt = ao_pat_n[pat];
+                const int pat = ao_pat_row(lc, mc, C), nt = ao_pat_n[pat];
                 const int c0 = ao_pat_c[3 * pat], c1 = ao_pat_c[3 * pat + 1], c2 = ao_pat_c[3 * pat + 2];
-                const double w0 = c_ao_xform[xform_off(lc) + mc * ncC + c0];
-                const double w1 = nt > 1 ? c_ao_xform[xform_off(lc) + mc * ncC + c1] : 0.0;
-                const double w2 = nt > 2 ? c_ao_xform[xform_off(lc) + mc * ncC + c2] : 0.0;
+                const double w0 = c_ao_xform[xform_off(lc, C) + mc * ncC + c0];
+                const double w1 = nt > 1 ? c_ao_xform[xform_off(lc, C) + mc * ncC + c1] : 0.0;
+                const double w2 = nt > 2 ? c_ao_xform[xform_off(lc, C) + mc * ncC + c2] : 0.0;
                 for (int o = lg; o < ncA * ncB * ND; o += LG) {
                     const int md = o % ND, r = o / ND;
                     const double* in = R1 + r * ncC * ND + md;
@@ -227,11 +227,11 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             }
             __syncthreads();
             for (int mb = 0; mb < NB; ++mb) {  // T3[ca][mb][mc][md]
-                const int pat = lb * lb + mb, nt = ao_pat_n[pat];
+                const int pat = ao_pat_row(lb, mb, C), nt = ao_pat_n[pat];
                 const int c0 = ao_pat_c[3 * pat], c1 = ao_pat_c[3 * pat + 1], c2 = ao_pat_c[3 * pat + 2];
-                const double w0 = c_ao_xform[xform_off(lb) + mb * ncB + c0];
-                const double w1 = nt > 1 ? c_ao_xform[xform_off(lb) + mb * ncB + c1] : 0.0;
-                const double w2 = nt > 2 ? c_ao_xform[xform_off(lb) + mb * ncB + c2] : 0.0;
+                const double w0 = c_ao_xform[xform_off(lb, C) + mb * ncB + c0];
+                const double w1 = nt > 1 ? c_ao_xform[xform_off(lb, C) + mb * ncB + c1] : 0.0;
+                const double w2 = nt > 2 ? c_ao_xform[xform_off(lb, C) + mb * ncB + c2] : 0.0;
                 for (int o = lg; o < ncA * NC * ND; o += LG) {
                     const int q = o % (NC * ND), ca = o / (NC * ND);
                     const double* in = R2 + ca * ncB * NC * ND + q;
@@ -243,11 +243,11 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             }
             __syncthreads();
             for (int ma = 0; ma < NA; ++ma) {  // V[ma][mb][mc][md]
-                const int pat = la * la + ma, nt = ao_pat_n[pat];
+                const int pat = ao_pat_row(la, ma, C), nt = ao_pat_n[pat];
                 const int c0 = ao_pat_c[3 * pat], c1 = ao_pat_c[3 * pat + 1], c2 = ao_pat_c[3 * pat + 2];
-                const double w0 = c_ao_xform[xform_off(la) + ma * ncA + c0];
-                const double w1 = nt > 1 ? c_ao_xform[xform_off(la) + ma * ncA + c1] : 0.0;
-                const double w2 = nt > 2 ? c_ao_xform[xform_off(la) + ma * ncA + c2] : 0.0;
+                const double w0 = c_ao_xform[xform_off(la, C) + ma * ncA + c0];
+                const double w1 = nt > 1 ? c_ao_xform[xform_off(la, C) + ma * ncA + c1] : 0.0;
+                const double w2 = nt > 2 ? c_ao_xform[xform_off(la, C) + ma * ncA + c2] : 0.0;
                 for (int q = lg; q < NB * NC * ND; q += LG) {
                     const double* in = R1 + q;
                     double sum = w0 * in[c0 * NB * NC * ND];

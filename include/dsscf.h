@@ -70,6 +70,11 @@ typedef struct {
                                  (distance from the bra from DSSCF_KET_DISTANCE_QUARTETS unscreened
                                  quartets on, else shuffled), 1 = shuffled, 2 = by distance; the
                                  results agree to rounding */
+    int cartesian;            /* 1: shells with l >= 2 are ncart(l) Cartesian functions x^a y^b z^c
+                                 (x-major: xx, xy, xz, yy, yz, zz; the "6D" convention of 6-31G*),
+                                 each primitive normalized; 0: the reference's 2l+1 real solid
+                                 harmonics (its only form, src/chem/GTO.cpp:15-94).  No reference
+                                 counterpart; s and p shells are the same either way */
 } dsscf_options;
 
 #define DSSCF_WORK_ITEM_FLOPS 16384.0
@@ -114,13 +119,16 @@ int dsscf_set_options(dsscf_ctx* ctx, const dsscf_options* opt);
 long long dsscf_ctx_device_bytes(dsscf_ctx* ctx);
 
 /* Shell data, shells in AO order (atom -> shell in basis-file order, each
- * shell contributing 2l+1 AOs m=-l..l: src/chem/scf/SCFBasis.cpp:120-140).
+ * shell contributing 2l+1 AOs m=-l..l: src/chem/scf/SCFBasis.cpp:120-140;
+ * with dsscf_options.cartesian, l >= 2 shells contribute ncart(l) AOs).
  *   shell_l[n_shell], shell_nprim[n_shell], shell_center[3*n_shell] (bohr)
  *   prim_exp[], prim_coef[]  -- the reference's normalized primitive weights
  *                               (GTOPrimitive::weight, src/chem/GTO.cpp:90-93)
  *   ao_xform[dsscf_xform_size()] -- for each l = 0..max_l, a (2l+1) x ncart(l)
  *       row-major matrix: weight of Cartesian monomial c (x-major order) in
- *       AO m (ContractedGaussian::spherical_harmonic, src/chem/GTO.cpp:15-88)
+ *       AO m (ContractedGaussian::spherical_harmonic, src/chem/GTO.cpp:15-88);
+ *       then for each l = 2..max_l an ncart(l) x ncart(l) matrix, the
+ *       weights of the Cartesian AOs (diagonal)
  * Replaces the std::vector<ContractedGaussian> basis that
  * assignExactRepulsionTensor reads (ExactCoulomb.cpp:41-72, SCFCommon.hpp:10). */
 int dsscf_set_basis(dsscf_ctx* ctx, int n_shell, const int* shell_l, const int* shell_nprim,

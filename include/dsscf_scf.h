@@ -37,6 +37,13 @@ int dsscf_job_set_precision(dsscf_job* job, double fp32_threshold, double prim_t
  * energies and tr(Pa G) come to the host.  The device step keeps at most 16
  * DIIS vectors.  Used by dsscf_job_solve. */
 int dsscf_job_set_step(dsscf_job* job, int on_device);
+/* on = 1: d shells become the six Cartesian functions xx, xy, xz, yy, yz, zz
+ * (each primitive normalized; "6D", as 6-31G* is often run) instead of the
+ * reference's five real solid harmonics (src/chem/GTO.cpp:15-94,
+ * src/chem/scf/SCFBasis.cpp:129-134, its only form).  No reference
+ * counterpart.  Before the job is constructed.  Molden output and nuclear
+ * gradients refuse such jobs. */
+int dsscf_job_set_cartesian(dsscf_job* job, int on);
 /* scf::assignMolecule (SCFBasis.cpp:17-37); coordinates in bohr. */
 int dsscf_job_set_molecule(dsscf_job* job, int natom, const int* Z, const double* xyz_bohr, int charge,
                            int multiplicity);

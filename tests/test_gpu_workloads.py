@@ -91,13 +91,15 @@ def test_mixed_precision_c20h42_def2svp():
 
 
 def test_batch_512_jobs_fused_equal_alone():
-    """Config 3's 512 seeded jobs (1-6 waters, 6-31G*), solved together with
-    fused, graph-replayed Fock builds and autotuned block sizes: every energy
-    within 1e-9 Eh of the same job solved alone, iterations within one."""
+    """Config 3's 512 seeded jobs (1-6 waters, 6-31G* with Cartesian d, 19-114
+    AOs), solved together with fused, graph-replayed Fock builds and
+    autotuned block sizes: every energy within 1e-9 Eh of the same job solved
+    alone, iterations within one."""
     specs = bench.batch_specs(512)
 
     def make(Z, xyz):
-        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12)
+        job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12,
+                                    cartesian=True)
         return job.set_basis("6-31GP").set_molecule(Z, xyz)
 
     jobs = ds.construct_jobs([make(Z, xyz) for Z, xyz in specs])

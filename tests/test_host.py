@@ -60,6 +60,27 @@ def test_host_aos_and_one_electron_match_oracle(basis):
     assert np.max(np.abs(job.get("V") - V)) < 1e-11
 
 
+@pytest.mark.parametrize("basis", ["6-31GP", "def2-SVP"])
+def test_host_cartesian_aos_and_one_electron_match_oracle(basis):
+    """Cartesian-d jobs: the host AO list (6 x-major components per d shell,
+    each with its normalizing weight) and S, T, V equal the oracle's
+    Cartesian construction (pinned in test_oracle.py)."""
+    Z, xyz = G.water_box(2, seed=5)
+    job = ds.Job().set_options(boys_mode=ds.BOYS_REFERENCE, cartesian=True).set_basis(basis).set_molecule(Z, xyz)
+    job.construct_host()
+    aos = O.build_aos(O.read_basis(ds.basis_path(basis)), Z, xyz, cartesian=True)
+    assert job.n_ao == len(aos) and sum(a.l == 2 for a in aos) == 12
+    for i, a in enumerate(aos):
+        h = job.ao(i)
+        assert (h["l"], h["m"]) == (a.l, a.m)
+        assert [t[1:] for t in h["terms"]] == [tuple(t[1:]) for t in a.terms]
+        assert np.allclose([t[0] for t in h["terms"]], [t[0] for t in a.terms], rtol=1e-15, atol=0)
+    S, T, V = O.OracleBasis(aos, Z, xyz).one_electron(0)
+    assert np.max(np.abs(job.get("S") - S)) < 1e-13
+    assert np.max(np.abs(job.get("T") - T)) < 1e-12
+    assert np.max(np.abs(job.get("V") - V)) < 1e-11
+
+
 def test_host_one_electron_accurate_mode_shift_small():
     """V with the accurate Boys differs from the reference series by < 1e-9."""
     Z, xyz = G.water()

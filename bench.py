@@ -348,13 +348,26 @@ def bench_batch(a):
 
     specs = batch_specs(a.jobs)
 
-    def make(Z, xyz):
+    def make(Z, xyz, schedule=None):
         job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-8, max_iterations=100, schwarz_threshold=1e-12,
                                     cartesian=True)
-        return job.set_basis("6-31GP").set_molecule(Z, xyz)
+        return job.set_basis("6-31GP").set_molecule(Z, xyz).set_schedule(schedule)
+
+    # the project's schedule autotuner on a sample of the same workload (every
+    # 16th job, 1-6 waters): per class sub-pair size, lane groups, kets per
+    # unit and block size, then applied to all jobs (set-up, not timed)
+    t0 = time.perf_counter()
+    sample = ds.construct_jobs([make(Z, xyz) for Z, xyz in specs[::16]])
+    tb = ds.Batch(0)
+    tb.set([j.context for j in sample], graph=False)
+    schedule = tb.autotune_schedule()
+    tb.close()
+    for j in sample:
+        j.close()
+    t_tune = time.perf_counter() - t0
 
     t0 = time.perf_counter()
-    jobs = ds.construct_jobs([make(Z, xyz) for Z, xyz in specs])
+    jobs = ds.construct_jobs([make(Z, xyz, schedule) for Z, xyz in specs])
     t_construct_fused = time.perf_counter() - t0
     n_ao = [j.n_ao for j in jobs]
     t0 = time.perf_counter()
@@ -384,6 +397,11 @@ def bench_batch(a):
         "config": {"workload": f"{len(specs)} RHF/6-31G* jobs (Cartesian d, 6D), 1-6 waters in a 6 A box, min O-O "
                                "2.6 A, seed 2024; Schwarz 1e-12; converge |dE| < 1e-8",
                    "n_ao_min": int(min(n_ao)), "n_ao_max": int(max(n_ao)), "parallelism": "single GPU"},
+        "schedule": {"tune_seconds": t_tune, "sample_jobs": len(specs[::16]),
+                     "per_class": {"".join("spdf"[l] for l in c): {k: (float(v[i]) if k != "kets_per_unit" and
+                                                                     k != "block_size" else int(v[i]))
+                                                                 for k, v in schedule.items()}
+                                   for i, c in enumerate(ds.class_list())}},
         "fused": {"solves_per_s": len(specs) / t_fused, "seconds": t_fused, "construct_seconds": t_construct_fused,
                   "iterations_total": int(np.sum(it_fused)), "failed": int(np.count_nonzero(st))},
         "loop": {"solves_per_s": len(specs) / t_loop, "seconds": t_loop, "construct_seconds": t_construct_loop,

@@ -95,7 +95,9 @@ def lib():
         for name, args in {"dsscf_batch_create": [p, i], "dsscf_batch_destroy": [p], "dsscf_batch_set": [p, p, i],
                            "dsscf_batch_set_graph": [p, i], "dsscf_batch_jk": [p, p, p, p, p],
                            "dsscf_batch_set_device": [p, p, i, p, p, p, p], "dsscf_batch_jk_device": [p],
-                           "dsscf_batch_time": [p, i, i, p], "dsscf_batch_autotune": [p, p]}.items():
+                           "dsscf_batch_time": [p, i, i, p], "dsscf_batch_autotune": [p, p],
+                           "dsscf_batch_autotune_schedule": [p, p, p, p, p], "dsscf_set_class_schedule": [p, p, p, p],
+                           "dsscf_job_set_schedule": [p, p, p, p]}.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = i
@@ -424,6 +426,17 @@ class Batch:
         _check(lib().dsscf_batch_autotune(self._h, _p(out)))
         return out
 
+    def autotune_schedule(self):
+        """The schedule autotuner (dsscf_batch_autotune_schedule): per class the
+        sub-pair FLOP size, lane-group target, kets per work unit and threads
+        per block that run the members' own lists fastest (0 = default).  The
+        members end prepared with it; returns a dict of per-class arrays."""
+        n = lib().dsscf_class_count()
+        wif, gp = np.zeros(n), np.zeros(n)
+        ch, bs = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        _check(lib().dsscf_batch_autotune_schedule(self._h, _p(wif), _p(gp), _p(ch), _p(bs)))
+        return {"work_item_flops": wif, "group_prims": gp, "kets_per_unit": ch, "block_size": bs}
+
 
 class Job:
     """One SCF job: molecule + basis -> AOs, 1e terms, device Fock builds, RHF."""
@@ -536,6 +549,19 @@ class Job:
         g1, gn = np.empty(3 * na), np.empty(3 * na)
         _check(lib().dsscf_job_gradient_terms(self._h, _p(Pt), _p(W), _p(g1), _p(gn)), self._err)
         return g1.reshape(na, 3), gn.reshape(na, 3)
+
+    def set_schedule(self, schedule):
+        """Per-class list schedule for this job's context (a dict as
+        Batch.autotune_schedule returns; None = defaults).  Before construct."""
+        if schedule is None:
+            _check(lib().dsscf_job_set_schedule(self._h, None, None, None), self._err)
+            return self
+        wif = np.ascontiguousarray(schedule["work_item_flops"], np.float64)
+        gp = np.ascontiguousarray(schedule["group_prims"], np.float64)
+        ch = np.ascontiguousarray(schedule["kets_per_unit"], np.int32)
+        self._sched = (wif, gp, ch)
+        _check(lib().dsscf_job_set_schedule(self._h, _p(wif), _p(gp), _p(ch)), self._err)
+        return self
 
     def write_molden(self, path: str, write_virtual: bool = True):
         """Molden file of the current orbitals (scf::writeOrbitalsMolden)."""

@@ -293,6 +293,11 @@ struct dsscf_ctx {
 
     std::vector<int> block_size;  // per class, thread-per-quartet kernels (autotuned)
     std::vector<int> chunk;       // per class, kets per work unit
+    // per-class schedule overrides for prepare (dsscf_set_class_schedule; 0 or
+    // empty: the options' value / the default): sub-pair FLOP size,
+    // lane-group target, kets per work unit
+    std::vector<double> sched_wif, sched_gp;
+    std::vector<int> sched_chunk;
     Plan plan;                    // fused launch plan over this context alone
 
     void free_geometry()
@@ -669,11 +674,12 @@ int dsscf_prepare(dsscf_ctx* c)
     struct ClassJob {
         int X, Y, cls, P, kChunk;
         bool wave, split;
+        double gp;  // lane-group target (primitive quartets per lane)
     };
     std::vector<ClassJob> cj;
     for (int X = 0; X < npc; ++X)
         for (int Y = 0; Y <= X; ++Y) {
-            ClassJob j{X, Y, X * (X + 1) / 2 + Y, 0, 0, false, false};
+            ClassJob j{X, Y, X * (X + 1) / 2 + Y, 0, 0, false, false, group_prims};
             j.wave = dsscf_class_is_wave(j.cls) != 0;
             j.split = thr32 > 0.0 && !j.wave;
             const size_t nx = by_class[X].size(), ny = by_class[Y].size();
@@ -681,6 +687,9 @@ int dsscf_prepare(dsscf_ctx* c)
             c->n_unscreened += est;
             // kets per work unit: enough units to fill the chip (~2048), 64..512, multiple of 64
             j.kChunk = (int)std::min<long long>(512, std::max<long long>(64, (est / 2048 + 63) / 64 * 64));
+            if (!c->sched_chunk.empty() && c->sched_chunk[j.cls] > 0)
+                j.kChunk = std::min(512, std::max(64, (c->sched_chunk[j.cls] + 63) / 64 * 64));
+            if (!c->sched_gp.empty() && c->sched_gp[j.cls] != 0.0) j.gp = c->sched_gp[j.cls];
             c->chunk[j.cls] = j.kChunk;
             // primitive pairs per sub-pair, both sides: P^2 primitive quartets ~ item_flops,
             // and at least kMinSubPrims (shorter runs cost more in per-item digestion and
@@ -688,7 +697,8 @@ int dsscf_prepare(dsscf_ctx* c)
             int l4[4];
             dk_class_l(j.cls, l4);
             const double fq = prim_quartet_flops(l4[0], l4[1], l4[2], l4[3]);
-            j.P = item_flops < 0.0 ? (1 << 30) : std::max(kMinSubPrims, (int)std::sqrt(item_flops / fq));
+            const double wif = !c->sched_wif.empty() && c->sched_wif[j.cls] != 0.0 ? c->sched_wif[j.cls] : item_flops;
+            j.P = wif < 0.0 ? (1 << 30) : std::max(kMinSubPrims, (int)std::sqrt(wif / fq));
             cj.push_back(j);
         }
     // every sub-pair any class will use, created before the parallel part
@@ -828,12 +838,12 @@ int dsscf_prepare(dsscf_ctx* c)
                     // kets come by primitive count, so lanes per quartet only fall along
                     // the list: a unit keeps one group size and about kChunk lanes
                     for (size_t o = 0; o < ksub.size();) {
-                        const int grp = quartet_group(work[sb].nprim, work[ksub[o]].nprim, group_prims);
+                        const int grp = quartet_group(work[sb].nprim, work[ksub[o]].nprim, j.gp);
                         const int G = (grp & 0xff) * (grp >> 8);
                         const size_t lim = std::max<size_t>(1, (size_t)j.kChunk / G);
                         size_t e = o + 1;
                         while (e < ksub.size() && e - o < lim &&
-                               quartet_group(work[sb].nprim, work[ksub[e]].nprim, group_prims) == grp)
+                               quartet_group(work[sb].nprim, work[ksub[e]].nprim, j.gp) == grp)
                             ++e;
                         const int cnt = (int)(e - o);
                         long long cost = 0;  // primitive quartets of the unit
@@ -897,7 +907,7 @@ int dsscf_prepare(dsscf_ctx* c)
         std::stable_sort(units32.begin(), units32.end(), by_cost);
         // the device reads the lanes per quartet where the sort key was
         auto set_group = [&](int4& u) {
-            u.w = quartet_group(work[u.x].nprim, work[kets[u.y]].nprim, group_prims);
+            u.w = quartet_group(work[u.x].nprim, work[kets[u.y]].nprim, cj[jn].gp);
         };
         for (size_t k = (size_t)u0; k < units.size(); ++k) set_group(units[k]);
         for (int4& u : units32) set_group(u);
@@ -2041,31 +2051,64 @@ int dsscf_batch_time(dsscf_batch* b, int warmup, int iters, double* ms_per_build
 // class kernel alone at 64, 128 and 256 threads per block (3 launches each,
 // HIP events) and keep the fastest; the result goes into the batch's plan
 // and, for a single-context batch, into the context.
+}  // extern "C"
+
+namespace {
+// A pair of timing events, destroyed on every path.
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~EventPair()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    int create()
+    {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        return DSSCF_OK;
+    }
+};
+
+// Device time (ms) of `reps` launches of one class's kernels on the batch's
+// current plan, after one warm-up launch.
+int time_class(dsscf_batch* b, int cls, EventPair& ev, int reps, float* ms)
+{
+    if (int rc = plan_launch_class(b->plan, cls, b->stream)) return rc;  // warm
+    HIP_TRY(hipEventRecord(ev.e0, b->stream));
+    for (int r = 0; r < reps; ++r)
+        if (int rc = plan_launch_class(b->plan, cls, b->stream)) return rc;
+    HIP_TRY(hipEventRecord(ev.e1, b->stream));
+    HIP_TRY(hipEventSynchronize(ev.e1));
+    HIP_TRY(hipEventElapsedTime(ms, ev.e0, ev.e1));
+    return DSSCF_OK;
+}
+
+bool class_in_plan(const Plan& p, int cls)
+{
+    return !dsscf_class_is_wave(cls) && (p.cls_nseg[2 * cls] > 0 || p.cls_nseg[2 * cls + 1] > 0);
+}
+}  // namespace
+
+extern "C" {
+
 int dsscf_batch_autotune(dsscf_batch* b, int* chosen)
 {
     if (!b || !b->plan.built) return fail(DSSCF_ERR_STATE, "dsscf_batch_autotune: call dsscf_batch_set first");
     HIP_TRY(hipSetDevice(b->device));
     const int ncls = dk_n_class();
     const int cand[3] = {64, 128, 256};
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    EventPair ev;
+    if (int rc = ev.create()) return rc;
     std::vector<int> best = b->block_size;
     for (int cls = 0; cls < ncls; ++cls) {
-        if (dsscf_class_is_wave(cls) || (b->plan.cls_nseg[2 * cls] == 0 && b->plan.cls_nseg[2 * cls + 1] == 0))
-            continue;
+        if (!class_in_plan(b->plan, cls)) continue;
         float best_ms = 1e30f;
         for (int ci = 0; ci < 3; ++ci) {
             b->block_size[cls] = cand[ci];
             if (int rc = batch_build_plan(b)) return rc;
-            if (int rc = plan_launch_class(b->plan, cls, b->stream)) return rc;  // warm
-            HIP_TRY(hipEventRecord(e0, b->stream));
-            for (int r = 0; r < 3; ++r)
-                if (int rc = plan_launch_class(b->plan, cls, b->stream)) return rc;
-            HIP_TRY(hipEventRecord(e1, b->stream));
-            HIP_TRY(hipEventSynchronize(e1));
             float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (int rc = time_class(b, cls, ev, 3, &ms)) return rc;
             if (ms < best_ms) {
                 best_ms = ms;
                 best[cls] = cand[ci];
@@ -2073,8 +2116,6 @@ int dsscf_batch_autotune(dsscf_batch* b, int* chosen)
         }
         b->block_size[cls] = best[cls];
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     b->block_size = best;
     if (int rc = batch_build_plan(b)) return rc;
     if (b->ctxs.size() == 1) {
@@ -2083,6 +2124,87 @@ int dsscf_batch_autotune(dsscf_batch* b, int* chosen)
     }
     if (chosen)
         for (int k = 0; k < ncls; ++k) chosen[k] = best[k];
+    return DSSCF_OK;
+}
+
+int dsscf_set_class_schedule(dsscf_ctx* c, const double* work_item_flops, const double* group_prims,
+                             const int* kets_per_unit)
+{
+    if (!c) return fail(DSSCF_ERR_ARG, "dsscf_set_class_schedule: null context");
+    const int ncls = dk_n_class();
+    if (work_item_flops) c->sched_wif.assign(work_item_flops, work_item_flops + ncls); else c->sched_wif.clear();
+    if (group_prims) c->sched_gp.assign(group_prims, group_prims + ncls); else c->sched_gp.clear();
+    if (kets_per_unit) c->sched_chunk.assign(kets_per_unit, kets_per_unit + ncls); else c->sched_chunk.clear();
+    for (int k = 0; k < (int)c->sched_chunk.size(); ++k)
+        if (c->sched_chunk[k] < 0 || c->sched_chunk[k] > 512)
+            return fail(DSSCF_ERR_ARG, "dsscf_set_class_schedule: kets per unit 0..512");
+    c->prepared = false;
+    return DSSCF_OK;
+}
+
+int dsscf_batch_autotune_schedule(dsscf_batch* b, double* work_item_flops, double* group_prims, int* kets_per_unit,
+                                  int* block_size)
+{
+    if (!b || !b->plan.built) return fail(DSSCF_ERR_STATE, "dsscf_batch_autotune_schedule: call dsscf_batch_set first");
+    HIP_TRY(hipSetDevice(b->device));
+    const int ncls = dk_n_class();
+    // candidates of each schedule parameter; 0 = the context's own default
+    const std::vector<double> c_wif = {0.0, 4096.0, 8192.0, 32768.0, 65536.0};
+    const std::vector<double> c_gp = {0.0, 4.0, 16.0, -1.0};
+    const std::vector<double> c_chunk = {0.0, 64.0, 128.0, 256.0, 512.0};
+    std::vector<double> wif(ncls, 0.0), gp(ncls, 0.0);
+    std::vector<int> chunk(ncls, 0);
+    EventPair ev;
+    if (int rc = ev.create()) return rc;
+    // apply the schedule to every member and rebuild its lists and the fused plan
+    auto apply = [&]() -> int {
+        for (dsscf_ctx* c : b->ctxs) {
+            c->sched_wif = wif;
+            c->sched_gp = gp;
+            c->sched_chunk = chunk;
+            c->prepared = false;
+            if (int rc = dsscf_prepare(c)) return rc;
+        }
+        return batch_build_plan(b);
+    };
+    // one parameter at a time (sub-pair size, lane groups, kets per unit),
+    // every class timed at each candidate; each class keeps its fastest
+    for (int param = 0; param < 3; ++param) {
+        const std::vector<double>& cand = param == 0 ? c_wif : (param == 1 ? c_gp : c_chunk);
+        std::vector<float> best_ms(ncls, 1e30f);
+        std::vector<double> best_v(ncls, 0.0);
+        for (double v : cand) {
+            for (int k = 0; k < ncls; ++k) {
+                if (param == 0) wif[k] = v;
+                else if (param == 1) gp[k] = v;
+                else chunk[k] = (int)v;
+            }
+            // the parameters already chosen stay at their per-class choice
+            if (int rc = apply()) return rc;
+            for (int cls = 0; cls < ncls; ++cls) {
+                if (!class_in_plan(b->plan, cls)) continue;
+                float ms = 0.0f;
+                if (int rc = time_class(b, cls, ev, 3, &ms)) return rc;
+                if (ms < best_ms[cls]) {
+                    best_ms[cls] = ms;
+                    best_v[cls] = v;
+                }
+            }
+        }
+        for (int k = 0; k < ncls; ++k) {
+            if (param == 0) wif[k] = best_v[k];
+            else if (param == 1) gp[k] = best_v[k];
+            else chunk[k] = (int)best_v[k];
+        }
+    }
+    if (int rc = apply()) return rc;
+    // block sizes for the chosen lists
+    if (int rc = dsscf_batch_autotune(b, block_size)) return rc;
+    for (int k = 0; k < ncls; ++k) {
+        if (work_item_flops) work_item_flops[k] = wif[k];
+        if (group_prims) group_prims[k] = gp[k];
+        if (kets_per_unit) kets_per_unit[k] = chunk[k];
+    }
     return DSSCF_OK;
 }
 

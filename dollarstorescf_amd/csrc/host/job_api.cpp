@@ -88,6 +88,19 @@ int dsscf_job_set_cartesian(dsscf_job* j, int on)
     return DSSCF_OK;
 }
 
+int dsscf_job_set_schedule(dsscf_job* j, const double* work_item_flops, const double* group_prims,
+                           const int* kets_per_unit)
+{
+    if (!j) return DSSCF_ERR_ARG;
+    if (j->job.ctx) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_set_schedule: set before constructing the job");
+    const int ncls = dsscf_class_count();
+    scf::Options& o = j->job.opt;
+    if (work_item_flops) o.sched_wif.assign(work_item_flops, work_item_flops + ncls); else o.sched_wif.clear();
+    if (group_prims) o.sched_gp.assign(group_prims, group_prims + ncls); else o.sched_gp.clear();
+    if (kets_per_unit) o.sched_chunk.assign(kets_per_unit, kets_per_unit + ncls); else o.sched_chunk.clear();
+    return DSSCF_OK;
+}
+
 int dsscf_job_set_molecule(dsscf_job* j, int natom, const int* Z, const double* xyz, int charge, int mult)
 {
     if (!j || natom <= 0 || !Z || !xyz || mult < 1) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_molecule: bad argument");

@@ -315,6 +315,10 @@ int constructDevice(Job& job)
     o.cartesian = job.opt.cartesian;
     int rc = dsscf_set_options(job.ctx, &o);
     if (!rc)
+        rc = dsscf_set_class_schedule(job.ctx, job.opt.sched_wif.empty() ? nullptr : job.opt.sched_wif.data(),
+                                      job.opt.sched_gp.empty() ? nullptr : job.opt.sched_gp.data(),
+                                      job.opt.sched_chunk.empty() ? nullptr : job.opt.sched_chunk.data());
+    if (!rc)
         rc = dsscf_set_basis(job.ctx, (int)sd.l.size(), sd.l.data(), sd.nprim.data(), sd.center.data(), sd.exps.data(),
                              sd.coefs.data(), sd.xform.data());
     if (!rc) rc = dsscf_prepare(job.ctx);

@@ -31,6 +31,9 @@ struct Options {
     int device_step = 0;               // 1: DIIS, X F X, diagonalization and density on the device
     double jcd_max_bytes = 0.0;        // dsscf_options::jcd_max_bytes of the job's context
     int cartesian = 0;                 // d shells as 6 Cartesian functions (dsscf_options::cartesian)
+    // per-class list schedule for the context (dsscf_set_class_schedule; empty = default)
+    std::vector<double> sched_wif, sched_gp;
+    std::vector<int> sched_chunk;
     int verbose = 0;                   // print the iteration table like SCFSolver.cpp:198-225
 };
 

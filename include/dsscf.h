@@ -251,6 +251,28 @@ int dsscf_batch_time(dsscf_batch* batch, int warmup, int iters, double* ms_per_b
  * receives the choice (may be null). */
 int dsscf_batch_autotune(dsscf_batch* batch, int* chosen);
 
+/* Per-class schedule of a context's quartet lists, used by its next
+ * dsscf_prepare (arrays of dsscf_class_count() entries; a null array or a 0
+ * entry keeps the options' value or the default):
+ *   work_item_flops -- sub-pair size (dsscf_options.work_item_flops) per class
+ *   group_prims     -- lane-group target (dsscf_options.group_prims) per class
+ *   kets_per_unit   -- quartets per work unit (one block), 64..512 (rounded up
+ *                      to a multiple of 64) instead of the size-based default
+ * No reference counterpart.  Results agree to rounding for any schedule. */
+int dsscf_set_class_schedule(dsscf_ctx* ctx, const double* work_item_flops, const double* group_prims,
+                             const int* kets_per_unit);
+
+/* The project's schedule autotuner: for each class of the batch's members,
+ * times the class kernels on the members' own lists at several sub-pair
+ * sizes, lane-group targets and kets per unit (one parameter at a time,
+ * re-preparing the members for each candidate) and keeps per class the
+ * fastest of each, then picks threads per block as dsscf_batch_autotune.
+ * The members end prepared with the chosen schedule; the arrays
+ * (dsscf_class_count() entries each, any may be null) receive it, ready for
+ * dsscf_set_class_schedule on other contexts of the same kind of workload. */
+int dsscf_batch_autotune_schedule(dsscf_batch* batch, double* work_item_flops, double* group_prims,
+                                  int* kets_per_unit, int* block_size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,12 @@ int dsscf_job_set_step(dsscf_job* job, int on_device);
  * counterpart.  Before the job is constructed.  Molden output and nuclear
  * gradients refuse such jobs. */
 int dsscf_job_set_cartesian(dsscf_job* job, int on);
+/* The per-class list schedule the job's context is prepared with
+ * (dsscf_set_class_schedule; arrays of dsscf_class_count() entries, null =
+ * default), e.g. from dsscf_batch_autotune_schedule on a sample of similar
+ * jobs.  Before the job is constructed. */
+int dsscf_job_set_schedule(dsscf_job* job, const double* work_item_flops, const double* group_prims,
+                           const int* kets_per_unit);
 /* scf::assignMolecule (SCFBasis.cpp:17-37); coordinates in bohr. */
 int dsscf_job_set_molecule(dsscf_job* job, int natom, const int* Z, const double* xyz_bohr, int charge,
                            int multiplicity);

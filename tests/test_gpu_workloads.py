@@ -175,3 +175,37 @@ def test_iteration_cap_reports_noconv_alone_and_served():
     srv.close()
     assert done[0]["status"] == 6
     assert done[1]["status"] == 0
+
+
+def test_schedule_autotuner_keeps_results():
+    """The schedule autotuner (sub-pair size, lane groups, kets per unit,
+    block size per class, chosen by timing the members' own lists) returns
+    values from its candidate sets, and the members' Fock builds with the
+    chosen schedule equal those with the default schedule (1e-12 of max |J|);
+    a job constructed with the schedule (Job.set_schedule) matches too."""
+    specs = [("6-31GP", G.water_box(3, seed=4)), ("6-31GP", G.water_box(5, seed=6)), ("def2-SVP", G.alkane(3))]
+    jobs, dens, ref = [], [], []
+    rng = np.random.default_rng(17)
+    for basis, (Z, xyz) in specs:
+        j = ds.Job(0).set_options(schwarz_threshold=1e-12).set_basis(basis).set_molecule(Z, xyz).construct()
+        C = rng.standard_normal((j.n_ao, 6)) * 0.3
+        Pa = O.pack(C @ C.T)
+        ref.append(j.context.jk(2 * Pa, Pa))
+        jobs.append(j)
+        dens.append(Pa)
+    b = ds.Batch(0)
+    b.set([j.context for j in jobs], graph=False)
+    sch = b.autotune_schedule()
+    assert set(sch["work_item_flops"]) <= {0.0, 4096.0, 8192.0, 32768.0, 65536.0}
+    assert set(sch["group_prims"]) <= {0.0, 4.0, 16.0, -1.0}
+    assert set(int(v) for v in sch["kets_per_unit"]) <= {0, 64, 128, 256, 512}
+    J, K = b.jk([2 * p for p in dens], dens)
+    for (Jr, Kr), Jb, Kb in zip(ref, J, K):
+        assert np.max(np.abs(Jb - Jr)) < 1e-12 * np.max(np.abs(Jr))
+        assert np.max(np.abs(Kb - Kr)) < 1e-12 * np.max(np.abs(Kr))
+    b.close()
+    basis, (Z, xyz) = specs[1]
+    j = ds.Job(0).set_options(schwarz_threshold=1e-12).set_basis(basis).set_molecule(Z, xyz).set_schedule(sch)
+    j.construct()
+    Jj, Kj = j.context.jk(2 * dens[1], dens[1])
+    assert np.max(np.abs(Jj - ref[1][0])) < 1e-12 * np.max(np.abs(ref[1][0]))

@@ -97,7 +97,7 @@ def lib():
                            "dsscf_batch_set_device": [p, p, i, p, p, p, p], "dsscf_batch_jk_device": [p],
                            "dsscf_batch_time": [p, i, i, p], "dsscf_batch_autotune": [p, p],
                            "dsscf_batch_autotune_schedule": [p, p, p, p, p], "dsscf_set_class_schedule": [p, p, p, p],
-                           "dsscf_job_set_schedule": [p, p, p, p]}.items():
+                           "dsscf_job_set_schedule": [p, p, p, p], "dsscf_job_set_guess": [p, p, ctypes.c_longlong]}.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = i
@@ -561,6 +561,16 @@ class Job:
         ch = np.ascontiguousarray(schedule["kets_per_unit"], np.int32)
         self._sched = (wif, gp, ch)
         _check(lib().dsscf_job_set_schedule(self._h, _p(wif), _p(gp), _p(ch)), self._err)
+        return self
+
+    def set_guess(self, Pa_packed):
+        """Warm start: the packed alpha density the SCF starts from (None = the
+        core guess), e.g. a neighbouring geometry's ``get("Pa")``."""
+        if Pa_packed is None:
+            _check(lib().dsscf_job_set_guess(self._h, None, 0), self._err)
+            return self
+        self._guess = np.ascontiguousarray(Pa_packed, np.float64)
+        _check(lib().dsscf_job_set_guess(self._h, _p(self._guess), len(self._guess)), self._err)
         return self
 
     def write_molden(self, path: str, write_virtual: bool = True):

@@ -75,14 +75,24 @@ def _newest(paths):
     return max((os.path.getmtime(p) for p in paths if os.path.exists(p)), default=0.0)
 
 
-def _deps(src: str):
-    """Files a unit may include: the headers and generated tables of csrc/
-    (and csrc/host/ for host units) and include/."""
-    d = os.path.dirname(src)
-    out = [src] + [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
-    for folder in {d, CSRC, GEN}:
-        out += [os.path.join(folder, f) for f in os.listdir(folder) if f.endswith((".hpp", ".inc", ".h"))]
-    return out
+def _deps(src: str, seen=None):
+    """The unit and every project file it includes, followed through
+    #include "..." lines (searched beside the includer, then in csrc/_gen,
+    csrc and include/)."""
+    import re
+    seen = set() if seen is None else seen
+    if src in seen or not os.path.exists(src):
+        return seen
+    seen.add(src)
+    with open(src, errors="replace") as f:
+        names = re.findall(r'^\s*#\s*include\s*"([^"]+)"', f.read(), flags=re.M)
+    for name in names:
+        for folder in (os.path.dirname(src), GEN, CSRC, os.path.join(ROOT, "include")):
+            cand = os.path.normpath(os.path.join(folder, name))
+            if os.path.exists(cand):
+                _deps(cand, seen)
+                break
+    return seen
 
 
 def build(maxl: int = DEFAULT_MAXL, jobs: int = 8, verbose: bool = False, incremental: bool = False) -> str:

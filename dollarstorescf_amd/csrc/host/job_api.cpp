@@ -101,6 +101,14 @@ int dsscf_job_set_schedule(dsscf_job* j, const double* work_item_flops, const do
     return DSSCF_OK;
 }
 
+int dsscf_job_set_guess(dsscf_job* j, const double* Pa, long long n)
+{
+    if (!j || n < 0 || (n > 0 && !Pa)) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_guess: bad argument");
+    j->job.opt.guess_Pa.assign(Pa, Pa + n);
+    if (!j->job.Pa.empty() && !j->job.dstep) scf::useInitialGuess(j->job);  // a constructed job starts from it too
+    return DSSCF_OK;
+}
+
 int dsscf_job_set_molecule(dsscf_job* j, int natom, const int* Z, const double* xyz, int charge, int mult)
 {
     if (!j || natom <= 0 || !Z || !xyz || mult < 1) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_molecule: bad argument");
@@ -149,7 +157,7 @@ int dsscf_job_construct(dsscf_job* j)
     if (!j->have_basis || !j->job.molecule.size()) return set_err(j, DSSCF_ERR_STATE, "molecule and basis must be set first");
     int rc = scf::constructBasis(j->job);
     if (rc) return rc;
-    scf::useCoreGuess(j->job);
+    scf::useInitialGuess(j->job);
     return DSSCF_OK;
 }
 
@@ -159,7 +167,7 @@ int dsscf_job_construct_host(dsscf_job* j)
     if (!j->have_basis || !j->job.molecule.size()) return set_err(j, DSSCF_ERR_STATE, "molecule and basis must be set first");
     int rc = scf::constructBasisHost(j->job);
     if (rc) return rc;
-    scf::useCoreGuess(j->job);
+    scf::useInitialGuess(j->job);
     return DSSCF_OK;
 }
 
@@ -327,7 +335,7 @@ int dsscf_jobs_construct(dsscf_job* const* jobs, int n)
     }
     const int rc = scf::constructTogether(js);
     for (scf::Job* j : js)
-        if (j->ctx) scf::useCoreGuess(*j);
+        if (j->ctx) scf::useInitialGuess(*j);
     return rc;
 }
 

@@ -349,6 +349,19 @@ void useCoreGuess(Job& job)
     job.jk_valid = false;
 }
 
+void useInitialGuess(Job& job)
+{
+    if (job.opt.guess_Pa.empty() || job.opt.guess_Pa.size() != job.Pa.size()) {
+        useCoreGuess(job);
+        return;
+    }
+    for (size_t i = 0; i < job.Pa.size(); ++i) {
+        job.Pa[i] = job.opt.guess_Pa[i];
+        job.Ptot[i] = 2.0 * job.opt.guess_Pa[i];
+    }
+    job.jk_valid = false;
+}
+
 int fockBuild(Job& job)
 {
     if (!job.ctx) return DSSCF_ERR_STATE;

@@ -31,6 +31,9 @@ struct Options {
     int device_step = 0;               // 1: DIIS, X F X, diagonalization and density on the device
     double jcd_max_bytes = 0.0;        // dsscf_options::jcd_max_bytes of the job's context
     int cartesian = 0;                 // d shells as 6 Cartesian functions (dsscf_options::cartesian)
+    // packed alpha density the SCF starts from (warm start from a neighbouring
+    // geometry's solution); empty, or of another size: the core guess
+    std::vector<double> guess_Pa;
     // per-class list schedule for the context (dsscf_set_class_schedule; empty = default)
     std::vector<double> sched_wif, sched_gp;
     std::vector<int> sched_chunk;
@@ -121,6 +124,10 @@ int constructBasis(Job& job);        // host part + device part
 int constructBasisHost(Job& job);    // AOs, S, T, V, H, Loewdin X
 int constructDevice(Job& job);       // shell upload, Schwarz, quartet lists
 void useCoreGuess(Job& job);
+// The job's starting density: opt.guess_Pa where it fits the AO count
+// (Optimization.cpp:179-181 keeps the previous geometry's density), else the
+// core guess.
+void useInitialGuess(Job& job);
 int fockBuild(Job& job);  // J[Ptot], K[Pa] on the device
 int runSCFIteration(Job& job);  // fockBuild + scfStep
 int scfStep(Job& job);          // G, DIIS, F' = X F X, diagonalization, density, E_elec

@@ -256,7 +256,7 @@ int Server::step(int* n_active, int* n_waiting)
                 Job& j = *r->job;
                 int rc = constructDevice(j);
                 if (!rc) {
-                    useCoreGuess(j);
+                    useInitialGuess(j);
                     if (j.opt.device_step) rc = beginDeviceSteps(j);
                 }
                 std::lock_guard<std::mutex> lk2(mu_);

@@ -50,6 +50,14 @@ int dsscf_job_set_cartesian(dsscf_job* job, int on);
  * jobs.  Before the job is constructed. */
 int dsscf_job_set_schedule(dsscf_job* job, const double* work_item_flops, const double* group_prims,
                            const int* kets_per_unit);
+/* Warm start: the packed-U alpha density (n = N(N+1)/2 doubles, e.g. the
+ * solution of a neighbouring geometry with the same atoms and basis) that
+ * the SCF starts from instead of the core guess, as the reference's
+ * geometry steps keep the previous density (src/chem/scf/Optimization.cpp:
+ * 179-181 around constructBasis/solveMOs).  Used at construction (and by a
+ * server at admission); a density of another size falls back to the core
+ * guess; n = 0 clears it. */
+int dsscf_job_set_guess(dsscf_job* job, const double* Pa, long long n);
 /* scf::assignMolecule (SCFBasis.cpp:17-37); coordinates in bohr. */
 int dsscf_job_set_molecule(dsscf_job* job, int natom, const int* Z, const double* xyz_bohr, int charge,
                            int multiplicity);

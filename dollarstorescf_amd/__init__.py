@@ -97,7 +97,8 @@ def lib():
                            "dsscf_batch_set_device": [p, p, i, p, p, p, p], "dsscf_batch_jk_device": [p],
                            "dsscf_batch_time": [p, i, i, p], "dsscf_batch_autotune": [p, p],
                            "dsscf_batch_autotune_schedule": [p, p, p, p, p], "dsscf_set_class_schedule": [p, p, p, p],
-                           "dsscf_job_set_schedule": [p, p, p, p], "dsscf_job_set_guess": [p, p, ctypes.c_longlong]}.items():
+                           "dsscf_job_set_schedule": [p, p, p, p], "dsscf_job_set_guess": [p, p, ctypes.c_longlong],
+                           "dsscf_jk_uhf": [p, p, p, p, p, p, p]}.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = i
@@ -143,7 +144,10 @@ def solve_jobs(jobs, use_graph=True, autotune=True):
     n = len(jobs)
     arr = (ctypes.c_void_p * n)(*[j._h.value for j in jobs])
     e, it, st = np.zeros(n), np.zeros(n, np.int32), np.zeros(n, np.int32)
-    lib().dsscf_jobs_solve(arr, n, 1 if use_graph else 0, 1 if autotune else 0, _p(e), _p(it), _p(st))
+    rc = lib().dsscf_jobs_solve(arr, n, 1 if use_graph else 0, 1 if autotune else 0, _p(e), _p(it), _p(st))
+    if rc and not np.any(st):  # refused before any job ran (a bad or unsupported job)
+        bad = [j._err() for j in jobs if j._err()]
+        raise DsscfError(rc, bad[0] if bad else "dsscf_jobs_solve failed")
     return e, it, st
 
 
@@ -286,6 +290,16 @@ class Context:
         J, K = np.empty_like(Pt), np.empty_like(Pt)
         _check(lib().dsscf_jk(self._h, _p(Pt), _p(Pa), _p(J), _p(K)))
         return J, K
+
+    def jk_uhf(self, Ptot_packed, Pa_packed, Pb_packed):
+        """Unrestricted Fock build (dsscf_jk_uhf): J[Ptot], K[Pa], K[Pb] from
+        one pass over the integrals."""
+        Pt = np.ascontiguousarray(Ptot_packed, np.float64)
+        Pa = np.ascontiguousarray(Pa_packed, np.float64)
+        Pb = np.ascontiguousarray(Pb_packed, np.float64)
+        J, Ka, Kb = np.empty_like(Pt), np.empty_like(Pt), np.empty_like(Pt)
+        _check(lib().dsscf_jk_uhf(self._h, _p(Pt), _p(Pa), _p(Pb), _p(J), _p(Ka), _p(Kb)))
+        return J, Ka, Kb
 
     def set_density(self, Ptot_packed, Pa_packed):
         self._Pt = np.ascontiguousarray(Ptot_packed, np.float64)
@@ -514,10 +528,10 @@ class Job:
 
     def get(self, name: str) -> np.ndarray:
         n = self.n_ao
-        size = {"eps": n, "C": n * n}.get(name, n * (n + 1) // 2)
+        size = {"eps": n, "epsb": n, "C": n * n, "Cb": n * n}.get(name, n * (n + 1) // 2)
         out = np.empty(size)
         _check(lib().dsscf_job_get(self._h, name.encode(), _p(out)), self._err)
-        return out.reshape(n, n) if name == "C" else out
+        return out.reshape(n, n) if name in ("C", "Cb") else out
 
     def set_orbitals(self, C, eps):
         """Replace the orbitals (C: n x n, column i = orbital i; eps: n energies)."""

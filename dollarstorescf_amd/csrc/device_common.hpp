@@ -97,6 +97,11 @@ struct JKSeg {
     double* js;
     const int* us;
     const int* jd;
+    // unrestricted (open-shell) contexts: the beta density and the beta
+    // half-accumulated K (nslice copies like Kh), else null.  K[Db] is
+    // digested from the same integrals as K[Da].
+    const double* Db;
+    double* Kbh;
 };
 
 // One context's J_cd reduction (k_jcd): one row per ket AO block; ri =
@@ -144,6 +149,12 @@ struct MatSeg {
     int row0;
     int nslice;
     int pad;
+    // unrestricted contexts: packed-U beta density, its full copy, the beta
+    // half-accumulated K and the packed-U K[Pb] output (else all null)
+    const double* Pbp;
+    double* Db;
+    double* Kbh;
+    double* Kbp;
 };
 
 // Read-only global-memory view of a pointer that arrived through memory (a

@@ -155,6 +155,7 @@ constexpr int kStreams = 4;  // streams per fused Fock build (GPU_MAX_HW_QUEUES 
 struct Plan {
     int boys_mode = DSSCF_BOYS_ACCURATE;
     int cart = 0;  // Cartesian d shells (every member context agrees)
+    int nks = 1;   // exchange sets per block: 2 when a member is unrestricted (K[Pa] and K[Pb])
     const double* btab = nullptr;
     int ncls = 0;
     std::vector<int> cls_nseg, cls_nblocks, cls_off, cls_bs, cls_maxN, cls_maxbra;
@@ -279,6 +280,9 @@ struct dsscf_ctx {
     int* d_plist = nullptr;
     double *d_Dt = nullptr, *d_Da = nullptr, *d_Jh = nullptr, *d_Kh = nullptr;
     double *d_Pt_pk = nullptr, *d_Pa_pk = nullptr, *d_J_pk = nullptr, *d_K_pk = nullptr;
+    // unrestricted builds (dsscf_jk_uhf), allocated on first use: the beta
+    // density (packed, full), K[Pb] half-accumulated (n_slice copies) and packed
+    double *d_Pb_pk = nullptr, *d_Db = nullptr, *d_Kbh = nullptr, *d_Kb_pk = nullptr;
     double* d_xform = nullptr;                   // AO transform (one-electron kernel)
     // J_cd slot records of the thread-per-quartet classes (null: atomics):
     // per unit its first double in d_js; per ket AO block (row) its records
@@ -299,10 +303,16 @@ struct dsscf_ctx {
     std::vector<double> sched_wif, sched_gp;
     std::vector<int> sched_chunk;
     Plan plan;                    // fused launch plan over this context alone
+    Plan plan_u;                  // the same for unrestricted builds (K[Pa] and K[Pb])
 
     void free_geometry()
     {
         plan.release();
+        plan_u.release();
+        dev_free(d_Pb_pk);
+        dev_free(d_Db);
+        dev_free(d_Kbh);
+        dev_free(d_Kb_pk);
         dev_free(d_pairs);
         dev_free(d_pp);
         dev_free(d_q);
@@ -1045,15 +1055,23 @@ namespace {
 
 // Build `p` over contexts ctxs[0..n): their packed inputs/outputs at
 // dPt/dPa/dJ/dK (device), block sizes per class from bs (nullptr: each
-// class's default of 256 or the first context's tuned value).
+// class's default of 256 or the first context's tuned value).  dPb/dKb
+// (both null, or arrays whose entry k is null for a restricted member): the
+// packed beta density and K[Pb] output of unrestricted members, whose beta
+// buffers (d_Db, d_Kbh) must exist.
 int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt, const double* const* dPa,
-               double* const* dJ, double* const* dK, const int* bs, hipStream_t stream)
+               double* const* dJ, double* const* dK, const int* bs, hipStream_t stream,
+               const double* const* dPb = nullptr, double* const* dKb = nullptr)
 {
     p.reset();
     const int ncls = dk_n_class();
     p.ncls = ncls;
     p.boys_mode = ctxs[0]->opt.boys_mode;
     p.cart = ctxs[0]->opt.cartesian;
+    auto is_u = [&](int k) { return dPb && dKb && dPb[k] && dKb[k]; };
+    p.nks = 1;
+    for (int k = 0; k < n; ++k)
+        if (is_u(k)) p.nks = 2;
     p.btab = ctxs[0]->d_btab;
     const int nslot = 2 * ncls;  // slot 2*cls: FP64 list, 2*cls+1: FP32 list
     p.cls_nseg.assign(nslot, 0);
@@ -1097,6 +1115,10 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
             sg.Da = c->d_Da;
             sg.Jh = c->d_Jh;
             sg.Kh = c->d_Kh;
+            if (is_u(k)) {
+                sg.Db = c->d_Db;
+                sg.Kbh = c->d_Kbh;
+            }
             sg.nslice = c->n_slice;
             if (!wave && c->d_js) {
                 sg.js = c->d_js;
@@ -1124,8 +1146,9 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
         if (!wave && p.cls_nseg[slot]) {
             int l4[4];
             dk_class_l(cls, l4);
-            const size_t lds = sizeof(double) * ((size_t)(nfun(l4[0], p.cart) + nfun(l4[1], p.cart)) * slot_maxN + 1) +
-                               sizeof(PrimPair) * (size_t)p.cls_maxbra[slot];
+            const size_t lds =
+                sizeof(double) * ((size_t)p.nks * (nfun(l4[0], p.cart) + nfun(l4[1], p.cart)) * slot_maxN + 1) +
+                sizeof(PrimPair) * (size_t)p.cls_maxbra[slot];
             if (lds > 64 * 1024) return fail(DSSCF_ERR_UNSUPPORTED, "basis too large for the LDS exchange rows");
         }
     }
@@ -1163,6 +1186,12 @@ int plan_build(Plan& p, dsscf_ctx* const* ctxs, int n, const double* const* dPt,
         m.Kh = c->d_Kh;
         m.Jp = dJ[k];
         m.Kp = dK[k];
+        if (is_u(k)) {
+            m.Pbp = dPb[k];
+            m.Db = c->d_Db;
+            m.Kbh = c->d_Kbh;
+            m.Kbp = dKb[k];
+        }
         m.N = c->n_ao;
         m.nslice = c->n_slice;
         m.row0 = row;
@@ -1224,7 +1253,7 @@ int plan_launch_slot(Plan& p, int slot, hipStream_t s)
 {
     if (p.cls_nseg[slot] == 0) return DSSCF_OK;
     HIP_TRY(dsscf_launch_jk(slot / 2, p.boys_mode, slot % 2, p.d_segs + p.cls_off[slot], p.d_blk0 + p.cls_off[slot],
-                            p.cls_nseg[slot], p.cls_nblocks[slot], p.cls_bs[slot], p.cls_maxN[slot],
+                            p.cls_nseg[slot], p.cls_nblocks[slot], p.cls_bs[slot], p.cls_maxN[slot], p.nks,
                             p.cls_maxbra[slot], p.cart, p.btab, kEriScale, s));
     return DSSCF_OK;
 }
@@ -1350,7 +1379,47 @@ int run_jk(dsscf_ctx* c, std::vector<float>* cls_ms)
     return plan_run(c->plan, c->stream, cls_ms == nullptr, cls_ms);
 }
 
+// The beta buffers of an unrestricted build and its plan (the restricted
+// plan and its graph stay as they are).
+int ctx_plan_u(dsscf_ctx* c)
+{
+    const size_t N = (size_t)c->n_ao, nn = N * N, np = N * (N + 1) / 2;
+    if (!c->d_Db) {
+        if (int rc = dev_alloc(&c->d_Pb_pk, np)) return rc;
+        if (int rc = dev_alloc(&c->d_Kb_pk, np)) return rc;
+        if (int rc = dev_alloc(&c->d_Db, nn)) return rc;
+        if (int rc = dev_alloc(&c->d_Kbh, nn * c->n_slice)) return rc;
+    }
+    if (c->plan_u.built) return DSSCF_OK;
+    const double* pt = c->d_Pt_pk;
+    const double* pa = c->d_Pa_pk;
+    const double* pb = c->d_Pb_pk;
+    double* j = c->d_J_pk;
+    double* k = c->d_K_pk;
+    double* kb = c->d_Kb_pk;
+    return plan_build(c->plan_u, &c, 1, &pt, &pa, &j, &k, nullptr, c->stream, &pb, &kb);
+}
+
 }  // namespace
+
+int dsscf_jk_uhf(dsscf_ctx* c, const double* Pt, const double* Pa, const double* Pb, double* J, double* Ka,
+                 double* Kb)
+{
+    if (!c || !Pt || !Pa || !Pb || !J || !Ka || !Kb) return fail(DSSCF_ERR_ARG, "dsscf_jk_uhf: null argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_jk_uhf: call dsscf_prepare first");
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = ctx_plan_u(c)) return rc;
+    const size_t np = (size_t)c->n_ao * (c->n_ao + 1) / 2;
+    HIP_TRY(hipMemcpyAsync(c->d_Pt_pk, Pt, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_Pa_pk, Pa, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_Pb_pk, Pb, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    if (int rc = plan_run(c->plan_u, c->stream, true, nullptr)) return rc;
+    HIP_TRY(hipMemcpyAsync(J, c->d_J_pk, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(Ka, c->d_K_pk, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(Kb, c->d_Kb_pk, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DSSCF_OK;
+}
 
 int dsscf_jk_device(dsscf_ctx* c, const double* d_Pt, const double* d_Pa, double* d_J, double* d_K)
 {
@@ -2121,6 +2190,7 @@ int dsscf_batch_autotune(dsscf_batch* b, int* chosen)
     if (b->ctxs.size() == 1) {
         b->ctxs[0]->block_size = best;
         b->ctxs[0]->plan.release();
+        b->ctxs[0]->plan_u.release();
     }
     if (chosen)
         for (int k = 0; k < ncls; ++k) chosen[k] = best[k];

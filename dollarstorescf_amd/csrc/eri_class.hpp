@@ -76,13 +76,14 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v)
 // grid of two resident blocks per CU that loop, and faster than blocks
 // taking units from an atomic counter, whose same-address adds serialize).
 //
-// Dynamic LDS: the Kh rows [(NA + NB) * maxN doubles, rounded to an even
-// count], then the bra's primitive records (at most maxBra of them, sized by
-// the launch rather than for the largest possible shell pair, so that more
-// blocks fit on a CU).
+// Dynamic LDS: the Kh rows [nks * (NA + NB) * maxN doubles, rounded to an
+// even count; nks = 2 when a segment of the launch is unrestricted, whose
+// blocks keep the K[Db] rows after the K[Da] rows], then the bra's primitive
+// records (at most maxBra of them, sized by the launch rather than for the
+// largest possible shell pair, so that more blocks fit on a CU).
 template <int la, int lb, int lc, int ld, int BM, typename R, int C>
 __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
-                                            int nseg, int nunits, int maxN,
+                                            int nseg, int nunits, int maxN, int nks,
                                             const double* __restrict__ btab, double scale)
 {
     typedef ClassInfo<la, lb, lc, ld, C> CI;
@@ -92,7 +93,7 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
     typedef typename std::conditional<kF32Rec, PrimPairF, PrimPair>::type BPT;
     extern __shared__ __align__(16) double lds_dyn[];
     double* krow = lds_dyn;  // Kh rows of shell a, then shell b
-    BPT* braP = reinterpret_cast<BPT*>(lds_dyn + (((NA + NB) * maxN + 1) & ~1));
+    BPT* braP = reinterpret_cast<BPT*>(lds_dyn + ((nks * (NA + NB) * maxN + 1) & ~1));
     // FP64 accurate mode: the Boys function from the block's LDS copy of the
     // interval table of m = L (a per-lane gather from the 801-row table
     // touches up to 64 cache lines per load)
@@ -131,12 +132,16 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
         const size_t slice = (size_t)jk_slice(S.nslice) * N * N;
         double* __restrict__ Jh = S.Jh + slice;
         double* __restrict__ Kh = S.Kh + slice;
+        // unrestricted segment: K[Db] from the same integrals into the rows
+        // after the K[Da] rows (uniform over the block)
+        const bool beta = S.Db != nullptr;
+        const int nrow = (beta ? 2 : 1) * (NA + NB);
         const GPtr<unsigned> kets = gbl(S.k);
         // J_cd: per-quartet slot records (summed by k_jcd) or atomics
         const int slot0 = S.js ? gbl(S.us)[vb - S.blk0] : 0;
         const ShellPair bra = ldg(pairs + u.x);
         const int a0 = bra.ao_a, b0 = bra.ao_b;
-        for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) krow[i] = 0.0;
+        for (int i = threadIdx.x; i < nrow * N; i += blockDim.x) krow[i] = 0.0;
         for (int i = threadIdx.x; i < bra.nprim; i += blockDim.x) {
             PrimPair r = ldg(pp + bra.prim_off + i);
             if (kLds)  // the LDS Boys mode's prim_setup takes P - A
@@ -219,48 +224,70 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
                     else
                         atomic_add_f64(&Jh[(size_t)(c0 + k) * N + d0 + l], 2.0 * g * s);
                 }
-            // Exchange into the LDS rows
+            // Exchange into the LDS rows: K[Da] (rows 0 .. NA+NB), then, for an
+            // unrestricted segment, K[Db] from the same integrals (rows NA+NB ..)
+            auto exchange = [&](double* __restrict__ kr) {
 #pragma unroll
-            for (int k = 0; k < NC; ++k) {
+                for (int k = 0; k < NC; ++k) {
 #pragma unroll
-                for (int i = 0; i < NA; ++i) {  // Kh[a][c] += g sum_bd v D_bd
-                    double s = 0.0;
+                    for (int i = 0; i < NA; ++i) {  // Kh[a][c] += g sum_bd v D_bd
+                        double s = 0.0;
 #pragma unroll
-                    for (int j = 0; j < NB; ++j)
+                        for (int j = 0; j < NB; ++j)
 #pragma unroll
-                        for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * dbd[j * ND + l];
-                    atomicAdd(&krow[i * N + c0 + k], g * s);
+                            for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * dbd[j * ND + l];
+                        atomicAdd(&kr[i * N + c0 + k], g * s);
+                    }
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {  // Kh[b][c] += g sum_ad v D_ad
+                        double s = 0.0;
+#pragma unroll
+                        for (int i = 0; i < NA; ++i)
+#pragma unroll
+                            for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * dad[i * ND + l];
+                        atomicAdd(&kr[(NA + j) * N + c0 + k], g * s);
+                    }
                 }
 #pragma unroll
-                for (int j = 0; j < NB; ++j) {  // Kh[b][c] += g sum_ad v D_ad
-                    double s = 0.0;
+                for (int l = 0; l < ND; ++l) {
 #pragma unroll
-                    for (int i = 0; i < NA; ++i)
+                    for (int i = 0; i < NA; ++i) {  // Kh[a][d] += g sum_bc v D_bc
+                        double s = 0.0;
 #pragma unroll
-                        for (int l = 0; l < ND; ++l) s += v[VIDX(i, j, k, l)] * dad[i * ND + l];
-                    atomicAdd(&krow[(NA + j) * N + c0 + k], g * s);
+                        for (int j = 0; j < NB; ++j)
+#pragma unroll
+                            for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * dbc[j * NC + k];
+                        atomicAdd(&kr[i * N + d0 + l], g * s);
+                    }
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {  // Kh[b][d] += g sum_ac v D_ac
+                        double s = 0.0;
+#pragma unroll
+                        for (int i = 0; i < NA; ++i)
+#pragma unroll
+                            for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * dac[i * NC + k];
+                        atomicAdd(&kr[(NA + j) * N + d0 + l], g * s);
+                    }
                 }
-            }
+            };
+            exchange(krow);
+            if (beta) {
+                const GPtr<double> Db = gbl(S.Db);
 #pragma unroll
-            for (int l = 0; l < ND; ++l) {
+                for (int l = 0; l < ND; ++l) {
 #pragma unroll
-                for (int i = 0; i < NA; ++i) {  // Kh[a][d] += g sum_bc v D_bc
-                    double s = 0.0;
+                    for (int j = 0; j < NB; ++j) dbd[j * ND + l] = Db[(size_t)(b0 + j) * N + d0 + l];
 #pragma unroll
-                    for (int j = 0; j < NB; ++j)
-#pragma unroll
-                        for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * dbc[j * NC + k];
-                    atomicAdd(&krow[i * N + d0 + l], g * s);
+                    for (int i = 0; i < NA; ++i) dad[i * ND + l] = Db[(size_t)(a0 + i) * N + d0 + l];
                 }
 #pragma unroll
-                for (int j = 0; j < NB; ++j) {  // Kh[b][d] += g sum_ac v D_ac
-                    double s = 0.0;
+                for (int k = 0; k < NC; ++k) {
 #pragma unroll
-                    for (int i = 0; i < NA; ++i)
+                    for (int j = 0; j < NB; ++j) dbc[j * NC + k] = Db[(size_t)(b0 + j) * N + c0 + k];
 #pragma unroll
-                        for (int k = 0; k < NC; ++k) s += v[VIDX(i, j, k, l)] * dac[i * NC + k];
-                    atomicAdd(&krow[(NA + j) * N + d0 + l], g * s);
+                    for (int i = 0; i < NA; ++i) dac[i * NC + k] = Db[(size_t)(a0 + i) * N + c0 + k];
                 }
+                exchange(krow + (NA + NB) * N);
             }
 #undef VIDX
         }
@@ -274,12 +301,18 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
                 atomic_add_f64(&Jh[(size_t)(a0 + k / NB) * N + b0 + k % NB], 2.0 * x);
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) {
+        for (int i = threadIdx.x; i < nrow * N; i += blockDim.x) {
             const double x = krow[i];
             if (x != 0.0) {
-                const int r = i / N, col = i % N;
+                int r = i / N;
+                const int col = i % N;
+                double* __restrict__ K = Kh;
+                if (r >= NA + NB) {  // K[Db] rows of an unrestricted segment
+                    r -= NA + NB;
+                    K = S.Kbh + slice;
+                }
                 const int row = r < NA ? a0 + r : b0 + (r - NA);
-                atomic_add_f64(&Kh[(size_t)row * N + col], x);
+                atomic_add_f64(&K[(size_t)row * N + col], x);
             }
         }
         __syncthreads();  // the flush has read krow before the next unit clears it
@@ -355,8 +388,8 @@ void ClassLaunch<a, b, c, d, wave>::wave_go(const void* items, int n, const Shel
 template <int a, int b, int c, int d, int wave>
 template <int C>
 hipError_t ClassLaunch<a, b, c, d, wave>::jk_c(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                                               int nblocks, int bs, int maxN, int maxBra, const double* btab,
-                                               double scale, hipStream_t s)
+                                               int nblocks, int bs, int maxN, int nks, int maxBra,
+                                               const double* btab, double scale, hipStream_t s)
 {
     if constexpr (wave) {
         if (bm == BOYS_ACCURATE)
@@ -367,35 +400,35 @@ hipError_t ClassLaunch<a, b, c, d, wave>::jk_c(int bm, int fp32, const JKSeg* se
                                nullptr, 0, nullptr, nullptr, btab, segs, blk0, nseg, nullptr, scale);
     } else {
         typedef ClassInfo<a, b, c, d, C> CI;
-        const size_t rows = ((size_t)(CI::NA + CI::NB) * maxN + 1) & ~(size_t)1;
+        const size_t rows = ((size_t)nks * (CI::NA + CI::NB) * maxN + 1) & ~(size_t)1;
         const size_t lds = sizeof(double) * rows + (fp32 ? sizeof(PrimPairF) : sizeof(PrimPair)) * (size_t)maxBra;
         const dim3 blk(bs), grd(nblocks);  // one block per work unit
         if (fp32)
             hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, float, C>), grd, blk, lds, s, segs, blk0, nseg,
-                               nblocks, maxN, btab, scale);
+                               nblocks, maxN, nks, btab, scale);
         else if (bm == BOYS_ACCURATE)
             hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_ACCURATE, double, C>), grd, blk, lds, s, segs, blk0, nseg,
-                               nblocks, maxN, btab, scale);
+                               nblocks, maxN, nks, btab, scale);
         else
             hipLaunchKernelGGL((k_jk<a, b, c, d, BOYS_REFERENCE, double, C>), grd, blk, lds, s, segs, blk0, nseg,
-                               nblocks, maxN, btab, scale);
+                               nblocks, maxN, nks, btab, scale);
     }
     return hipGetLastError();
 }
 
 // J/K over nseg context segments: thread-per-quartet classes one block of bs
-// threads per work unit (LDS: the Kh rows of the bra's two shells and its
-// primitive records), wave-cooperative classes one 64-lane block per
-// WaveGroup::G quartets.
+// threads per work unit (LDS: the Kh rows of the bra's two shells, nks = 2
+// sets of them when a segment is unrestricted, and its primitive records),
+// wave-cooperative classes one 64-lane block per WaveGroup::G quartets.
 template <int a, int b, int c, int d, int wave>
 hipError_t ClassLaunch<a, b, c, d, wave>::jk(int bm, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                                             int nblocks, int bs, int maxN, int maxBra, int cart,
+                                             int nblocks, int bs, int maxN, int nks, int maxBra, int cart,
                                              const double* btab, double scale, hipStream_t s)
 {
     if constexpr (kHasD) {
-        if (cart) return jk_c<1>(bm, fp32, segs, blk0, nseg, nblocks, bs, maxN, maxBra, btab, scale, s);
+        if (cart) return jk_c<1>(bm, fp32, segs, blk0, nseg, nblocks, bs, maxN, nks, maxBra, btab, scale, s);
     }
-    return jk_c<0>(bm, fp32, segs, blk0, nseg, nblocks, bs, maxN, maxBra, btab, scale, s);
+    return jk_c<0>(bm, fp32, segs, blk0, nseg, nblocks, bs, maxN, nks, maxBra, btab, scale, s);
 }
 
 template <int a, int b, int c, int d, int wave>

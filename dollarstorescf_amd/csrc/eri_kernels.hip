@@ -25,7 +25,8 @@ __device__ __forceinline__ int find_segment_by_row(const MatSeg* __restrict__ se
 }
 
 // Per context and row x: packed-U densities (element (y,x), y<=x, at
-// x(x+1)/2+y, Matrix.ipp:332-351) -> full row-major Dt, Da; zero Jh, Kh rows.
+// x(x+1)/2+y, Matrix.ipp:332-351) -> full row-major Dt, Da (and Db of an
+// unrestricted context); zero Jh, Kh (Kbh) rows.
 __global__ void k_prep(const MatSeg* __restrict__ segs, int nseg)
 {
     const MatSeg& S = segs[find_segment_by_row(segs, nseg, (int)blockIdx.y)];
@@ -36,6 +37,8 @@ __global__ void k_prep(const MatSeg* __restrict__ segs, int nseg)
             S.Jh[s * NN + (size_t)x * N + y] = 0.0;
             S.Kh[s * NN + (size_t)x * N + y] = 0.0;
         }
+        if (S.Kbh)
+            for (int s = 0; s < S.nslice; ++s) S.Kbh[s * NN + (size_t)x * N + y] = 0.0;
         if (y <= x) {
             const size_t o = (size_t)x * (x + 1) / 2 + y;
             const double t = S.Ptp[o], a = S.Pap[o];
@@ -43,11 +46,16 @@ __global__ void k_prep(const MatSeg* __restrict__ segs, int nseg)
             S.Dt[(size_t)x * N + y] = t;
             S.Da[(size_t)y * N + x] = a;
             S.Da[(size_t)x * N + y] = a;
+            if (S.Pbp) {
+                const double b = S.Pbp[o];
+                S.Db[(size_t)y * N + x] = b;
+                S.Db[(size_t)x * N + y] = b;
+            }
         }
     }
 }
 
-// J = sum over slices of Jh + Jh^T, likewise K -> packed-U, per context.
+// J = sum over slices of Jh + Jh^T, likewise K (and K[Db]) -> packed-U, per context.
 __global__ void k_fold(const MatSeg* __restrict__ segs, int nseg)
 {
     const MatSeg& S = segs[find_segment_by_row(segs, nseg, (int)blockIdx.y)];
@@ -63,6 +71,11 @@ __global__ void k_fold(const MatSeg* __restrict__ segs, int nseg)
         }
         S.Jp[o] = j;
         S.Kp[o] = k;
+        if (S.Kbp) {
+            double kb = 0.0;
+            for (int s = 0; s < S.nslice; ++s) kb += S.Kbh[s * NN + yx] + S.Kbh[s * NN + xy];
+            S.Kbp[o] = kb;
+        }
     }
 }
 
@@ -337,7 +350,7 @@ void dk_class_l(int c, int* l)
 }
 
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                           int nblocks, int block_size, int maxN, int maxBra, int cart, const double* btab,
+                           int nblocks, int block_size, int maxN, int nks, int maxBra, int cart, const double* btab,
                            double scale, hipStream_t s)
 {
     if (nseg <= 0 || nblocks <= 0) return hipSuccess;
@@ -345,7 +358,7 @@ hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, 
 #define DSSCF_LAUNCH_JK(a, b, cc, d, name, wave)                                                                 \
     if (c++ == cls)                                                                                              \
         return ClassLaunch<a, b, cc, d, wave>::jk(boys_mode, fp32, segs, blk0, nseg, nblocks, block_size, maxN,   \
-                                                  maxBra, cart, btab, scale, s);
+                                                  nks, maxBra, cart, btab, scale, s);
     DSSCF_FOR_EACH_CLASS(DSSCF_LAUNCH_JK)
 #undef DSSCF_LAUNCH_JK
     return hipErrorInvalidValue;

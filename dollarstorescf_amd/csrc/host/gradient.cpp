@@ -324,6 +324,10 @@ int gradient(Job& job, std::vector<double>& grad, double* parts)
         job.error = "gradient: solve the job first";
         return DSSCF_ERR_STATE;
     }
+    if (unrestricted(job)) {
+        job.error = "gradient: restricted jobs only";
+        return DSSCF_ERR_UNSUPPORTED;
+    }
     for (const auto& g : job.basis)
         if (g.l > 1) {
             job.error = "gradient: built for s and p shells only";

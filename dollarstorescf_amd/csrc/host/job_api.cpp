@@ -218,6 +218,11 @@ int dsscf_job_get(dsscf_job* j, const char* name, double* out)
     else if (n == "G") v = &b.G;
     else if (n == "eps") v = &b.eps;
     else if (n == "C") v = &b.C;
+    else if (n == "Pb") v = &b.Pb;  // unrestricted jobs: the beta spin's matrices
+    else if (n == "Kb") v = &b.Kb;
+    else if (n == "Gb") v = &b.Gb;
+    else if (n == "epsb") v = &b.epsb;
+    else if (n == "Cb") v = &b.Cb;
     if (!v) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_get: unknown matrix '" + n + "'");
     if (v->empty()) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_get: '" + n + "' not computed yet");
     std::memcpy(out, v->data(), sizeof(double) * v->size());
@@ -276,6 +281,7 @@ int dsscf_job_set_density(dsscf_job* j, const double* Pa)
         b.Pa[i] = Pa[i];
         b.Ptot[i] = 2.0 * Pa[i];
     }
+    if (scf::unrestricted(b)) b.Pb = b.Pa;  // both spins at the given density
     b.jk_valid = false;
     return DSSCF_OK;
 }
@@ -347,6 +353,8 @@ int dsscf_jobs_solve(dsscf_job* const* jobs, int n, int use_graph, int autotune,
     for (int i = 0; i < n; ++i) {
         if (!jobs[i]) return DSSCF_ERR_ARG;
         if (!jobs[i]->job.ctx) return set_err(jobs[i], DSSCF_ERR_STATE, "dsscf_jobs_solve: job not constructed");
+        if (scf::unrestricted(jobs[i]->job))
+            return set_err(jobs[i], DSSCF_ERR_UNSUPPORTED, "dsscf_jobs_solve: open-shell jobs are solved alone");
         for (int k = 0; k < i; ++k)
             if (jobs[k] == jobs[i]) return DSSCF_ERR_ARG;
         js.push_back(&jobs[i]->job);
@@ -390,6 +398,8 @@ int dsscf_server_submit(dsscf_server* s, dsscf_job* j, long long tag)
     if (!j->have_basis || !j->job.molecule.size()) return set_err(j, DSSCF_ERR_STATE, "molecule and basis must be set first");
     if (j->job.in_server.load()) return set_err(j, DSSCF_ERR_STATE, "dsscf_server_submit: the job is already held by a server");
     if (j->job.ctx) return set_err(j, DSSCF_ERR_STATE, "dsscf_server_submit: job already constructed");
+    if (scf::unrestricted(j->job))
+        return set_err(j, DSSCF_ERR_UNSUPPORTED, "dsscf_server_submit: open-shell jobs are solved alone");
     if (j->job.opt.device != s->s->device()) return set_err(j, DSSCF_ERR_ARG, "dsscf_server_submit: job on another device");
     return s->s->submit(j, &j->job, tag);
 }

@@ -272,6 +272,10 @@ int writeOrbitalsMolden(const Job& job, const std::string& path, bool write_virt
             if (error) *error = "writeOrbitalsMolden: Cartesian d functions are not written (the reference's [GTO]/[MO] layout is 5D)";
             return DSSCF_ERR_UNSUPPORTED;
         }
+    if (unrestricted(job)) {  // the reference's beta block (WriteOrbitals.cpp) is not restated
+        if (error) *error = "writeOrbitalsMolden: restricted jobs only";
+        return DSSCF_ERR_UNSUPPORTED;
+    }
     if (job.basis.empty() || job.C.size() != (size_t)job.n * job.n || job.eps.size() != (size_t)job.n) {
         if (error) *error = "writeOrbitalsMolden: no orbitals (solve the job or set them first)";
         return DSSCF_ERR_STATE;

@@ -209,10 +209,6 @@ static int buildAOs(Job& job)
             job.error = "no basis assigned to atom " + std::to_string(i + 1);
             return DSSCF_ERR_ARG;
         }
-    if (job.spin_treatment != SpinTreatment::restricted) {
-        job.error = "only closed-shell (restricted) jobs are supported";
-        return DSSCF_ERR_UNSUPPORTED;
-    }
     job.basis.clear();
     job.basis_atoms.clear();
     job.atom_basis.assign(1, 0);
@@ -261,6 +257,15 @@ static void finishHost(Job& job)
     job.J.assign(np, 0.0);
     job.K.assign(np, 0.0);
     job.G.assign(np, 0.0);
+    if (unrestricted(job)) {
+        job.Pb.assign(np, 0.0);
+        job.Kb.assign(np, 0.0);
+        job.Gb.assign(np, 0.0);
+    } else {
+        job.Pb.clear();
+        job.Kb.clear();
+        job.Gb.clear();
+    }
     job.jk_valid = false;
 }
 
@@ -346,12 +351,14 @@ void useCoreGuess(Job& job)
     // SCFSolver.cpp:179-194
     std::fill(job.Pa.begin(), job.Pa.end(), 0.0);
     std::fill(job.Ptot.begin(), job.Ptot.end(), 0.0);
+    std::fill(job.Pb.begin(), job.Pb.end(), 0.0);
     job.jk_valid = false;
 }
 
 void useInitialGuess(Job& job)
 {
-    if (job.opt.guess_Pa.empty() || job.opt.guess_Pa.size() != job.Pa.size()) {
+    // (an unrestricted job has no single previous density to start from here)
+    if (unrestricted(job) || job.opt.guess_Pa.empty() || job.opt.guess_Pa.size() != job.Pa.size()) {
         useCoreGuess(job);
         return;
     }
@@ -365,7 +372,9 @@ void useInitialGuess(Job& job)
 int fockBuild(Job& job)
 {
     if (!job.ctx) return DSSCF_ERR_STATE;
-    int rc = dsscf_jk(job.ctx, job.Ptot.data(), job.Pa.data(), job.J.data(), job.K.data());
+    int rc = unrestricted(job) ? dsscf_jk_uhf(job.ctx, job.Ptot.data(), job.Pa.data(), job.Pb.data(), job.J.data(),
+                                              job.K.data(), job.Kb.data())
+                               : dsscf_jk(job.ctx, job.Ptot.data(), job.Pa.data(), job.J.data(), job.K.data());
     if (rc) job.error = dsscf_last_error();
     job.jk_valid = rc == 0;
     return rc;
@@ -380,9 +389,85 @@ int runSCFIteration(Job& job)
     return scfStep(job);
 }
 
+namespace {
+// One spin's part of an iteration (SCFSolver.cpp:45-94): F0 = H + G,
+// Pulay DIIS from the second iteration (SCFDIIS.cpp:19-46, one history per
+// spin), F' = X F X, C = X C', P = C_occ C_occ^T into P (packed-U).
+int spinStep(Job& job, const std::vector<double>& G, const std::vector<double>& P_in, DIIS& diis, la::Mat& F,
+             std::vector<double>& eps, la::Mat& C, unsigned nocc, std::vector<double>& P_out)
+{
+    const int n = job.n;
+    std::vector<double> F0p(G.size());
+    for (size_t i = 0; i < F0p.size(); ++i) F0p[i] = job.H[i] + G[i];
+    la::Mat F0;
+    la::unpack(F0p.data(), n, F0);
+    if (job.iteration == 1) {
+        F = F0;
+    } else {
+        la::Mat Pd, PF, SPF, e;
+        la::unpack(P_in.data(), n, Pd);
+        la::gemm(Pd, F0, PF, n);
+        la::gemm(job.Sd, PF, SPF, n);
+        e.resize((size_t)n * n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) e[(size_t)i * n + j] = SPF[(size_t)i * n + j] - SPF[(size_t)j * n + i];
+        diis.addErrorVector(F0, e);
+        diis.solve();
+        diis.calculateResult(F);
+    }
+    la::Mat XF, Fp, Cp;
+    la::gemm(job.X, F, XF, n);
+    la::gemm(XF, job.X, Fp, n);
+    if (!la::syev(Fp, n, eps, Cp)) {
+        job.error = "eigensolver did not converge";
+        return DSSCF_ERR_NOCONV;
+    }
+    la::gemm(job.X, Cp, C, n);
+    // computeDensityMatrix (SCFSolver.cpp:77-94): a spin without electrons
+    // keeps its density (zero from the core guess)
+    if (nocc == 0) return DSSCF_OK;
+    for (int x = 0; x < n; ++x)
+        for (int y = 0; y <= x; ++y) {
+            double v = 0.0;
+            for (unsigned i = 0; i < nocc; ++i) v += C[(size_t)x * n + i] * C[(size_t)y * n + i];
+            P_out[(size_t)x * (x + 1) / 2 + y] = v;
+        }
+    return DSSCF_OK;
+}
+
+// runSCFIteration's open-shell branch (SCFSolver.cpp:145-172): both spins'
+// Fock matrices from the densities at the start of the iteration, then both
+// Roothaan-Hall solutions and densities; Ptot = Pa + Pb;
+// E_elec = sum eps_a,occ + sum eps_b,occ - (tr(Pa Ga) + tr(Pb Gb)) / 2
+// (SCFSolver.cpp:130-141).
+int scfStepUnrestricted(Job& job)
+{
+    const int n = job.n;
+    for (size_t i = 0; i < job.G.size(); ++i) {
+        job.G[i] = job.J[i] - job.K[i];
+        job.Gb[i] = job.J[i] - job.Kb[i];
+    }
+    const std::vector<double> Pa0 = job.Pa, Pb0 = job.Pb;
+    if (int rc = spinStep(job, job.G, Pa0, job.diis, job.F, job.eps, job.C, job.electron_count[0], job.Pa)) return rc;
+    if (int rc = spinStep(job, job.Gb, Pb0, job.diis_b, job.Fb, job.epsb, job.Cb, job.electron_count[1], job.Pb))
+        return rc;
+    for (size_t i = 0; i < job.Ptot.size(); ++i) job.Ptot[i] = job.Pa[i] + job.Pb[i];
+    job.jk_valid = false;
+    double E = 0.0;
+    for (unsigned i = 0; i < job.electron_count[0]; ++i) E += job.eps[i];
+    for (unsigned i = 0; i < job.electron_count[1]; ++i) E += job.epsb[i];
+    E -= 0.5 * (trace_packed(job.Pa, job.G, n) + trace_packed(job.Pb, job.Gb, n));
+    job.previous_electronic_energy = job.total_electronic_energy;
+    job.total_electronic_energy = E;
+    job.history.push_back(E);
+    return DSSCF_OK;
+}
+}  // namespace
+
 // The host part of an iteration, with job.J and job.K at the current density.
 int scfStep(Job& job)
 {
+    if (unrestricted(job)) return scfStepUnrestricted(job);
     const int n = job.n;
     for (size_t i = 0; i < job.G.size(); ++i) job.G[i] = job.J[i] - job.K[i];
     std::vector<double> F0p(job.G.size());
@@ -518,6 +603,10 @@ static int solveMOsDevice(Job& job)
 
 int solveMOs(Job& job)
 {
+    if (job.opt.device_step && unrestricted(job)) {
+        job.error = "the device iteration step is built for restricted jobs only";
+        return DSSCF_ERR_UNSUPPORTED;
+    }
     if (job.opt.device_step) {
         job.iteration = 0;
         job.total_electronic_energy = job.previous_electronic_energy = 0.0;
@@ -534,6 +623,7 @@ int solveMOs(Job& job)
     job.total_electronic_energy = job.previous_electronic_energy = 0.0;
     job.history.clear();
     job.diis.resize(job.opt.diis_size);
+    job.diis_b.resize(job.opt.diis_size);
     if (job.opt.verbose) std::printf("%9s %22s %22s %14s\n", "iteration", "E_elec", "dE", "[PF]^2");
     while (!converged(job)) {
         if ((int)job.iteration >= job.opt.max_iterations) {
@@ -558,8 +648,13 @@ int calculateEnergies(Job& job)
     Energies& e = job.energies;
     e.kinetic = trace_packed(job.Ptot, job.T, n);
     e.nuclear_attraction = trace_packed(job.Ptot, job.V, n);
-    e.electron_repulsion = trace_packed(job.Pa, job.J, n);
-    e.exchange = -trace_packed(job.Pa, job.K, n);
+    if (unrestricted(job)) {
+        e.electron_repulsion = 0.5 * trace_packed(job.Ptot, job.J, n);
+        e.exchange = -0.5 * (trace_packed(job.Pa, job.K, n) + trace_packed(job.Pb, job.Kb, n));
+    } else {
+        e.electron_repulsion = trace_packed(job.Pa, job.J, n);
+        e.exchange = -trace_packed(job.Pa, job.K, n);
+    }
     e.nuclear = 0.0;
     const auto& at = job.molecule.atoms;
     for (size_t i = 0; i < at.size(); ++i)
@@ -578,20 +673,30 @@ int constructBasis() { return constructBasis(current()); }
 void useCoreGuess() { useCoreGuess(current()); }
 int solveMOs() { return solveMOs(current()); }
 
-void computeCoulombTerms(Spin)
+void computeCoulombTerms(Spin spin)
 {
-    // SCFSolver.cpp:29-35: G = J.  One device pass yields J and K; K is kept
-    // for the computeExchangeTerms call that follows.
+    // SCFSolver.cpp:29-35: G_spin = J.  One device pass yields J and K (both
+    // spins' K for an unrestricted job); they are kept for the calls that
+    // follow in the same iteration.
     Job& job = current();
+    if (spin == Spin::beta && unrestricted(job)) {
+        if (!job.jk_valid && fockBuild(job)) return;
+        job.Gb = job.J;
+        return;
+    }
     if (fockBuild(job)) return;
     job.G = job.J;
 }
 
-void computeExchangeTerms(Spin)
+void computeExchangeTerms(Spin spin)
 {
-    // SCFSolver.cpp:37-43: G -= K
+    // SCFSolver.cpp:37-43: G -= K[P_spin]
     Job& job = current();
     if (!job.jk_valid && fockBuild(job)) return;
+    if (spin == Spin::beta && unrestricted(job)) {
+        for (size_t i = 0; i < job.Gb.size(); ++i) job.Gb[i] -= job.Kb[i];
+        return;
+    }
     for (size_t i = 0; i < job.G.size(); ++i) job.G[i] -= job.K[i];
 }
 

@@ -83,6 +83,14 @@ struct Job {
     la::Mat Sd, X, F, C;
     std::vector<double> eps;
     DIIS diis;
+    // unrestricted jobs (multiplicity > 1): the beta spin's density, K[Pb],
+    // G_b = J - K[Pb], Fock matrix, orbitals and DIIS history (the alpha
+    // ones are the fields above; SCFCommon.hpp:6-37 keeps both spins in
+    // two-element arrays)
+    std::vector<double> Pb, Kb, Gb;
+    la::Mat Fb, Cb;
+    std::vector<double> epsb;
+    DIIS diis_b;
     unsigned iteration = 0;
     double total_electronic_energy = 0, previous_electronic_energy = 0;
     Energies energies;
@@ -128,7 +136,8 @@ void useCoreGuess(Job& job);
 // (Optimization.cpp:179-181 keeps the previous geometry's density), else the
 // core guess.
 void useInitialGuess(Job& job);
-int fockBuild(Job& job);  // J[Ptot], K[Pa] on the device
+int fockBuild(Job& job);  // J[Ptot], K[Pa] (and K[Pb] for an unrestricted job) on the device
+inline bool unrestricted(const Job& job) { return job.spin_treatment == SpinTreatment::unrestricted; }
 int runSCFIteration(Job& job);  // fockBuild + scfStep
 int scfStep(Job& job);          // G, DIIS, F' = X F X, diagonalization, density, E_elec
 bool converged(const Job& job);  // the stopping test of solveMOs (SCFSolver.cpp:174-177)

@@ -20,8 +20,9 @@ void dsscf_upload_xform(const double* x, int n, hipStream_t s);
 // nblocks: work units (thread-per-quartet classes) or blocks
 // (wave-cooperative classes, which ignore maxBra).
 // cart: d shells as Cartesian functions (the classes' Cartesian-d form).
+// nks: 2 when a segment is unrestricted (JKSeg::Db set), else 1.
 hipError_t dsscf_launch_jk(int cls, int boys_mode, int fp32, const JKSeg* segs, const int* blk0, int nseg,
-                           int nblocks, int block_size, int maxN, int maxBra, int cart, const double* btab,
+                           int nblocks, int block_size, int maxN, int nks, int maxBra, int cart, const double* btab,
                            double scale, hipStream_t s);
 int dsscf_class_is_wave(int cls);
 // Shell quartets per wavefront of a wave-cooperative class (1 for the others).

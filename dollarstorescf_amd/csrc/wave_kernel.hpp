@@ -273,11 +273,9 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             }
         } else {
             const double* __restrict__ Dt = S->Dt;
-            const double* __restrict__ Da = S->Da;
             const int N = S->N;
             const size_t slice = (size_t)jk_slice(S->nslice) * N * N;
             double* __restrict__ Jh = S->Jh + slice;
-            double* __restrict__ Kh = S->Kh + slice;
             double g = 1.0;
             if (bra.ao_a == bra.ao_b) g *= 0.5;
             if (ket.ao_a == ket.ao_b) g *= 0.5;
@@ -285,8 +283,15 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             const int a0 = bra.ao_a, b0 = bra.ao_b, c0 = ket.ao_a, d0 = ket.ao_b;
             constexpr int T0 = NA * NB, T1 = T0 + NC * ND, T2 = T1 + NA * NC, T3 = T2 + NB * NC, T4 = T3 + NA * ND,
                           T5 = T4 + NB * ND;
-            for (int o = lg; o < T5; o += LG) {
+            // an unrestricted segment digests the exchange terms twice: K[Da]
+            // (outputs T1..T5), then K[Db] into Kbh (T5..T5+T5-T1)
+            const int Tend = S->Db ? T5 + (T5 - T1) : T5;
+            for (int o0 = lg; o0 < Tend; o0 += LG) {
                 double s = 0.0;
+                const bool bx = o0 >= T5;
+                const int o = bx ? o0 - (T5 - T1) : o0;
+                const double* __restrict__ Dx = bx ? S->Db : S->Da;
+                double* __restrict__ Kh = (bx ? S->Kbh : S->Kh) + slice;
                 if (o < T0) {  // Jh[a][b] += 2g sum_cd v Dt_cd
                     const int i = o / NB, j = o % NB;
                     for (int k = 0; k < NC; ++k)
@@ -297,25 +302,25 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
                     for (int i = 0; i < NA; ++i)
                         for (int j = 0; j < NB; ++j) s += VX(i, j, k, l) * Dt[(size_t)(a0 + i) * N + b0 + j];
                     unsafeAtomicAdd(&Jh[(size_t)(c0 + k) * N + d0 + l], 2.0 * g * s);
-                } else if (o < T2) {  // Kh[a][c] += g sum_bd v Da_bd
+                } else if (o < T2) {  // Kh[a][c] += g sum_bd v D_bd
                     const int i = (o - T1) / NC, k = (o - T1) % NC;
                     for (int j = 0; j < NB; ++j)
-                        for (int l = 0; l < ND; ++l) s += VX(i, j, k, l) * Da[(size_t)(b0 + j) * N + d0 + l];
+                        for (int l = 0; l < ND; ++l) s += VX(i, j, k, l) * Dx[(size_t)(b0 + j) * N + d0 + l];
                     unsafeAtomicAdd(&Kh[(size_t)(a0 + i) * N + c0 + k], g * s);
-                } else if (o < T3) {  // Kh[b][c] += g sum_ad v Da_ad
+                } else if (o < T3) {  // Kh[b][c] += g sum_ad v D_ad
                     const int j = (o - T2) / NC, k = (o - T2) % NC;
                     for (int i = 0; i < NA; ++i)
-                        for (int l = 0; l < ND; ++l) s += VX(i, j, k, l) * Da[(size_t)(a0 + i) * N + d0 + l];
+                        for (int l = 0; l < ND; ++l) s += VX(i, j, k, l) * Dx[(size_t)(a0 + i) * N + d0 + l];
                     unsafeAtomicAdd(&Kh[(size_t)(b0 + j) * N + c0 + k], g * s);
-                } else if (o < T4) {  // Kh[a][d] += g sum_bc v Da_bc
+                } else if (o < T4) {  // Kh[a][d] += g sum_bc v D_bc
                     const int i = (o - T3) / ND, l = (o - T3) % ND;
                     for (int j = 0; j < NB; ++j)
-                        for (int k = 0; k < NC; ++k) s += VX(i, j, k, l) * Da[(size_t)(b0 + j) * N + c0 + k];
+                        for (int k = 0; k < NC; ++k) s += VX(i, j, k, l) * Dx[(size_t)(b0 + j) * N + c0 + k];
                     unsafeAtomicAdd(&Kh[(size_t)(a0 + i) * N + d0 + l], g * s);
-                } else {  // Kh[b][d] += g sum_ac v Da_ac
+                } else {  // Kh[b][d] += g sum_ac v D_ac
                     const int j = (o - T4) / ND, l = (o - T4) % ND;
                     for (int i = 0; i < NA; ++i)
-                        for (int k = 0; k < NC; ++k) s += VX(i, j, k, l) * Da[(size_t)(a0 + i) * N + c0 + k];
+                        for (int k = 0; k < NC; ++k) s += VX(i, j, k, l) * Dx[(size_t)(a0 + i) * N + c0 + k];
                     unsafeAtomicAdd(&Kh[(size_t)(b0 + j) * N + d0 + l], g * s);
                 }
             }

@@ -148,6 +148,19 @@ int dsscf_prepare(dsscf_ctx* ctx);
  * computeExchangeTerms (SCFSolver.cpp:37-43). */
 int dsscf_jk(dsscf_ctx* ctx, const double* Ptot, const double* Palpha, double* J, double* K);
 
+/* Unrestricted (open-shell) Fock build: J[Ptot], K[Palpha] and K[Pbeta] from
+ * one pass over the integrals (host pointers, packed-U).  Replaces the
+ * open-shell sweep of runSCFIteration (SCFSolver.cpp:145-166):
+ * computeCoulombTerms for each spin (SCFSolver.cpp:29-35, both read the
+ * total density) and computeExchangeTerms(spin) (SCFSolver.cpp:37-43 over
+ * getExactExchangeMatrix(mu, nu, spin), ExactCoulomb.cpp:114-123).  The
+ * first call allocates the beta buffers (two packed-U arrays and 2 to 17
+ * full N x N arrays, as many as the context's J/K copies plus one); a
+ * context over the LDS limit of the exchange rows (about
+ * N = 400 with d shells) returns DSSCF_ERR_UNSUPPORTED. */
+int dsscf_jk_uhf(dsscf_ctx* ctx, const double* Ptot, const double* Palpha, const double* Pbeta, double* J,
+                 double* Kalpha, double* Kbeta);
+
 /* As dsscf_jk with device pointers (device-resident density / output). */
 int dsscf_jk_device(dsscf_ctx* ctx, const double* d_Ptot, const double* d_Palpha, double* d_J, double* d_K);
 

@@ -58,7 +58,14 @@ int dsscf_job_set_schedule(dsscf_job* job, const double* work_item_flops, const 
  * server at admission); a density of another size falls back to the core
  * guess; n = 0 clears it. */
 int dsscf_job_set_guess(dsscf_job* job, const double* Pa, long long n);
-/* scf::assignMolecule (SCFBasis.cpp:17-37); coordinates in bohr. */
+/* scf::assignMolecule (SCFBasis.cpp:17-37); coordinates in bohr.
+ * Multiplicity 1 gives a restricted job, > 1 an unrestricted one
+ * (SCFBasis.cpp:29-34): alpha and beta densities, Fock matrices and DIIS
+ * histories, K[Pa] and K[Pb] from one pass over the integrals
+ * (dsscf_jk_uhf).  Unrestricted jobs are solved alone with the host
+ * iteration step (dsscf_job_solve): batches, servers, the device step,
+ * gradients and Molden output refuse them with DSSCF_ERR_UNSUPPORTED, and
+ * they start from the core guess. */
 int dsscf_job_set_molecule(dsscf_job* job, int natom, const int* Z, const double* xyz_bohr, int charge,
                            int multiplicity);
 /* readXYZFile (XYZReader.cpp:6-35) + scf::assignMolecule. */
@@ -80,7 +87,10 @@ int dsscf_job_ao(dsscf_job* job, int i, int* l, int* m, double* center, int* npr
 /* Packed-U matrices by name: "S","T","V","H","Pa","Ptot","J","K","G"; "eps"
  * gives n orbital energies (ascending), "C" the n x n row-major MO
  * coefficients (column i = orbital i; scf::coefficient_matrix,
- * SCFCommon.hpp) once the job is solved. */
+ * SCFCommon.hpp) once the job is solved.  An unrestricted job (multiplicity
+ * > 1) also has the beta spin's "Pb","Kb","Gb","epsb","Cb" (the [1] entries
+ * of SCFCommon.hpp's per-spin arrays); its "Pa","K","G","eps","C" are the
+ * alpha spin's. */
 int dsscf_job_get(dsscf_job* job, const char* name, double* out);
 /* Replace the job's orbitals: C n x n row-major (column i = orbital i), eps
  * n orbital energies.  No reference counterpart (its orbitals come only from
@@ -91,7 +101,8 @@ int dsscf_job_set_orbitals(dsscf_job* job, const double* C, const double* eps);
  * orbitals (all n, or the occupied ones), byte-identical to the reference's
  * file for the same orbitals.  Restricted jobs, l <= 2. */
 int dsscf_job_write_molden(dsscf_job* job, const char* path, int write_virtual);
-/* Set the alpha density (packed-U); Ptot = 2 Pa. */
+/* Set the alpha density (packed-U); Ptot = 2 Pa (an unrestricted job:
+ * Pb = Pa as well). */
 int dsscf_job_set_density(dsscf_job* job, const double* Pa);
 /* One Fock build at the current density (computeCoulombTerms +
  * computeExchangeTerms, SCFSolver.cpp:29-43). */

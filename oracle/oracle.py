@@ -19,6 +19,8 @@ Contents, each restating the cited reference code:
                               Pulay error of scf::diis (src/chem/scf/SCFDIIS.cpp:19-52)
 * ``run_scf``              -- scf::solveMOs / runSCFIteration / calculateEnergies
                               (src/chem/scf/SCFSolver.cpp:45-237, src/chem/scf/Energy.cpp:21-78)
+* ``run_uhf``              -- the same for multiplicity > 1 (the unrestricted branches of
+                              SCFSolver.cpp:77-172, SCFDIIS.cpp:11-52, Energy.cpp:21-50)
 
 Pinning is in tests/test_oracle.py (reference build under oracle/_ref, golden
 fixtures under tests/golden, closed forms and invariants).
@@ -501,6 +503,67 @@ def run_scf(S, T, V, jk, n_alpha: int, e_nuc: float, diis_size=8, conv=1e-8, max
     return SCFResult(sum(comps.values()), E, it, hist, comps, Pa)
 
 
+def run_uhf(S, T, V, jk_u, n_alpha: int, n_beta: int, e_nuc: float, diis_size=8, conv=1e-8, max_iter=500):
+    """UHF as scf::solveMOs with spin_treatment unrestricted (SCFSolver.cpp:
+    145-237; multiplicity > 1 selects it, SCFBasis.cpp:29-34).
+
+    ``jk_u(Ptot, Pa, Pb) -> (J, Ka, Kb)`` full matrices.  Core guess Pa = Pb =
+    0 (:179-194).  Per iteration, from the densities at its start: G_s =
+    J[Pa + Pb] - K[P_s] for s = alpha, beta (computeCoulombTerms /
+    computeExchangeTerms per spin, :29-43, :145-159), F0_s = H + G_s, DIIS
+    from iteration 2 with one solver per spin and error S P_s F0_s - (S P_s
+    F0_s)^T (SCFDIIS.cpp:11-52); then both Roothaan-Hall solutions and
+    densities (:160-166); a spin without electrons keeps its density
+    (:77-89).  E_elec = sum eps_a,occ + sum eps_b,occ - (tr(Pa Ga) +
+    tr(Pb Gb)) / 2 (:130-141).  Final energies as calculateEnergies
+    (Energy.cpp:21-50): Coulomb tr(Ptot J)/2, exchange -(tr(Pa Ka) +
+    tr(Pb Kb))/2."""
+    H = T + V
+    s, U = np.linalg.eigh(S)
+    X = (U * (1.0 / np.sqrt(s))) @ U.T
+    n = S.shape[0]
+    P = [np.zeros((n, n)), np.zeros((n, n))]
+    occ = [n_alpha, n_beta]
+    diis = [DIIS(diis_size), DIIS(diis_size)]
+    it, E, Eprev = 0, 0.0, 0.0
+    hist = []
+    while not (it and abs(E - Eprev) < conv):
+        if it >= max_iter:
+            raise RuntimeError("SCF did not converge")
+        it += 1
+        J, Ka, Kb = jk_u(P[0] + P[1], P[0], P[1])
+        G = [J - Ka, J - Kb]
+        F = []
+        for sp in range(2):
+            F0 = H + G[sp]
+            if it == 1:
+                F.append(F0)
+            else:
+                spf = S @ (P[sp] @ F0)
+                diis[sp].add(F0, spf - spf.T)
+                diis[sp].solve()
+                F.append(diis[sp].result())
+        eps = []
+        for sp in range(2):
+            e_s, Cp = np.linalg.eigh(X @ F[sp] @ X)
+            C = X @ Cp
+            eps.append(e_s)
+            if occ[sp]:
+                P[sp] = C[:, :occ[sp]] @ C[:, :occ[sp]].T
+        Eprev = E
+        E = (float(np.sum(eps[0][:n_alpha])) + float(np.sum(eps[1][:n_beta]))
+             - 0.5 * (float(np.sum(P[0] * G[0])) + float(np.sum(P[1] * G[1]))))
+        hist.append(E)
+    Ptot = P[0] + P[1]
+    J, Ka, Kb = jk_u(Ptot, P[0], P[1])
+    comps = dict(kinetic=float(np.sum(Ptot * T)), nuclear_attraction=float(np.sum(Ptot * V)),
+                 coulomb=0.5 * float(np.sum(Ptot * J)),
+                 exchange=-0.5 * (float(np.sum(P[0] * Ka)) + float(np.sum(P[1] * Kb))), nuclear=e_nuc)
+    r = SCFResult(sum(comps.values()), E, it, hist, comps, P[0])
+    r.Pb = P[1]
+    return r
+
+
 def nuclear_repulsion(Z, xyz) -> float:
     """Energy.cpp:43-48 (element numbers as charges)."""
     e = 0.0
@@ -517,8 +580,18 @@ def tensor_jk(tensor, n):
     return jk
 
 
+def tensor_jk_u(tensor, n):
+    """Unrestricted J/K from the packed tensor (the same loops per spin)."""
+    def jk(Ptot, Pa, Pb):
+        return (unpack(coulomb(tensor, n, pack(Ptot)), n), unpack(exchange(tensor, n, pack(Pa)), n),
+                unpack(exchange(tensor, n, pack(Pb)), n))
+    return jk
+
+
 def scf_energy(basis_path: str, Z, xyz, charge=0, mult=1, mode=0, diis_size=8, conv=1e-8, cartesian=False):
-    """Whole oracle single point: basis, AOs, 1e, packed tensor, RHF."""
+    """Whole oracle single point: basis, AOs, 1e, packed tensor, RHF
+    (multiplicity 1) or UHF (SCFBasis.cpp:17-37: n_alpha = (N + mult) / 2,
+    n_beta = (N + 1 - mult) / 2)."""
     bas = read_basis(basis_path)
     aos = build_aos(bas, Z, xyz, cartesian)
     ob = OracleBasis(aos, Z, xyz)
@@ -526,4 +599,7 @@ def scf_energy(basis_path: str, Z, xyz, charge=0, mult=1, mode=0, diis_size=8, c
     t = ob.tensor(mode)
     ne = int(np.sum(Z)) - charge
     n_alpha = (ne + mult) // 2
+    if mult > 1:
+        return run_uhf(S, T, V, tensor_jk_u(t, ob.n), n_alpha, (ne + 1 - mult) // 2, nuclear_repulsion(Z, xyz),
+                       diis_size, conv)
     return run_scf(S, T, V, tensor_jk(t, ob.n), n_alpha, nuclear_repulsion(Z, xyz), diis_size, conv)

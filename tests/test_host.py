@@ -225,6 +225,19 @@ def test_server_refuses_duplicate_tag_second_submit_and_early_destroy():
     b.close()
 
 
+def test_server_refuses_open_shell_job():
+    """An unrestricted job (multiplicity 2) is solved alone: the server
+    refuses it at submit with DSSCF_ERR_UNSUPPORTED (4) and a message."""
+    srv = ds.Server(0, budget_bytes=1e9, max_active=2, workers=1)
+    job = ds.Job(0).set_basis("STO-3G").set_molecule(np.array([1], np.int32), np.zeros((1, 3)), multiplicity=2)
+    with pytest.raises(ds.DsscfError) as ei:
+        srv.submit(job, 3)
+    assert ei.value.code == 4
+    assert "open-shell" in str(ei.value)
+    srv.close()
+    job.close()
+
+
 def test_server_refuses_job_that_can_never_fit():
     """Forced failure: a job whose device-footprint estimate exceeds the whole
     budget is retired at admission with DSSCF_ERR_NOMEM (5) and a message,

@@ -184,6 +184,94 @@ def test_scf_invariants_and_fixture():
     assert np.trace(2 * r.Pa @ S) == pytest.approx(10.0, abs=1e-9)
 
 
+# --------------------------------------------------------- UHF (mult > 1) --
+def _full_tensor(ob, mode=0):
+    n = ob.n
+    idx = np.array([(i, j, k, l) for i in range(n) for j in range(n) for k in range(n) for l in range(n)], np.int32)
+    full = np.zeros((n, n, n, n))
+    full[tuple(idx.T)] = ob.eri_list(idx, mode)
+    return full
+
+
+@pytest.mark.parametrize("basis", ["STO-3G", "6-31G"])
+def test_uhf_one_electron_is_the_core_eigenproblem(basis):
+    """One electron (H atom, doublet): no two-electron energy survives
+    (J[Pa] = K[Pa] cancel in G_a and the beta spin is empty), so the UHF
+    energy is the lowest generalized eigenvalue of (H, S); STO-3G gives the
+    textbook -0.46658 Eh.  Fails a wrong sign of K, a beta density built
+    from zero electrons, or a J/K from the wrong spin."""
+    Z, xyz = np.array([1]), np.zeros((1, 3))
+    r = O.scf_energy(os.path.join(BASIS, basis), Z, xyz, mult=2, conv=1e-12)
+    ob = oracle_basis(basis, Z, xyz)
+    S, T, V = (O.unpack(x, ob.n) for x in ob.one_electron(0))
+    s, U = np.linalg.eigh(S)
+    X = (U / np.sqrt(s)) @ U.T
+    e0 = np.linalg.eigvalsh(X @ (T + V) @ X)[0]
+    assert r.energy == pytest.approx(e0, abs=1e-11)
+    assert np.max(np.abs(r.Pb)) == 0.0
+    assert abs(r.components["coulomb"] + r.components["exchange"]) < 1e-12
+    if basis == "STO-3G":
+        assert r.energy == pytest.approx(-0.46658185, abs=1e-7)
+
+
+def test_uhf_full_alpha_shell_closed_form():
+    """Triplet H2/STO-3G (the reference's test/hydrogen.xyz geometry): both
+    electrons are alpha and fill both orbitals, so Pa = S^-1 whatever the
+    SCF does, and E = tr(Pa H) + 1/2 sum Pa_mn Pa_ls [(mn|ls) - (ml|ns)] +
+    E_nuc from the full four-index tensor."""
+    Z = np.array([1, 1])
+    xyz = np.array([[0.37, 0, 0], [-0.37, 0, 0]]) * O.A_TO_A0
+    r = O.scf_energy(os.path.join(BASIS, "STO-3G"), Z, xyz, mult=3, conv=1e-12)
+    ob = oracle_basis("STO-3G", Z, xyz)
+    S, T, V = (O.unpack(x, ob.n) for x in ob.one_electron(0))
+    full = _full_tensor(ob)
+    Pa = np.linalg.inv(S)
+    e = (np.sum(Pa * (T + V)) + 0.5 * np.einsum("mn,ls,mnls->", Pa, Pa, full)
+         - 0.5 * np.einsum("mn,ls,mlns->", Pa, Pa, full) + O.nuclear_repulsion(Z, xyz))
+    assert np.allclose(r.Pa, Pa, atol=1e-12)
+    assert r.energy == pytest.approx(e, abs=1e-11)
+
+
+def test_uhf_closed_shell_equals_rhf():
+    """With n_alpha = n_beta the unrestricted iteration (two DIIS histories,
+    per-spin Fock matrices, the UHF energy expression) must retrace RHF:
+    water/6-31G, same iterations and energy to 1e-10 Eh."""
+    Z, xyz = G.water()
+    ob = oracle_basis("6-31G", Z, xyz)
+    S, T, V = (O.unpack(x, ob.n) for x in ob.one_electron(0))
+    t = ob.tensor(0)
+    en = O.nuclear_repulsion(Z, xyz)
+    rr = O.run_scf(S, T, V, O.tensor_jk(t, ob.n), 5, en, conv=1e-10)
+    ru = O.run_uhf(S, T, V, O.tensor_jk_u(t, ob.n), 5, 5, en, conv=1e-10)
+    assert ru.energy == pytest.approx(rr.energy, abs=1e-10)
+    assert ru.iterations == rr.iterations
+    assert np.allclose(ru.Pa, ru.Pb, atol=1e-10)
+
+
+def test_uhf_literature_and_invariants():
+    """N atom quartet, STO-3G: 5 alpha electrons fill all five functions
+    and the UHF energy is the textbook -53.71901 Eh.  NH2 radical (doublet,
+    N-H 1.024 A, 103.4 deg), 6-31G: tr(Pa S) = 5, tr(Pb S) = 4, each density
+    idempotent in the S metric, and <S^2> = S(S+1) + n_beta - tr(Pa S Pb S)
+    (the reference's spin-contamination estimate, SCFSolver.cpp:96-116) a
+    little above 3/4.  (The reference's iteration, DIIS from the second
+    step of a core guess, does not settle for radicals whose unpaired
+    electron sits in a degenerate pair, such as OH; NH2 is bent.)"""
+    e = O.scf_energy(os.path.join(BASIS, "STO-3G"), np.array([7]), np.zeros((1, 3)), mult=4, conv=1e-10).energy
+    assert e == pytest.approx(-53.71901, abs=1e-5)
+    th = math.radians(103.4)
+    Z = np.array([7, 1, 1])
+    xyz = np.array([[0, 0, 0], [1.024, 0, 0], [1.024 * math.cos(th), 1.024 * math.sin(th), 0]]) * O.A_TO_A0
+    r = O.scf_energy(os.path.join(BASIS, "6-31G"), Z, xyz, mult=2, conv=1e-10)
+    S = O.unpack(oracle_basis("6-31G", Z, xyz).one_electron(0)[0], len(r.Pa))
+    assert np.trace(r.Pa @ S) == pytest.approx(5.0, abs=1e-9)
+    assert np.trace(r.Pb @ S) == pytest.approx(4.0, abs=1e-9)
+    for P in (r.Pa, r.Pb):
+        assert np.allclose(P @ S @ P, P, atol=1e-7)
+    s2 = 0.75 + 4 - np.trace(r.Pa @ S @ r.Pb @ S)
+    assert 0.75 < s2 < 0.77
+
+
 # ------------------------------------------------- Cartesian d (6D) ----
 def test_cartesian_d_normalization_and_overlap_closed_forms():
     """One-primitive Cartesian d shell: every component has unit self-overlap

@@ -1469,13 +1469,6 @@ int dsscf_eri_gradient(dsscf_ctx* c, const double* Pa, int natom, const int* ao_
     if ((size_t)3 * natom * sizeof(double) > 48 * 1024)
         return fail(DSSCF_ERR_UNSUPPORTED, "dsscf_eri_gradient: more than 2048 atoms");
     const int ncls = dk_n_class();
-    for (int cls = 0; cls < ncls; ++cls) {
-        int l4[4];
-        dk_class_l(cls, l4);
-        const bool sp = l4[0] <= 1 && l4[1] <= 1 && l4[2] <= 1 && l4[3] <= 1;
-        if (!sp && (c->q_cnt[cls] > 0 || c->q_cnt32[cls] > 0 || c->u_cnt[cls] > 0 || c->u_cnt32[cls] > 0))
-            return fail(DSSCF_ERR_UNSUPPORTED, "dsscf_eri_gradient: gradients are built for s and p shells only");
-    }
     HIP_TRY(hipSetDevice(c->device));
     const size_t np = (size_t)N * (N + 1) / 2;
     std::vector<double> Pt(np);
@@ -1492,14 +1485,18 @@ int dsscf_eri_gradient(dsscf_ctx* c, const double* Pa, int natom, const int* ao_
     m.Kh = c->d_Kh;
     m.N = N;
     m.nslice = c->n_slice;
-    // per class: up to two segments (the FP64 and the FP32 work units; both run in FP64 here)
+    // per class: up to two segments (the FP64 and the FP32 work units; both
+    // run in FP64 here), or the flat quartet list of a wave-cooperative class
     std::vector<JKSeg> segs;
-    std::vector<int> blk0, cls_of;
+    std::vector<int> blk0, cls_of, nblk;
     for (int cls = 0; cls < ncls; ++cls)
         for (int pass = 0; pass < 2; ++pass) {
-            const long long cnt = pass ? c->u_cnt32[cls] : c->u_cnt[cls];
+            const bool wave = dsscf_class_is_wave(cls) != 0;
+            if (wave && pass) continue;
+            const long long cnt = wave ? c->q_cnt[cls] : (pass ? c->u_cnt32[cls] : c->u_cnt[cls]);
             if (cnt <= 0) continue;
             JKSeg sg{};
+            sg.q = wave ? c->d_q + c->q_off[cls] : nullptr;
             sg.u = c->d_units + (pass ? c->u_off32[cls] : c->u_off[cls]);
             sg.k = c->d_kets;
             sg.pairs = c->d_pairs;
@@ -1512,6 +1509,7 @@ int dsscf_eri_gradient(dsscf_ctx* c, const double* Pa, int natom, const int* ao_
             segs.push_back(sg);
             blk0.push_back(0);
             cls_of.push_back(cls);
+            nblk.push_back(wave ? (int)((cnt + dsscf_grad_flat_chunk() - 1) / dsscf_grad_flat_chunk()) : (int)cnt);
         }
     MatSeg* d_m = nullptr;
     JKSeg* d_segs = nullptr;
@@ -1544,8 +1542,14 @@ int dsscf_eri_gradient(dsscf_ctx* c, const double* Pa, int natom, const int* ao_
     for (size_t k = 0; e == hipSuccess && k < segs.size(); ++k) {
         int l4[4];
         dk_class_l(cls_of[k], l4);
-        e = dsscf_launch_grad(l4, c->opt.boys_mode, d_segs + k, d_blk0 + k, 1, segs[k].n, c->d_btab, c->d_xform, d_atom,
-                              d_grad, natom, kEriScale, c->stream);
+        const bool sp = l4[0] <= 1 && l4[1] <= 1 && l4[2] <= 1 && l4[3] <= 1;
+        if (sp)
+            e = dsscf_launch_grad(l4, c->opt.boys_mode, d_segs + k, d_blk0 + k, 1, segs[k].n, c->d_btab, c->d_xform,
+                                  d_atom, d_grad, natom, kEriScale, c->stream);
+        else
+            e = dsscf_launch_gradw(l4, c->opt.boys_mode, c->opt.cartesian, dsscf_class_is_wave(cls_of[k]), d_segs + k,
+                                   d_blk0 + k, 1, nblk[k], c->d_btab, c->d_xform, d_atom, d_grad, natom, kEriScale,
+                                   c->stream);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(grad, d_grad, sizeof(double) * 3 * natom, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

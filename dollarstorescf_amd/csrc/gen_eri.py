@@ -484,11 +484,14 @@ def is_wave(c):
     return acc > REG_MAX_ACC or len(hrr) > REG_MAX_HRR
 
 
-def vrr_structure(la, lb, lc, ld):
-    """Structured VRR graph with the same recursion choices as gen_vrr."""
+def vrr_structure(la, lb, lc, ld, targets=None):
+    """Structured VRR graph with the same recursion choices as gen_vrr, for
+    the [e0|f0] targets of class (la lb|lc ld), or for the given (e, f, 0)
+    targets."""
     E, Fk = la + lb, lc + ld
-    targets = [(e, f, 0) for le in range(la, E + 1) for e in cart(le)
-               for lf in range(lc, Fk + 1) for f in cart(lf)]
+    if targets is None:
+        targets = [(e, f, 0) for le in range(la, E + 1) for e in cart(le)
+                   for lf in range(lc, Fk + 1) for f in cart(lf)]
     info, level = {}, {}
 
     def need(key):

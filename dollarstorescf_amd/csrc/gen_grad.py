@@ -289,7 +289,183 @@ def gen_density_xform(c):
 GRAD_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 0), (1, 1, 1, 1)]
 
 
+# ------------------------------------------------------------------------
+# Wave-cooperative gradient classes (every class with a d shell).
+#
+# The derivative integrals of centre X are HRR transfers of [e0|f0] sums,
+# so each class needs, per primitive quartet, the VRR targets of four
+# accumulator kinds:
+#   kind 0 (weight 1)      the lowered sets (a-1x b|, (a b-1x|, |c-1x d)
+#   kind 1 (weight 2 alpha) the A-raised set (a+1x b|
+#   kind 2 (weight 2 beta)  the B-raised set (a b+1x|
+#   kind 3 (weight 2 gamma) the C-raised set |c+1x d)
+# Each derivative set is then one HRR over its kind's accumulators to the
+# Cartesian block of its shifted class, contracted with the Cartesian pair
+# density Gc into the nine derivatives of A, B and C (D follows from
+# translational invariance):
+#   dA_x: sum Gc[a b c d] ((a+1x b|cd)_2alpha - a_x (a-1x b|cd)),  likewise B, C.
+# The tables below drive one generic kernel (k_gradw, grad_kernels.hip).
+
+WAVE_LDS_BUDGET = 65536  # bytes of LDS tables per block up to which G > 1 quartets share a wavefront
+
+
+def lrange(lo, hi):
+    return [t for l in range(max(lo, 0), hi + 1) for t in cart(l)]
+
+
+def grad_sets(c):
+    """(name, kind, shifted class, centre, +1 raised / -1 lowered)."""
+    la, lb, lc, ld = c
+    s = [("A+", 1, (la + 1, lb, lc, ld), 0, 1)]
+    if la > 0:
+        s.append(("A-", 0, (la - 1, lb, lc, ld), 0, -1))
+    s.append(("B+", 2, (la, lb + 1, lc, ld), 1, 1))
+    if lb > 0:
+        s.append(("B-", 0, (la, lb - 1, lc, ld), 1, -1))
+    s.append(("C+", 3, (la, lb, lc + 1, ld), 2, 1))
+    if lc > 0:
+        s.append(("C-", 0, (la, lb, lc - 1, ld), 2, -1))
+    return s
+
+
+def grad_wave_class(c):
+    """Tables of one wave-cooperative gradient class (shared by its
+    spherical and Cartesian-d forms: everything here is Cartesian)."""
+    import gen_eri
+    la, lb, lc, ld = c
+    name = "".join("spdf"[l] for l in c)
+    L1 = la + lb + lc + ld + 1
+    sets = grad_sets(c)
+    kinds = [set() for _ in range(4)]
+    for _, k, (a, b, cc, d), _, _ in sets:
+        kinds[k] |= {(e, f) for e in lrange(a, a + b) for f in lrange(cc, cc + d)}
+    klist = [sorted(k) for k in kinds]
+    alltg = sorted({(e, f, 0) for k in klist for (e, f) in k})
+    keys, info, level, _ = gen_eri.vrr_structure(la, lb, lc, ld, targets=alltg)
+    xi = {k: n for n, k in enumerate(keys)}
+    nbase = sum(1 for k in keys if level[k] == 0)
+    zero = len(keys)
+    nlev = max(level.values())
+    vrr_rows, lev_off = [], [0]
+    for lev in range(1, nlev + 1):
+        for k in keys:
+            if level[k] != lev:
+                continue
+            kind, d, n1, n2, src = info[k]
+            s_ = [xi[t] if t is not None else zero for t in src]
+            assert xi[k] == nbase + len(vrr_rows)
+            vrr_rows.append((s_[0] | (s_[1] << 16), s_[2] | (s_[3] << 16),
+                             s_[4] | (kind << 16) | (d << 18) | (n1 << 20) | (n2 << 24)))
+        lev_off.append(len(vrr_rows))
+    base = [-1] * (L1 + 1)
+    for k in keys:
+        if level[k] == 0:
+            base[k[2]] = xi[k]
+    nx = zero + 1
+    nc4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+
+    def gidx(a, b, x, y):
+        return ((cart(la).index(a) * ncart(lb) + cart(lb).index(b)) * ncart(lc) + cart(lc).index(x)) * ncart(ld) + \
+            cart(ld).index(y)
+
+    # HRR and contraction of each set
+    hrr_rows, hrr_dir, hlev, set_rows = [], [], [], []
+    ce = []  # (set, comp, y, g, coef)
+    for si, (nm, k, (a_, b_, c_, d_), cen, sign) in enumerate(sets):
+        tidx = {ef: n for n, ef in enumerate(klist[k])}
+        ent, levs, cart_y, ny = gen_eri.hrr_structure(a_, b_, c_, d_, len(klist[k]), tidx)
+        h0, l0 = len(hrr_rows), len(hlev)
+        for (s1, s2, i, st) in ent:
+            hrr_rows.append(s1 | (s2 << 16))
+            hrr_dir.append(i | (st << 2))
+        for (e0, e1) in levs:
+            hlev.append((h0 + e0, h0 + e1))
+        set_rows.append((k, len(klist[k]), l0, len(hlev)))
+        outs = [(A, B, C, D) for A in cart(a_) for B in cart(b_) for C in cart(c_) for D in cart(d_)]
+        for (A, B, C, D), y in zip(outs, cart_y):
+            shifted = (A, B, C)[cen]
+            for x in range(3):
+                if sign > 0:
+                    if shifted[x] == 0:
+                        continue
+                    orig = sub1(shifted, x)
+                    coef = 1
+                else:
+                    orig = add1(shifted, x)
+                    coef = -orig[x]
+                q = [A, B, C]
+                q[cen] = orig
+                ce.append((si, 3 * cen + x, y, gidx(q[0], q[1], q[2], D), coef))
+    ce.sort(key=lambda t: (t[0], t[1]))
+    cstart = []
+    for si in range(len(sets)):
+        for comp in range(10):
+            cstart.append(next((n for n, t in enumerate(ce) if (t[0], t[1]) >= (si, comp)), len(ce)))
+    nacc = [len(k) for k in klist]
+    # LDS per quartet group: the Cartesian pair density, then a region used
+    # in turn for the density staging (two Cartesian blocks), the VRR table
+    # and each set's leaves + HRR entries
+    G = 8
+    while True:
+        LG = 64 // G
+        pad = [(n + LG - 1) // LG * LG for n in nacc]
+        ysz = 0
+        for si, (k, n, l0, l1) in enumerate(set_rows):
+            nent = sum(hlev[j][1] - hlev[j][0] for j in range(l0, l1))
+            ysz = max(ysz, pad[k] + nent)
+        region = max(nx, ysz, 2 * nc4)
+        lds = nc4 + region
+        if G == 1 or 8 * G * lds <= WAVE_LDS_BUDGET:
+            break
+        G //= 2
+    ks = [0]
+    for k in range(4):
+        ks.append(ks[-1] + pad[k] // LG)
+    tgt = []
+    for k in range(4):
+        lst = [xi[(e, f, 0)] for (e, f) in klist[k]]
+        tgt += lst + [zero] * (pad[k] - len(lst))
+    o = []
+    o.append("// ---- gradient class (%s): wave-cooperative, %d VRR entries (L = %d), accumulators %s, "
+             "%d HRR entries, %d contraction terms, G = %d, LDS %d doubles per quartet"
+             % (name, len(keys), L1, "/".join(str(n) for n in nacc), len(hrr_rows), len(ce), G, lds))
+
+    def arr(t, nm, vals):
+        return "__device__ const %s %s[%d] = {%s};" % (t, nm, max(len(vals), 1), ",".join(map(str, vals)) or "0")
+    o.append(arr("int", "gw_%s_vrr" % name, [v for r in vrr_rows for v in r]))
+    o.append(arr("int", "gw_%s_vlev" % name, lev_off))
+    o.append(arr("short", "gw_%s_base" % name, base))
+    o.append(arr("short", "gw_%s_tgt" % name, tgt))
+    o.append(arr("int", "gw_%s_hrr" % name, hrr_rows))
+    o.append(arr("char", "gw_%s_hdir" % name, hrr_dir))
+    o.append(arr("int", "gw_%s_hlev" % name, [v for lv in hlev for v in lv]))
+    o.append(arr("int", "gw_%s_set" % name, [v for r in set_rows for v in r]))
+    o.append(arr("int", "gw_%s_cstart" % name, cstart))
+    o.append(arr("short", "gw_%s_cey" % name, [t[2] for t in ce]))
+    o.append(arr("short", "gw_%s_ceg" % name, [t[3] for t in ce]))
+    o.append(arr("signed char", "gw_%s_cec" % name, [t[4] for t in ce]))
+    o.append("template <> struct GradWaveTab<%d,%d,%d,%d> {" % c)
+    o.append("  static constexpr int L1 = %d, G = %d, LG = %d, NBASE = %d, NVE = %d, ZERO = %d, NLEV = %d, NX = %d,"
+             % (L1, G, LG, nbase, len(vrr_rows), zero, nlev, nx))
+    o.append("      NSET = %d, NC4 = %d, REGION = %d, LDS = %d, SLOTS = %d, KS1 = %d, KS2 = %d, KS3 = %d;"
+             % (len(sets), nc4, region, lds, ks[4], ks[1], ks[2], ks[3]))
+    o.append("  static constexpr int ks(int k) { return k == 0 ? 0 : k == 1 ? KS1 : k == 2 ? KS2 : k == 3 ? KS3 : SLOTS; }")
+    for fld, t in (("vrr", "int"), ("vlev", "int"), ("base", "short"), ("tgt", "short"), ("hrr", "int"),
+                   ("hdir", "char"), ("hlev", "int"), ("set", "int"), ("cstart", "int"), ("cey", "short"),
+                   ("ceg", "short"), ("cec", "signed char")):
+        o.append("  static __device__ __forceinline__ const %s* %s() { return gw_%s_%s; }" % (t, fld, name, fld))
+    o.append("};")
+    return "\n".join(o)
+
+
+def grad_wave_classes(maxl=2):
+    import gen_eri
+    _, cls_list = gen_eri.classes(maxl)
+    return [c for c in cls_list if max(c) >= 2]
+
+
 def main():
+    import gen_eri
     out = sys.argv[1]
     text = ["// Generated by gen_grad.py -- do not edit.", "#pragma once"]
     for c in GRAD_CLASSES:
@@ -297,6 +473,20 @@ def main():
         text.append(gen_class(c))
     text.append("#define DSSCF_FOR_EACH_GRAD_CLASS(X) \\")
     text.append(" \\\n".join("    X(%d, %d, %d, %d, %s)" % (*c, "".join("spdf"[l] for l in c)) for c in GRAD_CLASSES))
+    # AO transform offsets (as the ERI classes' xform_off) for the wave classes' density transform
+    xs = [gen_eri.xform_offset(l) for l in range(gen_eri.MAXL + 1)]
+    xc = [gen_eri.xform_offset_cart(l) if l >= 2 else gen_eri.xform_offset(l) for l in range(gen_eri.MAXL + 1)]
+    text.append("__device__ __forceinline__ constexpr int grad_xform_off(int l, int C) {")
+    text.append("  constexpr int xs[%d] = {%s}, xc[%d] = {%s};" % (len(xs), ",".join(map(str, xs)), len(xc),
+                                                                 ",".join(map(str, xc))))
+    text.append("  return C && l >= 2 ? xc[l] : xs[l];")
+    text.append("}")
+    text.append("template <int la, int lb, int lc, int ld> struct GradWaveTab;")
+    wc = grad_wave_classes()
+    for c in wc:
+        text.append(grad_wave_class(c))
+    text.append("#define DSSCF_FOR_EACH_GRAD_WAVE_CLASS(X) \\")
+    text.append(" \\\n".join("    X(%d, %d, %d, %d, %s)" % (*c, "".join("spdf"[l] for l in c)) for c in wc))
     with open(out, "w") as f:
         f.write("\n".join(text) + "\n")
 

@@ -329,8 +329,8 @@ int gradient(Job& job, std::vector<double>& grad, double* parts)
         return DSSCF_ERR_UNSUPPORTED;
     }
     for (const auto& g : job.basis)
-        if (g.l > 1) {
-            job.error = "gradient: built for s and p shells only";
+        if (g.cartesian) {
+            job.error = "gradient: built for the reference's spherical AOs only (no Cartesian d)";
             return DSSCF_ERR_UNSUPPORTED;
         }
     const unsigned nocc = job.electron_count[0];

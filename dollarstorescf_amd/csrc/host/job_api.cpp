@@ -247,7 +247,8 @@ int dsscf_job_gradient_terms(dsscf_job* j, const double* Pt, const double* W, do
     scf::Job& b = j->job;
     if (b.basis.empty()) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_gradient_terms: construct first");
     for (const auto& g : b.basis)
-        if (g.l > 1) return set_err(j, DSSCF_ERR_UNSUPPORTED, "dsscf_job_gradient_terms: s and p shells only");
+        if (g.cartesian)
+            return set_err(j, DSSCF_ERR_UNSUPPORTED, "dsscf_job_gradient_terms: spherical AOs only (no Cartesian d)");
     const size_t np = (size_t)b.n * (b.n + 1) / 2;
     std::vector<double> pt(Pt, Pt + np), w(W, W + np), g1, gn;
     flo::oneElectronGradient(b.basis, b.basis_atoms, b.molecule, b.opt.boys_mode, pt, w, g1);

@@ -50,6 +50,15 @@ hipError_t dsscf_launch_boys_eval(int mode, int L, const double* T, int n, const
 // ERI part of the nuclear gradient for one s/p class (l4) over nseg context
 // segments of J/K work units: grad[3*natom] += its contribution (see
 // grad_kernels.hip); xform the AO transform, ao_atom[N] the atom of each AO.
+// The same for a class with a d shell (wave-cooperative, one 64-lane block
+// per work unit, or with flat != 0 per dsscf_grad_flat_chunk() quartets of
+// the segments' flat lists); cart: the Cartesian-d form.  Fails with
+// hipErrorInvalidValue when the class's LDS tables and the 3 natom doubles
+// of the block's gradient copy exceed the 160 KiB of a workgroup.
+hipError_t dsscf_launch_gradw(const int* l4, int boys_mode, int cart, int flat, const JKSeg* segs, const int* blk0,
+                              int nseg, int nblocks, const double* btab, const double* xform, const int* ao_atom,
+                              double* grad, int natom, double scale, hipStream_t s);
+int dsscf_grad_flat_chunk();
 hipError_t dsscf_launch_grad(const int* l4, int boys_mode, const JKSeg* segs, const int* blk0, int nseg, int nblocks,
                              const double* btab, const double* xform, const int* ao_atom, double* grad, int natom,
                              double scale, hipStream_t s);

@@ -214,8 +214,10 @@ int dsscf_boys_eval(dsscf_ctx* ctx, int mode, int L, int n, const double* T, dou
  * quantity scf::addExactTwoElectronGradient adds (ExactCoulomb.cpp:125-195,
  * over flo::electronRepulsionGradient, GaussianIntegral.cpp:1052-1459);
  * ao_atom[N] gives each AO's atom.  Uses the prepared quartet lists (Schwarz
- * screening as the Fock build; 0 keeps every quartet).  s and p shells only:
- * DSSCF_ERR_UNSUPPORTED if the basis has d shells.  Synchronous. */
+ * screening as the Fock build; 0 keeps every quartet).  s, p and d shells,
+ * spherical or Cartesian d; the class with the most LDS, (dd|dd), leaves
+ * room for the block's copy of about 1800 atoms (DSSCF_ERR_HIP beyond).
+ * Synchronous. */
 int dsscf_eri_gradient(dsscf_ctx* ctx, const double* Pa, int natom, const int* ao_atom, double* grad);
 
 /* Schwarz bounds Q_ab of all unique shell pairs (a >= b, index a(a+1)/2+b). */

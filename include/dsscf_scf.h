@@ -115,13 +115,14 @@ int dsscf_job_solve(dsscf_job* job, double* energy, int* iterations);
  * runs on the device (dsscf_eri_gradient), the one-electron and
  * nuclear-repulsion parts on the host.  parts (may be null) receives the
  * one-electron, two-electron and nuclear-repulsion parts, 3*natom each.
- * s and p shells only (DSSCF_ERR_UNSUPPORTED otherwise). */
+ * s, p and d shells of the reference's spherical AOs (a Cartesian-d job:
+ * DSSCF_ERR_UNSUPPORTED). */
 int dsscf_job_gradient(dsscf_job* job, double* grad, double* parts);
 /* The host terms of the gradient for given packed-U total density Pt and
  * energy-weighted density W: one_e[3*natom] the kinetic, overlap and
  * nuclear-attraction terms (Gradient.cpp:61-116), nuclear[3*natom] the
  * nuclear repulsion (Gradient.cpp:118-132).  Needs no GPU (a constructed
- * or host-constructed job).  s and p shells only. */
+ * or host-constructed job).  Spherical AOs (s, p, d). */
 int dsscf_job_gradient_terms(dsscf_job* job, const double* Pt, const double* W, double* one_e, double* nuclear);
 /* Atoms of the job's molecule (scf::molecule.size()). */
 int dsscf_job_n_atom(dsscf_job* job);

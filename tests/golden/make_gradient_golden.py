@@ -1,8 +1,10 @@
 """Writes tests/golden/gradient.json: the reference's RHF gradient terms
 (scf::getEnergyGradient's loops over its own derivative integrals,
-oracle/_ref, reference Boys series) for seeded synthetic densities on two
-s/p systems.  The densities are random (Pa = C_occ C_occ^T of a random C, W
+oracle/_ref, reference Boys series) for seeded synthetic densities on s/p and
+d-shell systems.  The densities are random (Pa = C_occ C_occ^T of a random C, W
 from random orbital energies): the fixture pins the formula, not an SCF.
+Systems: water/STO-3G and two waters/6-31G (s, p), water/def2-SVP,
+water/6-31G* and two waters/def2-SVP (d shells, up to (dd|dd)).
 
 Usage (in a container with the reference tree, after `make -C oracle`):
     python tests/golden/make_gradient_golden.py
@@ -29,7 +31,11 @@ def main():
     out = {}
     Zw, xw = G.water()
     Z2, x2 = G.water_box(2, seed=3)
-    for name, basis, Z, xyz, nocc in [("water_sto3g", "STO-3G", Zw, xw, 5), ("water2_631g", "6-31G", Z2, x2, 10)]:
+    cases = [("water_sto3g", "STO-3G", Zw, xw, 5), ("water2_631g", "6-31G", Z2, x2, 10),
+             # d shells: one water (d on O), and two waters whose O d shells meet in (dd|dd)
+             ("water_def2svp", "def2-SVP", Zw, xw, 5), ("water_631gs", "6-31GP", Zw, xw, 5),
+             ("water2_def2svp", "def2-SVP", Z2, x2, 10)]
+    for name, basis, Z, xyz, nocc in cases:
         rm = R.RefMolecule(ds.basis_path(basis), Z, xyz)
         n = rm.n
         rng = np.random.default_rng(17)

@@ -1,6 +1,6 @@
 """RHF nuclear gradients on the device (dsscf_eri_gradient, dsscf_job_gradient)
 against the reference's derivative integrals (fixtures from oracle/_ref) and
-against finite differences of the SCF energy."""
+against finite differences of the SCF energy, for s, p and d shells."""
 import json
 import os
 
@@ -24,13 +24,15 @@ def _ao_atom(job, xyz):
                     np.int32)
 
 
-@pytest.mark.parametrize("case", ["water_sto3g", "water2_631g"])
+@pytest.mark.parametrize("case", ["water_sto3g", "water2_631g", "water_def2svp", "water_631gs", "water2_def2svp"])
 @pytest.mark.parametrize("boys", [ds.BOYS_REFERENCE, ds.BOYS_ACCURATE])
 def test_eri_gradient_matches_reference(case, boys):
     """Two-electron gradient part (ExactCoulomb.cpp:125-195 over
     electronRepulsionGradient) for a fixed synthetic density, no screening:
     1e-11 relative to the largest component in the reference Boys mode, 1e-9
-    in the accurate mode (the two Boys forms differ by up to 3e-11)."""
+    in the accurate mode (the two Boys forms differ by up to 3e-11).  The
+    d-shell systems run every class with a d shell through the
+    wave-cooperative kernel, two waters/def2-SVP up to (dd|dd)."""
     g = _fixture(case)
     Z = np.array(g["Z"], np.int32)
     xyz = np.array(g["xyz_bohr"]).reshape(-1, 3)
@@ -39,7 +41,27 @@ def test_eri_gradient_matches_reference(case, boys):
     got = job.context.eri_gradient(np.array(g["Pa"]), _ao_atom(job, xyz), len(Z))
     want = np.array(g["two_electron"])
     tol = (1e-11 if boys == ds.BOYS_REFERENCE else 1e-9) * np.max(np.abs(want))
+    print(f"{case} boys={boys}: max |d| = {np.max(np.abs(got - want)):.2e} of max {np.max(np.abs(want)):.2e}", flush=True)
     assert np.max(np.abs(got - want)) < tol, np.max(np.abs(got - want))
+
+
+def test_eri_gradient_d_classes_with_sub_pairs_and_lane_groups():
+    """The same d-shell fixture with every shell pair split into sub-pairs of
+    at most 5 primitive pairs and lane groups forced on (the work units the
+    gradient shares with the Fock build): the sub-quartets add up to the
+    same derivatives (1e-11 relative, reference Boys)."""
+    g = _fixture("water2_def2svp")
+    Z = np.array(g["Z"], np.int32)
+    xyz = np.array(g["xyz_bohr"]).reshape(-1, 3)
+    job = ds.Job(0).set_options(boys_mode=ds.BOYS_REFERENCE, schwarz_threshold=0.0)
+    job.set_basis(g["basis"]).set_molecule(Z, xyz).construct()
+    ctx = job.context
+    ctx.set_options(boys_mode=ds.BOYS_REFERENCE, schwarz_threshold=0.0, work_item_flops=1.0, group_prims=1.0)
+    ctx.prepare()
+    assert ctx.stats().n_work_pairs > ctx.stats().n_pairs
+    got = ctx.eri_gradient(np.array(g["Pa"]), _ao_atom(job, xyz), len(Z))
+    want = np.array(g["two_electron"])
+    assert np.max(np.abs(got - want)) < 1e-11 * np.max(np.abs(want))
 
 
 def _energy(basis, Z, xyz):
@@ -49,7 +71,8 @@ def _energy(basis, Z, xyz):
     return e, job
 
 
-@pytest.mark.parametrize("basis,system", [("STO-3G", "water"), ("6-31G", "water2")])
+@pytest.mark.parametrize("basis,system", [("STO-3G", "water"), ("6-31G", "water2"), ("def2-SVP", "water"),
+                                          ("6-31GP", "water2")])
 def test_gradient_matches_finite_differences(basis, system):
     """dsscf_job_gradient at the converged density vs central differences of
     the total SCF energy (h = 1e-3 bohr, SCF converged to 1e-11): 2e-6 Eh/bohr;
@@ -72,9 +95,10 @@ def test_gradient_matches_finite_differences(basis, system):
         assert abs(fd - grad[a, d]) < 2e-6, (a, d, fd, grad[a, d])
 
 
-def test_gradient_rejects_d_shells():
+def test_gradient_rejects_cartesian_d():
     Z, xyz = G.water()
-    job = ds.Job(0).set_basis("def2-SVP").set_molecule(Z, xyz).construct()
+    job = ds.Job(0).set_options(cartesian=True).set_basis("6-31GP").set_molecule(Z, xyz).construct()
     job.solve()
-    with pytest.raises(ds.DsscfError):
+    with pytest.raises(ds.DsscfError) as ei:
         job.gradient()
+    assert ei.value.code == 4

@@ -290,7 +290,7 @@ def test_molden_needs_orbitals(tmp_path):
         job.set_orbitals(np.zeros((3, 3)), np.zeros(3))
 
 
-@pytest.mark.parametrize("case", ["water_sto3g", "water2_631g"])
+@pytest.mark.parametrize("case", ["water_sto3g", "water2_631g", "water_def2svp", "water_631gs", "water2_def2svp"])
 def test_gradient_host_terms_match_reference(case):
     """One-electron (kinetic, overlap, nuclear attraction) and nuclear-repulsion
     gradient terms (Gradient.cpp:61-132) for fixed synthetic densities vs the
@@ -311,9 +311,11 @@ def test_gradient_host_terms_match_reference(case):
     assert np.max(np.abs((g1 + gn).sum(axis=0))) < 1e-10
 
 
-def test_gradient_terms_reject_d_shells():
+def test_gradient_terms_reject_cartesian_d():
+    """The gradient follows the reference's AO set: a Cartesian-d job is refused."""
     Z, xyz = G.water()
-    job = ds.Job(0).set_basis("def2-SVP").set_molecule(Z, xyz).construct_host()
+    job = ds.Job(0).set_options(cartesian=True).set_basis("6-31GP").set_molecule(Z, xyz).construct_host()
     n = job.n_ao
-    with pytest.raises(ds.DsscfError):
+    with pytest.raises(ds.DsscfError) as ei:
         job.gradient_terms(np.zeros(n * (n + 1) // 2), np.zeros(n * (n + 1) // 2))
+    assert ei.value.code == 4

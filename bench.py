@@ -271,19 +271,19 @@ def bench_scf(a):
 
 
 def bench_gradient(a):
-    """RHF nuclear gradient of the 8-water/6-31G lattice (104 AOs, s/p shells)
-    at its converged density: the two-electron part on the device (every
-    surviving shell quartet, Schwarz 1e-12), the one-electron and
-    nuclear-repulsion parts on the host.  CPU baseline: the reference's
-    electronRepulsionGradient (oracle/_ref, 1 core) on a sample of unique AO
-    quartets, extrapolated to all of them."""
+    """RHF nuclear gradient of the 8-water lattice (6-31G: 104 AOs, s/p
+    shells; def2-SVP: 200 AOs with d shells) at its converged density: the
+    two-electron part on the device (every surviving shell quartet, Schwarz
+    1e-12), the one-electron and nuclear-repulsion parts on the host.  CPU
+    baseline: the reference's electronRepulsionGradient (oracle/_ref, 1
+    core) on a sample of unique AO quartets, extrapolated to all of them."""
     import dollarstorescf_amd as ds
     from dollarstorescf_amd import geometry
 
     Z, xyz = geometry.water_lattice(1)
     xyz = np.asarray(xyz, float).reshape(-1, 3)
     job = ds.Job(0).set_options(diis_size=8, energy_threshold=1e-10, max_iterations=60, schwarz_threshold=1e-12)
-    job.set_basis("6-31G").set_molecule(Z, xyz).construct()
+    job.set_basis(a.basis).set_molecule(Z, xyz).construct()
     energy, iters = job.solve()
     n = job.n_ao
     ao_atom = np.array([int(np.argmin(np.linalg.norm(xyz - job.ao(i)["center"], axis=1))) for i in range(n)], np.int32)
@@ -303,7 +303,7 @@ def bench_gradient(a):
     if not a.no_cpu:
         from oracle import ref
         if ref.available():
-            rm = ref.RefMolecule(ds.basis_path("6-31G"), Z, xyz.reshape(-1))
+            rm = ref.RefMolecule(ds.basis_path(a.basis), Z, xyz.reshape(-1))
             rng = np.random.default_rng(5)
             sample = max(1000, a.cpu_sample // 10)
             q = rng.integers(0, n, size=(sample, 4)).astype(np.int32)
@@ -316,7 +316,7 @@ def bench_gradient(a):
     out = {"metric": METRIC, "value": 1e3 / ms_eri, "unit": "ERI gradients/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": ms_eri, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded water lattice geometry)",
-           "config": {"workload": "RHF/6-31G nuclear gradient, 8-water 2x2x2 lattice, 104 AOs, Schwarz 1e-12, "
+           "config": {"workload": f"RHF/{a.basis} nuclear gradient, 8-water 2x2x2 lattice, {n} AOs, Schwarz 1e-12, "
                                   "converged density", "parallelism": "single GPU"},
            "ms_full_gradient": ms_total, "scf_energy": energy, "scf_iterations": iters,
            "max_abs_gradient": float(np.max(np.abs(g))), "sum_over_atoms": np.abs(g.sum(axis=0)).max(),
@@ -530,6 +530,8 @@ def main():
                          "job at a time; serve: Poisson stream of SCF requests, mixed precision vs FP64; scf: whole "
                          "SCF solves with the iteration step on the device vs the host; gradient: RHF nuclear "
                          "gradient of the water8 system")
+    ap.add_argument("--basis", default="6-31G", help="basis of the gradient workload (6-31G: s/p; def2-SVP or "
+                                                     "6-31GP: with d shells)")
     ap.add_argument("--jobs", type=int, default=512, help="jobs of the batch workload")
     ap.add_argument("--requests", type=int, default=200, help="requests of the serve workload")
     ap.add_argument("--rate", type=float, default=200.0, help="mean arrival rate of the serve workload (1/s)")

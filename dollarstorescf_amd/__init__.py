@@ -45,7 +45,9 @@ class Stats(ctypes.Structure):
                 ("class_quartets", ctypes.c_longlong * 21), ("class_prim_quartets", ctypes.c_longlong * 21),
                 ("n_quartets_fp32", ctypes.c_longlong), ("class_quartets_fp32", ctypes.c_longlong * 21),
                 ("class_prim_quartets_fp32", ctypes.c_longlong * 21), ("n_work_pairs", ctypes.c_longlong),
-                ("n_work_items", ctypes.c_longlong)]
+                ("n_work_items", ctypes.c_longlong), ("class_lanes", ctypes.c_longlong * 21),
+                ("class_lane_slots", ctypes.c_longlong * 21), ("class_lane_trips", ctypes.c_longlong * 21),
+                ("class_wave_trips", ctypes.c_longlong * 21)]
 
 
 _LIB = None

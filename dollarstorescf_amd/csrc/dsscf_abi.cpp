@@ -263,6 +263,8 @@ struct dsscf_ctx {
     // per class [u_off, u_off+u_cnt) FP64 and [u_off32, u_off32+u_cnt32) FP32
     std::vector<long long> u_off, u_cnt, u_off32, u_cnt32;
     std::vector<int> u_maxbra, u_maxbra32;  // per class: most primitive pairs of a unit's bra
+    // per class: wavefront fill of the work units (dsscf_stats class_lanes etc.)
+    std::vector<long long> fill_lanes, fill_slots, fill_lane_trips, fill_wave_trips;
     long long n_unscreened = 0;
     long long n_work_pairs = 0, n_work_items = 0;  // shell pairs + sub-pairs; (sub-)quartets
     int n_slice = 1;                                // copies of Jh/Kh
@@ -672,6 +674,7 @@ int dsscf_prepare(dsscf_ctx* c)
     };
 
     c->chunk.assign(ncls, 512);
+    for (auto* v : {&c->fill_lanes, &c->fill_slots, &c->fill_lane_trips, &c->fill_wave_trips}) v->assign(ncls, 0);
     c->u_maxbra.assign(ncls, 0);
     c->u_maxbra32.assign(ncls, 0);
     for (auto* v : {&c->q_off, &c->q_cnt, &c->q_prim, &c->q_off32, &c->q_cnt32, &c->q_prim32, &c->u_off, &c->u_cnt,
@@ -921,6 +924,29 @@ int dsscf_prepare(dsscf_ctx* c)
         };
         for (size_t k = (size_t)u0; k < units.size(); ++k) set_group(units[k]);
         for (int4& u : units32) set_group(u);
+        // wavefront fill of the lists (diagnostics for dsscf_get_stats, at
+        // 64-lane granularity): lanes the units keep busy against the lanes of
+        // the wavefronts they occupy, and lane-trips of the primitive loops
+        // against the trips the wavefronts run (each runs its longest lane)
+        auto fill = [&](const int4& u) {
+            const int gb = u.w & 0xff, gk = (u.w >> 8) & 0xff, G = gb * gk;
+            const int per_wave = std::max(1, 64 / G);
+            const long long tb = (work[u.x].nprim + gb - 1) / gb;
+            for (int w0 = 0; w0 < u.z; w0 += per_wave) {
+                const int w1 = std::min(u.z, w0 + per_wave);
+                long long mx = 0;
+                for (int q = w0; q < w1; ++q) {
+                    const long long trips = tb * ((work[kets[u.y + q]].nprim + gk - 1) / gk);
+                    c->fill_lane_trips[cls] += trips * G;
+                    mx = std::max(mx, trips);
+                }
+                c->fill_wave_trips[cls] += mx * 64;
+                c->fill_lanes[cls] += (long long)(w1 - w0) * G;
+                c->fill_slots[cls] += 64;
+            }
+        };
+        for (size_t k = (size_t)u0; k < units.size(); ++k) fill(units[k]);
+        for (const int4& u : units32) fill(u);
         c->u_maxbra[cls] = c->u_maxbra32[cls] = 0;
         for (size_t k = (size_t)u0; k < units.size(); ++k)
             c->u_maxbra[cls] = std::max(c->u_maxbra[cls], work[units[k].x].nprim);
@@ -1749,6 +1775,12 @@ int dsscf_get_stats(dsscf_ctx* c, dsscf_stats* st)
         st->n_quartets += c->q_cnt[k] + c->q_cnt32[k];
         st->n_prim_quartets += c->q_prim[k] + c->q_prim32[k];
         st->n_quartets_fp32 += c->q_cnt32[k];
+        if (k < (int)c->fill_lanes.size()) {
+            st->class_lanes[k] = c->fill_lanes[k];
+            st->class_lane_slots[k] = c->fill_slots[k];
+            st->class_lane_trips[k] = c->fill_lane_trips[k];
+            st->class_wave_trips[k] = c->fill_wave_trips[k];
+        }
     }
     st->n_work_pairs = c->n_work_pairs;
     st->n_work_items = c->n_work_items;

@@ -96,6 +96,15 @@ typedef struct {
     long long class_prim_quartets_fp32[21];
     long long n_work_pairs;        /* shell pairs plus the sub-pairs of split ones (work_item_flops) */
     long long n_work_items;        /* (sub-)quartets the thread-per-quartet kernels evaluate */
+    /* Wavefront fill of the thread-per-quartet classes' work units, per class
+     * (64-lane wavefronts): lanes with a quartet against the lanes of the
+     * wavefronts the units occupy, and the primitive-loop trips of those
+     * lanes against the trips the wavefronts run (each runs its longest
+     * lane).  Diagnostics; no reference counterpart. */
+    long long class_lanes[21];
+    long long class_lane_slots[21];
+    long long class_lane_trips[21];
+    long long class_wave_trips[21];
 } dsscf_stats;
 
 /* Last error message of the calling thread ("" if none). */

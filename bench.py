@@ -203,16 +203,25 @@ def bench_alkane(a):
                       "prim_quartets": int(st.n_prim_quartets),
                       "fock_build_tflops": flops.fock_build_flops(classes, cq, cp) / (ms * 1e-3) / 1e12,
                       "history": [float(x) for x in job.history()],
-                      "_cls": per_cls, "_q": cq, "_q32": cq32, "_jk": (J, K)}
+                      "_cls": per_cls, "_q": cq, "_q32": cq32, "_p": cp, "_jk": (J, K)}
     h0, h1 = runs["fp64"].pop("history"), runs["mixed"].pop("history")
     c64, c32 = runs["fp64"].pop("_cls"), runs["mixed"].pop("_cls")
     q64, q32 = runs["fp64"].pop("_q"), runs["mixed"].pop("_q32")
     runs["mixed"].pop("_q")
     runs["fp64"].pop("_q32")
+    p64 = runs["fp64"].pop("_p")
+    runs["mixed"].pop("_p")
     (J64, K64), (J32, K32) = runs["fp64"].pop("_jk"), runs["mixed"].pop("_jk")
-    per_class = [{"class": "".join("spdf"[l] for l in c), "quartets": int(q), "fp32_quartets": int(f),
-                  "fp64_ms": float(m64), "mixed_ms": float(m32)}
-                 for c, q, f, m64, m32 in zip(classes, q64, q32, c64, c32) if q > 0]
+    # per class, one stream, classes in turn; FP64 roofline: algorithmic FLOPs
+    # (dollarstorescf_amd/flops.py) / the class's FP64 duration / 78.6 TFLOP/s
+    per_class = []
+    for c, q, pq, f, m64, m32 in zip(classes, q64, p64, q32, c64, c32):
+        if q <= 0:
+            continue
+        tf = flops.class_flops(c, q, pq) / (m64 * 1e-3) / 1e12 if m64 > 0 else 0.0
+        per_class.append({"class": "".join("spdf"[l] for l in c), "quartets": int(q), "prim_quartets": int(pq),
+                          "fp32_quartets": int(f), "fp64_ms": float(m64), "mixed_ms": float(m32),
+                          "fp64_tflops": tf, "fp64_frac": tf / FP64_PEAK_TFLOPS})
     out = {
         "metric": METRIC, "value": runs["mixed"]["fock_builds_per_s"], "unit": "Fock builds/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": runs["mixed"]["ms_per_build"],

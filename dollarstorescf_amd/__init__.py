@@ -100,7 +100,8 @@ def lib():
                            "dsscf_batch_time": [p, i, i, p], "dsscf_batch_autotune": [p, p],
                            "dsscf_batch_autotune_schedule": [p, p, p, p, p], "dsscf_set_class_schedule": [p, p, p, p],
                            "dsscf_job_set_schedule": [p, p, p, p], "dsscf_job_set_guess": [p, p, ctypes.c_longlong],
-                           "dsscf_jk_uhf": [p, p, p, p, p, p, p]}.items():
+                           "dsscf_jk_uhf": [p, p, p, p, p, p, p],
+                           "dsscf_one_electron_gradient": [p, i, p, p, p, p, p, p]}.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = i
@@ -336,6 +337,18 @@ class Context:
         S, T, V = np.empty(npk), np.empty(npk), np.empty(npk)
         _check(lib().dsscf_one_electron(self._h, len(Z), _p(Z), _p(xyz), _p(S), _p(T), _p(V)))
         return S, T, V
+
+    def one_electron_gradient(self, Z, xyz_bohr, ao_atom, Pt_packed, W_packed):
+        """Kinetic, overlap and nuclear-attraction gradient terms on the device
+        (dsscf_one_electron_gradient): (natom, 3)."""
+        Z = np.ascontiguousarray(Z, np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, np.float64).reshape(-1)
+        at = np.ascontiguousarray(ao_atom, np.int32)
+        Pt = np.ascontiguousarray(Pt_packed, np.float64)
+        W = np.ascontiguousarray(W_packed, np.float64)
+        g = np.empty(3 * len(Z))
+        _check(lib().dsscf_one_electron_gradient(self._h, len(Z), _p(Z), _p(xyz), _p(at), _p(Pt), _p(W), _p(g)))
+        return g.reshape(-1, 3)
 
     def eri_gradient(self, Pa_packed, ao_atom, natom: int):
         """Two-electron part of the RHF gradient at alpha density Pa (packed-U):

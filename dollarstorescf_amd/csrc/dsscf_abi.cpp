@@ -1568,8 +1568,7 @@ int dsscf_eri_gradient(dsscf_ctx* c, const double* Pa, int natom, const int* ao_
     for (size_t k = 0; e == hipSuccess && k < segs.size(); ++k) {
         int l4[4];
         dk_class_l(cls_of[k], l4);
-        const bool sp = l4[0] <= 1 && l4[1] <= 1 && l4[2] <= 1 && l4[3] <= 1;
-        if (sp)
+        if (!dsscf_grad_is_wave(l4))
             e = dsscf_launch_grad(l4, c->opt.boys_mode, d_segs + k, d_blk0 + k, 1, segs[k].n, c->d_btab, c->d_xform,
                                   d_atom, d_grad, natom, kEriScale, c->stream);
         else
@@ -1742,6 +1741,61 @@ int dsscf_one_electron(dsscf_ctx* c, int n_atom, const int* Z, const double* xyz
     dev_free(d_at);
     dev_free(d_out);
     return rc;
+}
+
+int dsscf_one_electron_gradient(dsscf_ctx* c, int n_atom, const int* Z, const double* xyz, const int* ao_atom,
+                                const double* Pt, const double* W, double* grad)
+{
+    if (!c || n_atom <= 0 || !Z || !xyz || !ao_atom || !Pt || !W || !grad)
+        return fail(DSSCF_ERR_ARG, "dsscf_one_electron_gradient: bad argument");
+    if (!c->prepared) return fail(DSSCF_ERR_STATE, "dsscf_one_electron_gradient: call dsscf_prepare first");
+    if (c->opt.cartesian) return fail(DSSCF_ERR_UNSUPPORTED, "dsscf_one_electron_gradient: spherical AOs only");
+    const int N = c->n_ao;
+    for (int i = 0; i < N; ++i)
+        if (ao_atom[i] < 0 || ao_atom[i] >= n_atom) return fail(DSSCF_ERR_ARG, "dsscf_one_electron_gradient: AO atom index");
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t np = (size_t)N * (N + 1) / 2;
+    std::vector<double> at(4 * (size_t)n_atom);
+    for (int i = 0; i < n_atom; ++i) {
+        at[4 * i] = xyz[3 * i];
+        at[4 * i + 1] = xyz[3 * i + 1];
+        at[4 * i + 2] = xyz[3 * i + 2];
+        at[4 * i + 3] = (double)Z[i];
+    }
+    double *d_at = nullptr, *d_m = nullptr, *d_g = nullptr;
+    int* d_atom = nullptr;
+    auto cleanup = [&] {
+        dev_free(d_at);
+        dev_free(d_m);
+        dev_free(d_g);
+        dev_free(d_atom);
+    };
+    int rc = dev_alloc(&d_at, at.size());
+    if (!rc) rc = dev_alloc(&d_m, 2 * np);
+    if (!rc) rc = dev_alloc(&d_g, 3 * (size_t)n_atom);
+    if (!rc) rc = dev_alloc(&d_atom, (size_t)N);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_at, at.data(), sizeof(double) * at.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_m, Pt, sizeof(double) * np, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_m + np, W, sizeof(double) * np, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_atom, ao_atom, sizeof(int) * N, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = dsscf_launch_fill(d_g, 3 * (size_t)n_atom, 0.0, c->stream);
+    const int maxl = dk_max_l();
+    for (int l1 = 0; l1 <= maxl && e == hipSuccess; ++l1)
+        for (int l2 = 0; l2 <= l1 && e == hipSuccess; ++l2) {
+            const int pc = pair_class_of(l1, l2);
+            e = dsscf_launch_one_electron_grad(l1, l2, c->opt.boys_mode, c->d_plist + c->pc_start[pc], c->pc_count[pc],
+                                               c->d_pairs, c->d_pp, d_at, n_atom, c->d_btab, c->d_xform, d_atom, d_m,
+                                               d_m + np, d_g, c->stream);
+        }
+    if (e == hipSuccess) e = hipMemcpyAsync(grad, d_g, sizeof(double) * 3 * n_atom, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(DSSCF_ERR_HIP, std::string("dsscf_one_electron_gradient: ") + hipGetErrorString(e));
+    return DSSCF_OK;
 }
 
 long long dsscf_ctx_device_bytes(dsscf_ctx* c)

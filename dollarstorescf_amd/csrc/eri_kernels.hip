@@ -291,6 +291,253 @@ __global__ void __launch_bounds__(64) k_one_electron(const int* __restrict__ pli
         }
 }
 
+// ------------------------------------------ one-electron gradient terms ---
+// The one-electron part of the nuclear gradient (Gradient.cpp:61-116): per
+// unique shell pair, sum_mn Pt_mn dT_mn/dR - W_mn dS_mn/dR + Pt_mn dV_mn/dR,
+// with the reference's AO-pair weights (2 for m != n; a shell paired with
+// itself counts m >= n).  One thread per (shell pair, job): job 0 takes the
+// kinetic and overlap terms (they cancel within one atom; dB = -dA), job
+// 1 + C the attraction to nucleus C (skipped when the pair and C share one
+// atom; dC = -(dA + dB)).  Derivatives per primitive pair, d/dA_x <a|O|b> =
+// 2 alpha <a+1x|O|b> - a_x <a-1x|O|b> and likewise for B with 2 beta: S and
+// T from 1-D Obara-Saika overlaps, V from the vertical recursion on the
+// first centre up to la + lb + 1 accumulated unweighted, 2 alpha- and 2
+// beta-weighted, then the transfer to the second shell.  The pair densities
+// are taken to the Cartesian block once per thread; the contributions go to
+// grad with atomic adds.
+namespace one_e {
+__device__ __forceinline__ double binom3(int n, int k) { return (n == 3 && (k == 1 || k == 2)) ? 3.0 : binom(n, k); }
+}  // namespace one_e
+
+template <int la, int lb, int BM>
+__global__ void __launch_bounds__(64) k_one_electron_grad(const int* __restrict__ plist, int n,
+                                                          const ShellPair* __restrict__ pairs,
+                                                          const PrimPair* __restrict__ pp,
+                                                          const double* __restrict__ atoms, int natom,
+                                                          const double* __restrict__ btab,
+                                                          const double* __restrict__ xform,
+                                                          const int* __restrict__ ao_atom,
+                                                          const double* __restrict__ Pt,
+                                                          const double* __restrict__ W, double* __restrict__ grad)
+{
+    using namespace one_e;
+    constexpr double kPi = 3.14159265358979;
+    constexpr int NA = ncart(la), NB = ncart(lb), L1 = la + lb + 1;
+    constexpr int NFA = 2 * la + 1, NFB = 2 * lb + 1;
+    constexpr int E0 = la > 0 ? cart_off(la - 1) : 0, NE = cart_off(L1 + 1) - E0;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)n * (natom + 1)) return;
+    const int pi = (int)(t / (natom + 1)), c = (int)(t % (natom + 1)) - 1;
+    const ShellPair sp = pairs[plist[pi]];
+    const int atA = ao_atom[sp.ao_a], atB = ao_atom[sp.ao_b];
+    if (c < 0 ? atA == atB : (atA == atB && c == atA)) return;
+    const double A[3] = {sp.A[0], sp.A[1], sp.A[2]};
+    const double B[3] = {sp.A[0] - sp.AB[0], sp.A[1] - sp.AB[1], sp.A[2] - sp.AB[2]};
+    // AO pair densities -> Cartesian block
+    double Dt[NA * NB], Dw[NA * NB];
+    for (int i = 0; i < NA * NB; ++i) Dt[i] = Dw[i] = 0.0;
+    const double* xa = xform + xform_off(la, 0);
+    const double* xb = xform + xform_off(lb, 0);
+    const bool same = sp.ao_a == sp.ao_b;
+    for (int ma = 0; ma < NFA; ++ma)
+        for (int mb = 0; mb < NFB; ++mb) {
+            if (same && mb > ma) continue;
+            const int i = sp.ao_a + ma, j = sp.ao_b + mb;
+            const size_t o = i >= j ? (size_t)i * (i + 1) / 2 + j : (size_t)j * (j + 1) / 2 + i;
+            const double w = i == j ? 1.0 : 2.0;
+            const double pt = w * Pt[o], pw = c < 0 ? w * W[o] : 0.0;
+            for (int ia = 0; ia < NA; ++ia) {
+                const double fa = xa[ma * NA + ia];
+                if (fa == 0.0) continue;
+                for (int ib = 0; ib < NB; ++ib) {
+                    const double f = fa * xb[mb * NB + ib];
+                    Dt[ia * NB + ib] += f * pt;
+                    Dw[ia * NB + ib] += f * pw;
+                }
+            }
+        }
+    if (c < 0) {
+        // kinetic and overlap terms
+        double g[3] = {0.0, 0.0, 0.0};
+        for (int ip = 0; ip < sp.nprim; ++ip) {
+            const PrimPair P = pp[sp.prim_off + ip];
+            const double p = P.p, al = P.alpha, o2p = 0.5 * P.inv_p;
+            const double K = P.Kp * p;  // w_a w_b exp(-ab/p |AB|^2)
+            double S1[3][la + 4][lb + 1];
+            for (int x = 0; x < 3; ++x) {
+                const double PA = P.P[x] - A[x], PB = P.P[x] - B[x];
+                S1[x][0][0] = sqrt(kPi * P.inv_p);
+                for (int i = 1; i <= la + 3; ++i)
+                    S1[x][i][0] = PA * S1[x][i - 1][0] + (i > 1 ? (i - 1) * S1[x][i - 2][0] * o2p : 0.0);
+                for (int j = 1; j <= lb; ++j)
+                    for (int i = 0; i <= la + 3; ++i) {
+                        double w = PB * S1[x][i][j - 1];
+                        if (i > 0) w += i * S1[x][i - 1][j - 1] * o2p;
+                        if (j > 1) w += (j - 1) * S1[x][i][j - 2] * o2p;
+                        S1[x][i][j] = w;
+                    }
+            }
+            // S and T of the Cartesian pair (e | b): T applies -1/2 nabla^2 to the bra
+            auto st = [&](const int* e, const int* b, double& sv, double& tv) {
+                double sd[3], dd[3];
+                for (int x = 0; x < 3; ++x) {
+                    const int i = e[x], j = b[x];
+                    sd[x] = S1[x][i][j];
+                    dd[x] = 4.0 * al * al * S1[x][i + 2][j] - 2.0 * al * (2 * i + 1) * S1[x][i][j] +
+                            (i > 1 ? (double)(i * (i - 1)) * S1[x][i - 2][j] : 0.0);
+                }
+                sv = sd[0] * sd[1] * sd[2];
+                tv = -0.5 * (dd[0] * sd[1] * sd[2] + sd[0] * dd[1] * sd[2] + sd[0] * sd[1] * dd[2]);
+            };
+            for (int ia = 0; ia < NA; ++ia)
+                for (int ib = 0; ib < NB; ++ib) {
+                    const double dt = Dt[ia * NB + ib], dw = Dw[ia * NB + ib];
+                    if (dt == 0.0 && dw == 0.0) continue;
+                    int a[3], b[3];
+                    cart_xyz(la, ia, a[0], a[1], a[2]);
+                    cart_xyz(lb, ib, b[0], b[1], b[2]);
+                    for (int d = 0; d < 3; ++d) {
+                        int e[3] = {a[0], a[1], a[2]};
+                        e[d] += 1;
+                        double su, tu, sl = 0.0, tl = 0.0;
+                        st(e, b, su, tu);
+                        if (a[d] > 0) {
+                            e[d] -= 2;
+                            st(e, b, sl, tl);
+                        }
+                        const double dS = 2.0 * al * su - a[d] * sl, dT = 2.0 * al * tu - a[d] * tl;
+                        g[d] += K * (dt * dT - dw * dS);
+                    }
+                }
+        }
+        for (int d = 0; d < 3; ++d) {
+            unsafeAtomicAdd(&grad[3 * atA + d], g[d]);
+            unsafeAtomicAdd(&grad[3 * atB + d], -g[d]);
+        }
+        return;
+    }
+    // attraction to nucleus c
+    const double Cc[3] = {atoms[4 * c], atoms[4 * c + 1], atoms[4 * c + 2]};
+    const double Zc = atoms[4 * c + 3];
+    double U[NE], RA[NE], RB[NE];
+    for (int i = 0; i < NE; ++i) U[i] = RA[i] = RB[i] = 0.0;
+    double th[L1 + 1][cart_off(L1 + 1)];
+    for (int ip = 0; ip < sp.nprim; ++ip) {
+        const PrimPair P = pp[sp.prim_off + ip];
+        const double p = P.p, al = P.alpha, be = p - al, o2p = 0.5 * P.inv_p;
+        const double K = P.Kp * p;
+        const double PA[3] = {P.P[0] - A[0], P.P[1] - A[1], P.P[2] - A[2]};
+        const double PC[3] = {P.P[0] - Cc[0], P.P[1] - Cc[1], P.P[2] - Cc[2]};
+        const double R2 = PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2];
+        double F[L1 + 1];
+        boys<BM, L1>(p * R2, btab, F);
+        for (int m = 0; m <= L1; ++m) th[m][0] = F[m];
+        for (int l = 1; l <= L1; ++l)
+            for (int kk = 0; kk < ncart(l); ++kk) {
+                int e[3];
+                cart_xyz(l, kk, e[0], e[1], e[2]);
+                const int d = e[0] > 0 ? 0 : (e[1] > 0 ? 1 : 2);
+                int e1[3] = {e[0], e[1], e[2]};
+                e1[d] -= 1;
+                const int i1 = cart_off(l - 1) + cart_pos(l - 1, e1[0], e1[1]);
+                int i2 = -1;
+                if (e1[d] > 0) {
+                    int e2[3] = {e1[0], e1[1], e1[2]};
+                    e2[d] -= 1;
+                    i2 = cart_off(l - 2) + cart_pos(l - 2, e2[0], e2[1]);
+                }
+                const int io = cart_off(l) + kk;
+                for (int m = 0; m <= L1 - l; ++m) {
+                    double w = PA[d] * th[m][i1] - PC[d] * th[m + 1][i1];
+                    if (i2 >= 0) w += e1[d] * o2p * (th[m][i2] - th[m + 1][i2]);
+                    th[m][io] = w;
+                }
+            }
+        const double pref = -Zc * 2.0 * kPi * P.inv_p * K;
+        for (int i = 0; i < NE; ++i) {
+            const double v = pref * th[0][E0 + i];
+            U[i] += v;
+            RA[i] += 2.0 * al * v;
+            RB[i] += 2.0 * be * v;
+        }
+    }
+    // (e|f) from (e+k|0): sum_k C(f,k) AB^(f-k) X(e+k|0) per direction
+    const double AB[3] = {sp.AB[0], sp.AB[1], sp.AB[2]};
+    auto xfer = [&](const double* X, const int* e, const int* f) {
+        double w = 0.0;
+        for (int kx = 0; kx <= f[0]; ++kx)
+            for (int ky = 0; ky <= f[1]; ++ky)
+                for (int kz = 0; kz <= f[2]; ++kz) {
+                    const int ex = e[0] + kx, ey = e[1] + ky, ez = e[2] + kz, le = ex + ey + ez;
+                    double cf = binom3(f[0], kx) * binom3(f[1], ky) * binom3(f[2], kz);
+                    for (int q = 0; q < f[0] - kx; ++q) cf *= AB[0];
+                    for (int q = 0; q < f[1] - ky; ++q) cf *= AB[1];
+                    for (int q = 0; q < f[2] - kz; ++q) cf *= AB[2];
+                    w += cf * X[cart_off(le) - E0 + cart_pos(le, ex, ey)];
+                }
+        return w;
+    };
+    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+    for (int ia = 0; ia < NA; ++ia)
+        for (int ib = 0; ib < NB; ++ib) {
+            const double D = Dt[ia * NB + ib];
+            if (D == 0.0) continue;
+            int a[3], b[3];
+            cart_xyz(la, ia, a[0], a[1], a[2]);
+            cart_xyz(lb, ib, b[0], b[1], b[2]);
+            for (int d = 0; d < 3; ++d) {
+                int e[3] = {a[0], a[1], a[2]}, f[3] = {b[0], b[1], b[2]};
+                e[d] += 1;
+                double vA = xfer(RA, e, b);
+                if (a[d] > 0) {
+                    e[d] -= 2;
+                    vA -= a[d] * xfer(U, e, b);
+                }
+                f[d] += 1;
+                double vB = xfer(RB, a, f);
+                if (b[d] > 0) {
+                    f[d] -= 2;
+                    vB -= b[d] * xfer(U, a, f);
+                }
+                gA[d] += D * vA;
+                gB[d] += D * vB;
+            }
+        }
+    for (int d = 0; d < 3; ++d) {
+        unsafeAtomicAdd(&grad[3 * atA + d], gA[d]);
+        unsafeAtomicAdd(&grad[3 * atB + d], gB[d]);
+        unsafeAtomicAdd(&grad[3 * c + d], -(gA[d] + gB[d]));
+    }
+}
+
+hipError_t dsscf_launch_one_electron_grad(int la, int lb, int boys_mode, const int* plist, int n,
+                                          const ShellPair* pairs, const PrimPair* pp, const double* atoms, int natom,
+                                          const double* btab, const double* xform, const int* ao_atom,
+                                          const double* Pt, const double* W, double* grad, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const long long nt = (long long)n * (natom + 1);
+    const dim3 blk(64), grd((unsigned)((nt + 63) / 64));
+#define DSSCF_ONE_EG(A_, B_)                                                                                       \
+    if (la == A_ && lb == B_) {                                                                                  \
+        if (boys_mode == BOYS_ACCURATE)                                                                          \
+            hipLaunchKernelGGL((k_one_electron_grad<A_, B_, BOYS_ACCURATE>), grd, blk, 0, s, plist, n, pairs, pp,  \
+                               atoms, natom, btab, xform, ao_atom, Pt, W, grad);                                 \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_one_electron_grad<A_, B_, BOYS_REFERENCE>), grd, blk, 0, s, plist, n, pairs, pp, \
+                               atoms, natom, btab, xform, ao_atom, Pt, W, grad);                                 \
+        return hipGetLastError();                                                                                \
+    }
+    DSSCF_ONE_EG(0, 0)
+    DSSCF_ONE_EG(1, 0)
+    DSSCF_ONE_EG(1, 1)
+    DSSCF_ONE_EG(2, 0)
+    DSSCF_ONE_EG(2, 1)
+    DSSCF_ONE_EG(2, 2)
+#undef DSSCF_ONE_EG
+    return hipErrorInvalidValue;
+}
+
 hipError_t dsscf_launch_one_electron(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
                                      const PrimPair* pp, const double* atoms, int natom, const double* btab,
                                      const double* xform, int cart, double* S, double* T, double* V, hipStream_t s)

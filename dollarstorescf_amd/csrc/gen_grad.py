@@ -458,10 +458,17 @@ def grad_wave_class(c):
     return "\n".join(o)
 
 
+# s/p classes that would also run wave-cooperative: none.  Measured on the
+# 8-water/6-31G gradient, the wave form was slower for every s/p class it was
+# tried on, (pp|pp) included although its register form spills: (ps|ps) 350
+# -> 1640 us, (pp|ss) 202 -> 772, (pp|ps) 462 -> 1035, (pp|pp) 1709 -> 2423.
+GRAD_WAVE_SP = []
+
+
 def grad_wave_classes(maxl=2):
     import gen_eri
     _, cls_list = gen_eri.classes(maxl)
-    return [c for c in cls_list if max(c) >= 2]
+    return [c for c in cls_list if max(c) >= 2 or c in GRAD_WAVE_SP]
 
 
 def main():

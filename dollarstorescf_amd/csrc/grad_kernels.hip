@@ -393,6 +393,15 @@ static hipError_t launch_gradw(int flat, const JKSeg* segs, const int* blk0, int
 
 int dsscf_grad_flat_chunk() { return kGradFlatChunk; }
 
+int dsscf_grad_is_wave(const int* l4)
+{
+#define DSSCF_GRADW_IS(a, b, c, d, name) \
+    if (l4[0] == a && l4[1] == b && l4[2] == c && l4[3] == d) return 1;
+    DSSCF_FOR_EACH_GRAD_WAVE_CLASS(DSSCF_GRADW_IS)
+#undef DSSCF_GRADW_IS
+    return 0;
+}
+
 hipError_t dsscf_launch_gradw(const int* l4, int boys_mode, int cart, int flat, const JKSeg* segs, const int* blk0,
                               int nseg, int nblocks, const double* btab, const double* xform, const int* ao_atom,
                               double* grad, int natom, double scale, hipStream_t s)

@@ -1,7 +1,8 @@
 // gradient.cpp -- RHF nuclear gradient of a solved job: scf::getEnergyGradient
-// (src/chem/scf/Gradient.cpp:14-133) restricted case.  The two-electron part
-// runs on the device (dsscf_eri_gradient); the one-electron and
-// nuclear-repulsion parts, O(N^2 natom), run here.
+// (src/chem/scf/Gradient.cpp:14-133) restricted case.  A job's gradient runs
+// its one- and two-electron parts on the device (dsscf_one_electron_gradient,
+// dsscf_eri_gradient) and the nuclear repulsion here; the host restatement
+// of the one-electron terms below serves dsscf_job_gradient_terms (no GPU).
 //
 //   dE/dR = sum_mn Pt_mn dT_mn/dR - sum_mn W_mn dS_mn/dR + sum_mn Pt_mn dV_mn/dR
 //           + two-electron part + dV_nn/dR
@@ -342,9 +343,23 @@ int gradient(Job& job, std::vector<double>& grad, double* parts)
             for (unsigned i = 0; i < nocc; ++i) v += job.C[(size_t)x * n + i] * job.C[(size_t)y * n + i] * job.eps[i];
             W[(size_t)x * (x + 1) / 2 + y] = 2.0 * v;
         }
-    std::vector<double> g1, g2((size_t)3 * natom), gn;
-    flo::oneElectronGradient(job.basis, job.basis_atoms, job.molecule, job.opt.boys_mode, job.Ptot, W, g1);
+    // one- and two-electron parts on the device
+    std::vector<double> g1((size_t)3 * natom), g2((size_t)3 * natom), gn;
     std::vector<int> ao_atom(job.basis_atoms.begin(), job.basis_atoms.end());
+    std::vector<int> Z(natom);
+    std::vector<double> xyz((size_t)3 * natom);
+    for (int i = 0; i < natom; ++i) {
+        const flo::Atom& a = job.molecule.atoms[i];
+        Z[i] = (int)a.charge;
+        xyz[3 * i] = a.position.x;
+        xyz[3 * i + 1] = a.position.y;
+        xyz[3 * i + 2] = a.position.z;
+    }
+    if (int rc = dsscf_one_electron_gradient(job.ctx, natom, Z.data(), xyz.data(), ao_atom.data(), job.Ptot.data(),
+                                             W.data(), g1.data())) {
+        job.error = dsscf_last_error();
+        return rc;
+    }
     if (int rc = dsscf_eri_gradient(job.ctx, job.Pa.data(), natom, ao_atom.data(), g2.data())) {
         job.error = dsscf_last_error();
         return rc;

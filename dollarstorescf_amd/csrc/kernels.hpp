@@ -39,6 +39,14 @@ hipError_t dsscf_launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t dsscf_launch_one_electron(int la, int lb, int boys_mode, const int* plist, int n, const ShellPair* pairs,
                                      const PrimPair* pp, const double* atoms, int natom, const double* btab,
                                      const double* xform, int cart, double* S, double* T, double* V, hipStream_t s);
+// One-electron part of the nuclear gradient over the pairs plist[0..n) of
+// pair class (la, lb) (spherical AOs): grad[3*natom] += the kinetic, overlap
+// and nuclear-attraction terms for the packed-U total density Pt and
+// energy-weighted density W; ao_atom[N] the atom of each AO.
+hipError_t dsscf_launch_one_electron_grad(int la, int lb, int boys_mode, const int* plist, int n,
+                                          const ShellPair* pairs, const PrimPair* pp, const double* atoms, int natom,
+                                          const double* btab, const double* xform, const int* ao_atom,
+                                          const double* Pt, const double* W, double* grad, hipStream_t s);
 hipError_t dsscf_launch_prep(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);
 hipError_t dsscf_launch_fold(const MatSeg* segs, int nseg, int total_rows, int maxN, hipStream_t s);
 // Sum the J_cd slot records into copy 0 of Jh: nblocks = rows over all segments.
@@ -59,6 +67,9 @@ hipError_t dsscf_launch_gradw(const int* l4, int boys_mode, int cart, int flat, 
                               int nseg, int nblocks, const double* btab, const double* xform, const int* ao_atom,
                               double* grad, int natom, double scale, hipStream_t s);
 int dsscf_grad_flat_chunk();
+// 1 if class l4's gradient runs in the wave-cooperative kernel (every class
+// with a d shell and the s/p classes with many accumulators), else 0.
+int dsscf_grad_is_wave(const int* l4);
 hipError_t dsscf_launch_grad(const int* l4, int boys_mode, const JKSeg* segs, const int* blk0, int nseg, int nblocks,
                              const double* btab, const double* xform, const int* ao_atom, double* grad, int natom,
                              double scale, hipStream_t s);

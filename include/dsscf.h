@@ -229,6 +229,17 @@ int dsscf_boys_eval(dsscf_ctx* ctx, int mode, int L, int n, const double* T, dou
  * Synchronous. */
 int dsscf_eri_gradient(dsscf_ctx* ctx, const double* Pa, int natom, const int* ao_atom, double* grad);
 
+/* One-electron part of the nuclear gradient on the device: grad[3*natom]
+ * receives sum_mn Pt_mn dT_mn/dR - W_mn dS_mn/dR + Pt_mn dV_mn/dR for the
+ * packed-U total density Pt and energy-weighted density W, the kinetic,
+ * overlap and nuclear-attraction terms of scf::getEnergyGradient
+ * (Gradient.cpp:61-116, over overlapGradient / kineticEnergyGradient /
+ * nuclearPotentialGradient, GaussianIntegral.cpp:707-1050); nuclei Z[natom]
+ * at xyz[3*natom] (bohr), ao_atom[N] each AO's atom.  Spherical AOs
+ * (DSSCF_ERR_UNSUPPORTED for a Cartesian-d context).  Synchronous. */
+int dsscf_one_electron_gradient(dsscf_ctx* ctx, int natom, const int* Z, const double* xyz, const int* ao_atom,
+                                const double* Pt, const double* W, double* grad);
+
 /* Schwarz bounds Q_ab of all unique shell pairs (a >= b, index a(a+1)/2+b). */
 int dsscf_schwarz(dsscf_ctx* ctx, double* Q);
 

@@ -111,9 +111,9 @@ int dsscf_job_fock_build(dsscf_job* job);
  * total energy (getTotalEnergy, Energy.cpp:76-78) and iteration count. */
 int dsscf_job_solve(dsscf_job* job, double* energy, int* iterations);
 /* scf::getEnergyGradient (Gradient.cpp:14-133), restricted, of a solved job:
- * grad[3*natom] = dE/dR (Eh/bohr, x y z per atom).  The two-electron part
- * runs on the device (dsscf_eri_gradient), the one-electron and
- * nuclear-repulsion parts on the host.  parts (may be null) receives the
+ * grad[3*natom] = dE/dR (Eh/bohr, x y z per atom).  The one- and
+ * two-electron parts run on the device (dsscf_one_electron_gradient,
+ * dsscf_eri_gradient), the nuclear repulsion on the host.  parts (may be null) receives the
  * one-electron, two-electron and nuclear-repulsion parts, 3*natom each.
  * s, p and d shells of the reference's spherical AOs (a Cartesian-d job:
  * DSSCF_ERR_UNSUPPORTED). */

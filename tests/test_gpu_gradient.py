@@ -45,6 +45,31 @@ def test_eri_gradient_matches_reference(case, boys):
     assert np.max(np.abs(got - want)) < tol, np.max(np.abs(got - want))
 
 
+@pytest.mark.parametrize("case", ["water_sto3g", "water2_631g", "water_def2svp", "water_631gs", "water2_def2svp"])
+@pytest.mark.parametrize("boys", [ds.BOYS_REFERENCE, ds.BOYS_ACCURATE])
+def test_one_electron_gradient_device_matches_reference(case, boys):
+    """Kinetic, overlap and nuclear-attraction gradient terms on the device
+    (dsscf_one_electron_gradient, Gradient.cpp:61-116) for the fixture's
+    synthetic densities: 1e-11 relative to the reference build's terms in the
+    reference Boys mode (the accurate mode moves the attraction terms by the
+    Boys difference, below 1e-9 relative), and equal to the host
+    restatement (dsscf_job_gradient_terms) in the same mode."""
+    g = _fixture(case)
+    Z = np.array(g["Z"], np.int32)
+    xyz = np.array(g["xyz_bohr"]).reshape(-1, 3)
+    job = ds.Job(0).set_options(boys_mode=boys, schwarz_threshold=0.0).set_basis(g["basis"]).set_molecule(Z, xyz)
+    job.construct()
+    Pa, W = np.array(g["Pa"]), np.array(g["W"])
+    got = job.context.one_electron_gradient(Z, xyz, _ao_atom(job, xyz), 2 * Pa, W)
+    want = np.array(g["one_electron"])
+    host, _ = job.gradient_terms(2 * Pa, W)
+    scale = np.max(np.abs(want))
+    print(f"{case} boys={boys}: device vs reference {np.max(np.abs(got - want)):.2e}, vs host "
+          f"{np.max(np.abs(got - host)):.2e} of max {scale:.2e}", flush=True)
+    assert np.max(np.abs(got - host)) < 1e-11 * scale
+    assert np.max(np.abs(got - want)) < (1e-11 if boys == ds.BOYS_REFERENCE else 1e-9) * scale
+
+
 def test_eri_gradient_d_classes_with_sub_pairs_and_lane_groups():
     """The same d-shell fixture with every shell pair split into sub-pairs of
     at most 5 primitive pairs and lane groups forced on (the work units the

@@ -543,7 +543,7 @@ class Job:
 
     def get(self, name: str) -> np.ndarray:
         n = self.n_ao
-        size = {"eps": n, "epsb": n, "C": n * n, "Cb": n * n}.get(name, n * (n + 1) // 2)
+        size = {"eps": n, "epsb": n, "C": n * n, "Cb": n * n, "device_steps": 3}.get(name, n * (n + 1) // 2)
         out = np.empty(size)
         _check(lib().dsscf_job_get(self._h, name.encode(), _p(out)), self._err)
         return out.reshape(n, n) if name in ("C", "Cb") else out

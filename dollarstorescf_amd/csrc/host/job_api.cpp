@@ -223,6 +223,10 @@ int dsscf_job_get(dsscf_job* j, const char* name, double* out)
     else if (n == "Gb") v = &b.Gb;
     else if (n == "epsb") v = &b.epsb;
     else if (n == "Cb") v = &b.Cb;
+    else if (n == "device_steps") {
+        std::memcpy(out, b.device_steps, sizeof(b.device_steps));
+        return DSSCF_OK;
+    }
     if (!v) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_get: unknown matrix '" + n + "'");
     if (v->empty()) return set_err(j, DSSCF_ERR_STATE, "dsscf_job_get: '" + n + "' not computed yet");
     std::memcpy(out, v->data(), sizeof(double) * v->size());

@@ -34,6 +34,7 @@ void setCurrent(Job* job) { g_current = job; }
 Job::~Job()
 {
     scfdev::destroy(dstep);
+    scfdev::group_destroy(dgroup);
     if (ctx) dsscf_ctx_destroy(ctx);
 }
 
@@ -539,38 +540,82 @@ int beginDeviceSteps(Job& job)
     if (int rc = scfdev::create(&job.dstep, job.opt.device, job.n, (int)job.electron_count[0], job.opt.diis_size,
                                 job.H.data(), job.Sd.data(), job.X.data(), &job.error))
         return rc;
+    for (double& c : job.device_steps) c = 0.0;
     return scfdev::set_density(job.dstep, job.Pa.data(), &job.error);
+}
+
+int deviceIterations(scfdev::Group* g, Job* const* jobs, int m, int* rcs)
+{
+    constexpr int W = scfdev::kMaxHistory;
+    std::vector<scfdev::Step*> steps(m);
+    std::vector<int> slot(m), count(m);
+    std::vector<double> dots((size_t)W * m, 0.0), coef((size_t)W * m, 0.0), se(m), tr(m);
+    std::vector<int> psteps(m, 0);
+    for (int i = 0; i < m; ++i) {
+        Job& job = *jobs[i];
+        steps[i] = job.dstep;
+        rcs[i] = DSSCF_OK;
+        if (job.iteration == 1) {  // F = F0: no DIIS yet
+            slot[i] = -1;
+            count[i] = 0;
+        } else {
+            slot[i] = job.diis.reserveSlot();
+            count[i] = job.diis.count();
+        }
+    }
+    std::string err;
+    auto fail_all = [&](int rc) {
+        for (int i = 0; i < m; ++i) {
+            jobs[i]->error = err;
+            rcs[i] = rc;
+        }
+        return rc;
+    };
+    if (int rc = scfdev::group_pre(g, steps.data(), m, slot.data(), count.data(), dots.data(), &err))
+        return fail_all(rc);
+    for (int i = 0; i < m; ++i) {
+        if (slot[i] < 0) continue;
+        DIIS& d = jobs[i]->diis;
+        d.setRow(slot[i], dots.data() + (size_t)W * i);
+        d.solve();
+        for (int c = 0; c < count[i]; ++c) coef[(size_t)W * i + c] = d.coef[c];
+    }
+    if (int rc = scfdev::group_post(g, steps.data(), m, coef.data(), se.data(), tr.data(), rcs, psteps.data(), &err))
+        return fail_all(rc);
+    for (int i = 0; i < m; ++i) {
+        Job& job = *jobs[i];
+        if (rcs[i]) {
+            job.error = err;
+            continue;
+        }
+        job.device_steps[0] += 1.0;
+        if (psteps[i] >= 0)
+            job.device_steps[1] += psteps[i];
+        else
+            job.device_steps[2] += 1.0;
+        // E_elec = 2 sum eps_occ - tr(Pa G) (SCFSolver.cpp:118-143), sum eps_occ = tr(P' F')
+        const double E = 2.0 * se[i] - tr[i];
+        job.previous_electronic_energy = job.total_electronic_energy;
+        job.total_electronic_energy = E;
+        job.history.push_back(E);
+    }
+    return DSSCF_OK;
 }
 
 int deviceIteration(Job& job)
 {
-    double trPG = 0.0;
-    if (job.iteration == 1) {
-        if (int rc = scfdev::assemble(job.dstep, -1, 0, nullptr, nullptr, &job.error)) return rc;
-        if (int rc = scfdev::diagonalize_direct(job.dstep, job.eps, &trPG, &job.error)) return rc;
-    } else {
-        std::vector<double> dots(job.opt.diis_size);
-        const int slot = job.diis.reserveSlot();
-        const int count = job.diis.count();
-        double rms = 0.0;
-        if (int rc = scfdev::assemble(job.dstep, slot, count, dots.data(), &rms, &job.error)) return rc;
-        job.diis.setRow(slot, dots.data());
-        job.diis.solve();
-        if (int rc = scfdev::extrapolate(job.dstep, job.diis.coef.data(), count, job.eps, &trPG, &job.error))
-            return rc;
-    }
-    double E = 0.0;
-    for (unsigned i = 0; i < job.electron_count[0]; ++i) E += 2.0 * job.eps[i];
-    E -= trPG;
-    job.previous_electronic_energy = job.total_electronic_energy;
-    job.total_electronic_energy = E;
-    job.history.push_back(E);
-    return DSSCF_OK;
+    if (!job.dgroup)
+        if (int rc = scfdev::group_create(&job.dgroup, job.opt.device, &job.error)) return rc;
+    Job* jp = &job;
+    int rcj = DSSCF_OK;
+    if (int rc = deviceIterations(job.dgroup, &jp, 1, &rcj)) return rc;
+    return rcj;
 }
 
 int finishDeviceSteps(Job& job)
 {
     const int n = job.n;
+    if (int rc = scfdev::orbitals(job.dstep, job.eps, &job.error)) return rc;
     job.C.resize((size_t)n * n);
     if (int rc = scfdev::download(job.dstep, job.Pa.data(), job.Ptot.data(), job.J.data(), job.K.data(), nullptr,
                                   job.C.data(), &job.error))

@@ -98,7 +98,10 @@ struct Job {
     bool jk_valid = false;
 
     dsscf_ctx* ctx = nullptr;
-    scfdev::Step* dstep = nullptr;  // device SCF step state (opt.device_step)
+    scfdev::Step* dstep = nullptr;    // device SCF step state (opt.device_step)
+    scfdev::Group* dgroup = nullptr;  // its grouped-step stream when solved alone
+    // device iterations, purification steps summed over them, eigensolver fallbacks
+    double device_steps[3] = {0.0, 0.0, 0.0};
     std::string error;
     // set while a Server holds the job (from submit until it is retired):
     // a second submit and a destroy are refused meanwhile
@@ -143,10 +146,14 @@ int scfStep(Job& job);          // G, DIIS, F' = X F X, diagonalization, density
 bool converged(const Job& job);  // the stopping test of solveMOs (SCFSolver.cpp:174-177)
 // The same iteration with its host part on the device (job.dstep):
 // beginDeviceSteps creates the step state at the job's density;
-// deviceIteration is scfStep with J/K already in the step's buffers;
-// finishDeviceSteps, with J/K at the final density there, downloads the
+// deviceIterations is scfStep for m jobs at once with J/K already in their
+// step buffers (one grouped launch sequence on g's stream; rcs[i] is job i's
+// status, the return value the group's), deviceIteration the same for one
+// job on its own group; finishDeviceSteps, with J/K at the final density
+// there, computes the orbitals of the last Fock matrix, downloads the
 // densities, J, K and C and computes the energies.
 int beginDeviceSteps(Job& job);
+int deviceIterations(scfdev::Group* g, Job* const* jobs, int m, int* rcs);
 int deviceIteration(Job& job);
 int finishDeviceSteps(Job& job);
 int solveMOs(Job& job);

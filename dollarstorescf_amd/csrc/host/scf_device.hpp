@@ -1,11 +1,13 @@
 // scf_device.hpp -- the host part of an SCF iteration run on the device
 // (DIIS extrapolation, F' = X F X, diagonalization, density, energy terms):
 // the roles of SCFSolver.cpp:45-94 and 118-143, DIIS.ipp:18-100 and
-// Lapack.cpp:12-39 (dspev_) with the matrices resident in HBM.  Products go
-// through rocBLAS dgemm, the eigenproblem through rocSOLVER dsyevd, the
-// element-wise parts through small kernels; only the DIIS bookkeeping (which
-// slot to replace, the bordered B system of at most 17 x 17) and a few
-// scalars per iteration cross to the host.
+// Lapack.cpp:12-39 (dspev_) with the matrices resident in HBM.  Iterations
+// run grouped (group_pre / group_post: batched products and purification
+// for the density); the per-job functions (rocBLAS dgemm, rocSOLVER dsyevd)
+// give the final orbitals and cover an iteration whose purification does
+// not finish.  Only the DIIS bookkeeping (which slot to replace, the
+// bordered B system of at most 17 x 17) and a few scalars per iteration cross
+// to the host.
 //
 // No HIP types here: scf.cpp (host compiler) includes it.
 #pragma once
@@ -16,7 +18,10 @@
 
 namespace scfdev {
 
-struct Step;  // device buffers, rocBLAS handle and stream of one job
+constexpr int kMaxHistory = 16;  // DIIS vectors a device step can hold
+
+struct Step;   // device buffers, rocBLAS handle and stream of one job
+struct Group;  // a stream and job tables for grouped steps
 
 // n AOs, nocc doubly occupied orbitals, DIIS history of kmax vectors; Hp
 // packed-U core Hamiltonian, S and X full n x n symmetric (row- or
@@ -53,5 +58,33 @@ long long device_bytes_for(int n, int kmax);
 // The packed-U Fock-build buffers of the step (densities in, J and K out),
 // for fused builds that read and write them in place.
 void buffers(Step* s, const double** Pt, const double** Pa, double** J, double** K);
+
+// Orbitals of the step's last Fock matrix F (the DIIS extrapolation of its
+// last iteration): eps (n values, ascending) to the host and C = X C' into
+// the step's C for download; the densities are left alone.
+int orbitals(Step* s, std::vector<double>& eps, std::string* err);
+
+// Grouped iterations: the iteration of every job in steps[0..m) after its
+// Fock build, one launch per stage for all of them on the group's stream.
+// The density comes from trace-correcting purification of F' = X F X (the
+// projector onto the nocc lowest orbitals: the density of the
+// diagonalization route, without the eigenvectors), the sum of occupied
+// orbital energies from tr(P' F').  A job whose purification does not reach
+// idempotency within its launch budget (orbitals nocc and nocc+1 too close)
+// takes the rocSOLVER route instead.
+//
+// group_pre: F0 = H + G and, for jobs with slot[i] >= 0, the Pulay error into
+// history slot slot[i] and <e_new, e_c> for c < count[i] into
+// dots[kMaxHistory i + c]; slot[i] < 0 marks a first iteration (F = F0).
+// group_post, with the same steps in the same order: F = sum_c
+// coef[kMaxHistory i + c] F_c, the new densities, sum_eps[i] (occupied
+// orbital energies) and tr_PG[i] = tr(Pa_new G); rcs[i] is the job's own
+// status (the fallback eigensolver's), the return value the group's;
+// psteps[i] (may be null) the purification steps taken, -1 for a fallback.
+int group_create(Group** g, int device, std::string* err);
+void group_destroy(Group* g);
+int group_pre(Group* g, Step* const* steps, int m, const int* slot, const int* count, double* dots, std::string* err);
+int group_post(Group* g, Step* const* steps, int m, const double* coef, double* sum_eps, double* tr_PG, int* rcs,
+               int* psteps, std::string* err);
 
 }  // namespace scfdev

@@ -12,10 +12,12 @@
 //     diagonalization, density; or, after convergence, the energies of
 //     Energy.cpp:21-50), and returns: the next round fuses whichever jobs are
 //     ready then, so a large job's host step holds back no one else.
-// With device steps (set_device_steps) a job's densities, J/K, DIIS history
-// and eigenproblem stay in HBM: the fused build reads and writes the job's
-// step buffers in place, and the worker runs the step on the job's own
-// stream (rocBLAS, rocSOLVER), with only DIIS bookkeeping on the host.
+// With device steps (set_device_steps) a job's densities, J/K, Fock
+// matrices and DIIS history stay in HBM: the fused build reads and writes the
+// job's step buffers in place, and the server thread then runs the
+// iterations of every job of the round as one grouped launch sequence
+// (batched products, density by purification), with only the DIIS
+// bookkeeping on the host; a finished job's worker computes its orbitals.
 // A finished job's worker frees its device memory and queues it for poll().
 // The fused set changes from round to round, so launches are made directly
 // (no graph); the batch keeps its buffers and tables between rounds.
@@ -42,6 +44,7 @@ Server::~Server()
     cv_.notify_all();
     for (auto& t : workers_) t.join();
     if (batch_) dsscf_batch_destroy(batch_);
+    scfdev::group_destroy(group_);
     for (Req* r : active_) {
         free_device(*r->job);
         r->job->in_server.store(false);
@@ -59,6 +62,8 @@ void Server::free_device(Job& job)
         scfdev::destroy(job.dstep);
         job.dstep = nullptr;
     }
+    scfdev::group_destroy(job.dgroup);
+    job.dgroup = nullptr;
     if (job.ctx) {
         dsscf_ctx_destroy(job.ctx);
         job.ctx = nullptr;
@@ -226,6 +231,55 @@ void Server::host_step(Req* r, bool final)
     ready_cv_.notify_all();
 }
 
+// The iterations of a round's iterating jobs with device steps, after their
+// fused Fock build: one grouped launch sequence (DIIS bookkeeping on the
+// host between its two halves), in the server thread, so every job is ready
+// for the next round when it returns.  Finished jobs go to the workers.
+int Server::device_round(const std::vector<Req*>& rs)
+{
+    const double t_start = clock();
+    if (!group_) {
+        std::string err;
+        if (int rc = scfdev::group_create(&group_, device_, &err)) {
+            for (Req* r : rs) {
+                r->job->error = err;
+                retire(r, rc);
+            }
+            return rc;
+        }
+    }
+    std::vector<Req*> go;
+    std::vector<Job*> jobs;
+    for (Req* r : rs) {
+        Job& job = *r->job;
+        job.jk_valid = true;
+        if ((int)job.iteration >= job.opt.max_iterations) {
+            job.error = "SCF did not converge within " + std::to_string(job.opt.max_iterations) + " iterations";
+            retire(r, DSSCF_ERR_NOCONV);
+            continue;
+        }
+        ++job.iteration;
+        go.push_back(r);
+        jobs.push_back(&job);
+    }
+    std::vector<int> rcs(jobs.size(), DSSCF_OK);
+    const int rc = jobs.empty() ? DSSCF_OK : deviceIterations(group_, jobs.data(), (int)jobs.size(), rcs.data());
+    for (size_t i = 0; i < go.size(); ++i) {
+        if (rcs[i]) {
+            retire(go[i], rcs[i]);
+            continue;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        go[i]->st = converged(*jobs[i]) ? FINAL : ITERATING;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        st_host_ += clock() - t_start;
+    }
+    ready_cv_.notify_all();
+    return rc;
+}
+
 bool Server::idle()
 {
     std::lock_guard<std::mutex> lk(mu_);
@@ -341,12 +395,16 @@ int Server::step(int* n_active, int* n_waiting)
             if ((rc = dsscf_batch_jk(batch_, pt.data(), pa.data(), jj.data(), kk.data()))) return rc;
         }
         const double tc = clock();
+        std::vector<Req*> iterating;  // device steps: this round's grouped iterations
         {
             std::lock_guard<std::mutex> lk(mu_);
             for (Req* r : members) {
                 const bool final = r->st == FINAL;
                 r->st = STEPPING;
-                steps_.push_back([this, r, final] { host_step(r, final); });
+                if (dev && !final)
+                    iterating.push_back(r);
+                else
+                    steps_.push_back([this, r, final] { host_step(r, final); });
             }
             st_rounds_ += 1.0;
             st_set_ += tb - ta;
@@ -354,6 +412,7 @@ int Server::step(int* n_active, int* n_waiting)
             st_members_ += (double)members.size();
         }
         cv_.notify_all();
+        if (!iterating.empty()) rc = device_round(iterating);
     }
     std::unique_lock<std::mutex> lk(mu_);
     if (members.empty() && !active_.empty())  // nothing ready: wait briefly for a worker to finish something

@@ -40,13 +40,16 @@ public:
     // footprint fits; handle and tag come back from poll().
     int submit(void* handle, Job* job, long long tag);
     // Device-resident iterations: every job submitted afterwards keeps its
-    // DIIS history, Fock matrix and eigenproblem in HBM as well (the device
-    // SCF step), and the fused builds read and write its device buffers in
-    // place.  Only before the first submit.
+    // DIIS history and Fock matrices in HBM as well (the device SCF step),
+    // the fused builds read and write its device buffers in place, and each
+    // round runs the iterations of all its jobs as one grouped launch
+    // sequence.  Only before the first submit.
     int set_device_steps(bool on);
     // One round: admit ready jobs, one fused Fock build over the jobs whose
     // next density is ready, then hand each of them to a worker for its host
-    // step (the next round does not wait for those).
+    // step (the next round does not wait for those); with device steps the
+    // iterating jobs' steps run here, grouped, and only finished jobs go to
+    // the workers.
     int step(int* n_active, int* n_waiting);
     // Finished jobs, oldest first, at most cap.
     void poll(std::vector<Done>& out, int cap);
@@ -75,6 +78,7 @@ private:
     void retire_locked(Req* r, int status);
     void retire(Req* r, int status);  // frees the context outside the lock
     void host_step(Req* r, bool final);
+    int device_round(const std::vector<Req*>& rs);  // grouped device steps of a round's iterating jobs
     double estimate_bytes(const Job& job) const;
 
     void free_device(Job& job);  // context and device step state
@@ -96,6 +100,7 @@ private:
     std::vector<Req*> active_;   // PREPARING, ITERATING or FINAL
     std::deque<Done> done_;
     dsscf_batch* batch_ = nullptr;
+    scfdev::Group* group_ = nullptr;  // grouped device steps (server thread only)
     std::vector<Req*> members_;  // the fused set of the last round
     bool batch_dirty_ = true;    // a job retired since the fused set was built
     bool tuned_ = false;         // block sizes chosen by the autotuner (once)

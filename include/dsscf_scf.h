@@ -90,7 +90,10 @@ int dsscf_job_ao(dsscf_job* job, int i, int* l, int* m, double* center, int* npr
  * SCFCommon.hpp) once the job is solved.  An unrestricted job (multiplicity
  * > 1) also has the beta spin's "Pb","Kb","Gb","epsb","Cb" (the [1] entries
  * of SCFCommon.hpp's per-spin arrays); its "Pa","K","G","eps","C" are the
- * alpha spin's. */
+ * alpha spin's.  "device_steps" gives three counts of a job iterated on the
+ * device (dsscf_job_set_step, device-step serving): its device iterations,
+ * the purification steps summed over them, and the iterations whose density
+ * fell back to the eigensolver. */
 int dsscf_job_get(dsscf_job* job, const char* name, double* out);
 /* Replace the job's orbitals: C n x n row-major (column i = orbital i), eps
  * n orbital energies.  No reference counterpart (its orbitals come only from

@@ -1,6 +1,6 @@
-"""The SCF iteration's host part on the device (DIIS, X F X, rocSOLVER dsyevd,
-density, energy terms) against the host step and the oracle, through the C-ABI
-(dsscf_job_set_step)."""
+"""The SCF iteration's host part on the device (DIIS, X F X, the density by
+purification, energy terms; rocSOLVER dsyevd for the final orbitals) against
+the host step and the oracle, through the C-ABI (dsscf_job_set_step)."""
 import numpy as np
 import pytest
 
@@ -43,6 +43,10 @@ def test_device_step_matches_host_step(case):
     }[case]
     jh, eh, ith = _solve(basis, Z, xyz, False)
     jd, ed, itd = _solve(basis, Z, xyz, True)
+    steps = jd.get("device_steps")  # device iterations, purification steps, eigensolver fallbacks
+    print(f"{case}: {ith} host / {itd} device iterations, {steps[1] / steps[0]:.1f} purification steps per "
+          f"iteration, {steps[2]:.0f} fallbacks, dE {ed - eh:.2e}", flush=True)
+    assert steps[0] == itd and steps[2] == 0
     assert abs(ith - itd) <= 1, (ith, itd)
     assert abs(eh - ed) < 1e-9, (eh, ed)
     assert np.max(np.abs(jh.get("Pa") - jd.get("Pa"))) < 1e-5

@@ -487,16 +487,17 @@ def bench_serve(a):
     one fused Fock build per round over every job on the device).  The
     stream runs with FP32 below Schwarz 1e-5 (value) and again with every
     quartet in FP64; reported: p50/p99 latency, solves/s, |dE| per request.
-    --device-steps: every job's densities, J/K, Fock matrix, DIIS history and
-    eigenproblem stay in HBM (device SCF step); otherwise the iterations'
-    linear algebra runs on host worker threads."""
+    Every job's densities, J/K, Fock matrices and DIIS history stay in HBM and
+    each round runs the iterations of all its jobs as one grouped launch
+    sequence (density by purification); --host-steps runs the iterations'
+    linear algebra on host worker threads instead."""
     import dollarstorescf_amd as ds
 
     arrivals, reqs = serve_requests(a.requests, a.rate)
     runs = {}
     for mode, thr in (("mixed", 1e-5), ("fp64", 0.0)):
         lat, e, st, wall, stats = serve_stream(ds, arrivals, reqs, thr, max_active=64, budget=32e9,
-                                               device_steps=a.device_steps)
+                                               device_steps=not a.host_steps)
         runs[mode] = {"p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                       "mean_latency_s": float(lat.mean()), "solves_per_s": len(reqs) / wall, "wall_s": wall,
                       "failed": int(np.count_nonzero(st)), "server": stats, "_e": e}
@@ -515,7 +516,8 @@ def bench_serve(a):
         "config": {"workload": f"{len(reqs)} RHF requests, Poisson arrivals at {a.rate:g}/s (seed 7), STO-3G / "
                                "6-31G / def2-SVP, 1-12 waters or C1-C12 alkanes; server: 64 slots, 32 GB budget, "
                                "12 host workers",
-                   "iterations": "device-resident (DIIS, diagonalization on the device)" if a.device_steps
+                   "iterations": "device-resident (grouped device steps, density by purification)"
+                   if not a.host_steps
                    else "host steps on the workers",
                    "n_ao_min": int(min(n_ao)), "n_ao_max": int(max(n_ao)), "parallelism": "single GPU"},
         "mixed": runs["mixed"], "fp64": runs["fp64"],
@@ -544,8 +546,9 @@ def main():
     ap.add_argument("--jobs", type=int, default=512, help="jobs of the batch workload")
     ap.add_argument("--requests", type=int, default=200, help="requests of the serve workload")
     ap.add_argument("--rate", type=float, default=200.0, help="mean arrival rate of the serve workload (1/s)")
-    ap.add_argument("--device-steps", action="store_true",
-                    help="serve workload: keep each job's iterations (DIIS, diagonalization) on the device")
+    ap.add_argument("--host-steps", action="store_true",
+                    help="serve workload: run the iterations' linear algebra on host worker threads instead of "
+                         "the grouped device steps")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     a = ap.parse_args()

@@ -490,7 +490,8 @@ class Job:
                     cartesian=False):
         """prim_threshold defaults to 1e-17 in the accurate Boys mode and to 0 (every
         primitive quartet, as the reference) in the reference mode.  device_step runs
-        each iteration's DIIS, X F X, diagonalization and density on the device.
+        each iteration's DIIS, X F X and density on the device (True or 1: the
+        density by purification; 2: by rocSOLVER eigenvectors every iteration).
         cartesian: d shells as six Cartesian functions (6D) instead of the
         reference's five real solid harmonics."""
         if prim_threshold is None:
@@ -498,7 +499,7 @@ class Job:
         _check(lib().dsscf_job_set_options(self._h, diis_size, energy_threshold, max_iterations, boys_mode,
                                            schwarz_threshold, verbose), self._err)
         _check(lib().dsscf_job_set_precision(self._h, fp32_threshold, prim_threshold), self._err)
-        _check(lib().dsscf_job_set_step(self._h, 1 if device_step else 0), self._err)
+        _check(lib().dsscf_job_set_step(self._h, int(device_step)), self._err)
         _check(lib().dsscf_job_set_cartesian(self._h, 1 if cartesian else 0), self._err)
         return self
 

@@ -71,7 +71,7 @@ int dsscf_job_set_precision(dsscf_job* j, double fp32_threshold, double prim_thr
 
 int dsscf_job_set_step(dsscf_job* j, int on_device)
 {
-    if (!j || (on_device != 0 && on_device != 1)) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_step: bad argument");
+    if (!j || on_device < 0 || on_device > 2) return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_step: bad argument");
     if (on_device && (j->job.opt.diis_size < 1 || j->job.opt.diis_size > 16))
         return set_err(j, DSSCF_ERR_ARG, "dsscf_job_set_step: the device step keeps at most 16 DIIS vectors");
     j->job.opt.device_step = on_device;

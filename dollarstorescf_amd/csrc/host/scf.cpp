@@ -541,6 +541,7 @@ int beginDeviceSteps(Job& job)
                                 job.H.data(), job.Sd.data(), job.X.data(), &job.error))
         return rc;
     for (double& c : job.device_steps) c = 0.0;
+    scfdev::set_purify(job.dstep, job.opt.device_step != 2);
     return scfdev::set_density(job.dstep, job.Pa.data(), &job.error);
 }
 

@@ -82,6 +82,9 @@ int orbitals(Step* s, std::vector<double>& eps, std::string* err);
 // status (the fallback eigensolver's), the return value the group's;
 // psteps[i] (may be null) the purification steps taken, -1 for a fallback.
 int group_create(Group** g, int device, std::string* err);
+// on = false: the step's grouped iterations take the eigensolver route
+// every time (the route of the per-job functions; default on).
+void set_purify(Step* s, bool on);
 void group_destroy(Group* g);
 int group_pre(Group* g, Step* const* steps, int m, const int* slot, const int* count, double* dots, std::string* err);
 int group_post(Group* g, Step* const* steps, int m, const double* coef, double* sum_eps, double* tr_PG, int* rcs,

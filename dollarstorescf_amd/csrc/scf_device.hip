@@ -55,6 +55,7 @@ struct Step {
     double *w = nullptr, *work = nullptr;               // eigenvalues, dsyevd's E
     double *scal = nullptr;                             // [0]: tr(Pa G), [1..kmax]: dots
     double *part = nullptr, *pstate = nullptr;          // purification: tile partial sums, state (two parities each)
+    bool purify = true;                                 // grouped iterations: purification (else the eigensolver)
     rocblas_int* info = nullptr;
     double* h_scal = nullptr;                           // pinned host mirror of scal
     size_t blas_bytes = 0;                              // the handle's workspace (sized at creation)
@@ -90,7 +91,7 @@ constexpr int kResStride = kResDots + kMaxHist * kChunks;
 enum GemmOp { OP_PF0, OP_ST1, OP_XF, OP_T1X, OP_XPP, OP_T2X };
 
 struct GJob {
-    int n, nocc, slot, count;
+    int n, nocc, slot, count, purify;
     double *H, *S, *X, *Pa, *Pt, *J, *K, *G, *F0, *P, *T1, *T2, *F, *C, *Fh, *Eh;
     double *part, *st;
     double* coef;  // count coefficients (slot >= 0)
@@ -336,6 +337,7 @@ __global__ void __launch_bounds__(256) k_g_gemm(const GJob* __restrict__ jobs, i
     double* C = nullptr;
     bool twice = false;  // PURIFY: 2X - X^2
     if (PURIFY) {
+        if (!q.purify) return;
         const double* sin = q.st + 4 * (k_iter & 1);
         double* sout = q.st + 4 * ((k_iter + 1) & 1);
         if (sin[0] != 0.0) {  // finished earlier: carry the state forward
@@ -562,7 +564,7 @@ __global__ void k_g_purify_status(const GJob* __restrict__ jobs, int m, int laun
 {
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
         const double* st = jobs[i].st + 4 * (launches & 1);
-        jobs[i].res[kResStatus] = st[0] != 0.0 ? st[1] : -1.0;
+        jobs[i].res[kResStatus] = jobs[i].purify && st[0] != 0.0 ? st[1] : -1.0;
     }
 }
 
@@ -572,7 +574,7 @@ __global__ void __launch_bounds__(256) k_g_purify_end(const GJob* __restrict__ j
 {
     const GJob& q = jobs[blockIdx.y];
     const double* st = q.st + 4 * (launches & 1);
-    const bool done = st[0] != 0.0;
+    const bool done = q.purify && st[0] != 0.0;
     const int kf = (int)st[1];
     const double* Pp = (kf & 1) ? q.C : q.T1;
     size_t b, e;
@@ -904,6 +906,8 @@ void buffers(Step* s, const double** Pt, const double** Pa, double** J, double**
     *K = s->K;
 }
 
+void set_purify(Step* s, bool on) { s->purify = on; }
+
 int orbitals(Step* s, std::vector<double>& eps, std::string* err)
 {
     STEP_HIP(hipSetDevice(s->device));
@@ -1013,6 +1017,7 @@ int group_pre(Group* g, Step* const* steps, int m, const int* slot, const int* c
         o.nocc = s->nocc;
         o.slot = slot[i];
         o.count = slot[i] >= 0 ? count[i] : 0;
+        o.purify = s->purify ? 1 : 0;
         o.H = s->H, o.S = s->S, o.X = s->X, o.Pa = s->Pa, o.Pt = s->Pt, o.J = s->J, o.K = s->K, o.G = s->G;
         o.F0 = s->F0, o.P = s->P, o.T1 = s->T1, o.T2 = s->T2, o.F = s->F, o.C = s->C, o.Fh = s->Fh, o.Eh = s->Eh;
         o.part = s->part;
@@ -1086,7 +1091,8 @@ int group_post(Group* g, Step* const* steps, int m, const double* coef, double* 
                                 g->stream));
         STEP_HIP(hipStreamSynchronize(g->stream));
         bool all = true;
-        for (int i = 0; i < m && all; ++i) all = g->h_res[(size_t)kResStride * i + kResStatus] >= 0.0;
+        for (int i = 0; i < m && all; ++i)
+            all = g->h_res[(size_t)kResStride * i + kResStatus] >= 0.0 || !g->h_jobs[i].purify;
         if (all) break;
     }
     k_g_purify_end<<<dim3(kChunks, m), 256, 0, g->stream>>>(g->d_jobs, launches);
@@ -1113,7 +1119,7 @@ int group_post(Group* g, Step* const* steps, int m, const double* coef, double* 
             continue;
         }
         // purification did not reach its limit (orbitals nocc and nocc+1 too
-        // close): this job takes the eigensolver route on its own stream
+        // close) or is off for this job: the eigensolver route on its own stream
         Step* s = steps[i];
         const GJob& o = g->h_jobs[i];
         std::vector<double> eps;

@@ -31,11 +31,13 @@ int dsscf_job_set_precision(dsscf_job* job, double fp32_threshold, double prim_t
 /* Where the host part of each iteration runs (no reference counterpart):
  * 0 = host (default): G, DIIS, F' = X F X, the eigensolver and the density
  * on the CPU, J/K copied across per iteration; 1 = device: the same steps
- * (SCFSolver.cpp:45-94 and 118-143, DIIS.ipp:18-100, dspev_ in
- * Lapack.cpp:12-39) with every matrix resident in HBM (rocBLAS dgemm,
- * rocSOLVER dsyevd, small kernels); only the DIIS B matrix row, the orbital
- * energies and tr(Pa G) come to the host.  The device step keeps at most 16
- * DIIS vectors.  Used by dsscf_job_solve. */
+ * (SCFSolver.cpp:45-94 and 118-143, DIIS.ipp:18-100) with every matrix
+ * resident in HBM, the density of F' = X F X by purification (the projector
+ * onto its nocc lowest orbitals, the density dspev_ in Lapack.cpp:12-39
+ * leads to) and rocSOLVER dsyevd where purification does not finish and for
+ * the final orbitals; only the DIIS B matrix row and the energy terms come
+ * to the host; 2 = device, with the dsyevd eigenvectors every iteration.
+ * The device step keeps at most 16 DIIS vectors.  Used by dsscf_job_solve. */
 int dsscf_job_set_step(dsscf_job* job, int on_device);
 /* on = 1: d shells become the six Cartesian functions xx, xy, xz, yy, yz, zz
  * (each primitive normalized; "6D", as 6-31G* is often run) instead of the

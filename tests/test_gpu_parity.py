@@ -387,7 +387,7 @@ def _serve_job(basis, Z, xyz, fp32=0.0):
     return job.set_basis(basis).set_molecule(Z, xyz)
 
 
-@pytest.mark.parametrize("fp32,device_steps", [(0.0, False), (1e-5, False), (0.0, True)])
+@pytest.mark.parametrize("fp32,device_steps", [(0.0, False), (1e-5, False), (0.0, True), (1e-5, True)])
 def test_server_result_same_alone_and_in_mix(fp32, device_steps):
     """A job served alone and the same job served inside a mix of other
     molecules and bases (arriving while others iterate, with a footprint
@@ -395,8 +395,9 @@ def test_server_result_same_alone_and_in_mix(fp32, device_steps):
     every job in the mix equals its own Job.solve energy within 1e-9 Eh
     (FP64) and, with FP32 below Schwarz 1e-5, within 1e-6 Eh of it.  With
     device steps the densities, J/K, Fock matrices and DIIS histories stay
-    in HBM and the fused builds read and write them in place; the reference
-    energies are still the host-step Job.solve ones."""
+    in HBM, the fused builds read and write them in place and each round's
+    iterations run as one grouped launch sequence (density by purification);
+    the reference energies are still the host-step Job.solve ones."""
     probe = ("6-31G", G.water_box(3, seed=4))
     others = [("STO-3G", G.water()), ("def2-SVP", G.water()), ("6-31G", G.alkane(3)),
               ("STO-3G", G.water_box(5, seed=2)), ("6-31GP", G.water_box(2, seed=7)), ("def2-SVP", G.alkane(1))]

@@ -57,6 +57,25 @@ def test_device_step_matches_host_step(case):
         assert abs(v - jd.energies()[k]) < 1e-5, k
 
 
+@pytest.mark.parametrize("case", ["water8_631g", "butane_def2svp"])
+def test_device_step_purification_matches_eigensolver_route(case):
+    """The device step's two routes to the density: purification of
+    F' = X F X (device_step=1) and rocSOLVER eigenvectors (device_step=2, the
+    route an unfinished purification falls back to) give the same
+    iterations, per-iteration energies within 1e-9 Eh and final densities
+    within 1e-8; the counts show which route ran."""
+    Z, xyz, basis = {"water8_631g": (*G.water_lattice(1), "6-31G"), "butane_def2svp": (*G.alkane(4), "def2-SVP")}[case]
+    jp, ep, itp = _solve(basis, Z, xyz, 1)
+    je, ee, ite = _solve(basis, Z, xyz, 2)
+    cp, ce = jp.get("device_steps"), je.get("device_steps")
+    assert cp[0] == itp and cp[2] == 0 and cp[1] >= 10 * itp
+    assert ce[0] == ite and ce[1] == 0 and ce[2] == ite
+    assert itp == ite
+    assert np.max(np.abs(jp.history() - je.history())) < 1e-9
+    assert abs(ep - ee) < 1e-10
+    assert np.max(np.abs(jp.get("Pa") - je.get("Pa"))) < 1e-8
+
+
 def test_device_step_orbitals():
     """Orbitals of the device step solve the Roothaan-Hall equations
     F C = S C eps with C^T S C = 1 (F = H + G at the converged density), and

@@ -159,16 +159,19 @@ def test_served_stream_equals_alone(stream_alone, fp32):
     assert de.max() < (5e-9 if fp32 == 0.0 else 1e-6)
 
 
-def test_iteration_cap_reports_noconv_alone_and_served():
+@pytest.mark.parametrize("device_steps", [False, True])
+def test_iteration_cap_reports_noconv_alone_and_served(device_steps):
     """Forced failure: an iteration cap of 1 ends a solve with
     DSSCF_ERR_NOCONV (6), both from Job.solve and as a served job's status,
-    while a job beside it in the same server still converges."""
+    while a job beside it in the same server still converges; with host
+    and with device-resident iterations."""
     Z, xyz = G.water()
-    job = ds.Job(0).set_options(max_iterations=1).set_basis("STO-3G").set_molecule(Z, xyz).construct()
+    job = ds.Job(0).set_options(max_iterations=1, device_step=device_steps).set_basis("STO-3G")
+    job.set_molecule(Z, xyz).construct()
     with pytest.raises(ds.DsscfError) as ei:
         job.solve()
     assert ei.value.code == 6
-    srv = ds.Server(0, budget_bytes=1e9, max_active=4, workers=2)
+    srv = ds.Server(0, budget_bytes=1e9, max_active=4, workers=2, device_steps=device_steps)
     srv.submit(ds.Job(0).set_options(max_iterations=1).set_basis("STO-3G").set_molecule(Z, xyz), 0)
     srv.submit(ds.Job(0).set_options(max_iterations=60).set_basis("STO-3G").set_molecule(Z, xyz), 1)
     done = {d["tag"]: d for d in srv.run_until_idle()}

@@ -84,6 +84,9 @@ const double kEriScale = 2.0 * std::pow(3.14159265358979, 2.5);
 // Smallest primitive-pair run of a sub-pair.
 constexpr int kMinSubPrims = 5;
 constexpr int kMaxGroup = 16;  // lanes per quartet, at most
+// Kets per work unit that a class schedule may ask for (the default stays at
+// 64..512).
+constexpr int kMaxKetsPerUnit = 4096;
 
 // Lanes per quartet for a (sub-)quartet of nb x nk primitive quartets: the
 // largest power of two G <= kMaxGroup with nb*nk/G >= target, split into
@@ -701,7 +704,7 @@ int dsscf_prepare(dsscf_ctx* c)
             // kets per work unit: enough units to fill the chip (~2048), 64..512, multiple of 64
             j.kChunk = (int)std::min<long long>(512, std::max<long long>(64, (est / 2048 + 63) / 64 * 64));
             if (!c->sched_chunk.empty() && c->sched_chunk[j.cls] > 0)
-                j.kChunk = std::min(512, std::max(64, (c->sched_chunk[j.cls] + 63) / 64 * 64));
+                j.kChunk = std::min(kMaxKetsPerUnit, std::max(64, (c->sched_chunk[j.cls] + 63) / 64 * 64));
             if (!c->sched_gp.empty() && c->sched_gp[j.cls] != 0.0) j.gp = c->sched_gp[j.cls];
             c->chunk[j.cls] = j.kChunk;
             // primitive pairs per sub-pair, both sides: P^2 primitive quartets ~ item_flops,
@@ -2296,8 +2299,8 @@ int dsscf_set_class_schedule(dsscf_ctx* c, const double* work_item_flops, const 
     if (group_prims) c->sched_gp.assign(group_prims, group_prims + ncls); else c->sched_gp.clear();
     if (kets_per_unit) c->sched_chunk.assign(kets_per_unit, kets_per_unit + ncls); else c->sched_chunk.clear();
     for (int k = 0; k < (int)c->sched_chunk.size(); ++k)
-        if (c->sched_chunk[k] < 0 || c->sched_chunk[k] > 512)
-            return fail(DSSCF_ERR_ARG, "dsscf_set_class_schedule: kets per unit 0..512");
+        if (c->sched_chunk[k] < 0 || c->sched_chunk[k] > kMaxKetsPerUnit)
+            return fail(DSSCF_ERR_ARG, "dsscf_set_class_schedule: kets per unit 0..4096");
     c->prepared = false;
     return DSSCF_OK;
 }

@@ -668,6 +668,12 @@ def gen_wave_class(c, cartd=False):
     o.append("  static constexpr int L = %d, NBASE = %d, NVE = %d, ZERO = %d, NLEV = %d, NT = %d, NHE = %d, NHLEV = %d,"
              " NY = %d, NCART = %d, NAO = %d, LDS = %d;" % (L, nbase, len(vrr_rows), zero, nlev, len(tlist), len(hent),
                                                            len(hlev), ny, len(cart_y), NAO, lds))
+    # level bounds as constants, so that a kernel can unroll its walk over
+    # the levels and keep each lane's table entries in registers
+    o.append("  static constexpr int VLEV[%d] = {%s};" % (len(lev_off), ",".join("%d" % v for v in lev_off)))
+    o.append("  static constexpr short BASE[%d] = {%s};" % (len(base), ",".join("%d" % v for v in base)))
+    o.append("  static constexpr int HLEV[%d] = {%s};" % (max(2 * len(hlev), 1),
+                                                         ",".join("%d" % v for lv in hlev for v in lv) or "0"))
     for fld, nm in (("vrr", "int"), ("vlev", "int"), ("base", "short"), ("tgt", "short"), ("hrr", "int"),
                     ("hdir", "char"), ("hlev", "int"), ("cart", "short")):
         o.append("  static __device__ __forceinline__ const %s* %s() { return wv_%s_%s; }" % (nm, fld, name, fld))

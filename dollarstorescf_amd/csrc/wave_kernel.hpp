@@ -14,11 +14,17 @@
 // transformed to the reference's AO functions, and the result is digested
 // into J/K (each output element owned by one lane: one atomic per element and
 // quartet), written out, or reduced to a Schwarz bound.
+//
+// The level walks are unrolled at compile time (the level bounds are
+// constants of the generated tables): each lane loads its VRR and HRR table
+// entries into registers once per block, and reads an entry's coefficients
+// (A, W, 1/2zeta, rho/zeta by the entry's side and direction; A-B or C-D)
+// from a small per-group LDS block written once per primitive quartet or
+// quartet, so no table read waits on global memory inside the walks.
 #pragma once
+#include <utility>
 
 enum { WAVE_JK = 0, WAVE_BLOCK = 1, WAVE_SCHWARZ = 2 };
-
-__device__ __forceinline__ double pick3(int d, double x, double y, double z) { return d == 0 ? x : (d == 1 ? y : z); }
 
 // Quartets per wavefront: the largest power of two up to 8 whose G LDS
 // tables fit in DSSCF_WAVE_LDS_BUDGET bytes.
@@ -34,6 +40,45 @@ struct WaveGroup {
                                                                   : 1;
 };
 
+// Compile-time loop: f(integral_constant<int, 0>) .. f(integral_constant<int, N-1>).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// A lane's share of the VRR and HRR tables: level l's entries are walked by
+// the LG lanes of a group in slots of LG entries; slot j of level l holds
+// entry start(l) + lg + LG j when that is below the level's end.  The slots of
+// all levels are numbered consecutively, so each lane keeps its table entries
+// in registers (loaded once per block) instead of reading them per entry.
+template <typename W, int LG>
+struct WaveSlots {
+    static constexpr int vcnt(int l) { return W::VLEV[l + 1] - W::VLEV[l]; }
+    static constexpr int vslots(int l) { return (vcnt(l) + LG - 1) / LG; }
+    static constexpr int v0(int l)
+    {
+        int s = 0;
+        for (int i = 0; i < l; ++i) s += vslots(i);
+        return s;
+    }
+    static constexpr int NV = v0(W::NLEV);
+    static constexpr int hcnt(int l) { return W::HLEV[2 * l + 1] - W::HLEV[2 * l]; }
+    static constexpr int hslots(int l) { return (hcnt(l) + LG - 1) / LG; }
+    static constexpr int h0(int l)
+    {
+        int s = 0;
+        for (int i = 0; i < l; ++i) s += hslots(i);
+        return s;
+    }
+    static constexpr int NH = h0(W::NHLEV);
+};
+
 __device__ __forceinline__ int wave_max(int v)
 {
 #pragma unroll
@@ -44,8 +89,12 @@ __device__ __forceinline__ int wave_max(int v)
 // MODE WAVE_JK: block b covers quartets G*(b - blk0) .. +G-1 of the segment
 // its index falls in (items, n, pairs, pp unused).  WAVE_BLOCK / WAVE_SCHWARZ:
 // items over one context, grid-stride over groups of G.
+// At least two wavefronts per SIMD: the register file split in two leaves
+// 256 VGPRs a lane (a few spill), which lets a CU hold as many blocks as its
+// LDS allows ((dd|dp): six, 0.86 -> 0.60 ms on C20H42/def2-SVP, where one
+// wavefront per SIMD held four).
 template <int la, int lb, int lc, int ld, int BM, int MODE, int C>
-__global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int n,
+__global__ void __launch_bounds__(64, 2) k_wave(const void* __restrict__ items, int n,
                                              const ShellPair* __restrict__ pairs_in,
                                              const PrimPair* __restrict__ pp_in, const double* __restrict__ btab,
                                              const JKSeg* __restrict__ segs, const int* __restrict__ seg_blk0,
@@ -60,16 +109,58 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
     constexpr int ncA = (la + 1) * (la + 2) / 2, ncB = (lb + 1) * (lb + 2) / 2;
     constexpr int ncC = (lc + 1) * (lc + 2) / 2, ncD = (ld + 1) * (ld + 2) / 2;
+    typedef WaveSlots<W, LG> WS;
     __shared__ double smem[G * W::LDS];
+    // per group: the VRR coefficients of the current primitive quartet as
+    // (A, W) pairs [bra x, y, z, ket x, y, z] and (1/2zeta, rho/zeta) pairs
+    // [bra, ket], then 1/2(p+q); the HRR distances A-B (x, y, z) and C-D
+    __shared__ __align__(16) double wcoef[G][24];
     const int grp = (int)threadIdx.x / LG, lg = (int)threadIdx.x % LG;
     double* __restrict__ sm = smem + grp * W::LDS;
-    const int* __restrict__ vrr = W::vrr();
-    const int* __restrict__ vlev = W::vlev();
-    const short* __restrict__ tgt = W::tgt();
-    const int* __restrict__ hrr = W::hrr();
-    const char* __restrict__ hdir = W::hdir();
-    const int* __restrict__ hlev = W::hlev();
-    const short* __restrict__ carty = W::cart();
+    double* __restrict__ cf = wcoef[grp];
+    // this lane's VRR entries (three words each) and HRR entries (two 13-bit
+    // operand indices, then direction and stage in bits 26..28), all levels;
+    // the positions in Y of its [e0|f0] targets; the group's copy of the
+    // Cartesian-component map of the AO transform
+    int vr0[WS::NV > 0 ? WS::NV : 1], vr1[WS::NV > 0 ? WS::NV : 1], vr2[WS::NV > 0 ? WS::NV : 1];
+    int hr[WS::NH > 0 ? WS::NH : 1];
+    static_assert(W::NY < 8192, "HRR operand indices take 13 bits");
+    int tg[SLOTS];
+    __shared__ short scart[W::NCART];
+    for (int i = (int)threadIdx.x; i < W::NCART; i += (int)blockDim.x) scart[i] = W::cart()[i];
+    const short* __restrict__ carty = scart;
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+        const int tt = lg + LG * s;
+        tg[s] = tt < W::NT ? W::tgt()[tt] : W::ZERO;
+    }
+    {
+        const int* __restrict__ vrr = W::vrr();
+        const int* __restrict__ hrr = W::hrr();
+        const char* __restrict__ hdir = W::hdir();
+        static_for<W::NLEV>([&](auto lev) {
+            constexpr int l = decltype(lev)::value;
+            static_for<WS::vslots(l)>([&](auto jj) {
+                constexpr int j = decltype(jj)::value;
+                constexpr int s = WS::v0(l) + j;
+                // an empty slot reads the level's first entry (never used)
+                const int k = W::VLEV[l] + (lg + LG * j < WS::vcnt(l) ? lg + LG * j : 0);
+                vr0[s] = vrr[3 * k];
+                vr1[s] = vrr[3 * k + 1];
+                vr2[s] = vrr[3 * k + 2];
+            });
+        });
+        static_for<W::NHLEV>([&](auto lev) {
+            constexpr int l = decltype(lev)::value;
+            static_for<WS::hslots(l)>([&](auto jj) {
+                constexpr int j = decltype(jj)::value;
+                constexpr int s = WS::h0(l) + j;
+                const int k = W::HLEV[2 * l] + (lg + LG * j < WS::hcnt(l) ? lg + LG * j : 0);
+                const int h = hrr[k];
+                hr[s] = (h & 0x1fff) | (((h >> 16) & 0x1fff) << 13) | ((int)hdir[k] << 26);
+            });
+        });
+    }
 
     const JKSeg* S = nullptr;
     const ShellPair* __restrict__ pairs = pairs_in;
@@ -127,42 +218,53 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
                     sm[W::ZERO] = 0.0;
 #pragma unroll
                     for (int m = 0; m <= L; ++m) {
-                        const int b = W::base()[m];
+                        const int b = W::BASE[m];
                         if (b >= 0) sm[b] = Fm[m];
                     }
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        cf[2 * d] = g.PA[d];
+                        cf[2 * d + 1] = g.WP[d];
+                        cf[6 + 2 * d] = g.QC[d];
+                        cf[6 + 2 * d + 1] = g.WQ[d];
+                    }
+                    cf[12] = g.o2p;
+                    cf[13] = g.rp;
+                    cf[14] = g.o2q;
+                    cf[15] = g.rq;
+                    cf[16] = g.o2pq;
                 }
                 __syncthreads();
-                for (int lev = 0; lev < W::NLEV; ++lev) {
-                    const int e1 = vlev[lev + 1];
-                    for (int k = vlev[lev] + lg; k < e1; k += LG) {
-                        const int r0 = vrr[3 * k], r1 = vrr[3 * k + 1], r2 = vrr[3 * k + 2];
-                        const int s0 = r0 & 0xffff, s1 = (r0 >> 16) & 0xffff;
-                        const int s2 = r1 & 0xffff, s3 = (r1 >> 16) & 0xffff, s4 = r2 & 0xffff;
-                        const int kind = (r2 >> 16) & 3, dir = (r2 >> 18) & 3;
-                        const double n1 = (double)((r2 >> 20) & 15), n2 = (double)((r2 >> 24) & 15);
-                        double A, Wc, o2, rr;
-                        if (kind == 1) {
-                            A = pick3(dir, g.PA[0], g.PA[1], g.PA[2]);
-                            Wc = pick3(dir, g.WP[0], g.WP[1], g.WP[2]);
-                            o2 = g.o2p;
-                            rr = g.rp;
-                        } else {
-                            A = pick3(dir, g.QC[0], g.QC[1], g.QC[2]);
-                            Wc = pick3(dir, g.WQ[0], g.WQ[1], g.WQ[2]);
-                            o2 = g.o2q;
-                            rr = g.rq;
+                // the levels and slots unrolled: per entry its coefficients
+                // from the group's LDS block (the entry's kind and direction
+                // select them), its operands from the levels below
+                const double o2pq = cf[16];
+                static_for<W::NLEV>([&](auto lev) {
+                    constexpr int l = decltype(lev)::value;
+                    static_for<WS::vslots(l)>([&](auto jj) {
+                        constexpr int j = decltype(jj)::value;
+                        constexpr int s = WS::v0(l) + j;
+                        if (lg + LG * j < WS::vcnt(l)) {
+                            int r0 = vr0[s], r1 = vr1[s], r2 = vr2[s];
+                            asm volatile("" : "+v"(r0), "+v"(r1), "+v"(r2));  // decode per use
+                            const int s0 = r0 & 0xffff, s1 = (r0 >> 16) & 0xffff;
+                            const int s2 = r1 & 0xffff, s3 = (r1 >> 16) & 0xffff, s4 = r2 & 0xffff;
+                            const int ket_side = ((r2 >> 16) & 3) != 1, dir = (r2 >> 18) & 3;
+                            const double n1 = (double)((r2 >> 20) & 15), n2 = (double)((r2 >> 24) & 15);
+                            const double2 aw = reinterpret_cast<const double2*>(cf)[3 * ket_side + dir];
+                            const double2 orr = reinterpret_cast<const double2*>(cf)[6 + ket_side];
+                            double v = aw.x * sm[s0] + aw.y * sm[s1];
+                            v += n1 * orr.x * (sm[s2] - orr.y * sm[s3]);
+                            v += n2 * o2pq * sm[s4];
+                            sm[W::NBASE + W::VLEV[l] + lg + LG * j] = v;
                         }
-                        double v = A * sm[s0] + Wc * sm[s1];
-                        v += n1 * o2 * (sm[s2] - rr * sm[s3]);
-                        v += n2 * g.o2pq * sm[s4];
-                        sm[W::NBASE + k] = v;
-                    }
+                    });
                     __syncthreads();
-                }
+                });
 #pragma unroll
                 for (int s = 0; s < SLOTS; ++s) {
                     const int tt = lg + LG * s;
-                    if (tt < W::NT) acc[s] += sm[tgt[tt]];
+                    if (tt < W::NT) acc[s] += sm[tg[s]];
                 }
                 __syncthreads();  // (a block is one wavefront: idle groups need no barrier of their own)
             }
@@ -173,18 +275,30 @@ __global__ void __launch_bounds__(64) k_wave(const void* __restrict__ items, int
             const int tt = lg + LG * s;
             if (tt < W::NT) sm[tt] = acc[s];
         }
-        __syncthreads();
-        for (int hl = 0; hl < W::NHLEV; ++hl) {
-            const int e1 = hlev[2 * hl + 1];
-            for (int k = hlev[2 * hl] + lg; k < e1; k += LG) {
-                const int h = hrr[k];
-                const int d = hdir[k];
-                const double c = (d >> 2) ? pick3(d & 3, ket.AB[0], ket.AB[1], ket.AB[2])
-                                          : pick3(d & 3, bra.AB[0], bra.AB[1], bra.AB[2]);
-                sm[W::NT + k] = sm[h & 0xffff] + c * sm[(h >> 16) & 0xffff];
+        if (lg == 0) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                cf[17 + d] = bra.AB[d];
+                cf[20 + d] = ket.AB[d];
             }
-            __syncthreads();
         }
+        __syncthreads();
+        static_for<W::NHLEV>([&](auto lev) {
+            constexpr int l = decltype(lev)::value;
+            static_for<WS::hslots(l)>([&](auto jj) {
+                constexpr int j = decltype(jj)::value;
+                constexpr int s = WS::h0(l) + j;
+                if (lg + LG * j < WS::hcnt(l)) {
+                    int h = hr[s];
+                    asm volatile("" : "+v"(h));  // decode here, not hoisted out of the quartet loop
+                    const int d = (h >> 26) & 7;
+                    // direction d & 3 of A-B (stage 0) or C-D (stage 1)
+                    const double c = cf[17 + 3 * (d >> 2) + (d & 3)];
+                    sm[W::NT + W::HLEV[2 * l] + lg + LG * j] = sm[h & 0x1fff] + c * sm[(h >> 13) & 0x1fff];
+                }
+            });
+            __syncthreads();
+        });
         // Cartesian -> AO, one index at a time (d, c, b, a): each stage output
         // sums the at most three Cartesian components of one AO.  The AO index
         // m is the outer, wavefront-uniform loop, so its terms (weight and

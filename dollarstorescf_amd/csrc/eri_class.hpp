@@ -301,28 +301,18 @@ __global__ void __launch_bounds__(256) k_jk(const JKSeg* __restrict__ segs, cons
                 atomic_add_f64(&Jh[(size_t)(a0 + k / NB) * N + b0 + k % NB], 2.0 * x);
         }
         __syncthreads();
-        // element i = r N + col of the rows; (r, col) advance by the block
-        // size without a per-element integer division
-        const int step_r = (int)blockDim.x / N, step_c = (int)blockDim.x % N;
-        int r = (int)threadIdx.x / N, col = (int)threadIdx.x % N;
-        double* __restrict__ const Kbh = beta ? S.Kbh + slice : nullptr;
         for (int i = threadIdx.x; i < nrow * N; i += blockDim.x) {
             const double x = krow[i];
             if (x != 0.0) {
+                int r = i / N;
+                const int col = i % N;
                 double* __restrict__ K = Kh;
-                int rr = r;
-                if (rr >= NA + NB) {  // K[Db] rows of an unrestricted segment
-                    rr -= NA + NB;
-                    K = Kbh;
+                if (r >= NA + NB) {  // K[Db] rows of an unrestricted segment
+                    r -= NA + NB;
+                    K = S.Kbh + slice;
                 }
-                const int row = rr < NA ? a0 + rr : b0 + (rr - NA);
+                const int row = r < NA ? a0 + r : b0 + (r - NA);
                 atomic_add_f64(&K[(size_t)row * N + col], x);
-            }
-            r += step_r;
-            col += step_c;
-            if (col >= N) {
-                col -= N;
-                ++r;
             }
         }
         __syncthreads();  // the flush has read krow before the next unit clears it

@@ -291,8 +291,9 @@ int dsscf_batch_autotune(dsscf_batch* batch, int* chosen);
  * entry keeps the options' value or the default):
  *   work_item_flops -- sub-pair size (dsscf_options.work_item_flops) per class
  *   group_prims     -- lane-group target (dsscf_options.group_prims) per class
- *   kets_per_unit   -- quartets per work unit (one block), 64..512 (rounded up
- *                      to a multiple of 64) instead of the size-based default
+ *   kets_per_unit   -- quartets per work unit (one block), 64..4096 (rounded
+ *                      up to a multiple of 64) instead of the size-based
+ *                      default (64..512); larger values are refused
  * No reference counterpart.  Results agree to rounding for any schedule. */
 int dsscf_set_class_schedule(dsscf_ctx* ctx, const double* work_item_flops, const double* group_prims,
                              const int* kets_per_unit);
